@@ -1,0 +1,161 @@
+"""Python face of the C-ABI: one :class:`Context` per device owning the
+scene, the BVH and the render buffers (a gi_ctx)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import COUNTERS, HIT, LIGHT, MATERIAL, OPTS, TRI, TUNING, VIEW, GiError, ptr
+
+
+def _check(lib, ctx, rc):
+    if rc != _lib.GI_OK:
+        msg = lib.gi_last_error(ctx).decode() if ctx else ""
+        raise GiError(rc, msg)
+
+
+class Context:
+    """A gi_ctx: scene upload, tracing primitives, renderer, telemetry."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        rc = self.lib.gi_create(C.byref(h), device)
+        if rc != _lib.GI_OK:
+            raise GiError(rc, f"gi_create(device={device})")
+        self.h = h
+        self.scene = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc):
+        _check(self.lib, self.h, rc)
+
+    # ---- scene ---------------------------------------------------------
+    def upload(self, scene, max_leaf: int = 4, bvh2: tuple | None = None):
+        tris = np.ascontiguousarray(scene.tris, TRI)
+        self._ck(self.lib.gi_upload_triangles(self.h, ptr(tris), len(tris)))
+        if scene.mats is not None and len(scene.mats):
+            mats = np.ascontiguousarray(scene.mats, MATERIAL)
+            self._ck(self.lib.gi_upload_materials(self.h, ptr(mats), len(mats)))
+        if scene.lights is not None and len(scene.lights):
+            lights = np.ascontiguousarray(scene.lights, LIGHT)
+            self._ck(self.lib.gi_upload_lights(self.h, ptr(lights), len(lights)))
+        if bvh2 is None:
+            self._ck(self.lib.gi_build_bvh(self.h, max_leaf))
+        else:
+            nodes, idx = bvh2
+            nodes = np.ascontiguousarray(nodes, np.float32)
+            idx = np.ascontiguousarray(idx, np.uint32)
+            self._ck(self.lib.gi_upload_bvh2(self.h, ptr(nodes), len(nodes) // 8, ptr(idx), len(idx)))
+        self.scene = scene
+        return self
+
+    # ---- tracing primitives ---------------------------------------------
+    def trace_closest(self, orig, dirs, max_t) -> np.ndarray:
+        o = np.ascontiguousarray(orig, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(max_t, np.float32).reshape(-1)
+        out = np.zeros(len(t), HIT)
+        self._ck(self.lib.gi_trace_closest(self.h, ptr(o), ptr(d), ptr(t), len(t), ptr(out)))
+        return out
+
+    def trace_any(self, orig, dirs, max_t) -> np.ndarray:
+        o = np.ascontiguousarray(orig, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(max_t, np.float32).reshape(-1)
+        out = np.zeros(len(t), np.uint8)
+        self._ck(self.lib.gi_trace_any(self.h, ptr(o), ptr(d), ptr(t), len(t), ptr(out)))
+        return out
+
+    def camera_rays(self, view, sample: int, seed: int = 0):
+        v = np.ascontiguousarray(view, VIEW).reshape(1)
+        n = int(v["w"][0]) * int(v["h"][0])
+        o = np.zeros((n, 3), np.float32)
+        d = np.zeros((n, 3), np.float32)
+        t = np.zeros(n, np.float32)
+        self._ck(self.lib.gi_generate_camera_rays(self.h, ptr(v), sample, seed, ptr(o), ptr(d), ptr(t)))
+        return o, d, t
+
+    # ---- renderer -------------------------------------------------------
+    def render(self, views=None, opts=None, **kw):
+        views = np.ascontiguousarray(self.scene.views if views is None else views, VIEW).reshape(-1)
+        o = np.array(self.scene.opts if opts is None else opts, OPTS).reshape(1).copy()
+        for k, v in kw.items():
+            o[k] = v
+        self._views = views
+        self._ck(self.lib.gi_render(self.h, ptr(views), len(views), ptr(o)))
+        return self
+
+    def image(self, view: int = 0) -> np.ndarray:
+        v = self._views[view]
+        out = np.zeros((int(v["h"]), int(v["w"]), 3), np.float32)
+        self._ck(self.lib.gi_get_image(self.h, view, ptr(out)))
+        return out
+
+    def images(self):
+        return [self.image(i) for i in range(len(self._views))]
+
+    def progress_begin(self, views=None, opts=None):
+        views = np.ascontiguousarray(self.scene.views if views is None else views, VIEW).reshape(-1)
+        o = np.array(self.scene.opts if opts is None else opts, OPTS).reshape(1).copy()
+        self._views = views
+        self._ck(self.lib.gi_progress_begin(self.h, ptr(views), len(views), ptr(o)))
+
+    def progress_step(self) -> bool:
+        done = C.c_int32(0)
+        self._ck(self.lib.gi_progress_step(self.h, C.byref(done)))
+        return bool(done.value)
+
+    def counters(self) -> dict:
+        c = np.zeros(1, COUNTERS)
+        self._ck(self.lib.gi_get_counters(self.h, ptr(c)))
+        return {k: c[k][0].item() for k in COUNTERS.names}
+
+    def set_tuning(self, **kw):
+        t = np.zeros(1, TUNING)
+        self._ck(self.lib.gi_get_tuning(self.h, ptr(t)))
+        for k, v in kw.items():
+            t[k] = v
+        self._ck(self.lib.gi_set_tuning(self.h, ptr(t)))
+
+    def tuning(self) -> dict:
+        t = np.zeros(1, TUNING)
+        self._ck(self.lib.gi_get_tuning(self.h, ptr(t)))
+        return {k: int(t[k][0]) for k in TUNING.names}
+
+
+def build_bvh2(tris: np.ndarray, max_leaf: int = 4):
+    """Host binned-SAH BVH (no device needed): (nodes float32[8*n], indices uint32[m])
+    in the reference's 2 x float4 layout (src/tracers.cu:26-36)."""
+    lib = _lib.load()
+    tris = np.ascontiguousarray(tris, TRI)
+    h = C.c_void_p()
+    rc = lib.gi_bvh_build(ptr(tris), len(tris), max_leaf, C.byref(h))
+    if rc != _lib.GI_OK:
+        raise GiError(rc, "gi_bvh_build")
+    try:
+        nn, ni = C.c_int64(), C.c_int64()
+        lib.gi_bvh_sizes(h, C.byref(nn), C.byref(ni))
+        nodes = np.zeros(8 * nn.value, np.float32)
+        idx = np.zeros(max(1, ni.value), np.uint32)
+        rc = lib.gi_bvh_export2(h, ptr(nodes), ptr(idx))
+        if rc != _lib.GI_OK:
+            raise GiError(rc, "gi_bvh_export2")
+        return nodes, idx[: ni.value]
+    finally:
+        lib.gi_bvh_free(h)

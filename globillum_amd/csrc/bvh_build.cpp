@@ -1,0 +1,437 @@
+// Host BVH construction: binned SAH (stands in for rta's "-b bsah",
+// src/hl-main.cpp:377-379), import/export of the reference's 2 x float4
+// node layout (src/tracers.cu:26-36), and the collapse to the 4-wide
+// 128-byte nodes the gfx950 traversal kernels stream.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "gi_internal.h"
+
+namespace gi {
+namespace {
+
+struct Box {
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+  float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  void grow(const float *p) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], p[k]);
+      hi[k] = std::max(hi[k], p[k]);
+    }
+  }
+  void grow(const Box &b) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], b.lo[k]);
+      hi[k] = std::max(hi[k], b.hi[k]);
+    }
+  }
+  bool empty() const { return lo[0] > hi[0]; }
+  float area() const {
+    if (empty()) return 0.f;
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.f * (dx * dy + dy * dz + dz * dx);
+  }
+};
+
+// Relative padding that keeps float slab tests conservative for points on
+// the triangles: 2^-20 of the largest coordinate magnitude / extent.
+void pad_box(float *lo, float *hi) {
+  float m = 0.f;
+  for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+  for (int k = 0; k < 3; ++k) m = std::max(m, hi[k] - lo[k]);
+  float p = m * 0x1p-20f + 1e-30f;
+  for (int k = 0; k < 3; ++k) {
+    lo[k] -= p;
+    hi[k] += p;
+  }
+}
+
+constexpr int kBins = 32;
+constexpr int kSahDepth = 48;  // beyond this depth, split at the object median
+
+struct Builder {
+  const gi_tri *tris;
+  std::vector<Box> tbox;
+  std::vector<float> cen;  // 3 per triangle
+  std::vector<uint32_t> &perm;
+  std::vector<Bvh2Node> &nodes;
+  int32_t max_leaf;
+  int32_t max_depth = 0;
+
+  Builder(const gi_tri *t, int64_t n, int32_t ml, std::vector<uint32_t> &p, std::vector<Bvh2Node> &nd)
+      : tris(t), tbox(n), cen(3 * n), perm(p), nodes(nd), max_leaf(ml) {
+    for (int64_t i = 0; i < n; ++i) {
+      tbox[i].grow(t[i].a);
+      tbox[i].grow(t[i].b);
+      tbox[i].grow(t[i].c);
+      for (int k = 0; k < 3; ++k) cen[3 * i + k] = 0.5f * (tbox[i].lo[k] + tbox[i].hi[k]);
+    }
+  }
+
+  int32_t make_leaf(int32_t node, uint32_t b, uint32_t e) {
+    nodes[node].left = nodes[node].right = -1;
+    nodes[node].first = (int32_t)b;
+    nodes[node].count = (int32_t)(e - b);
+    return node;
+  }
+
+  void build(int32_t node, uint32_t b, uint32_t e, int depth) {
+    max_depth = std::max(max_depth, depth);
+    Box bb, cb;
+    for (uint32_t i = b; i < e; ++i) {
+      bb.grow(tbox[perm[i]]);
+      cb.grow(&cen[3 * perm[i]]);
+    }
+    std::memcpy(nodes[node].lo, bb.lo, 12);
+    std::memcpy(nodes[node].hi, bb.hi, 12);
+    pad_box(nodes[node].lo, nodes[node].hi);
+    const uint32_t n = e - b;
+    if (n <= (uint32_t)max_leaf) {
+      make_leaf(node, b, e);
+      return;
+    }
+    int axis = -1;
+    int split_bin = -1;
+    float best = FLT_MAX;
+    const float leaf_cost = (float)n;
+    uint32_t mid = b;
+    if (depth < kSahDepth) {
+      for (int k = 0; k < 3; ++k) {
+        float ext = cb.hi[k] - cb.lo[k];
+        if (!(ext > 0.f)) continue;
+        Box bin[kBins];
+        uint32_t cnt[kBins] = {0};
+        float scale = (float)kBins / ext;
+        for (uint32_t i = b; i < e; ++i) {
+          int bi = std::min(kBins - 1, (int)((cen[3 * perm[i] + k] - cb.lo[k]) * scale));
+          bin[bi].grow(tbox[perm[i]]);
+          ++cnt[bi];
+        }
+        float ra[kBins];
+        uint32_t rc[kBins];
+        Box acc;
+        uint32_t c = 0;
+        for (int i = kBins - 1; i > 0; --i) {
+          acc.grow(bin[i]);
+          c += cnt[i];
+          ra[i] = acc.area();
+          rc[i] = c;
+        }
+        Box accl;
+        uint32_t cl = 0;
+        for (int i = 0; i < kBins - 1; ++i) {
+          accl.grow(bin[i]);
+          cl += cnt[i];
+          if (cl == 0 || rc[i + 1] == 0) continue;
+          float cost = accl.area() * (float)cl + ra[i + 1] * (float)rc[i + 1];
+          if (cost < best) {
+            best = cost;
+            axis = k;
+            split_bin = i;
+          }
+        }
+      }
+      float parent_area = bb.area();
+      if (axis >= 0) {
+        float sah = 1.0f + best / std::max(parent_area, 1e-30f);
+        if (sah >= leaf_cost && n <= (uint32_t)kMaxLeafTris) {
+          make_leaf(node, b, e);
+          return;
+        }
+        float ext = cb.hi[axis] - cb.lo[axis];
+        float scale = (float)kBins / ext;
+        auto it = std::partition(perm.begin() + b, perm.begin() + e, [&](uint32_t id) {
+          int bi = std::min(kBins - 1, (int)((cen[3 * id + axis] - cb.lo[axis]) * scale));
+          return bi <= split_bin;
+        });
+        mid = (uint32_t)(it - perm.begin());
+      }
+    }
+    if (mid == b || mid == e) {
+      // degenerate centroids or depth limit: object median on the longest axis
+      int ax = 0;
+      for (int k = 1; k < 3; ++k)
+        if (cb.hi[k] - cb.lo[k] > cb.hi[ax] - cb.lo[ax]) ax = k;
+      mid = (b + e) / 2;
+      std::nth_element(perm.begin() + b, perm.begin() + mid, perm.begin() + e, [&](uint32_t x, uint32_t y) {
+        float cx = cen[3 * x + ax], cy = cen[3 * y + ax];
+        return cx < cy || (cx == cy && x < y);
+      });
+    }
+    int32_t l = (int32_t)nodes.size();
+    nodes.resize(nodes.size() + 2);
+    nodes[node].left = l;
+    nodes[node].right = l + 1;
+    nodes[node].first = 0;
+    nodes[node].count = 0;
+    build(l, b, mid, depth + 1);
+    build(l + 1, mid, e, depth + 1);
+  }
+};
+
+inline int32_t fbits(float f) {
+  int32_t i;
+  std::memcpy(&i, &f, 4);
+  return i;
+}
+inline float bitsf(int32_t i) {
+  float f;
+  std::memcpy(&f, &i, 4);
+  return f;
+}
+
+}  // namespace
+
+void build_bvh2(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh &out) {
+  out.nodes.clear();
+  out.perm.resize(n);
+  std::iota(out.perm.begin(), out.perm.end(), 0u);
+  max_leaf = std::max(1, std::min(max_leaf, kMaxLeafTris));
+  out.nodes.reserve(2 * n + 1);
+  out.nodes.resize(1);
+  if (n == 0) {
+    out.nodes[0] = Bvh2Node{{0, 0, 0}, {0, 0, 0}, -1, -1, 0, 0};
+    out.max_depth = 0;
+    return;
+  }
+  Builder bld(tris, n, max_leaf, out.perm, out.nodes);
+  bld.build(0, 0, (uint32_t)n, 0);
+  out.max_depth = bld.max_depth;
+}
+
+bool import_bvh2(const float *nodes, int64_t n_nodes, const uint32_t *indices, int64_t n_indices,
+                 int64_t n_tris, gi_bvh &out, std::string &err) {
+  // Reference layout (src/tracers.cu:26-36): x bit0 = inner flag,
+  // inner: left = x>>1, right = y; leaf: count = x>>1, offset = y;
+  // box min = (n0.z, n0.w, n1.x), box max = (n1.y, n1.z, n1.w).
+  // Leaves are re-expressed over a fresh permutation so each leaf owns a
+  // contiguous range (the reference's index array may share entries).
+  out.nodes.assign(n_nodes, Bvh2Node{});
+  out.perm.clear();
+  out.max_depth = 0;
+  std::vector<std::pair<int64_t, int>> st{{0, 0}};
+  std::vector<uint8_t> seen(n_nodes, 0);
+  while (!st.empty()) {
+    auto [i, depth] = st.back();
+    st.pop_back();
+    if (i < 0 || i >= n_nodes || seen[i]) {
+      err = "bvh2: node index out of range or shared";
+      return false;
+    }
+    seen[i] = 1;
+    out.max_depth = std::max(out.max_depth, depth);
+    const float *p = nodes + 8 * i;
+    Bvh2Node &o = out.nodes[i];
+    o.lo[0] = p[2]; o.lo[1] = p[3]; o.lo[2] = p[4];
+    o.hi[0] = p[5]; o.hi[1] = p[6]; o.hi[2] = p[7];
+    int32_t x = fbits(p[0]), y = fbits(p[1]);
+    if (x & 1) {
+      o.left = x >> 1;
+      o.right = y;
+      o.first = o.count = 0;
+      st.push_back({o.right, depth + 1});
+      st.push_back({o.left, depth + 1});
+    } else {
+      int64_t cnt = x >> 1, off = y;
+      if (cnt < 0 || off < 0 || off + cnt > n_indices) {
+        err = "bvh2: leaf range outside the index array";
+        return false;
+      }
+      if (cnt > kMaxLeafTris) {
+        err = "bvh2: leaf with more than 15 triangles";
+        return false;
+      }
+      o.left = o.right = -1;
+      o.first = (int32_t)out.perm.size();
+      o.count = (int32_t)cnt;
+      for (int64_t k = 0; k < cnt; ++k) {
+        if (indices[off + k] >= (uint64_t)n_tris) {
+          err = "bvh2: triangle index out of range";
+          return false;
+        }
+        out.perm.push_back(indices[off + k]);
+      }
+    }
+  }
+  return true;
+}
+
+void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &wnodes,
+                   std::vector<IsectTri> &leaf_tris, int32_t &wide_depth) {
+  // Leaf-ordered intersection triangles: e1 = b - a, e2 = c - a in float,
+  // the same rounding the oracle's per-test subtraction produces.
+  leaf_tris.resize(b.perm.size());
+  for (size_t i = 0; i < b.perm.size(); ++i) {
+    const gi_tri &t = tris[b.perm[i]];
+    IsectTri &o = leaf_tris[i];
+    for (int k = 0; k < 3; ++k) {
+      o.a[k] = t.a[k];
+      o.e1[k] = t.b[k] - t.a[k];
+      o.e2[k] = t.c[k] - t.a[k];
+    }
+    o.a[3] = bitsf((int32_t)b.perm[i]);
+    o.e1[3] = 0.f;
+    o.e2[3] = 0.f;
+  }
+  wnodes.clear();
+  wide_depth = 0;
+  const auto &N = b.nodes;
+  auto area = [&](int32_t i) {
+    float dx = N[i].hi[0] - N[i].lo[0], dy = N[i].hi[1] - N[i].lo[1], dz = N[i].hi[2] - N[i].lo[2];
+    return dx * dy + dy * dz + dz * dx;
+  };
+  auto leaf_ref = [&](int32_t i) -> uint32_t {
+    if (N[i].count == 0) return kEmptyRef;
+    return kLeafBit | ((uint32_t)N[i].count << 27) | (uint32_t)N[i].first;
+  };
+  // BFS over binary nodes that become wide nodes.
+  std::vector<std::pair<int32_t, int32_t>> work;  // (binary node, wide slot)
+  wnodes.emplace_back();
+  std::vector<int32_t> wdepth{0};
+  work.push_back({0, 0});
+  for (size_t w = 0; w < work.size(); ++w) {
+    int32_t bi = work[w].first, wi = work[w].second;
+    std::vector<int32_t> kids;
+    if (N[bi].left < 0) {
+      kids.push_back(bi);  // root is a leaf: one child slot
+    } else {
+      kids = {N[bi].left, N[bi].right};
+      while (kids.size() < 4) {
+        int best = -1;
+        float ba = -1.f;
+        for (int k = 0; k < (int)kids.size(); ++k)
+          if (N[kids[k]].left >= 0 && area(kids[k]) > ba) {
+            ba = area(kids[k]);
+            best = k;
+          }
+        if (best < 0) break;
+        int32_t open = kids[best];
+        kids.erase(kids.begin() + best);
+        kids.push_back(N[open].left);
+        kids.push_back(N[open].right);
+      }
+    }
+    WideNode nd;
+    std::memset(&nd, 0, sizeof(nd));
+    for (int k = 0; k < 4; ++k) {
+      if (k >= (int)kids.size()) {
+        nd.lox[k] = nd.loy[k] = nd.loz[k] = 1.f;
+        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = -1.f;
+        nd.child[k] = kEmptyRef;
+        continue;
+      }
+      int32_t c = kids[k];
+      nd.lox[k] = N[c].lo[0]; nd.loy[k] = N[c].lo[1]; nd.loz[k] = N[c].lo[2];
+      nd.hix[k] = N[c].hi[0]; nd.hiy[k] = N[c].hi[1]; nd.hiz[k] = N[c].hi[2];
+      if (N[c].left < 0) {
+        nd.child[k] = leaf_ref(c);
+      } else {
+        int32_t slot = (int32_t)wnodes.size();
+        wnodes.emplace_back();
+        wdepth.push_back(wdepth[wi] + 1);
+        wide_depth = std::max(wide_depth, wdepth.back());
+        nd.child[k] = (uint32_t)slot;
+        work.push_back({c, slot});
+      }
+    }
+    wnodes[wi] = nd;
+  }
+}
+
+// ------------------------------------------------------------- Halton ----
+namespace {
+struct PrimeTable {
+  std::vector<int32_t> p;
+  PrimeTable() {
+    for (int32_t c = 2; (int)p.size() < 1024; ++c) {
+      bool ok = true;
+      for (int32_t q : p) {
+        if (q * q > c) break;
+        if (c % q == 0) {
+          ok = false;
+          break;
+        }
+      }
+      if (ok) p.push_back(c);
+    }
+  }
+};
+const PrimeTable &prime_table() {
+  static PrimeTable t;
+  return t;
+}
+// PCG32 (O'Neill 2014): srandom / random / boundedrand.
+struct Pcg {
+  uint64_t state = 0, inc = 1;
+  uint32_t next() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+  }
+  Pcg(uint64_t initstate, uint64_t initseq) {
+    inc = (initseq << 1u) | 1u;
+    next();
+    state += initstate;
+    next();
+  }
+  uint32_t bounded(uint32_t bound) {
+    uint32_t threshold = (0u - bound) % bound;
+    for (;;) {
+      uint32_t r = next();
+      if (r >= threshold) return r % bound;
+    }
+  }
+};
+}  // namespace
+
+int32_t prime_of(int32_t dim) { return prime_table().p.at(dim); }
+
+void halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm) {
+  const int32_t p = prime_of(dim);
+  for (int32_t i = 0; i < p; ++i) perm[i] = (uint16_t)i;
+  Pcg r(((uint64_t)seed << 32) ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(dim + 1)), (uint64_t)(dim + 1));
+  for (int32_t i = p - 1; i >= 1; --i) std::swap(perm[i], perm[r.bounded((uint32_t)(i + 1))]);
+}
+
+void halton_tables(uint32_t seed, int32_t ndims, std::vector<HaltonDim> &dims, std::vector<uint16_t> &perms) {
+  dims.resize(ndims);
+  perms.clear();
+  for (int32_t d = 0; d < ndims; ++d) {
+    const uint32_t p = (uint32_t)prime_of(d);
+    HaltonDim &h = dims[d];
+    h.prime = p;
+    h.perm_off = (uint32_t)perms.size();
+    perms.resize(perms.size() + p);
+    halton_permutation(d, seed, perms.data() + h.perm_off);
+    // floor(2^64 / p) + 1: exact floor(n / p) = mulhi(n, magic) for n < 2^32
+    h.magic = (uint64_t)(~0ULL / p) + 1ULL;
+    h.inv = 1.0 / (double)p;
+    h.tail = (double)perms[h.perm_off] / (double)(p - 1);
+  }
+}
+
+float halton_host(uint64_t n, int32_t dim, uint32_t seed) {
+  const int32_t p = prime_of(dim);
+  std::vector<uint16_t> perm(p);
+  halton_permutation(dim, seed, perm.data());
+  const double inv = 1.0 / (double)p;
+  uint64_t rev = 0;
+  double inv_n = 1.0;
+  while (n > 0) {
+    uint64_t q = n / (uint64_t)p;
+    uint64_t dg = n - q * (uint64_t)p;
+    rev = rev * (uint64_t)p + perm[dg];
+    inv_n *= inv;
+    n = q;
+  }
+  double r = inv_n * ((double)rev + (double)perm[0] / (double)(p - 1));
+  float f = (float)r;
+  return f >= 1.0f ? 0x1.fffffep-1f : f;
+}
+
+}  // namespace gi

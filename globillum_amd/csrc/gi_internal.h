@@ -1,0 +1,95 @@
+// Internal declarations shared by the host sources of libgi_hip.so.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gi_hip.h"
+
+namespace gi {
+
+// ---------------------------------------------------------------- BVH ----
+// Binary BVH node of the host builder.  Leaves hold [first, first+count) of
+// the builder's triangle permutation.
+struct Bvh2Node {
+  float lo[3], hi[3];
+  int32_t left, right;   // children (inner), -1 for leaves
+  int32_t first, count;  // leaf range, count == 0 for inner nodes
+};
+
+// 4-wide node streamed by the traversal kernels: the six child-box planes
+// as float4 (one lane per child) plus the child references -- 112 bytes,
+// padded to one 128-byte cache line.
+//   ref >= 0                 : inner node index
+//   ref & 0x80000000         : leaf, bits 27..30 = triangle count (1..15),
+//                              bits 0..26 = first triangle (leaf order)
+//   ref == kEmptyRef         : empty slot
+struct alignas(128) WideNode {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  uint32_t child[4];
+  uint32_t pad[4];
+};
+static_assert(sizeof(WideNode) == 128, "wide node must be one cache line");
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmptyRef = 0x80000000u;  // a leaf with zero triangles
+constexpr int kMaxLeafTris = 15;
+constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1;
+
+// Triangle in leaf order for intersection: vertex a (w = original id as
+// bits), edges b-a and c-a.  48 bytes.
+struct alignas(16) IsectTri {
+  float a[4];
+  float e1[4];
+  float e2[4];
+};
+static_assert(sizeof(IsectTri) == 48, "isect tri layout");
+
+}  // namespace gi
+
+struct gi_bvh {
+  std::vector<gi::Bvh2Node> nodes;  // binary tree, root = 0
+  std::vector<uint32_t> perm;       // leaf-order triangle ids
+  int32_t max_depth = 0;
+};
+
+namespace gi {
+
+// Binned-SAH build over the triangle list (host).  Boxes are padded by a
+// relative 2^-20 so the float slab tests stay conservative.
+void build_bvh2(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh &out);
+// Import a binary BVH in the reference's 2 x float4 layout.
+bool import_bvh2(const float *nodes, int64_t n_nodes, const uint32_t *indices, int64_t n_indices,
+                 int64_t n_tris, gi_bvh &out, std::string &err);
+// Collapse to the 4-wide layout (largest-surface-area child opened first)
+// and emit the leaf-ordered intersection triangles.
+void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &nodes,
+                   std::vector<IsectTri> &leaf_tris, int32_t &wide_depth);
+
+// ------------------------------------------------------------- Halton ----
+// Per-dimension constants of the scrambled radical inverse, laid out for
+// the device: prime, offset of its digit permutation, floor(2^64/p)+1 for
+// the division, 1/p and perm[0]/(p-1) in FP64.
+struct HaltonDim {
+  uint32_t prime;
+  uint32_t perm_off;
+  uint64_t magic;
+  double inv;
+  double tail;
+};
+static_assert(sizeof(HaltonDim) == 32, "halton dim layout");
+
+int32_t prime_of(int32_t dim);
+void halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm);
+void halton_tables(uint32_t seed, int32_t ndims, std::vector<HaltonDim> &dims,
+                   std::vector<uint16_t> &perms);
+float halton_host(uint64_t n, int32_t dim, uint32_t seed);
+
+// Sample dimension map: jitter x/y, lens x/y, then per path vertex b:
+// NEE x, y, light choice; BSDF x, y, lobe; Russian roulette.
+constexpr int kDimJitterX = 0, kDimJitterY = 1, kDimLensX = 2, kDimLensY = 3;
+constexpr int kDimVertex0 = 4, kDimsPerVertex = 7;
+constexpr int kMaxDepth = 32;
+inline int dims_for_depth(int depth) { return kDimVertex0 + kDimsPerVertex * depth; }
+
+}  // namespace gi

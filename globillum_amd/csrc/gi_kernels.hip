@@ -1,0 +1,1500 @@
+// gfx950 kernels of the wavefront path tracer and the C-ABI (include/gi_hip.h).
+//
+// Data layout in HBM (all arrays 16-byte aligned, structure of arrays):
+//   WideNode   nodes[]      128 B per 4-wide node (one cache line)
+//   IsectTri   itris[]      48 B per triangle, BVH leaf order
+//   ShadeTri   stris[]      128 B per triangle, original order (shading)
+//   ray queue  ray_o[], ray_d[], tp[]  float4 each: (o, halton index),
+//              (d, path slot), (throughput, 0); compacted per bounce
+//   hits[]     float4 (t, beta, gamma, triangle id) per queue entry
+//   shadow     sh_o[] (o, max_t), sh_d[] (d, slot), sh_c[] (contribution)
+//   path_l[]   float4 per path slot: the path's accumulated colour
+//   images     float4 per pixel per view: running mean over samples
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gi_device.h"
+#include "gi_internal.h"
+
+using namespace gi;
+using namespace gi::dev;
+
+namespace {
+
+// ------------------------------------------------------------- records ----
+struct alignas(16) ShadeTri {
+  float4 a, b, c;     // a.w = material index (bits)
+  float4 na, nb, nc;
+  float4 tab;         // ta.xy, tb.xy
+  float4 tcm;         // tc.xy, pad
+};
+static_assert(sizeof(ShadeTri) == 128, "shade tri");
+
+struct alignas(16) DevView {
+  float U[3], V[3], W[3], pos[3], vd[3];
+  float width, height, fw, fh, aperture, focus;
+  int32_t w, h, spp;
+  uint32_t index_base;
+  float4 *image;
+  int32_t pad[2];
+};
+
+constexpr int kMaxChunks = 64;
+struct PassChunk {
+  int32_t view, s0, k, npix;
+  uint32_t slot_base, pix_base;
+  int32_t pad[2];
+};
+struct PassDesc {
+  int32_t nchunks;
+  uint32_t total_slots;
+  uint32_t total_pix;
+  int32_t pad;
+  PassChunk chunk[kMaxChunks];
+};
+
+struct alignas(16) DevLight {
+  float center[3], power;
+  float dir[3], w;
+  float right[3], h;
+  float up[3], pad;
+  float col[3], pad2;
+};
+
+struct ShadeParams {
+  const ShadeTri *tris;
+  const DevMat *mats;
+  const DevLight *lights;
+  int32_t nlights;
+  float power_total;
+  HaltonDev H;
+  float off_b, off_s;
+  float bg[3];
+  int32_t depth, rr_start;
+};
+
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
+
+// Append under a wave-wide ballot: one atomic per wave, lane offsets from
+// the mask prefix.  Called uniformly by every lane of the wave.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return 0;
+  const uint32_t lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, leader);
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// -------------------------------------------------------------- traversal --
+// Ordered 4-wide BVH traversal, one ray per lane.  The traversal stack lives
+// in LDS (kLdsDepth entries per lane, interleaved by lane so every access is
+// bank-conflict free) and continues in a per-thread global-memory region for
+// deep trees.  Closest hit = minimum (t, id); any-hit returns on the first
+// triangle with 0 < t <= tmax.
+template <int kLdsDepth>
+struct Stack {
+  uint32_t *lds;
+  uint32_t *glb;
+  uint32_t lds_stride, glb_stride, cap;
+  int sp;
+  __device__ __forceinline__ bool push(uint32_t v) {
+    if ((uint32_t)sp >= cap) return false;
+    if (sp < kLdsDepth) lds[sp * lds_stride] = v;
+    else glb[(sp - kLdsDepth) * glb_stride] = v;
+    ++sp;
+    return true;
+  }
+  __device__ __forceinline__ uint32_t pop() {
+    --sp;
+    return sp < kLdsDepth ? lds[sp * lds_stride] : glb[(sp - kLdsDepth) * glb_stride];
+  }
+};
+
+template <bool kAny, int kLdsDepth>
+__device__ __forceinline__ bool traverse(const WideNode *__restrict__ nodes, const float4 *__restrict__ itris, f3 o,
+                                         f3 d, float tmax, float &bt, float &bu, float &bv, uint32_t &bid,
+                                         Stack<kLdsDepth> &st, uint32_t *overflow) {
+  const f3 idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
+  bt = FLT_MAX;
+  bu = bv = 0.0f;
+  bid = GI_INVALID_REF;
+  uint32_t ref = 0;
+  st.sp = 0;
+  for (;;) {
+    if (!(ref & kLeafBit)) {
+      const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
+      const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+      const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
+      const float lim = kAny ? tmax : smin(bt, tmax);
+      float dist[4];
+      uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+      const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+      const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+      const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float x0 = (lx[k] - o.x) * idir.x, x1 = (hx[k] - o.x) * idir.x;
+        float y0 = (ly[k] - o.y) * idir.y, y1 = (hy[k] - o.y) * idir.y;
+        float z0 = (lz[k] - o.z) * idir.z, z1 = (hz[k] - o.z) * idir.z;
+        float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+        float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), lim));
+        bool hit = tn <= tf && cref[k] != kEmptyRef;
+        dist[k] = hit ? tn : INFINITY;
+      }
+      // sort (dist, ref) ascending: network (0,1)(2,3)(0,2)(1,3)(1,2)
+#define GI_CSWAP(i, j)                                      \
+  if (dist[j] < dist[i]) {                                  \
+    float td = dist[i]; dist[i] = dist[j]; dist[j] = td;    \
+    uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
+  }
+      GI_CSWAP(0, 1) GI_CSWAP(2, 3) GI_CSWAP(0, 2) GI_CSWAP(1, 3) GI_CSWAP(1, 2)
+#undef GI_CSWAP
+      if (dist[0] != INFINITY) {
+        bool ok = true;
+        if (dist[3] != INFINITY) ok &= st.push(cref[3]);
+        if (dist[2] != INFINITY) ok &= st.push(cref[2]);
+        if (dist[1] != INFINITY) ok &= st.push(cref[1]);
+        if (!ok) atomicAdd(overflow, 1u);
+        ref = cref[0];
+        continue;
+      }
+    } else {
+      const uint32_t cnt = (ref >> 27) & 15u, first = ref & kMaxLeafFirst;
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const float4 ta = itris[3 * (first + j) + 0];
+        const float4 t1 = itris[3 * (first + j) + 1];
+        const float4 t2 = itris[3 * (first + j) + 2];
+        float t, u, v;
+        if (intersect_tri(xyz(ta), xyz(t1), xyz(t2), o, d, t, u, v) && t <= tmax) {
+          const uint32_t id = fbits(ta.w);
+          if (kAny) {
+            bt = t; bu = u; bv = v; bid = id;
+            return true;
+          }
+          if (t < bt || (t == bt && id < bid)) {
+            bt = t; bu = u; bv = v; bid = id;
+          }
+        }
+      }
+    }
+    if (st.sp == 0) break;
+    ref = st.pop();
+  }
+  return bid != GI_INVALID_REF;
+}
+
+constexpr int kLdsDepth = 16;
+
+// Closest-hit kernel over a ray queue.  kHasTmax: max_t in ray_o.w (the
+// primitive entry point); otherwise ray_o.w carries the Halton index and
+// max_t is FLT_MAX (camera and bounce rays, src/hybrid-pt.cpp:558).
+template <bool kHasTmax>
+__global__ __launch_bounds__(256) void k_trace_closest(const uint32_t *__restrict__ count_ptr, uint32_t count_host,
+                                                       const float4 *__restrict__ ray_o,
+                                                       const float4 *__restrict__ ray_d, float4 *__restrict__ hits,
+                                                       const WideNode *__restrict__ nodes,
+                                                       const float4 *__restrict__ itris, uint32_t *__restrict__ gstack,
+                                                       uint32_t stack_cap, uint32_t *overflow) {
+  extern __shared__ uint32_t smem[];
+  const uint32_t count = count_ptr ? *count_ptr : count_host;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
+  Stack<kLdsDepth> st{smem + threadIdx.x, gstack + gtid, blockDim.x, gstride, stack_cap, 0};
+  for (uint32_t i = gtid; i < count; i += gstride) {
+    const float4 ro = ray_o[i], rd = ray_d[i];
+    const float tmax = kHasTmax ? ro.w : FLT_MAX;
+    float t, u, v;
+    uint32_t id;
+    traverse<false>(nodes, itris, xyz(ro), xyz(rd), tmax, t, u, v, id, st, overflow);
+    hits[i] = make_float4(t, u, v, bitsf(id));
+  }
+}
+
+// Any-hit kernel.  kDeposit: shadow rays of the renderer -- an unoccluded
+// ray adds its precomputed NEE contribution to its path's colour (the
+// !is.valid() branch of src/hybrid-pt.cpp:355-365).  Otherwise writes the
+// occlusion byte (primitive entry point).
+template <bool kDeposit>
+__global__ __launch_bounds__(256) void k_trace_any(const uint32_t *__restrict__ count_ptr, uint32_t count_host,
+                                                   const float4 *__restrict__ ray_o, const float4 *__restrict__ ray_d,
+                                                   const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
+                                                   uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes,
+                                                   const float4 *__restrict__ itris, uint32_t *__restrict__ gstack,
+                                                   uint32_t stack_cap, uint32_t *overflow) {
+  extern __shared__ uint32_t smem[];
+  const uint32_t count = count_ptr ? *count_ptr : count_host;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
+  Stack<kLdsDepth> st{smem + threadIdx.x, gstack + gtid, blockDim.x, gstride, stack_cap, 0};
+  for (uint32_t i = gtid; i < count; i += gstride) {
+    const float4 ro = ray_o[i], rd = ray_d[i];
+    float t, u, v;
+    uint32_t id;
+    const bool occ = traverse<true>(nodes, itris, xyz(ro), xyz(rd), ro.w, t, u, v, id, st, overflow);
+    if (kDeposit) {
+      if (!occ) {
+        const uint32_t slot = fbits(rd.w);
+        const float4 c = contrib[i];
+        float4 l = path_l[slot];
+        l.x = l.x + c.x;
+        l.y = l.y + c.y;
+        l.z = l.z + c.z;
+        path_l[slot] = l;
+      }
+    } else {
+      occluded[i] = occ ? 1 : 0;
+    }
+  }
+}
+
+__global__ void k_pack_rays(const float *__restrict__ orig, const float *__restrict__ dir,
+                            const float *__restrict__ maxt, uint32_t n, float4 *__restrict__ ray_o,
+                            float4 *__restrict__ ray_d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ray_o[i] = make_float4(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2], maxt[i]);
+  ray_d[i] = make_float4(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2], bitsf(i));
+}
+
+// ------------------------------------------------------------- camera ----
+// Jittered pinhole ray (src/raygen.cu:17-58) or thin-lens ray
+// (src/raygen.cu:99-117; lens jitter uniform on [-1/2,1/2]^2).
+__device__ __forceinline__ void camera_ray(const DevView &v, int32_t x, int32_t y, float jx, float jy, float lx,
+                                           float ly, f3 &o, f3 &d) {
+  float u_s = ((float(x) + jx) / v.fw) * 2.0f - 1.0f;
+  float v_s = ((float(y) + jy) / v.fh) * 2.0f - 1.0f;
+  u_s = v.width * u_s;
+  v_s = v.height * v_s;
+  const f3 U = ld3(v.U), V = ld3(v.V), W = ld3(v.W);
+  d = normalize(add(add(scl(U, u_s), scl(V, v_s)), W));
+  o = ld3(v.pos);
+  if (v.aperture > 0.0f) {
+    const f3 vd = ld3(v.vd);
+    const f3 focal = add(o, scl(d, (1.0f / dot(d, vd)) * v.focus));
+    const float ox = lx - 0.5f, oy = ly - 0.5f;
+    const f3 jp = add(add(o, scl(U, ox * v.aperture)), scl(V, oy * v.aperture));
+    d = normalize(sub(focal, jp));
+    o = jp;
+  }
+}
+
+__device__ __forceinline__ int find_chunk(const PassDesc *pass, uint32_t slot) {
+  int c = 0;
+  while (c + 1 < pass->nchunks && slot >= pass->chunk[c + 1].slot_base) ++c;
+  return c;
+}
+
+// Primary rays of every path slot of a pass (new_path: generate_rays and
+// the throughput / colour reset, src/gpu-pt.h:110-118).
+__global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pass, const DevView *__restrict__ views,
+                                                HaltonDev H, float4 *__restrict__ ray_o, float4 *__restrict__ ray_d,
+                                                float4 *__restrict__ tp, float4 *__restrict__ path_l) {
+  const uint32_t total = pass->total_slots;
+  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
+    const PassChunk &ck = pass->chunk[find_chunk(pass, slot)];
+    const DevView &v = views[ck.view];
+    const uint32_t local = slot - ck.slot_base;
+    const uint32_t s = (uint32_t)ck.s0 + local / (uint32_t)ck.npix;
+    const uint32_t p = local % (uint32_t)ck.npix;
+    const uint32_t n = v.index_base + p * (uint32_t)v.spp + s;
+    f3 o, d;
+    camera_ray(v, (int32_t)(p % (uint32_t)v.w), (int32_t)(p / (uint32_t)v.w), halton(n, kDimJitterX, H),
+               halton(n, kDimJitterY, H), halton(n, kDimLensX, H), halton(n, kDimLensY, H), o, d);
+    ray_o[slot] = make_float4(o.x, o.y, o.z, bitsf(n));
+    ray_d[slot] = make_float4(d.x, d.y, d.z, bitsf(slot));
+    tp[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+    path_l[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Shading of path vertex b: NEE (pixel_generate_arealight_sample_cpu,
+// src/arealight-sampler-cpu.cpp:25-119) and bounce
+// (compute_path_contribution_and_bounce, src/hybrid-pt.cpp:450-577) fused.
+// The NEE contribution is evaluated here with the lobe the bounce sample
+// selects (as evaluate_material does after sample_material) and travels
+// with the shadow ray; the shadow kernel adds it when unoccluded.
+__global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__restrict__ count_ptr, ShadeParams P,
+                                               const float4 *__restrict__ ray_o, const float4 *__restrict__ ray_d,
+                                               const float4 *__restrict__ tp_in, const float4 *__restrict__ hits,
+                                               float4 *__restrict__ path_l, float4 *__restrict__ nro,
+                                               float4 *__restrict__ nrd, float4 *__restrict__ ntp,
+                                               uint32_t *__restrict__ next_count, float4 *__restrict__ sho,
+                                               float4 *__restrict__ shd, float4 *__restrict__ shc,
+                                               uint32_t *__restrict__ shadow_count) {
+  const uint32_t count = *count_ptr;
+  const int dim = kDimVertex0 + kDimsPerVertex * b;
+  const bool last = b + 1 >= P.depth;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    bool emit_next = false, emit_shadow = false;
+    float4 o_next, d_next, tp_next, s_o, s_d, s_c;
+    if (i < count) {
+      const float4 ro = ray_o[i], rd = ray_d[i], tpv = tp_in[i], h = hits[i];
+      const uint32_t n = fbits(ro.w), slot = fbits(rd.w), id = fbits(h.w);
+      const f3 TP = xyz(tpv);
+      const f3 org_dir = normalize(xyz(rd));
+      if (id == GI_INVALID_REF) {
+        // handle_no_hit with NO_BACKGROUND (src/hybrid-pt.cpp:196-206,567-574)
+        if (TP.x == 1.0f && TP.y == 1.0f && TP.z == 1.0f) {
+          float4 l = path_l[slot];
+          l.x = l.x + TP.x * P.bg[0];
+          l.y = l.y + TP.y * P.bg[1];
+          l.z = l.z + TP.z * P.bg[2];
+          path_l[slot] = l;
+        }
+      } else {
+        const ShadeTri &T = P.tris[id];
+        const float beta = h.y, gamma = h.z;
+        const f3 bc = f3{(1.0f - beta) - gamma, beta, gamma};
+        const f3 Pp = add(add(scl(xyz(T.a), bc.x), scl(xyz(T.b), bc.y)), scl(xyz(T.c), bc.z));
+        const f3 Ni = add(add(scl(xyz(T.na), bc.x), scl(xyz(T.nb), bc.y)), scl(xyz(T.nc), bc.z));
+        const DevMat m = P.mats[fbits(T.a.w)];
+
+        // ---- NEE: power-proportional light choice, point on the rect
+        const f3 xl = f3{halton(n, dim + 0, P.H), halton(n, dim + 1, P.H), halton(n, dim + 2, P.H)};
+        const float choice = xl.z * P.power_total;
+        float acc = 0.0f;
+        int k = 0;
+        while (k < P.nlights && choice > acc + P.lights[k].power) {
+          acc += P.lights[k].power;
+          ++k;
+        }
+        if (k >= P.nlights) k = P.nlights - 1;
+        const DevLight &lt = P.lights[k];
+        const f3 ldir = ld3(lt.dir);
+        const float offx = (xl.x - 0.5f) * lt.w, offy = (xl.y - 0.5f) * lt.h;
+        const f3 ls = add(add(ld3(lt.center), scl(ld3(lt.right), offx)), scl(ld3(lt.up), offy));
+        f3 sd = sub(ls, Pp);
+        const float len = length(sd);
+        sd = divs(sd, len);
+        const float ndotl = smax(dot(Ni, sd), 0.0f);
+        const float lcos = smax(dot(ldir, neg(sd)), 0.0f);
+        const float factor = (((lt.w * lt.h) * ndotl) * lcos) / (len * len);
+        const f3 contrib = scl(ld3(lt.col), factor);
+        sd = normalize(sd);
+        const f3 so = add(Pp, scl(sd, P.off_s));
+        const f3 psc = scl(contrib, P.power_total / lt.power);
+
+        // ---- bounce: tangent frame, normal flip, BRDF sample
+        f3 N = normalize(Ni);
+        f3 Tt, Bt;
+        tangent_frame(N, Tt, Bt);
+        if (dot(org_dir, N) > 0.0f) N = neg(N);
+        const f3 inv = neg(org_dir);
+        const f3 wo_ts = f3{dot(inv, Tt), dot(inv, Bt), dot(inv, N)};
+        const f3 xp = f3{halton(n, dim + 3, P.H), halton(n, dim + 4, P.H), halton(n, dim + 5, P.H)};
+        f3 wi_ts;
+        float pdf = 1.0f;
+        const int type = brdf_sample(m, wo_ts, xp, wi_ts, pdf);
+        const f3 wi = normalize(add(add(scl(Tt, wi_ts.x), scl(Bt, wi_ts.y)), scl(N, wi_ts.z)));
+        const bool glass = m.principled && m.opacity < 1.0f;
+        if (!glass) {
+          const f3 C = mul(brdf_eval(m, type, inv, normalize(sd), N), mul(TP, psc));
+          if (C.x > 0.0f || C.y > 0.0f || C.z > 0.0f) {
+            emit_shadow = true;
+            s_o = make_float4(so.x, so.y, so.z, len);
+            s_d = make_float4(sd.x, sd.y, sd.z, bitsf(slot));
+            s_c = make_float4(C.x, C.y, C.z, 0.0f);
+          }
+        }
+        if (!last) {
+          const f3 f = brdf_eval(m, type, inv, wi, N);
+          const f3 np = glass ? add(Pp, scl(org_dir, P.off_b)) : add(Pp, scl(N, P.off_b));
+          f3 t = f3{(TP.x * f.x) / pdf, (TP.y * f.y) / pdf, (TP.z * f.z) / pdf};
+          if (t.x > 1.0f) t.x = 1.0f;
+          if (t.y > 1.0f) t.y = 1.0f;
+          if (t.z > 1.0f) t.z = 1.0f;
+          bool alive = true;
+          if (P.rr_start > 0 && b + 1 >= P.rr_start) {
+            const float q = smax(t.x, smax(t.y, t.z));
+            const float xr = halton(n, dim + 6, P.H);
+            if (!(xr < q)) alive = false;
+            else t = divs(t, q);
+          }
+          if (alive) {
+            emit_next = true;
+            o_next = make_float4(np.x, np.y, np.z, bitsf(n));
+            d_next = make_float4(wi.x, wi.y, wi.z, bitsf(slot));
+            tp_next = make_float4(t.x, t.y, t.z, 0.0f);
+          }
+        }
+      }
+    }
+    const uint32_t qn = wave_append(emit_next, next_count);
+    if (emit_next) {
+      nro[qn] = o_next;
+      nrd[qn] = d_next;
+      ntp[qn] = tp_next;
+    }
+    const uint32_t qs = wave_append(emit_shadow, shadow_count);
+    if (emit_shadow) {
+      sho[qs] = s_o;
+      shd[qs] = s_d;
+      shc[qs] = s_c;
+    }
+  }
+}
+
+// Running mean over the pass's samples in sample order
+// (combine_color_samples, src/util-kernels.cu:44-51).
+__global__ __launch_bounds__(256) void k_accumulate(const PassDesc *__restrict__ pass,
+                                                    const DevView *__restrict__ views,
+                                                    const float4 *__restrict__ path_l) {
+  const uint32_t total = pass->total_pix;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    int c = 0;
+    while (c + 1 < pass->nchunks && g >= pass->chunk[c + 1].pix_base) ++c;
+    const PassChunk &ck = pass->chunk[c];
+    const uint32_t p = g - ck.pix_base;
+    float4 *img = views[ck.view].image;
+    float4 acc = img[p];
+    for (int j = 0; j < ck.k; ++j) {
+      const float4 l = path_l[ck.slot_base + (uint32_t)j * (uint32_t)ck.npix + p];
+      const float s = float(ck.s0 + j), s1 = float(ck.s0 + j + 1);
+      acc.x = (s * acc.x + l.x) / s1;
+      acc.y = (s * acc.y + l.y) / s1;
+      acc.z = (s * acc.z + l.z) / s1;
+    }
+    img[p] = acc;
+  }
+}
+
+// Per-pass queue counters: count[0] = all slots, everything else zero.
+// A kernel rather than a memset so the sequence can sit in a hipGraph.
+__global__ void k_pass_setup(const PassDesc *__restrict__ pass, uint32_t *__restrict__ counters, int32_t n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = (i == 0) ? pass->total_slots : 0u;
+}
+
+__global__ void k_fill4(float4 *__restrict__ p, uint32_t n, float4 v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_unpack_image(const float4 *__restrict__ img, float *__restrict__ rgb, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = img[i];
+  rgb[3 * i] = v.x;
+  rgb[3 * i + 1] = v.y;
+  rgb[3 * i + 2] = v.z;
+}
+
+// ------------------------------------------------------------- buffers ----
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipMalloc(&p, std::max<size_t>(n, 256));
+    if (e == hipSuccess) bytes = std::max<size_t>(n, 256);
+    return e;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace
+
+// ================================================================ context ==
+struct gi_ctx {
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // scene
+  std::vector<gi_tri> tris;
+  std::vector<gi_material> mats;
+  std::vector<gi_light> lights;
+  std::unique_ptr<gi_bvh> bvh;
+  int64_t ntri = 0, nnodes = 0;
+  int32_t wide_depth = 0;
+  bool have_bvh = false;
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, diag = 0.f, power_total = 0.f;
+  DBuf d_stris, d_itris, d_nodes, d_mats, d_lights;
+  // Halton tables
+  uint32_t hal_seed = 0;
+  int32_t hal_dims = 0;
+  DBuf d_hdims, d_hperm;
+  // traversal stack + overflow flag
+  DBuf d_gstack, d_overflow;
+  uint32_t stack_cap = 0;
+  int32_t trace_grid = 0;
+  // render state
+  std::vector<gi_view> views;
+  gi_render_opts opts{};
+  std::vector<DBuf> images;
+  DBuf d_views, d_passes, d_counters;
+  DBuf q_o[2], q_d[2], q_tp[2], q_hit, s_o, s_d, s_c, path_l;
+  size_t slot_cap = 0;
+  // primitives
+  DBuf p_orig, p_dir, p_maxt, p_ro, p_rd, p_hit, p_occ;
+  // progressive
+  int32_t prog_sample = 0, prog_max = 0;
+  bool prog_active = false;
+  gi_counters counters{};
+  gi_tuning tuning{256, 8, 256, 0};
+  std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+#define GI_HIP(ctx, expr)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);              \
+      return e_ == hipErrorOutOfMemory ? GI_ERR_OUT_OF_MEMORY : GI_ERR_HIP;        \
+    }                                                                              \
+  } while (0)
+
+int fail(gi_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+float rect_power(const gi_light &l) {
+  // add-rectlight, src/lights.cpp:92
+  return l.wh[0] * l.wh[1] * (l.col[0] + l.col[1] + l.col[2]) * .333f;
+}
+
+// host copies of the device tangent frame (same formula, same rounding)
+void host_frame(const float *n, float *t, float *b) {
+  float sign = std::copysign(1.0f, n[2]);
+  float a = -1.0f / (sign + n[2]);
+  float bb = n[0] * n[1] * a;
+  float T[3] = {1.0f + sign * n[0] * n[0] * a, sign * bb, -sign * n[0]};
+  float B[3] = {bb, sign + n[1] * n[1] * a, -n[1]};
+  float it = 1.0f / std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+  float ib = 1.0f / std::sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+  for (int k = 0; k < 3; ++k) {
+    t[k] = T[k] * it;
+    b[k] = B[k] * ib;
+  }
+}
+void host_tangential(const float *v, const float *n, float *out) {
+  float t[3], b[3];
+  host_frame(n, t, b);
+  for (int k = 0; k < 3; ++k) out[k] = (t[k] * v[0] + b[k] * v[1]) + n[k] * v[2];
+}
+
+DevMat make_devmat(const gi_material &m) {
+  DevMat d;
+  std::memset(&d, 0, sizeof(d));
+  for (int k = 0; k < 3; ++k) d.diffuse[k] = m.diffuse_color[k];
+  d.principled = m.principled;
+  float cd[3];
+  for (int k = 0; k < 3; ++k) cd[k] = std::pow(m.color[k], 2.2f);  // mon2lin
+  float lum = 0.3f * cd[0] + 0.6f * cd[1] + 0.1f * cd[2];
+  float tint[3];
+  for (int k = 0; k < 3; ++k) tint[k] = lum > 0.f ? cd[k] / lum : 1.f;
+  const float sp = m.specular * 0.08f;
+  for (int k = 0; k < 3; ++k) {
+    float mt = (1.0f - m.specular_tint) * 1.0f + m.specular_tint * tint[k];
+    float x = sp * mt;
+    d.cspec0[k] = (1.0f - m.metallic) * x + m.metallic * cd[k];
+    d.csheen[k] = (1.0f - m.sheen_tint) * 1.0f + m.sheen_tint * tint[k];
+    d.cdlin[k] = cd[k];
+    d.color[k] = m.color[k];
+  }
+  d.metallic = m.metallic;
+  d.roughness = m.roughness;
+  d.sheen = m.sheen;
+  d.subsurface = m.subsurface;
+  d.clearcoat = m.clearcoat;
+  d.cc_a = (1.0f - m.clearcoat_gloss) * 0.1f + m.clearcoat_gloss * 0.001f;
+  d.cc_log_a2 = std::log(d.cc_a * d.cc_a);
+  d.opacity = m.opacity;
+  d.specular = m.specular;
+  float aspect = std::sqrt(1.0f - m.anisotropic * 0.9f);
+  float r2 = m.roughness * m.roughness;
+  d.ax = std::max(0.001f, r2 / aspect);
+  d.ay = std::max(0.001f, r2 * aspect);
+  float rg = d.ax * d.ay * 0.5f + 0.5f;
+  rg = rg * rg;
+  d.roughg2 = rg * rg;
+  return d;
+}
+
+int ensure_halton(gi_ctx *ctx, uint32_t seed, int32_t ndims) {
+  if (ctx->hal_dims >= ndims && ctx->hal_seed == seed && ctx->d_hdims.p) return GI_OK;
+  std::vector<HaltonDim> dims;
+  std::vector<uint16_t> perms;
+  halton_tables(seed, ndims, dims, perms);
+  GI_HIP(ctx, ctx->d_hdims.ensure(dims.size() * sizeof(HaltonDim)));
+  GI_HIP(ctx, ctx->d_hperm.ensure(perms.size() * sizeof(uint16_t)));
+  GI_HIP(ctx, hipMemcpy(ctx->d_hdims.p, dims.data(), dims.size() * sizeof(HaltonDim), hipMemcpyHostToDevice));
+  GI_HIP(ctx, hipMemcpy(ctx->d_hperm.p, perms.data(), perms.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  ctx->hal_seed = seed;
+  ctx->hal_dims = ndims;
+  return GI_OK;
+}
+
+int ensure_gstack(gi_ctx *ctx);
+
+int upload_wide(gi_ctx *ctx) {
+  std::vector<WideNode> wn;
+  std::vector<IsectTri> it;
+  int32_t wd = 0;
+  collapse_wide(*ctx->bvh, ctx->tris.data(), wn, it, wd);
+  GI_HIP(ctx, ctx->d_nodes.ensure(wn.size() * sizeof(WideNode)));
+  GI_HIP(ctx, ctx->d_itris.ensure(std::max<size_t>(1, it.size()) * sizeof(IsectTri)));
+  GI_HIP(ctx, hipMemcpy(ctx->d_nodes.p, wn.data(), wn.size() * sizeof(WideNode), hipMemcpyHostToDevice));
+  if (!it.empty())
+    GI_HIP(ctx, hipMemcpy(ctx->d_itris.p, it.data(), it.size() * sizeof(IsectTri), hipMemcpyHostToDevice));
+  ctx->nnodes = (int64_t)wn.size();
+  ctx->wide_depth = wd;
+  // at most three pushes per wide level on the current path
+  ctx->stack_cap = (uint32_t)(3 * (wd + 1) + 4);
+  const uint32_t gdepth = ctx->stack_cap > (uint32_t)kLdsDepth ? ctx->stack_cap - kLdsDepth : 1;
+  (void)gdepth;
+  ctx->trace_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
+  int rc = ensure_gstack(ctx);
+  if (rc) return rc;
+  ctx->have_bvh = true;
+  return GI_OK;
+}
+
+uint32_t read_overflow(gi_ctx *ctx) {
+  uint32_t v = 0;
+  (void)hipMemcpy(&v, ctx->d_overflow.p, 4, hipMemcpyDeviceToHost);
+  return v;
+}
+
+// Global part of the traversal stacks: one column per thread of the
+// largest (persistent) traversal grid.  Allocated before any launch or
+// graph capture; launches never exceed that grid.
+int ensure_gstack(gi_ctx *ctx) {
+  const int block = std::max(64, (std::min(1024, ctx->tuning.trace_block) / 64) * 64);
+  const size_t threads = (size_t)ctx->num_cus * (size_t)std::max(1, ctx->tuning.trace_blocks_per_cu) * block;
+  const uint32_t gdepth = ctx->stack_cap > (uint32_t)kLdsDepth ? ctx->stack_cap - kLdsDepth : 1;
+  GI_HIP(ctx, ctx->d_gstack.ensure(threads * gdepth * sizeof(uint32_t)));
+  return GI_OK;
+}
+
+int trace_grid_for(gi_ctx *ctx, uint64_t n, int &grid, int &block) {
+  block = std::max(64, (std::min(1024, ctx->tuning.trace_block) / 64) * 64);
+  const int max_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
+  uint64_t need = (n + (uint64_t)block - 1) / (uint64_t)block;
+  grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(need, (uint64_t)max_grid));
+  return GI_OK;
+}
+
+int check_scene(gi_ctx *ctx, bool need_shading) {
+  if (!ctx) return GI_ERR_INVALID_ARG;
+  if (!ctx->have_bvh || ctx->ntri <= 0) return fail(ctx, GI_ERR_STATE, "no triangles / BVH uploaded");
+  if (need_shading) {
+    if (ctx->mats.empty()) return fail(ctx, GI_ERR_STATE, "no materials uploaded");
+    if (ctx->lights.empty()) return fail(ctx, GI_ERR_STATE, "no lights uploaded");
+  }
+  return GI_OK;
+}
+
+int launch_closest(gi_ctx *ctx, bool has_tmax, const uint32_t *count_ptr, uint32_t count_host, uint64_t n_bound,
+                   const float4 *ro, const float4 *rd, float4 *hits) {
+  int grid, block;
+  int rc = trace_grid_for(ctx, n_bound, grid, block);
+  if (rc) return rc;
+  const size_t lds = (size_t)kLdsDepth * block * sizeof(uint32_t);
+  if (has_tmax)
+    hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(block), lds, ctx->stream, count_ptr, count_host, ro,
+                       rd, hits, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
+                       ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+  else
+    hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(block), lds, ctx->stream, count_ptr, count_host, ro,
+                       rd, hits, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
+                       ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+  GI_HIP(ctx, hipGetLastError());
+  return GI_OK;
+}
+
+int launch_any(gi_ctx *ctx, bool deposit, const uint32_t *count_ptr, uint32_t count_host, uint64_t n_bound,
+               const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ) {
+  int grid, block;
+  int rc = trace_grid_for(ctx, n_bound, grid, block);
+  if (rc) return rc;
+  const size_t lds = (size_t)kLdsDepth * block * sizeof(uint32_t);
+  if (deposit)
+    hipLaunchKernelGGL(k_trace_any<true>, dim3(grid), dim3(block), lds, ctx->stream, count_ptr, count_host, ro, rd,
+                       contrib, pl, occ, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
+                       ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+  else
+    hipLaunchKernelGGL(k_trace_any<false>, dim3(grid), dim3(block), lds, ctx->stream, count_ptr, count_host, ro, rd,
+                       contrib, pl, occ, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
+                       ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+  GI_HIP(ctx, hipGetLastError());
+  return GI_OK;
+}
+
+// ---------------------------------------------------------------- render --
+struct PassPlan {
+  std::vector<PassDesc> passes;
+  size_t max_slots = 0;
+};
+
+// Pack (view, sample range) chunks into passes of at most `target` slots.
+// batched: chunks of different views share a pass; otherwise every pass
+// holds one view (the per-view wavefront).
+void plan_passes(const std::vector<gi_view> &views, const std::vector<int32_t> &s_begin,
+                 const std::vector<int32_t> &s_end, bool batched, size_t target, int32_t spp_per_pass,
+                 PassPlan &plan) {
+  plan.passes.clear();
+  plan.max_slots = 0;
+  PassDesc cur;
+  std::memset(&cur, 0, sizeof(cur));
+  auto flush = [&]() {
+    if (cur.nchunks == 0) return;
+    plan.max_slots = std::max<size_t>(plan.max_slots, cur.total_slots);
+    plan.passes.push_back(cur);
+    std::memset(&cur, 0, sizeof(cur));
+  };
+  for (size_t v = 0; v < views.size(); ++v) {
+    const size_t npix = (size_t)views[v].w * (size_t)views[v].h;
+    int32_t s = s_begin[v];
+    while (s < s_end[v]) {
+      int32_t k;
+      if (spp_per_pass > 0) k = spp_per_pass;
+      else k = (int32_t)std::max<size_t>(1, target / std::max<size_t>(1, npix));
+      k = std::min(k, s_end[v] - s);
+      size_t slots = npix * (size_t)k;
+      if (cur.nchunks > 0 && (!batched || cur.nchunks == kMaxChunks || cur.total_slots + slots > std::max(target, npix)))
+        flush();
+      PassChunk &c = cur.chunk[cur.nchunks++];
+      c.view = (int32_t)v;
+      c.s0 = s;
+      c.k = k;
+      c.npix = (int32_t)npix;
+      c.slot_base = cur.total_slots;
+      c.pix_base = cur.total_pix;
+      cur.total_slots += (uint32_t)slots;
+      cur.total_pix += (uint32_t)npix;
+      s += k;
+    }
+    if (!batched) flush();
+  }
+  flush();
+}
+
+int ensure_workspace(gi_ctx *ctx, size_t slots) {
+  if (slots <= ctx->slot_cap && ctx->path_l.p) return GI_OK;
+  const size_t b4 = slots * sizeof(float4);
+  for (int i = 0; i < 2; ++i) {
+    GI_HIP(ctx, ctx->q_o[i].ensure(b4));
+    GI_HIP(ctx, ctx->q_d[i].ensure(b4));
+    GI_HIP(ctx, ctx->q_tp[i].ensure(b4));
+  }
+  GI_HIP(ctx, ctx->q_hit.ensure(b4));
+  GI_HIP(ctx, ctx->s_o.ensure(b4));
+  GI_HIP(ctx, ctx->s_d.ensure(b4));
+  GI_HIP(ctx, ctx->s_c.ensure(b4));
+  GI_HIP(ctx, ctx->path_l.ensure(b4));
+  ctx->slot_cap = slots;
+  return GI_OK;
+}
+
+ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
+  ShadeParams P;
+  P.tris = ctx->d_stris.as<ShadeTri>();
+  P.mats = ctx->d_mats.as<DevMat>();
+  P.lights = ctx->d_lights.as<DevLight>();
+  P.nlights = (int32_t)ctx->lights.size();
+  P.power_total = ctx->power_total;
+  P.H = HaltonDev{ctx->d_hdims.as<HaltonDim>(), ctx->d_hperm.as<uint16_t>()};
+  P.off_b = ctx->opts.offset_bounce > 0.f ? ctx->opts.offset_bounce : 1e-4f * ctx->diag;
+  P.off_s = ctx->opts.offset_shadow > 0.f ? ctx->opts.offset_shadow : 1e-4f * ctx->diag;
+  for (int k = 0; k < 3; ++k) P.bg[k] = ctx->opts.background[k];
+  P.depth = depth;
+  P.rr_start = ctx->opts.rr_start;
+  return P;
+}
+
+hipEvent_t ev(gi_ctx *ctx, size_t i) {
+  while (ctx->events.size() <= i) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->events.push_back(e);
+  }
+  return ctx->events[i];
+}
+
+// Run the passes.  sync_counts: read every bounce's queue length back to
+// the host and size the launches exactly (the reference's synchronous,
+// per-view flow); otherwise every launch uses the persistent grid and reads
+// its length from device memory, and nothing synchronises until the end.
+int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_counts, bool use_graph, bool timing) {
+  const int32_t ncnt = 2 * depth + 1;  // ray queue counts [0..depth], shadow counts
+  const size_t npass = plan.passes.size();
+  GI_HIP(ctx, ctx->d_passes.ensure(npass * sizeof(PassDesc)));
+  GI_HIP(ctx, hipMemcpyAsync(ctx->d_passes.p, plan.passes.data(), npass * sizeof(PassDesc), hipMemcpyHostToDevice,
+                             ctx->stream));
+  GI_HIP(ctx, ctx->d_counters.ensure(npass * ncnt * sizeof(uint32_t)));
+  int rc = ensure_workspace(ctx, plan.max_slots);
+  if (rc) return rc;
+  if ((rc = ensure_gstack(ctx))) return rc;
+  ShadeParams SP = shade_params(ctx, depth);
+  const PassDesc *dpass = ctx->d_passes.as<PassDesc>();
+  const DevView *dviews = ctx->d_views.as<DevView>();
+  const int sblock = std::max(64, (ctx->tuning.shade_block / 64) * 64);
+  const int persist = ctx->num_cus * 8;
+  double ms_c = 0, ms_a = 0, ms_s = 0;
+  int launches = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  if (use_graph) {
+    sync_counts = false;
+    timing = false;
+    GI_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  }
+  size_t evi = 0;
+  std::vector<std::pair<size_t, int>> marks;  // (event index, category)
+  auto mark = [&](int cat) -> int {
+    hipEvent_t e = ev(ctx, evi);
+    if (!e) return fail(ctx, GI_ERR_HIP, "hipEventCreate");
+    GI_HIP(ctx, hipEventRecord(e, ctx->stream));
+    marks.push_back({evi++, cat});
+    return GI_OK;
+  };
+  for (size_t pi = 0; pi < npass; ++pi) {
+    const PassDesc &pd = plan.passes[pi];
+    const PassDesc *dp = dpass + pi;
+    uint32_t *cnt = ctx->d_counters.as<uint32_t>() + pi * ncnt;
+    hipLaunchKernelGGL(k_pass_setup, dim3(1), dim3(256), 0, ctx->stream, dp, cnt, ncnt);
+    const int rg = (int)std::min<uint64_t>((pd.total_slots + 255) / 256, (uint64_t)persist);
+    hipLaunchKernelGGL(k_raygen, dim3(std::max(1, rg)), dim3(256), 0, ctx->stream, dp, dviews, SP.H,
+                       ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->q_tp[0].as<float4>(),
+                       ctx->path_l.as<float4>());
+    uint32_t host_count = pd.total_slots;
+    for (int32_t b = 0; b < depth; ++b) {
+      const int cur = b & 1, nxt = cur ^ 1;
+      if (sync_counts && b > 0) {
+        GI_HIP(ctx, hipMemcpyAsync(&host_count, cnt + b, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (host_count == 0) break;
+      }
+      const uint64_t bound = sync_counts ? host_count : pd.total_slots;
+      if (timing && (rc = mark(-1))) return rc;
+      rc = launch_closest(ctx, false, cnt + b, 0, bound, ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(),
+                          ctx->q_hit.as<float4>());
+      if (rc) return rc;
+      ++launches;
+      if (timing && (rc = mark(0))) return rc;
+      const int sg = (int)std::max<uint64_t>(1, std::min<uint64_t>((bound + sblock - 1) / sblock, (uint64_t)persist));
+      hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b, cnt + b, SP, ctx->q_o[cur].as<float4>(),
+                         ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(), ctx->q_hit.as<float4>(),
+                         ctx->path_l.as<float4>(), ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(),
+                         ctx->q_tp[nxt].as<float4>(), cnt + b + 1, ctx->s_o.as<float4>(), ctx->s_d.as<float4>(),
+                         ctx->s_c.as<float4>(), cnt + depth + 1 + b);
+      GI_HIP(ctx, hipGetLastError());
+      if (timing && (rc = mark(2))) return rc;
+      uint32_t shadow_host = 0;
+      uint64_t sbound = bound;
+      if (sync_counts) {
+        GI_HIP(ctx, hipMemcpyAsync(&shadow_host, cnt + depth + 1 + b, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        sbound = shadow_host;
+      }
+      if (sbound > 0) {
+        rc = launch_any(ctx, true, cnt + depth + 1 + b, 0, sbound, ctx->s_o.as<float4>(), ctx->s_d.as<float4>(),
+                        ctx->s_c.as<float4>(), ctx->path_l.as<float4>(), nullptr);
+        if (rc) return rc;
+      }
+      if (timing && (rc = mark(1))) return rc;
+    }
+    const int ag = (int)std::max<uint64_t>(1, std::min<uint64_t>((pd.total_pix + 255) / 256, (uint64_t)persist));
+    hipLaunchKernelGGL(k_accumulate, dim3(ag), dim3(256), 0, ctx->stream, dp, dviews, ctx->path_l.as<float4>());
+    GI_HIP(ctx, hipGetLastError());
+  }
+  if (use_graph) {
+    GI_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(gexec, ctx->stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(ctx, GI_ERR_HIP, std::string("hipGraph: ") + hipGetErrorString(e));
+  }
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (timing) {
+    for (size_t i = 1; i < marks.size(); ++i) {
+      if (marks[i].second < 0) continue;
+      float ms = 0.f;
+      GI_HIP(ctx, hipEventElapsedTime(&ms, ctx->events[marks[i - 1].first], ctx->events[marks[i].first]));
+      if (marks[i].second == 0) ms_c += ms;
+      else if (marks[i].second == 1) ms_a += ms;
+      else ms_s += ms;
+    }
+  }
+  // telemetry
+  std::vector<uint32_t> hc(npass * ncnt);
+  GI_HIP(ctx, hipMemcpy(hc.data(), ctx->d_counters.p, hc.size() * 4, hipMemcpyDeviceToHost));
+  for (size_t pi = 0; pi < npass; ++pi) {
+    for (int32_t b = 0; b < depth; ++b) {
+      ctx->counters.rays_closest += hc[pi * ncnt + b];
+      ctx->counters.rays_shadow += hc[pi * ncnt + depth + 1 + b];
+    }
+  }
+  ctx->counters.ms_trace_closest += ms_c;
+  ctx->counters.ms_trace_any += ms_a;
+  ctx->counters.ms_shade += ms_s;
+  ctx->counters.launches_closest += launches;
+  ctx->counters.passes += (int32_t)npass;
+  if (read_overflow(ctx) != 0) return fail(ctx, GI_ERR_LIMIT, "traversal stack overflow");
+  return GI_OK;
+}
+
+int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_render_opts *opts) {
+  if (!views || n_views <= 0 || !opts) return fail(ctx, GI_ERR_INVALID_ARG, "views/opts");
+  if (opts->max_depth < 1 || opts->max_depth > kMaxDepth)
+    return fail(ctx, GI_ERR_INVALID_ARG, "max_depth must be in [1, 32]");
+  uint64_t total = 0;
+  for (int32_t v = 0; v < n_views; ++v) {
+    const gi_view &w = views[v];
+    if (w.w <= 0 || w.h <= 0 || w.spp <= 0) return fail(ctx, GI_ERR_INVALID_ARG, "view size / spp must be > 0");
+    if (!(w.fovy > 0.f && w.fovy < 180.f)) return fail(ctx, GI_ERR_INVALID_ARG, "fovy must be in (0, 180)");
+    const float dl = w.dir[0] * w.dir[0] + w.dir[1] * w.dir[1] + w.dir[2] * w.dir[2];
+    if (!(dl > 0.f) || !std::isfinite(dl)) return fail(ctx, GI_ERR_INVALID_ARG, "view dir");
+    total += (uint64_t)w.w * (uint64_t)w.h * (uint64_t)w.spp;
+  }
+  if (total >= (1ull << 32)) return fail(ctx, GI_ERR_LIMIT, "more than 2^32 samples in one render");
+  int rc = ensure_halton(ctx, opts->seed, dims_for_depth(opts->max_depth));
+  if (rc) return rc;
+  ctx->views.assign(views, views + n_views);
+  ctx->opts = *opts;
+  ctx->images.resize(n_views);
+  std::vector<DevView> dv(n_views);
+  uint32_t base = 0;
+  for (int32_t v = 0; v < n_views; ++v) {
+    const gi_view &w = views[v];
+    DevView &d = dv[v];
+    std::memset(&d, 0, sizeof(d));
+    // image-plane extent and camera basis: src/raygen.cu:17-39
+    float half = w.fovy / 2.0f;
+    d.height = (float)std::tan(3.14159265358979323846 * (double)half / 180.0);
+    d.width = ((float)w.w / (float)w.h) * d.height;
+    float vl = std::sqrt(w.dir[0] * w.dir[0] + w.dir[1] * w.dir[1] + w.dir[2] * w.dir[2]);
+    float W[3] = {w.dir[0] / vl, w.dir[1] / vl, w.dir[2] / vl};
+    float txw[3] = {w.up[1] * W[2] - w.up[2] * W[1], w.up[2] * W[0] - w.up[0] * W[2], w.up[0] * W[1] - w.up[1] * W[0]};
+    float tl = std::sqrt(txw[0] * txw[0] + txw[1] * txw[1] + txw[2] * txw[2]);
+    if (!(tl > 0.f)) return fail(ctx, GI_ERR_INVALID_ARG, "view up parallel to dir");
+    float U[3] = {txw[0] / tl, txw[1] / tl, txw[2] / tl};
+    float V[3] = {W[1] * U[2] - W[2] * U[1], W[2] * U[0] - W[0] * U[2], W[0] * U[1] - W[1] * U[0]};
+    for (int k = 0; k < 3; ++k) {
+      d.U[k] = U[k]; d.V[k] = V[k]; d.W[k] = W[k];
+      d.pos[k] = w.pos[k]; d.vd[k] = w.dir[k];
+    }
+    d.fw = (float)w.w;
+    d.fh = (float)w.h;
+    d.aperture = w.aperture;
+    d.focus = w.focus_distance;
+    d.w = w.w; d.h = w.h; d.spp = w.spp;
+    d.index_base = base;
+    base += (uint32_t)((uint64_t)w.w * w.h * w.spp);
+    const size_t npix = (size_t)w.w * w.h;
+    GI_HIP(ctx, ctx->images[v].ensure(npix * sizeof(float4)));
+    d.image = ctx->images[v].as<float4>();
+    const uint32_t nb = (uint32_t)((npix + 255) / 256);
+    hipLaunchKernelGGL(k_fill4, dim3(nb), dim3(256), 0, ctx->stream, d.image, (uint32_t)npix,
+                       make_float4(0.f, 0.f, 0.f, 0.f));
+    GI_HIP(ctx, hipGetLastError());
+  }
+  GI_HIP(ctx, ctx->d_views.ensure(dv.size() * sizeof(DevView)));
+  GI_HIP(ctx, hipMemcpyAsync(ctx->d_views.p, dv.data(), dv.size() * sizeof(DevView), hipMemcpyHostToDevice,
+                             ctx->stream));
+  GI_HIP(ctx, hipMemsetAsync(ctx->d_overflow.p, 0, 4, ctx->stream));
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GI_OK;
+}
+
+size_t target_slots(gi_ctx *ctx) {
+  (void)ctx;
+  return (size_t)4 << 20;
+}
+
+}  // namespace
+
+// ================================================================= C ABI ==
+extern "C" {
+
+int gi_abi_version(void) { return GI_ABI_VERSION; }
+
+const char *gi_status_string(int s) {
+  switch (s) {
+    case GI_OK: return "ok";
+    case GI_ERR_INVALID_ARG: return "invalid argument";
+    case GI_ERR_NO_DEVICE: return "no HIP device";
+    case GI_ERR_HIP: return "HIP runtime error";
+    case GI_ERR_STATE: return "invalid call order";
+    case GI_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case GI_ERR_LIMIT: return "size beyond a documented limit";
+    case GI_ERR_UNKNOWN_NAME: return "unknown name";
+    default: return "unknown status";
+  }
+}
+
+int gi_create(gi_ctx **out, int device) {
+  if (!out) return GI_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GI_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return GI_ERR_INVALID_ARG;
+  auto ctx = std::make_unique<gi_ctx>();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) return GI_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return GI_ERR_NO_DEVICE;
+  ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return GI_ERR_NO_DEVICE;
+  if (ctx->d_overflow.ensure(256) != hipSuccess) return GI_ERR_OUT_OF_MEMORY;
+  if (hipMemset(ctx->d_overflow.p, 0, 4) != hipSuccess) return GI_ERR_HIP;
+  *out = ctx.release();
+  return GI_OK;
+}
+
+void gi_destroy(gi_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto e : ctx->events) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *gi_last_error(const gi_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gi_bvh_build(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh **out) {
+  if (!out || n < 0 || (n > 0 && !tris)) return GI_ERR_INVALID_ARG;
+  if (n > (int64_t)kMaxLeafFirst) return GI_ERR_LIMIT;
+  auto b = std::make_unique<gi_bvh>();
+  build_bvh2(tris, n, max_leaf > 0 ? max_leaf : 4, *b);
+  *out = b.release();
+  return GI_OK;
+}
+
+void gi_bvh_free(gi_bvh *bvh) { delete bvh; }
+
+int gi_bvh_sizes(const gi_bvh *bvh, int64_t *n_nodes, int64_t *n_indices) {
+  if (!bvh || !n_nodes || !n_indices) return GI_ERR_INVALID_ARG;
+  *n_nodes = (int64_t)bvh->nodes.size();
+  *n_indices = (int64_t)bvh->perm.size();
+  return GI_OK;
+}
+
+int gi_bvh_export2(const gi_bvh *bvh, float *nodes, uint32_t *indices) {
+  if (!bvh || !nodes || (!indices && !bvh->perm.empty())) return GI_ERR_INVALID_ARG;
+  for (size_t i = 0; i < bvh->nodes.size(); ++i) {
+    const Bvh2Node &n = bvh->nodes[i];
+    float *p = nodes + 8 * i;
+    int32_t x, y;
+    if (n.left >= 0) {
+      x = (n.left << 1) | 1;
+      y = n.right;
+    } else {
+      x = n.count << 1;
+      y = n.first;
+    }
+    std::memcpy(&p[0], &x, 4);
+    std::memcpy(&p[1], &y, 4);
+    p[2] = n.lo[0]; p[3] = n.lo[1]; p[4] = n.lo[2];
+    p[5] = n.hi[0]; p[6] = n.hi[1]; p[7] = n.hi[2];
+  }
+  if (!bvh->perm.empty()) std::memcpy(indices, bvh->perm.data(), bvh->perm.size() * 4);
+  return GI_OK;
+}
+
+int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n) {
+  if (!ctx || !tris || n <= 0) return fail(ctx, GI_ERR_INVALID_ARG, "triangles");
+  if (n > (int64_t)kMaxLeafFirst) return fail(ctx, GI_ERR_LIMIT, "more than 2^27 triangles");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  ctx->tris.assign(tris, tris + n);
+  ctx->ntri = n;
+  ctx->have_bvh = false;
+  ctx->bvh.reset();
+  for (int k = 0; k < 3; ++k) {
+    ctx->lo[k] = FLT_MAX;
+    ctx->hi[k] = -FLT_MAX;
+  }
+  std::vector<ShadeTri> st(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const gi_tri &t = tris[i];
+    for (int k = 0; k < 3; ++k) {
+      ctx->lo[k] = std::min(ctx->lo[k], std::min(t.a[k], std::min(t.b[k], t.c[k])));
+      ctx->hi[k] = std::max(ctx->hi[k], std::max(t.a[k], std::max(t.b[k], t.c[k])));
+    }
+    ShadeTri &s = st[i];
+    int32_t mi = t.material_index;
+    float mf;
+    std::memcpy(&mf, &mi, 4);
+    s.a = make_float4(t.a[0], t.a[1], t.a[2], mf);
+    s.b = make_float4(t.b[0], t.b[1], t.b[2], 0.f);
+    s.c = make_float4(t.c[0], t.c[1], t.c[2], 0.f);
+    s.na = make_float4(t.na[0], t.na[1], t.na[2], 0.f);
+    s.nb = make_float4(t.nb[0], t.nb[1], t.nb[2], 0.f);
+    s.nc = make_float4(t.nc[0], t.nc[1], t.nc[2], 0.f);
+    s.tab = make_float4(t.ta[0], t.ta[1], t.tb[0], t.tb[1]);
+    s.tcm = make_float4(t.tc[0], t.tc[1], 0.f, 0.f);
+  }
+  float dx = ctx->hi[0] - ctx->lo[0], dy = ctx->hi[1] - ctx->lo[1], dz = ctx->hi[2] - ctx->lo[2];
+  ctx->diag = std::sqrt(dx * dx + dy * dy + dz * dz);
+  GI_HIP(ctx, ctx->d_stris.ensure(n * sizeof(ShadeTri)));
+  GI_HIP(ctx, hipMemcpy(ctx->d_stris.p, st.data(), n * sizeof(ShadeTri), hipMemcpyHostToDevice));
+  return GI_OK;
+}
+
+int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n) {
+  if (!ctx || !mats || n <= 0) return fail(ctx, GI_ERR_INVALID_ARG, "materials");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  ctx->mats.assign(mats, mats + n);
+  std::vector<DevMat> dm(n);
+  for (int32_t i = 0; i < n; ++i) dm[i] = make_devmat(mats[i]);
+  GI_HIP(ctx, ctx->d_mats.ensure(n * sizeof(DevMat)));
+  GI_HIP(ctx, hipMemcpy(ctx->d_mats.p, dm.data(), n * sizeof(DevMat), hipMemcpyHostToDevice));
+  return GI_OK;
+}
+
+int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n) {
+  if (!ctx || !lights || n <= 0) return fail(ctx, GI_ERR_INVALID_ARG, "lights");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  std::vector<DevLight> dl(n);
+  float total = 0.f;
+  for (int32_t i = 0; i < n; ++i) {
+    const gi_light &l = lights[i];
+    if (l.type != GI_LIGHT_RECT) return fail(ctx, GI_ERR_INVALID_ARG, "only rect lights are supported");
+    if (!(l.power > 0.f)) return fail(ctx, GI_ERR_INVALID_ARG, "light power must be > 0");
+    DevLight &d = dl[i];
+    std::memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 3; ++k) {
+      d.center[k] = l.center[k];
+      d.dir[k] = l.dir[k];
+      d.col[k] = l.col[k];
+    }
+    d.power = l.power;
+    d.w = l.wh[0];
+    d.h = l.wh[1];
+    const float ex[3] = {1, 0, 0}, ey[3] = {0, 1, 0};
+    host_tangential(ex, l.dir, d.right);
+    host_tangential(ey, l.dir, d.up);
+    total += l.power;
+  }
+  ctx->lights.assign(lights, lights + n);
+  ctx->power_total = total;
+  GI_HIP(ctx, ctx->d_lights.ensure(n * sizeof(DevLight)));
+  GI_HIP(ctx, hipMemcpy(ctx->d_lights.p, dl.data(), n * sizeof(DevLight), hipMemcpyHostToDevice));
+  return GI_OK;
+}
+
+int gi_build_bvh(gi_ctx *ctx, int32_t max_leaf) {
+  if (!ctx || ctx->tris.empty()) return fail(ctx, GI_ERR_STATE, "upload triangles first");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  ctx->bvh = std::make_unique<gi_bvh>();
+  build_bvh2(ctx->tris.data(), (int64_t)ctx->tris.size(), max_leaf > 0 ? max_leaf : 4, *ctx->bvh);
+  return upload_wide(ctx);
+}
+
+int gi_upload_bvh2(gi_ctx *ctx, const float *nodes, int64_t n_nodes, const uint32_t *indices, int64_t n_indices) {
+  if (!ctx || !nodes || n_nodes <= 0 || (!indices && n_indices > 0) || n_indices < 0)
+    return fail(ctx, GI_ERR_INVALID_ARG, "bvh2 arrays");
+  if (ctx->tris.empty()) return fail(ctx, GI_ERR_STATE, "upload triangles first");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  auto b = std::make_unique<gi_bvh>();
+  std::string err;
+  if (!import_bvh2(nodes, n_nodes, indices, n_indices, (int64_t)ctx->tris.size(), *b, err))
+    return fail(ctx, GI_ERR_INVALID_ARG, err);
+  ctx->bvh = std::move(b);
+  return upload_wide(ctx);
+}
+
+int gi_trace_closest_device(gi_ctx *ctx, const float *d_orig, const float *d_dir, const float *d_max_t, int64_t n,
+                            gi_hit *d_out) {
+  int rc = check_scene(ctx, false);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!d_orig || !d_dir || !d_max_t || !d_out))) return fail(ctx, GI_ERR_INVALID_ARG, "rays");
+  if (n == 0) return GI_OK;
+  if (n >= (int64_t)1 << 32) return fail(ctx, GI_ERR_LIMIT, "more than 2^32 rays");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  if ((rc = ensure_gstack(ctx))) return rc;
+  GI_HIP(ctx, ctx->p_ro.ensure(n * sizeof(float4)));
+  GI_HIP(ctx, ctx->p_rd.ensure(n * sizeof(float4)));
+  hipLaunchKernelGGL(k_pack_rays, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, d_orig, d_dir,
+                     d_max_t, (uint32_t)n, ctx->p_ro.as<float4>(), ctx->p_rd.as<float4>());
+  GI_HIP(ctx, hipGetLastError());
+  rc = launch_closest(ctx, true, nullptr, (uint32_t)n, (uint64_t)n, ctx->p_ro.as<float4>(), ctx->p_rd.as<float4>(),
+                      reinterpret_cast<float4 *>(d_out));
+  if (rc) return rc;
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (read_overflow(ctx) != 0) return fail(ctx, GI_ERR_LIMIT, "traversal stack overflow");
+  return GI_OK;
+}
+
+int gi_trace_any_device(gi_ctx *ctx, const float *d_orig, const float *d_dir, const float *d_max_t, int64_t n,
+                        uint8_t *d_occluded) {
+  int rc = check_scene(ctx, false);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!d_orig || !d_dir || !d_max_t || !d_occluded)))
+    return fail(ctx, GI_ERR_INVALID_ARG, "rays");
+  if (n == 0) return GI_OK;
+  if (n >= (int64_t)1 << 32) return fail(ctx, GI_ERR_LIMIT, "more than 2^32 rays");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  if ((rc = ensure_gstack(ctx))) return rc;
+  GI_HIP(ctx, ctx->p_ro.ensure(n * sizeof(float4)));
+  GI_HIP(ctx, ctx->p_rd.ensure(n * sizeof(float4)));
+  hipLaunchKernelGGL(k_pack_rays, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, d_orig, d_dir,
+                     d_max_t, (uint32_t)n, ctx->p_ro.as<float4>(), ctx->p_rd.as<float4>());
+  GI_HIP(ctx, hipGetLastError());
+  rc = launch_any(ctx, false, nullptr, (uint32_t)n, (uint64_t)n, ctx->p_ro.as<float4>(), ctx->p_rd.as<float4>(),
+                  nullptr, nullptr, d_occluded);
+  if (rc) return rc;
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (read_overflow(ctx) != 0) return fail(ctx, GI_ERR_LIMIT, "traversal stack overflow");
+  return GI_OK;
+}
+
+int gi_trace_closest(gi_ctx *ctx, const float *orig, const float *dir, const float *max_t, int64_t n, gi_hit *out) {
+  int rc = check_scene(ctx, false);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!orig || !dir || !max_t || !out))) return fail(ctx, GI_ERR_INVALID_ARG, "rays");
+  if (n == 0) return GI_OK;
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  GI_HIP(ctx, ctx->p_orig.ensure(n * 12));
+  GI_HIP(ctx, ctx->p_dir.ensure(n * 12));
+  GI_HIP(ctx, ctx->p_maxt.ensure(n * 4));
+  GI_HIP(ctx, ctx->p_hit.ensure(n * sizeof(gi_hit)));
+  GI_HIP(ctx, hipMemcpy(ctx->p_orig.p, orig, n * 12, hipMemcpyHostToDevice));
+  GI_HIP(ctx, hipMemcpy(ctx->p_dir.p, dir, n * 12, hipMemcpyHostToDevice));
+  GI_HIP(ctx, hipMemcpy(ctx->p_maxt.p, max_t, n * 4, hipMemcpyHostToDevice));
+  rc = gi_trace_closest_device(ctx, ctx->p_orig.as<float>(), ctx->p_dir.as<float>(), ctx->p_maxt.as<float>(), n,
+                               ctx->p_hit.as<gi_hit>());
+  if (rc) return rc;
+  GI_HIP(ctx, hipMemcpy(out, ctx->p_hit.p, n * sizeof(gi_hit), hipMemcpyDeviceToHost));
+  return GI_OK;
+}
+
+int gi_trace_any(gi_ctx *ctx, const float *orig, const float *dir, const float *max_t, int64_t n, uint8_t *occluded) {
+  int rc = check_scene(ctx, false);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!orig || !dir || !max_t || !occluded))) return fail(ctx, GI_ERR_INVALID_ARG, "rays");
+  if (n == 0) return GI_OK;
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  GI_HIP(ctx, ctx->p_orig.ensure(n * 12));
+  GI_HIP(ctx, ctx->p_dir.ensure(n * 12));
+  GI_HIP(ctx, ctx->p_maxt.ensure(n * 4));
+  GI_HIP(ctx, ctx->p_occ.ensure(n));
+  GI_HIP(ctx, hipMemcpy(ctx->p_orig.p, orig, n * 12, hipMemcpyHostToDevice));
+  GI_HIP(ctx, hipMemcpy(ctx->p_dir.p, dir, n * 12, hipMemcpyHostToDevice));
+  GI_HIP(ctx, hipMemcpy(ctx->p_maxt.p, max_t, n * 4, hipMemcpyHostToDevice));
+  rc = gi_trace_any_device(ctx, ctx->p_orig.as<float>(), ctx->p_dir.as<float>(), ctx->p_maxt.as<float>(), n,
+                           ctx->p_occ.as<uint8_t>());
+  if (rc) return rc;
+  GI_HIP(ctx, hipMemcpy(occluded, ctx->p_occ.p, n, hipMemcpyDeviceToHost));
+  return GI_OK;
+}
+
+int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint32_t seed, float *orig, float *dir,
+                            float *max_t) {
+  if (!ctx || !v || !orig || !dir || !max_t) return fail(ctx, GI_ERR_INVALID_ARG, "camera rays");
+  if (sample < 0 || sample >= v->spp) return fail(ctx, GI_ERR_INVALID_ARG, "sample index");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  gi_render_opts o{};
+  o.max_depth = 1;
+  o.seed = seed;
+  int rc = setup_views(ctx, v, 1, &o);
+  if (rc) return rc;
+  const uint32_t npix = (uint32_t)((uint64_t)v->w * v->h);
+  PassDesc pd;
+  std::memset(&pd, 0, sizeof(pd));
+  pd.nchunks = 1;
+  pd.total_slots = npix;
+  pd.total_pix = npix;
+  pd.chunk[0] = PassChunk{0, sample, 1, (int32_t)npix, 0, 0, {0, 0}};
+  GI_HIP(ctx, ctx->d_passes.ensure(sizeof(PassDesc)));
+  GI_HIP(ctx, hipMemcpy(ctx->d_passes.p, &pd, sizeof(pd), hipMemcpyHostToDevice));
+  rc = ensure_workspace(ctx, npix);
+  if (rc) return rc;
+  HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hperm.as<uint16_t>()};
+  hipLaunchKernelGGL(k_raygen, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_passes.as<PassDesc>(),
+                     ctx->d_views.as<DevView>(), H, ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(),
+                     ctx->q_tp[0].as<float4>(), ctx->path_l.as<float4>());
+  GI_HIP(ctx, hipGetLastError());
+  std::vector<float4> ro(npix), rd(npix);
+  GI_HIP(ctx, hipMemcpyAsync(ro.data(), ctx->q_o[0].p, npix * 16, hipMemcpyDeviceToHost, ctx->stream));
+  GI_HIP(ctx, hipMemcpyAsync(rd.data(), ctx->q_d[0].p, npix * 16, hipMemcpyDeviceToHost, ctx->stream));
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (uint32_t i = 0; i < npix; ++i) {
+    orig[3 * i] = ro[i].x; orig[3 * i + 1] = ro[i].y; orig[3 * i + 2] = ro[i].z;
+    dir[3 * i] = rd[i].x; dir[3 * i + 1] = rd[i].y; dir[3 * i + 2] = rd[i].z;
+    max_t[i] = FLT_MAX;
+  }
+  return GI_OK;
+}
+
+int gi_render(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_render_opts *opts) {
+  int rc = check_scene(ctx, true);
+  if (rc) return rc;
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  auto t0 = std::chrono::steady_clock::now();
+  std::memset(&ctx->counters, 0, sizeof(ctx->counters));
+  rc = setup_views(ctx, views, n_views, opts);
+  if (rc) return rc;
+  ctx->prog_active = false;
+  std::vector<int32_t> sb(n_views, 0), se(n_views);
+  for (int32_t v = 0; v < n_views; ++v) {
+    se[v] = views[v].spp;
+    ctx->counters.samples += (uint64_t)views[v].w * views[v].h * views[v].spp;
+  }
+  const bool batched = (opts->flags & GI_FLAG_BATCHED) != 0;
+  const bool graph = (opts->flags & GI_FLAG_GRAPH) != 0;
+  const int32_t spp_pp = opts->samples_per_pass > 0 ? opts->samples_per_pass : ctx->tuning.samples_per_pass;
+  PassPlan plan;
+  plan_passes(ctx->views, sb, se, batched, target_slots(ctx), spp_pp, plan);
+  rc = run_passes(ctx, plan, opts->max_depth, !batched, graph, !graph);
+  if (rc) return rc;
+  ctx->counters.ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return GI_OK;
+}
+
+int gi_progress_begin(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_render_opts *opts) {
+  int rc = check_scene(ctx, true);
+  if (rc) return rc;
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  std::memset(&ctx->counters, 0, sizeof(ctx->counters));
+  rc = setup_views(ctx, views, n_views, opts);
+  if (rc) return rc;
+  ctx->prog_sample = 0;
+  ctx->prog_max = 0;
+  for (int32_t v = 0; v < n_views; ++v) ctx->prog_max = std::max(ctx->prog_max, views[v].spp);
+  ctx->prog_active = true;
+  return GI_OK;
+}
+
+int gi_progress_step(gi_ctx *ctx, int32_t *done) {
+  if (!ctx || !done) return fail(ctx, GI_ERR_INVALID_ARG, "progress_step");
+  if (!ctx->prog_active) return fail(ctx, GI_ERR_STATE, "gi_progress_begin first");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  const int32_t nv = (int32_t)ctx->views.size();
+  std::vector<int32_t> sb(nv), se(nv);
+  for (int32_t v = 0; v < nv; ++v) {
+    sb[v] = std::min(ctx->prog_sample, ctx->views[v].spp);
+    se[v] = std::min(ctx->prog_sample + 1, ctx->views[v].spp);
+    ctx->counters.samples += (uint64_t)ctx->views[v].w * ctx->views[v].h * (uint64_t)(se[v] - sb[v]);
+  }
+  PassPlan plan;
+  plan_passes(ctx->views, sb, se, true, target_slots(ctx), 1, plan);
+  if (!plan.passes.empty()) {
+    int rc = run_passes(ctx, plan, ctx->opts.max_depth, false, false, true);
+    if (rc) return rc;
+  }
+  ++ctx->prog_sample;
+  *done = ctx->prog_sample >= ctx->prog_max ? 1 : 0;
+  if (*done) ctx->prog_active = false;
+  return GI_OK;
+}
+
+int gi_get_image(gi_ctx *ctx, int32_t view, float *rgb) {
+  if (!ctx || !rgb) return fail(ctx, GI_ERR_INVALID_ARG, "get_image");
+  if (view < 0 || view >= (int32_t)ctx->views.size()) return fail(ctx, GI_ERR_INVALID_ARG, "view index");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t npix = (size_t)ctx->views[view].w * ctx->views[view].h;
+  std::vector<float4> tmp(npix);
+  GI_HIP(ctx, hipMemcpy(tmp.data(), ctx->images[view].p, npix * sizeof(float4), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < npix; ++i) {
+    rgb[3 * i] = tmp[i].x;
+    rgb[3 * i + 1] = tmp[i].y;
+    rgb[3 * i + 2] = tmp[i].z;
+  }
+  return GI_OK;
+}
+
+int gi_get_image_device(gi_ctx *ctx, int32_t view, float **d_rgb) {
+  if (!ctx || !d_rgb) return fail(ctx, GI_ERR_INVALID_ARG, "get_image_device");
+  if (view < 0 || view >= (int32_t)ctx->views.size()) return fail(ctx, GI_ERR_INVALID_ARG, "view index");
+  *d_rgb = ctx->images[view].as<float>();
+  return GI_OK;
+}
+
+int gi_get_counters(const gi_ctx *ctx, gi_counters *out) {
+  if (!ctx || !out) return GI_ERR_INVALID_ARG;
+  *out = ctx->counters;
+  return GI_OK;
+}
+
+int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t) {
+  if (!ctx || !t) return fail(ctx, GI_ERR_INVALID_ARG, "tuning");
+  gi_tuning n = *t;
+  if (n.trace_block <= 0) n.trace_block = 256;
+  if (n.trace_blocks_per_cu <= 0) n.trace_blocks_per_cu = 8;
+  if (n.shade_block <= 0) n.shade_block = 256;
+  if (n.trace_block % 64 || n.trace_block > 1024 || n.shade_block % 64 || n.shade_block > 1024)
+    return fail(ctx, GI_ERR_INVALID_ARG, "block sizes must be multiples of 64, at most 1024");
+  ctx->tuning = n;
+  return GI_OK;
+}
+
+int gi_get_tuning(const gi_ctx *ctx, gi_tuning *t) {
+  if (!ctx || !t) return GI_ERR_INVALID_ARG;
+  *t = ctx->tuning;
+  return GI_OK;
+}
+
+int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, float *d_emission) {
+  (void)dL_dimage;
+  (void)d_mats;
+  (void)d_emission;
+  return fail(ctx, GI_ERR_STATE, "adjoint not available in this build");
+}
+
+float gi_halton_sample(uint64_t n, int32_t dim, uint32_t seed) {
+  if (dim < 0 || dim >= 1024) return -1.0f;
+  return halton_host(n, dim, seed);
+}
+
+float gi_radical_inverse(uint64_t index, int32_t base) {
+  // gi::halton, src/util.cpp:22-32
+  if (base < 2) return -1.0f;
+  float result = 0.0f;
+  float f = 1.0f / float(base);
+  uint64_t i = index;
+  while (i > 0) {
+    result += f * (float)(i % (uint64_t)base);
+    i = i / (uint64_t)base;
+    f = f / float(base);
+  }
+  return result;
+}
+
+int32_t gi_halton_prime(int32_t dim) {
+  if (dim < 0 || dim >= 1024) return -1;
+  return prime_of(dim);
+}
+
+int gi_halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm) {
+  if (dim < 0 || dim >= 1024 || !perm) return GI_ERR_INVALID_ARG;
+  halton_permutation(dim, seed, perm);
+  return GI_OK;
+}
+
+float gi_rect_light_power(const gi_light *l) { return l ? rect_power(*l) : 0.0f; }
+
+}  // extern "C"

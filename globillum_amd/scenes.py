@@ -1,0 +1,283 @@
+"""Procedural scenes of the benchmark configurations (seeded, numpy).
+
+There is no scene file for these workloads; each generator returns a
+:class:`Scene` -- triangle, material and light records in the C-ABI layout
+(the simple_triangle field set of src/hl-main.cpp:332-348, material_t /
+PrincipledBRDFParameters, gi::light rect lights) plus its views.
+
+  cornell_boxes      closed box of 5 walls + 2 boxes, 1 rect light (34 tris)
+  terrain_spheres    fBm heightfield + UV spheres, principled materials
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import LIGHT, MATERIAL, OPTS, TRI, VIEW
+
+
+@dataclass
+class Scene:
+    tris: np.ndarray
+    mats: np.ndarray
+    lights: np.ndarray
+    views: np.ndarray
+    opts: np.ndarray
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+
+def rect_light_power(w: float, h: float, col) -> np.float32:
+    """add-rectlight's selection weight (src/lights.cpp:92), in float32."""
+    col = np.asarray(col, np.float32)
+    return np.float32(np.float32(w) * np.float32(h) * (col[0] + col[1] + col[2]) * np.float32(0.333))
+
+
+def make_light(center, direction, col, w, h) -> np.ndarray:
+    l = np.zeros(1, LIGHT)
+    l["center"] = center
+    l["dir"] = direction
+    l["up"] = (0.0, 0.0, 1.0)
+    l["col"] = col
+    l["wh"] = (w, h)
+    l["power"] = rect_light_power(w, h, col)
+    l["type"] = 0
+    return l
+
+
+def lambert(rgb) -> np.ndarray:
+    m = np.zeros(1, MATERIAL)
+    m["diffuse_color"] = rgb
+    m["alpha"] = 1.0
+    m["principled"] = 0
+    m["opacity"] = 1.0
+    m["transmission"] = 1.0
+    m["color"] = rgb
+    m["texture"] = -1
+    return m
+
+
+def principled(color, roughness, metallic, **kw) -> np.ndarray:
+    """PrincipledBRDFParameters defaults (src/principledParameters.h:23-24)
+    with the given base colour, roughness and metallic."""
+    m = np.zeros(1, MATERIAL)
+    m["diffuse_color"] = color
+    m["alpha"] = 1.0
+    m["principled"] = 1
+    m["color"] = color
+    m["roughness"] = roughness
+    m["metallic"] = metallic
+    m["opacity"] = 1.0
+    m["transmission"] = 1.0
+    m["texture"] = -1
+    for k, v in kw.items():
+        m[k] = v
+    return m
+
+
+def make_view(pos, look_at, up, fovy, w, h, spp, aperture=0.0, focus=1.0) -> np.ndarray:
+    v = np.zeros(1, VIEW)
+    pos = np.asarray(pos, np.float32)
+    d = np.asarray(look_at, np.float32) - pos
+    v["pos"] = pos
+    v["dir"] = d / np.linalg.norm(d)
+    v["up"] = up
+    v["fovy"] = fovy
+    v["w"], v["h"], v["spp"] = w, h, spp
+    v["aperture"] = aperture
+    v["focus_distance"] = focus
+    return v
+
+
+def make_opts(depth, rr_start=0, seed=0, flags=0, offset_bounce=0.0, offset_shadow=0.0,
+              background=(1.0, 1.0, 1.0), samples_per_pass=0) -> np.ndarray:
+    o = np.zeros(1, OPTS)
+    o["max_depth"] = depth
+    o["rr_start"] = rr_start
+    o["seed"] = seed
+    o["flags"] = flags
+    o["offset_bounce"] = offset_bounce
+    o["offset_shadow"] = offset_shadow
+    o["background"] = background
+    o["samples_per_pass"] = samples_per_pass
+    return o
+
+
+def _quads_to_tris(quads: np.ndarray, normals: np.ndarray, mat: np.ndarray) -> np.ndarray:
+    """quads (n,4,3) counter-clockwise, normals (n,3), mat (n,) -> 2n tris."""
+    n = len(quads)
+    t = np.zeros(2 * n, TRI)
+    t["a"][0::2], t["b"][0::2], t["c"][0::2] = quads[:, 0], quads[:, 1], quads[:, 2]
+    t["a"][1::2], t["b"][1::2], t["c"][1::2] = quads[:, 0], quads[:, 2], quads[:, 3]
+    for k in ("na", "nb", "nc"):
+        t[k][0::2] = normals
+        t[k][1::2] = normals
+    t["material_index"][0::2] = mat
+    t["material_index"][1::2] = mat
+    return t
+
+
+def _box_quads(lo, hi, inward: bool):
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    faces = [  # (quad, outward normal)
+        ([(x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1)], (0, -1, 0)),
+        ([(x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0)], (0, 1, 0)),
+        ([(x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0)], (0, 0, -1)),
+        ([(x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)], (0, 0, 1)),
+        ([(x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0)], (-1, 0, 0)),
+        ([(x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1)], (1, 0, 0)),
+    ]
+    q = np.array([f[0] for f in faces], np.float32)
+    n = np.array([f[1] for f in faces], np.float32)
+    return q, (-n if inward else n)
+
+
+def cornell_boxes(seed: int = 1, w: int = 256, h: int = 256, spp: int = 16, depth: int = 4) -> Scene:
+    """Closed box [0,1]^3 of 5 walls (open towards -z, 10 triangles), two
+    axis-aligned boxes of seeded size U[0.1,0.35]^3 resting on the floor (12
+    triangles each), one 0.25 x 0.25 rect light of emission 10 just below the
+    ceiling, Lambert albedo U[0.2,0.8]^3 per object; pinhole camera at
+    (0.5, 0.5, -1.3) looking +z, fovy 40."""
+    rng = np.random.default_rng(seed)
+    q, n = _box_quads((0, 0, 0), (1, 1, 1), inward=True)
+    walls = [0, 1, 3, 4, 5]  # floor, ceiling, back, left, right (no front)
+    parts = [_quads_to_tris(q[walls], n[walls], np.arange(5, dtype=np.int32))]
+    mats = [lambert(rng.uniform(0.2, 0.8, 3).astype(np.float32)) for _ in range(5)]
+    for b in range(2):
+        ext = rng.uniform(0.1, 0.35, 3).astype(np.float32)
+        x = rng.uniform(0.05, 0.95 - ext[0])
+        z = rng.uniform(0.05, 0.95 - ext[2])
+        lo = np.array([x, 0.0, z], np.float32)
+        bq, bn = _box_quads(lo, lo + ext, inward=False)
+        parts.append(_quads_to_tris(bq, bn, np.full(6, 5 + b, np.int32)))
+        mats.append(lambert(rng.uniform(0.2, 0.8, 3).astype(np.float32)))
+    tris = np.concatenate(parts)
+    lights = make_light((0.5, 0.999, 0.5), (0.0, -1.0, 0.0), (10.0, 10.0, 10.0), 0.25, 0.25)
+    view = make_view((0.5, 0.5, -1.3), (0.5, 0.5, 0.0), (0, 1, 0), 40.0, w, h, spp)
+    return Scene(tris, np.concatenate(mats), lights, view, make_opts(depth), name="cornell_boxes")
+
+
+# ------------------------------------------------------------- terrain ----
+def _value_noise(rng, res: int):
+    return rng.uniform(-1.0, 1.0, (res + 1, res + 1))
+
+
+def _fbm(x, z, rng, octaves=6, gain=0.5, lacunarity=2.0, base_res=4):
+    """Fractal sum of bilinearly interpolated value noise on [0,1]^2."""
+    h = np.zeros_like(x, dtype=np.float64)
+    amp, res = 1.0, base_res
+    for _ in range(octaves):
+        g = _value_noise(rng, res)
+        fx, fz = x * res, z * res
+        ix = np.clip(np.floor(fx).astype(int), 0, res - 1)
+        iz = np.clip(np.floor(fz).astype(int), 0, res - 1)
+        tx, tz = fx - ix, fz - iz
+        tx = tx * tx * (3 - 2 * tx)
+        tz = tz * tz * (3 - 2 * tz)
+        v = (g[ix, iz] * (1 - tx) * (1 - tz) + g[ix + 1, iz] * tx * (1 - tz) + g[ix, iz + 1] * (1 - tx) * tz
+             + g[ix + 1, iz + 1] * tx * tz)
+        h += amp * v
+        amp *= gain
+        res = int(res * lacunarity)
+    return h
+
+
+def heightfield(rng, nx: int, nz: int, size_x: float, size_z: float, amplitude: float, n_patch_mats: int,
+                mat_offset: int = 0):
+    """nx x nz cells -> 2*nx*nz triangles with smooth vertex normals; the
+    material index cycles over 8x8 patches."""
+    xs = np.linspace(0.0, 1.0, nx + 1)
+    zs = np.linspace(0.0, 1.0, nz + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    H = amplitude * _fbm(X, Z, rng)
+    P = np.stack([X * size_x, H, Z * size_z], -1).astype(np.float64)
+    dx = np.gradient(H, size_x / nx, axis=0)
+    dz = np.gradient(H, size_z / nz, axis=1)
+    N = np.stack([-dx, np.ones_like(dx), -dz], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    i, k = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    i, k = i.ravel(), k.ravel()
+    v00, v10, v11, v01 = (i, k), (i + 1, k), (i + 1, k + 1), (i, k + 1)
+    t = np.zeros(2 * len(i), TRI)
+    t["a"][0::2], t["b"][0::2], t["c"][0::2] = P[v00], P[v01], P[v11]
+    t["na"][0::2], t["nb"][0::2], t["nc"][0::2] = N[v00], N[v01], N[v11]
+    t["a"][1::2], t["b"][1::2], t["c"][1::2] = P[v00], P[v11], P[v10]
+    t["na"][1::2], t["nb"][1::2], t["nc"][1::2] = N[v00], N[v11], N[v10]
+    patch = ((i * 8) // nx) * 8 + (k * 8) // nz
+    mi = (mat_offset + patch % n_patch_mats).astype(np.int32)
+    t["material_index"][0::2] = mi
+    t["material_index"][1::2] = mi
+    return t, H
+
+
+def uv_sphere(center, radius, segments=10, rings=9, material=0) -> np.ndarray:
+    """UV sphere of 2*segments*(rings-1) triangles (160 for 10 x 9) with
+    smooth normals."""
+    th = np.linspace(0.0, np.pi, rings + 1)
+    ph = np.linspace(0.0, 2 * np.pi, segments + 1)
+    T, Pp = np.meshgrid(th, ph, indexing="ij")
+    N = np.stack([np.sin(T) * np.cos(Pp), np.cos(T), np.sin(T) * np.sin(Pp)], -1)
+    V = np.asarray(center)[None, None, :] + radius * N
+    tris = []
+    for r in range(rings):
+        for s in range(segments):
+            a, b, c, d = (r, s), (r + 1, s), (r + 1, s + 1), (r, s + 1)
+            if r > 0:
+                tris.append((a, b, d))
+            if r < rings - 1:
+                tris.append((b, c, d))
+    tris = tris
+    out = np.zeros(len(tris), TRI)
+    for j, key in enumerate(("a", "b", "c")):
+        idx = np.array([tt[j] for tt in tris])
+        out[key] = V[idx[:, 0], idx[:, 1]]
+        out["n" + key] = N[idx[:, 0], idx[:, 1]]
+    out["material_index"] = material
+    return out
+
+
+def terrain_spheres(seed: int = 2, cells=(256, 256), n_spheres: int = 1000, n_mats: int = 32, n_lights: int = 4,
+                    w: int = 1280, h: int = 720, spp: int = 64, depth: int = 8, size: float = 100.0,
+                    radius=(0.1, 2.0)) -> Scene:
+    """fBm heightfield (2*cells triangles, 6 octaves, gain 0.5, lacunarity 2,
+    amplitude 10) over [0,size]^2 plus n_spheres 160-triangle UV spheres of
+    radius U[radius], principled materials (baseColor U[0,1]^3, roughness
+    U[0.05,1], metallic in {0,1}), n_lights rect lights above the terrain."""
+    rng = np.random.default_rng(seed)
+    nx, nz = cells
+    ter, H = heightfield(rng, nx, nz, size, size, 10.0, n_mats)
+    parts = [ter]
+    for _ in range(n_spheres):
+        cx, cz = rng.uniform(0.0, size, 2)
+        r = rng.uniform(*radius)
+        hx = min(int(cx / size * nx), nx)
+        hz = min(int(cz / size * nz), nz)
+        cy = H[hx, hz] + 0.5 * r
+        parts.append(uv_sphere((cx, cy, cz), r, material=int(rng.integers(0, n_mats))))
+    tris = np.concatenate(parts)
+    mats = np.concatenate([
+        principled(rng.uniform(0.0, 1.0, 3).astype(np.float32), rng.uniform(0.05, 1.0),
+                   float(rng.random() < 0.5)) for _ in range(n_mats)])
+    g = int(np.ceil(np.sqrt(n_lights)))
+    lights = []
+    for li in range(n_lights):
+        lx = (li % g + 0.5) / g * size
+        lz = (li // g + 0.5) / g * size
+        lights.append(make_light((lx, 40.0, lz), (0.0, -1.0, 0.0), (40.0, 40.0, 40.0), 8.0, 8.0))
+    view = make_view((0.5 * size, 35.0, -0.3 * size), (0.5 * size, 0.0, 0.5 * size), (0, 1, 0), 50.0, w, h, spp)
+    return Scene(tris, mats, np.concatenate(lights), view, make_opts(depth), name="terrain_spheres")
+
+
+def config_scene(idx: int, **kw) -> Scene:
+    """Scenes of the benchmark configurations by number (1-based)."""
+    if idx == 1:
+        return cornell_boxes(**kw)
+    if idx == 2:
+        return terrain_spheres(**kw)
+    if idx == 4:
+        args = dict(seed=4, cells=(200, 250), n_spheres=50, n_mats=64, n_lights=4, w=512, h=512, spp=16, depth=4)
+        args.update(kw)
+        return terrain_spheres(**args)
+    raise ValueError(f"no scene generator for configuration {idx}")

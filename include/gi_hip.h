@@ -1,0 +1,264 @@
+/*
+ * gi_hip.h -- C-ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * This is the drop-in boundary that replaces globillum's GPU path-tracing
+ * hot path behind its gi_algorithm / rta tracer interface:
+ *
+ *   reference piece (file:line)                         replaced by
+ *   ---------------------------------------------------------------------------
+ *   rta::plugin_create_rt_set upload (src/hl-main.cpp:426)  gi_upload_triangles,
+ *                                                          gi_upload_bvh2 / gi_build_bvh
+ *   convert_and_upload_materials (src/material-conversion.cpp:59-151)
+ *                                                          gi_upload_materials
+ *   convert_and_upload_lights / add-rectlight (src/lights.cpp:84-96,
+ *     src/arealight-sampler.cu:25-62)                     gi_upload_lights
+ *   k::trace_cis_ailabox_indexed_with_alphamaps (src/tracers.cu:38-169),
+ *     bbvh-cuda closest-hit tracer (src/hl-main.cpp:375-385)
+ *                                                          gi_trace_closest
+ *   matching_any_hit_tracer() (src/gpu-pt.cpp:40)          gi_trace_any
+ *   k::setup_jittered_shirley / _dof_shirley (src/raygen.cu:10-145),
+ *     k::generate_rectlight_sample / generate_arealight_sample
+ *     (src/arealight-sampler.cu:110-327),
+ *     k::compute_path_contribution_and_bounce (src/gpu-pt-kernels.cu:43-272),
+ *     k::reset_data / k::combine_color_samples (src/util-kernels.cu:36-71),
+ *     gpu_pt_bouncer::{new_pass,new_path,bounce,trace_further_bounces}
+ *     (src/gpu-pt.h:102-226)                             gi_render, gi_progress_step
+ *   output_color readback (src/gpu-pt.cpp:53-55)           gi_get_image
+ *   gi::halton / generate_halton_pool_on_gpu (src/util.cpp:22-51)
+ *                                                          gi_halton_sample
+ *
+ * Conventions: integer status codes, no exceptions cross the ABI.  Host
+ * buffers are caller-owned and copied in or out; device buffers are owned
+ * by the context.  A context is externally synchronised (one caller thread)
+ * and owns one HIP stream; every call returns after its work completed.
+ */
+#ifndef GI_HIP_H
+#define GI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GI_ABI_VERSION 1
+
+typedef enum {
+  GI_OK = 0,
+  GI_ERR_INVALID_ARG = 1,   /* null pointer, zero/negative size, bad enum      */
+  GI_ERR_NO_DEVICE = 2,     /* no HIP device / HIP runtime error at create     */
+  GI_ERR_HIP = 3,           /* HIP runtime error during a call                 */
+  GI_ERR_STATE = 4,         /* call order violated (e.g. render before upload) */
+  GI_ERR_OUT_OF_MEMORY = 5,
+  GI_ERR_LIMIT = 6,         /* size beyond a documented limit                  */
+  GI_ERR_UNKNOWN_NAME = 7   /* unknown algorithm name (gi_algorithm::select)   */
+} gi_status;
+
+/* Triangle record: the field set of rta::cuda::simple_triangle as filled in
+ * src/hl-main.cpp:332-348 (positions, vertex normals, texture coordinates,
+ * material index).  100 bytes, copied as-is to the device. */
+typedef struct {
+  float a[3], b[3], c[3];
+  float na[3], nb[3], nc[3];
+  float ta[2], tb[2], tc[2];
+  int32_t material_index;
+} gi_tri;
+
+/* Hit record: rta::triangle_intersection {t, beta, gamma, ref}
+ * (src/hybrid-pt.cpp:288-289, src/tracers.cu:110-113).
+ * Barycentrics: P = (1-beta-gamma)*a + beta*b + gamma*c.
+ * Invalid hit: t = FLT_MAX and ref = GI_INVALID_REF. */
+typedef struct {
+  float t, beta, gamma;
+  uint32_t ref;
+} gi_hit;
+#define GI_INVALID_REF 0xFFFFFFFFu
+
+/* Material: rta::cuda::material_t (src/material.h:142-184) with the
+ * PrincipledBRDFParameters (src/principledParameters.h:7-21) inlined.
+ * principled == 0 selects the LambertianMaterial path
+ * (src/material-wrapper.h:11-28, ALL_MATERIAL_LAMBERT). */
+typedef struct {
+  float diffuse_color[3];
+  float specular_color[3];
+  float alpha;
+  int32_t principled;
+  float metallic, subsurface, specular, roughness;
+  float specular_tint, anisotropic, sheen, sheen_tint;
+  float clearcoat, clearcoat_gloss, opacity, transmission;
+  float color[3];
+  int32_t texture;          /* reserved: -1 (textures are not on this path) */
+} gi_material;
+
+/* Light: gi::light with the rect_light member (src/lights.h:94-154).
+ * power is the selection weight; gi_rect_light_power() gives the value the
+ * reference computes in add-rectlight (src/lights.cpp:92). */
+typedef struct {
+  float center[3], dir[3], up[3], col[3];
+  float wh[2];
+  float power;
+  int32_t type;             /* GI_LIGHT_RECT */
+} gi_light;
+#define GI_LIGHT_RECT 0
+
+/* One view of a render: camera as given to camera_ray_generator_shirley::
+ * setup(pos, dir, up, fovy) (src/gpu-pt.cpp:67) plus the per-view image
+ * size and sample count.  fovy is the setup() argument in degrees; the ray
+ * generator uses tan(pi*(fovy/2)/180) as the half-height of the image plane
+ * (src/raygen.cu:18-19).  aperture > 0 selects the thin-lens generator
+ * (src/raygen.cu:61-118). */
+typedef struct {
+  float pos[3], dir[3], up[3];
+  float fovy;
+  int32_t w, h, spp;
+  float aperture, focus_distance;
+} gi_view;
+
+/* Render options.  max_depth is max_path_len (src/gpu-pt.h:215-226): each
+ * sample traces max_depth closest-hit rays and up to max_depth shadow rays.
+ * rr_start > 0 enables Russian roulette once rr_start segments were traced
+ * (survival = max channel of the throughput).  offset_bounce / offset_shadow
+ * replace the scene-scale constants 0.3 (src/hybrid-pt.cpp:551) and 0.9
+ * (src/arealight-sampler-cpu.cpp:115); <= 0 selects 1e-4 * scene diagonal. */
+typedef struct {
+  int32_t max_depth;
+  int32_t rr_start;
+  uint32_t seed;
+  uint32_t flags;
+  float offset_bounce;
+  float offset_shadow;
+  float background[3];
+  int32_t samples_per_pass; /* samples per pixel kept in flight; 0 = auto    */
+} gi_render_opts;
+
+#define GI_FLAG_BATCHED   0x1u  /* all views share one set of queues       */
+#define GI_FLAG_GRAPH     0x2u  /* bounce loop captured into a hipGraph    */
+
+/* Telemetry of the last gi_render / gi_adjoint call. */
+typedef struct {
+  uint64_t rays_closest;    /* closest-hit rays traced (primary + bounce) */
+  uint64_t rays_shadow;     /* any-hit rays traced                         */
+  uint64_t samples;         /* sum over views of w*h*spp                   */
+  uint64_t path_vertices;   /* shaded hits                                 */
+  double ms_total;          /* wall time of the call, host clock           */
+  double ms_trace_closest;  /* HIP-event time of the closest-hit kernels   */
+  double ms_trace_any;
+  double ms_shade;
+  double ms_other;
+  int32_t launches_closest; /* kernel launches of the closest-hit kernel   */
+  int32_t passes;
+} gi_counters;
+
+/* Tunable launch parameters (chosen by the autotuner). 0 = default. */
+typedef struct {
+  int32_t trace_block;      /* threads per block of the traversal kernels  */
+  int32_t trace_blocks_per_cu;
+  int32_t shade_block;
+  int32_t samples_per_pass;
+} gi_tuning;
+
+/* Per-material gradient of the adjoint pass. */
+typedef struct {
+  float d_color[3];
+  float d_roughness;
+  float d_metallic;
+} gi_material_grad;
+
+typedef struct gi_ctx gi_ctx;
+typedef struct gi_bvh gi_bvh;
+
+/* ---- version / errors ---------------------------------------------------- */
+int gi_abi_version(void);
+const char *gi_status_string(int status);
+
+/* ---- context (gi_algorithm::activate, src/gpu-pt.cpp:11-47) --------------- */
+int gi_create(gi_ctx **out, int device);
+void gi_destroy(gi_ctx *ctx);
+const char *gi_last_error(const gi_ctx *ctx);
+
+/* ---- host BVH build (stands in for rta's -b bsah, src/hl-main.cpp:377-379) -
+ * Binned-SAH binary BVH over the triangle list.  Host only; no device call. */
+int gi_bvh_build(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh **out);
+void gi_bvh_free(gi_bvh *bvh);
+/* Size of the binary BVH in the reference's 2 x float4 node layout
+ * (src/tracers.cu:26-36) and of its triangle index array. */
+int gi_bvh_sizes(const gi_bvh *bvh, int64_t *n_nodes, int64_t *n_indices);
+/* Export in the reference layout: nodes[8*n_nodes] floats, indices[n_indices]. */
+int gi_bvh_export2(const gi_bvh *bvh, float *nodes, uint32_t *indices);
+
+/* ---- scene upload (src/hl-main.cpp:426-428) -------------------------------- */
+int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n);
+int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n);
+int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n);
+/* Build the binned-SAH BVH on the host and upload it. */
+int gi_build_bvh(gi_ctx *ctx, int32_t max_leaf);
+/* Upload an externally built binary BVH in the reference's 2 x float4 layout
+ * (src/tracers.cu:26-36): nodes[8*n_nodes], indices[n_indices]. */
+int gi_upload_bvh2(gi_ctx *ctx, const float *nodes, int64_t n_nodes,
+                   const uint32_t *indices, int64_t n_indices);
+
+/* ---- tracing primitives (trace_rays of the closest / any-hit tracers) ------
+ * orig[3n], dir[3n], max_t[n] are host arrays in the reference's AoS float
+ * layout (src/raygen.cu:53-58).  Closest hit is the minimum t, ties broken by
+ * the lower triangle id; a triangle counts when 0 < t <= max_t. */
+int gi_trace_closest(gi_ctx *ctx, const float *orig, const float *dir,
+                     const float *max_t, int64_t n, gi_hit *out);
+/* occluded[i] = 1 iff some triangle is hit with 0 < t <= max_t[i]. */
+int gi_trace_any(gi_ctx *ctx, const float *orig, const float *dir,
+                 const float *max_t, int64_t n, uint8_t *occluded);
+/* Same as gi_trace_* on device pointers (orig/dir as float[3n]); runs on the
+ * context stream and returns after completion. */
+int gi_trace_closest_device(gi_ctx *ctx, const float *d_orig, const float *d_dir,
+                            const float *d_max_t, int64_t n, gi_hit *d_out);
+int gi_trace_any_device(gi_ctx *ctx, const float *d_orig, const float *d_dir,
+                        const float *d_max_t, int64_t n, uint8_t *d_occluded);
+
+/* ---- camera rays (setup_jittered_shirley, src/raygen.cu:10-59) ------------
+ * Writes the w*h primary rays of sample `sample` of view `v` (host arrays). */
+int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample,
+                            uint32_t seed, float *orig, float *dir, float *max_t);
+
+/* ---- renderer ------------------------------------------------------------- */
+int gi_render(gi_ctx *ctx, const gi_view *views, int32_t n_views,
+              const gi_render_opts *opts);
+/* Running-mean image of view v (float rgb[3*w*h], row y*w+x). */
+int gi_get_image(gi_ctx *ctx, int32_t view, float *rgb);
+/* Device pointer of the image of view v (valid until the next gi_render). */
+int gi_get_image_device(gi_ctx *ctx, int32_t view, float **d_rgb);
+int gi_get_counters(const gi_ctx *ctx, gi_counters *out);
+int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);
+int gi_get_tuning(const gi_ctx *ctx, gi_tuning *t);
+
+/* Progressive mode (gpu_pt::compute / update, src/gpu-pt.cpp:49-73):
+ * begin resets the accumulation, each step renders one more sample per
+ * pixel of every view; returns GI_OK while samples remain, and sets *done. */
+int gi_progress_begin(gi_ctx *ctx, const gi_view *views, int32_t n_views,
+                      const gi_render_opts *opts);
+int gi_progress_step(gi_ctx *ctx, int32_t *done);
+
+/* ---- adjoint (path-replay backprop; no counterpart in the reference) -------
+ * Gradient of sum_i dL_dimage[i] * image[i] of view 0 with respect to the
+ * material parameters and the light emissions, replaying the same Halton
+ * samples as the last gi_render.  d_emission[3*n_lights]. */
+int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats,
+               float *d_emission);
+
+/* ---- Halton (gi::halton, src/util.cpp:22-32) ------------------------------
+ * Scrambled radical inverse of index n in the prime base of dimension dim,
+ * with the digit permutations derived from seed.  Host function. */
+float gi_halton_sample(uint64_t n, int32_t dim, uint32_t seed);
+/* Unscrambled radical inverse (the reference's gi::halton). */
+float gi_radical_inverse(uint64_t n, int32_t base);
+/* Prime base of dimension dim, and the permutation of its digits. */
+int32_t gi_halton_prime(int32_t dim);
+int gi_halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm);
+
+/* Light selection weight of a rect light, as add-rectlight computes it
+ * (src/lights.cpp:92). */
+float gi_rect_light_power(const gi_light *l);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GI_HIP_H */

@@ -1,0 +1,1010 @@
+/*
+ * ORACLE -- test infrastructure only (see oracle.h).  CPU restatement of
+ * globillum's CPU path tracer, templated on the arithmetic type so the same
+ * code gives the FP32 checker and the FP64 reference.  Compiled with
+ * -ffp-contract=off: every a*b+c rounds twice, as in the device code.
+ *
+ * Reference files restated (all under src/ of kiselgra/globillum):
+ *   halton                 util.cpp:22-32
+ *   camera rays            raygen.cu:10-118, raygen.h:122-181
+ *   BVH node layout        tracers.cu:26-36
+ *   closest-hit traversal  tracers.cu:42-138
+ *   NEE                    arealight-sampler-cpu.cpp:25-143
+ *   shading + bounce       hybrid-pt.cpp:196-206, 330-371, 421-448, 450-577
+ *   BRDFs                  simpleMaterial.h:82-113, principledMaterial.h:22-394,
+ *                          materialBase.h:16-40, material-wrapper.h:15-51
+ *   accumulate             util-kernels.cu:17-25
+ *   state machine          hybrid-pt.h:115-249, gpu-pt.h:102-226
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+template <class R> struct V3 {
+  R x, y, z;
+};
+template <class R> inline V3<R> mk(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> inline V3<R> mul(V3<R> a, V3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class R> inline V3<R> divs(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline V3<R> normalize(V3<R> a) { return a * (R(1) / std::sqrt(dot(a, a))); }
+template <class R> inline V3<R> ld(const float *p) { return {R(p[0]), R(p[1]), R(p[2])}; }
+template <class R> inline R maxc(V3<R> a) { return std::max(a.x, std::max(a.y, a.z)); }
+
+template <class R> struct K {
+  static constexpr R pi = R(3.14159265358979323846);
+  static constexpr R inv_pi = R(0.31830988618379067154);
+  static constexpr R two_pi = R(6.28318530717958647692);
+};
+
+/* ---------------------------------------------------------------- Halton -- */
+struct Primes {
+  std::vector<int32_t> p;
+  Primes() {
+    for (int32_t c = 2; (int)p.size() < 1024; ++c) {
+      bool ok = true;
+      for (int32_t q : p) {
+        if (q * q > c) break;
+        if (c % q == 0) { ok = false; break; }
+      }
+      if (ok) p.push_back(c);
+    }
+  }
+};
+const Primes &primes() {
+  static Primes P;
+  return P;
+}
+
+/* PCG32 (O'Neill 2014, pcg32_srandom_r / pcg32_random_r /
+ * pcg32_boundedrand_r), used to draw the per-dimension digit permutations. */
+struct Pcg32 {
+  uint64_t state = 0, inc = 1;
+  uint32_t next() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+  }
+  void seed(uint64_t initstate, uint64_t initseq) {
+    state = 0;
+    inc = (initseq << 1u) | 1u;
+    next();
+    state += initstate;
+    next();
+  }
+  uint32_t bounded(uint32_t bound) {
+    uint32_t threshold = (0u - bound) % bound;
+    for (;;) {
+      uint32_t r = next();
+      if (r >= threshold) return r % bound;
+    }
+  }
+};
+
+void halton_perm(int32_t dim, uint32_t seed, uint16_t *perm) {
+  int32_t p = primes().p[dim];
+  for (int32_t i = 0; i < p; ++i) perm[i] = (uint16_t)i;
+  Pcg32 r;
+  r.seed(((uint64_t)seed << 32) ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(dim + 1)), (uint64_t)(dim + 1));
+  for (int32_t i = p - 1; i >= 1; --i) {
+    uint32_t j = r.bounded((uint32_t)(i + 1));
+    std::swap(perm[i], perm[j]);
+  }
+}
+
+/* Scrambled radical inverse: digits d_k of n in base p map to perm[d_k]; the
+ * infinitely many leading zero digits contribute perm[0]/(p-1) * p^-K.
+ * Evaluated in FP64 and rounded once to FP32. */
+float scrambled_ri(uint64_t n, int32_t p, const uint16_t *perm) {
+  const double inv = 1.0 / (double)p;
+  uint64_t rev = 0;
+  double inv_n = 1.0;
+  while (n > 0) {
+    uint64_t q = n / (uint64_t)p;
+    uint64_t d = n - q * (uint64_t)p;
+    rev = rev * (uint64_t)p + perm[d];
+    inv_n *= inv;
+    n = q;
+  }
+  double r = inv_n * ((double)rev + (double)perm[0] / (double)(p - 1));
+  float f = (float)r;
+  if (f >= 1.0f) f = 0x1.fffffep-1f;
+  return f;
+}
+
+struct HaltonTable {
+  uint32_t seed;
+  int32_t ndims;
+  std::vector<uint32_t> off;
+  std::vector<uint16_t> perm;
+  HaltonTable(uint32_t s, int32_t nd) : seed(s), ndims(nd) {
+    off.resize(nd + 1);
+    off[0] = 0;
+    for (int32_t d = 0; d < nd; ++d) off[d + 1] = off[d] + primes().p[d];
+    perm.resize(off[nd]);
+    for (int32_t d = 0; d < nd; ++d) halton_perm(d, s, perm.data() + off[d]);
+  }
+  float sample(uint64_t n, int32_t dim) const {
+    return scrambled_ri(n, primes().p[dim], perm.data() + off[dim]);
+  }
+};
+
+/* Dimension map of one sample: (jitter x, y), (lens x, y), then 7 per path
+ * vertex b: NEE x, y, light choice; BSDF x, y, lobe; Russian roulette. */
+enum { DIM_JX = 0, DIM_JY = 1, DIM_LX = 2, DIM_LY = 3, DIM_VTX0 = 4, DIMS_PER_VTX = 7 };
+inline int32_t dims_for_depth(int32_t depth) { return DIM_VTX0 + DIMS_PER_VTX * depth; }
+
+/* -------------------------------------------------------- tangent frames -- */
+/* Orthonormal basis around a unit vector n: Duff et al., "Building an
+ * Orthonormal Basis, Revisited", JCGT 6(1), 2017.  Stands in for libhyb's
+ * make_tangent_frame (used at src/hybrid-pt.cpp:526); T and B are normalised
+ * afterwards as the reference does (src/hybrid-pt.cpp:527-528). */
+template <class R> void tangent_frame(V3<R> n, V3<R> &t, V3<R> &b) {
+  R sign = std::copysign(R(1), n.z);
+  R a = R(-1) / (sign + n.z);
+  R bb = n.x * n.y * a;
+  t = mk<R>(R(1) + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+  b = mk<R>(bb, sign + n.y * n.y * a, -n.y);
+  t = normalize(t);
+  b = normalize(b);
+}
+/* libhyb make_tangential(v, n): v given in the frame (T, B, n) to world. */
+template <class R> V3<R> tangential(V3<R> v, V3<R> n) {
+  V3<R> t, b;
+  tangent_frame(n, t, b);
+  return (t * v.x + b * v.y) + n * v.z;
+}
+
+/* ------------------------------------------------------------ intersection */
+/* Moller & Trumbore 1997, no back-face culling; hit iff det != 0,
+ * u,v >= 0, u+v <= 1, t > 0.  Stands in for rta's intersect_tri_opt
+ * (src/tracers.cu:112).  beta = u weights vertex b, gamma = v weights c. */
+template <class R>
+inline bool intersect_tri(V3<R> a, V3<R> e1, V3<R> e2, V3<R> o, V3<R> d, R &t, R &u, R &v) {
+  V3<R> pvec = cross(d, e2);
+  R det = dot(e1, pvec);
+  if (det == R(0)) return false;
+  R inv_det = R(1) / det;
+  V3<R> tvec = o - a;
+  u = dot(tvec, pvec) * inv_det;
+  if (!(u >= R(0)) || u > R(1)) return false;
+  V3<R> qvec = cross(tvec, e1);
+  v = dot(d, qvec) * inv_det;
+  if (!(v >= R(0)) || u + v > R(1)) return false;
+  t = dot(e2, qvec) * inv_det;
+  return t > R(0);
+}
+
+/* Slab test after Aila & Laine 2009 with the idir clamp of
+ * src/tracers.cu:48-51; returns the entry distance. */
+template <class R>
+inline bool slab(const float *bmin, const float *bmax, V3<R> o, V3<R> idir, R tfar, R &dist) {
+  R x0 = (R(bmin[0]) - o.x) * idir.x, x1 = (R(bmax[0]) - o.x) * idir.x;
+  R y0 = (R(bmin[1]) - o.y) * idir.y, y1 = (R(bmax[1]) - o.y) * idir.y;
+  R z0 = (R(bmin[2]) - o.z) * idir.z, z1 = (R(bmax[2]) - o.z) * idir.z;
+  R tmin = std::max(std::max(std::min(x0, x1), std::min(y0, y1)), std::max(std::min(z0, z1), R(0)));
+  R tmax = std::min(std::min(std::max(x0, x1), std::max(y0, y1)), std::min(std::max(z0, z1), tfar));
+  dist = tmin;
+  return tmin <= tmax;
+}
+
+template <class R> inline R clamp_idir(R d) {
+  const R ooeps = R(8.2718061255302767e-25); /* 2^-80 */
+  return R(1) / (std::fabs(d) > ooeps ? d : std::copysign(ooeps, d));
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------ scene -- */
+struct or_scene {
+  std::vector<gi_tri> tris;
+  std::vector<gi_material> mats;
+  std::vector<gi_light> lights;
+  std::vector<float> nodes; /* 8 floats per node, reference 2f4 layout */
+  std::vector<uint32_t> indices;
+  float lo[3], hi[3];
+  float power_total = 0.f;
+};
+
+namespace {
+
+inline int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
+inline float bitsf(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
+
+/* Object-median binary BVH (the oracle's own, deliberately unlike the
+ * product's binned SAH): split the centroid bounds' longest axis at the
+ * median.  Emits the reference 2f4 layout of src/tracers.cu:26-36. */
+struct MedianBuilder {
+  const std::vector<gi_tri> &tris;
+  std::vector<uint32_t> &idx;
+  std::vector<float> &nodes;
+  int32_t max_leaf;
+  void bounds(uint32_t b, uint32_t e, float *lo, float *hi) {
+    for (int k = 0; k < 3; ++k) { lo[k] = FLT_MAX; hi[k] = -FLT_MAX; }
+    for (uint32_t i = b; i < e; ++i) {
+      const gi_tri &t = tris[idx[i]];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], std::min(t.a[k], std::min(t.b[k], t.c[k])));
+        hi[k] = std::max(hi[k], std::max(t.a[k], std::max(t.b[k], t.c[k])));
+      }
+    }
+  }
+  void set_box(int64_t n, const float *lo, const float *hi) {
+    float *p = &nodes[8 * n];
+    p[2] = lo[0]; p[3] = lo[1]; p[4] = lo[2];
+    p[5] = hi[0]; p[6] = hi[1]; p[7] = hi[2];
+  }
+  void build(int64_t node, uint32_t b, uint32_t e) {
+    float lo[3], hi[3];
+    bounds(b, e, lo, hi);
+    set_box(node, lo, hi);
+    if (e - b <= (uint32_t)max_leaf) {
+      nodes[8 * node + 0] = bitsf((int32_t)((e - b) << 1));
+      nodes[8 * node + 1] = bitsf((int32_t)b);
+      return;
+    }
+    float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    auto cen = [&](uint32_t id, int k) {
+      const gi_tri &t = tris[id];
+      return (t.a[k] + t.b[k] + t.c[k]) * (1.0f / 3.0f);
+    };
+    for (uint32_t i = b; i < e; ++i)
+      for (int k = 0; k < 3; ++k) {
+        clo[k] = std::min(clo[k], cen(idx[i], k));
+        chi[k] = std::max(chi[k], cen(idx[i], k));
+      }
+    int ax = 0;
+    for (int k = 1; k < 3; ++k)
+      if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+    uint32_t m = (b + e) / 2;
+    std::nth_element(idx.begin() + b, idx.begin() + m, idx.begin() + e, [&](uint32_t x, uint32_t y) {
+      float cx = cen(x, ax), cy = cen(y, ax);
+      return cx < cy || (cx == cy && x < y);
+    });
+    int64_t left = (int64_t)nodes.size() / 8;
+    nodes.resize(nodes.size() + 16);
+    nodes[8 * node + 0] = bitsf((int32_t)((left << 1) | 1));
+    nodes[8 * node + 1] = bitsf((int32_t)(left + 1));
+    build(left, b, m);
+    build(left + 1, m, e);
+  }
+};
+
+/* --------------------------------------------------------------- traverse -- */
+/* Ordered traversal of the binary 2f4 BVH after src/tracers.cu:42-138:
+ * near child first, far child pushed, both children's boxes tested per inner
+ * node.  Differences: the hit record starts invalid (t = FLT_MAX, see
+ * src/tracers.cu:61-62), boxes are culled only when their entry distance is
+ * strictly beyond the closest hit or t_max, and a triangle replaces the
+ * closest hit when (t, id) is lexicographically smaller -- this makes the
+ * result independent of traversal order and BVH topology. */
+template <class R, bool ANY>
+bool traverse(const or_scene &s, V3<R> o, V3<R> d, R tmax, R &bt, R &bu, R &bv, uint32_t &bid,
+              uint64_t &nbox, uint64_t &ntri) {
+  V3<R> idir = mk<R>(clamp_idir(d.x), clamp_idir(d.y), clamp_idir(d.z));
+  bt = R(FLT_MAX);
+  bid = GI_INVALID_REF;
+  const float *N = s.nodes.data();
+  if (s.nodes.empty()) return false;
+  uint32_t stack[64];
+  int sp = -1;
+  int64_t cur = 0;
+  for (;;) {
+    const float *n = N + 8 * cur;
+    int32_t x = fbits(n[0]), y = fbits(n[1]);
+    if (x & 1) {
+      int64_t l = x >> 1, r = y;
+      R dl, dr;
+      R lim = std::min(bt, tmax);
+      bool hl = slab<R>(N + 8 * l + 2, N + 8 * l + 5, o, idir, lim, dl);
+      bool hr = slab<R>(N + 8 * r + 2, N + 8 * r + 5, o, idir, lim, dr);
+      nbox += 2;
+      if (hl && hr) {
+        if (dl <= dr) { stack[++sp] = (uint32_t)r; cur = l; }
+        else { stack[++sp] = (uint32_t)l; cur = r; }
+        continue;
+      } else if (hl) { cur = l; continue; }
+      else if (hr) { cur = r; continue; }
+    } else {
+      uint32_t cnt = (uint32_t)(x >> 1), off = (uint32_t)y;
+      for (uint32_t i = 0; i < cnt; ++i) {
+        uint32_t id = s.indices[off + i];
+        const gi_tri &T = s.tris[id];
+        V3<R> a = ld<R>(T.a), b = ld<R>(T.b), c = ld<R>(T.c);
+        R t, u, v;
+        ++ntri;
+        if (intersect_tri<R>(a, b - a, c - a, o, d, t, u, v) && t <= tmax) {
+          if (ANY) { bt = t; bu = u; bv = v; bid = id; return true; }
+          if (t < bt || (t == bt && id < bid)) { bt = t; bu = u; bv = v; bid = id; }
+        }
+      }
+    }
+    if (sp < 0) break;
+    cur = stack[sp--];
+  }
+  return bid != GI_INVALID_REF;
+}
+
+template <class R>
+bool brute(const or_scene &s, V3<R> o, V3<R> d, R tmax, R &bt, R &bu, R &bv, uint32_t &bid) {
+  bt = R(FLT_MAX);
+  bid = GI_INVALID_REF;
+  for (uint32_t id = 0; id < s.tris.size(); ++id) {
+    const gi_tri &T = s.tris[id];
+    V3<R> a = ld<R>(T.a), b = ld<R>(T.b), c = ld<R>(T.c);
+    R t, u, v;
+    if (intersect_tri<R>(a, b - a, c - a, o, d, t, u, v) && t <= tmax &&
+        (t < bt || (t == bt && id < bid))) {
+      bt = t; bu = u; bv = v; bid = id;
+    }
+  }
+  return bid != GI_INVALID_REF;
+}
+
+/* ------------------------------------------------------------------ BRDFs -- */
+enum { T_DIFFUSE = 1, T_SPECULAR = 0, T_SPEC_REFL = 3, T_SPEC_TRANS = 4, T_DISNEY_REFL = 5 };
+
+template <class R> inline R saturate(R a) { return std::min(std::max(a, R(0)), R(1)); }
+template <class R> inline R schlick(R u) { R m = saturate(R(1) - u); R m2 = m * m; return m2 * m2 * m; }
+template <class R> inline R mixf(R x, R y, R s) { return (R(1) - s) * x + s * y; }
+template <class R> inline V3<R> mixv(V3<R> x, V3<R> y, R s) { return (R(1) - s) * x + s * y; }
+template <class R> inline R sqr(R a) { return a * a; }
+template <class R> inline R clamp01(R a) { return a < R(0) ? R(0) : a; }
+
+/* sin(2 pi v), cos(2 pi v) for v in [0,1).  FP32: quadrant q = floor(4v)
+ * and remainder f = 4v - q (both exact), Taylor series of sin(pi/2 f) to
+ * degree 13 and cos(pi/2 f) to degree 14 (truncation error < 1e-9), rotated
+ * into the quadrant -- basic operations only, so any IEEE machine gives the
+ * same bits.  FP64: the C library. */
+constexpr double kH = 1.5707963267948966;
+inline void sincos2pi(float v, float &s, float &c) {
+  const float S1 = (float)kH, S3 = (float)(-kH * kH * kH / 6.0), S5 = (float)(kH * kH * kH * kH * kH / 120.0);
+  const float S7 = (float)(-kH * kH * kH * kH * kH * kH * kH / 5040.0);
+  const float S9 = (float)(kH * kH * kH * kH * kH * kH * kH * kH * kH / 362880.0);
+  const float S11 = (float)(-kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH / 39916800.0);
+  const float S13 = (float)(kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH / 6227020800.0);
+  const float C2 = (float)(-kH * kH / 2.0), C4 = (float)(kH * kH * kH * kH / 24.0);
+  const float C6 = (float)(-kH * kH * kH * kH * kH * kH / 720.0);
+  const float C8 = (float)(kH * kH * kH * kH * kH * kH * kH * kH / 40320.0);
+  const float C10 = (float)(-kH * kH * kH * kH * kH * kH * kH * kH * kH * kH / 3628800.0);
+  const float C12 = (float)(kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH / 479001600.0);
+  const float C14 = (float)(-kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH * kH / 87178291200.0);
+  float x = v * 4.0f;
+  float q = std::floor(x);
+  float f = x - q;
+  float f2 = f * f;
+  float ps = f * (S1 + f2 * (S3 + f2 * (S5 + f2 * (S7 + f2 * (S9 + f2 * (S11 + f2 * S13))))));
+  float pc = 1.0f + f2 * (C2 + f2 * (C4 + f2 * (C6 + f2 * (C8 + f2 * (C10 + f2 * (C12 + f2 * C14))))));
+  int qi = (int)q & 3;
+  if (qi == 0) { s = ps; c = pc; }
+  else if (qi == 1) { s = pc; c = -ps; }
+  else if (qi == 2) { s = -ps; c = -pc; }
+  else { s = -pc; c = ps; }
+}
+inline void sincos2pi(double v, double &s, double &c) {
+  s = std::sin(K<double>::two_pi * v);
+  c = std::cos(K<double>::two_pi * v);
+}
+
+/* cosineSampleHemisphere, src/materialBase.h:16-24 */
+template <class R> V3<R> cosine_sample(R u, R v) {
+  R r = std::sqrt(u);
+  R s, c;
+  sincos2pi(v, s, c);
+  return mk<R>(r * c, r * s, std::sqrt(std::max(R(0), R(1) - u)));
+}
+
+template <class R> V3<R> mon2lin(V3<R> p) {
+  return mk<R>(std::pow(p.x, R(2.2)), std::pow(p.y, R(2.2)), std::pow(p.z, R(2.2)));
+}
+
+template <class R> R smith_ggx(R ndv, R ag) {
+  R a = ag * ag, b = ndv * ndv;
+  return R(1) / (ndv + std::sqrt(a + b - a * b));
+}
+template <class R> R gtr1(R ndh, R a) {
+  if (a >= R(1)) return K<R>::inv_pi;
+  R a2 = a * a;
+  R t = R(1) + (a2 - R(1)) * ndh * ndh;
+  return (a2 - R(1)) / (K<R>::pi * std::log(a2) * t);
+}
+
+template <class R> struct MatP {
+  R metallic, subsurface, specular, roughness, spec_tint, aniso, sheen, sheen_tint, clearcoat,
+      cc_gloss, opacity;
+  V3<R> color, diffuse;
+  bool principled;
+  explicit MatP(const gi_material &m)
+      : metallic(m.metallic), subsurface(m.subsurface), specular(m.specular), roughness(m.roughness),
+        spec_tint(m.specular_tint), aniso(m.anisotropic), sheen(m.sheen), sheen_tint(m.sheen_tint),
+        clearcoat(m.clearcoat), cc_gloss(m.clearcoat_gloss), opacity(m.opacity),
+        color(ld<R>(m.color)), diffuse(ld<R>(m.diffuse_color)), principled(m.principled != 0) {}
+  bool glass() const { return principled && opacity < R(1); }
+};
+
+/* evaluatePrincipledBRDF_specular, src/principledMaterial.h:210-252 */
+template <class R> V3<R> eval_spec(const MatP<R> &p, V3<R> V, V3<R> N, V3<R> L) {
+  R ndl = dot(N, L), ndv = dot(N, V);
+  if (ndl < R(0) || ndv < R(0)) return mk<R>(0, 0, 0);
+  V3<R> H = normalize(L + V);
+  R ndh = dot(N, H), ldh = dot(L, H);
+  V3<R> cd = mon2lin(p.color);
+  R lum = R(0.3) * cd.x + R(0.6) * cd.y + R(0.1) * cd.z;
+  V3<R> one = mk<R>(1, 1, 1);
+  V3<R> tint = lum > R(0) ? divs(cd, lum) : one;
+  V3<R> cs0 = mixv((p.specular * R(0.08)) * mixv(one, tint, p.spec_tint), cd, p.metallic);
+  R fh = schlick(ldh);
+  V3<R> fs = mixv(cs0, one, fh);
+  R cg = R(0);
+  if (p.clearcoat > R(0)) {
+    R dr = gtr1(ndh, mixf(R(0.1), R(0.001), p.cc_gloss));
+    R fr = mixf(R(0.04), R(1), fh);
+    R gr = smith_ggx(ndl, R(0.25)) * smith_ggx(ndv, R(0.25));
+    cg = R(0.25) * p.clearcoat * gr * fr * dr;
+  }
+  V3<R> sum = fs + mk<R>(cg, cg, cg);
+  return (sum * ndl) * p.metallic;
+}
+
+/* evaluatePrincipledBRDF_diffuse, src/principledMaterial.h:254-301 */
+template <class R> V3<R> eval_diff(const MatP<R> &p, V3<R> V, V3<R> N, V3<R> L) {
+  R ndl = dot(N, L), ndv = dot(N, V);
+  if (ndl < R(0) || ndv < R(0)) return mk<R>(0, 0, 0);
+  V3<R> H = normalize(L + V);
+  R ldh = dot(L, H);
+  V3<R> cd = mon2lin(p.color);
+  R lum = R(0.3) * cd.x + R(0.6) * cd.y + R(0.1) * cd.z;
+  V3<R> one = mk<R>(1, 1, 1);
+  V3<R> tint = lum > R(0) ? divs(cd, lum) : one;
+  V3<R> csheen = mixv(one, tint, p.sheen_tint);
+  R fl = schlick(ndl), fv = schlick(ndv);
+  R fd90 = R(0.5) + R(2) * ldh * ldh * p.roughness;
+  R fd = mixf(R(1), fd90, fl) * mixf(R(1), fd90, fv);
+  R fh = schlick(ldh);
+  V3<R> fsheen = (fh * p.sheen) * csheen;
+  R fss90 = ldh * ldh * p.roughness;
+  R fss = mixf(R(1), fss90, fl) * mixf(R(1), fss90, fv);
+  R ss = R(1.25) * (fss * (R(1) / (ndl + ndv) - R(0.5)) + R(0.5));
+  V3<R> dc = (K<R>::inv_pi * mixf(fd, ss, p.subsurface) * cd + fsheen) * (R(1) - p.metallic);
+  return dc * ndl;
+}
+
+/* materialBRDF::evaluate, src/material-wrapper.h:34-38 with
+ * LambertianMaterial::evaluate (src/simpleMaterial.h:99-101) and
+ * PrincipledMaterial::evaluate (src/principledMaterial.h:336-352). */
+template <class R> V3<R> brdf_eval(const MatP<R> &p, int type, V3<R> wo, V3<R> wi, V3<R> N) {
+  if (!p.principled) return (p.diffuse * K<R>::inv_pi) * std::fabs(dot(wi, N));
+  if (type == T_DISNEY_REFL) return eval_spec(p, wo, N, wi);
+  if (type == T_SPEC_REFL) return mk<R>(1, 1, 1);
+  if (type == T_SPEC_TRANS) return mk<R>(1, 1, 1) * (R(1) / std::fabs(dot(wi, N)));
+  return eval_diff(p, wo, N, wi);
+}
+
+/* sampleGTR2Aniso (src/principledMaterial.h:121-176) with the halfway
+ * sample of :46-61 in tangent space (T = x, B = y, N = z). */
+template <class R> R sample_gtr2(R u, R v, R ax, R ay, V3<R> V, V3<R> &L) {
+  R s, c;
+  sincos2pi(u, s, c);
+  R fac = std::sqrt(v / (R(1) - v));
+  V3<R> m = normalize(mk<R>(fac * (ax * c), fac * (ay * s), R(1)));
+  L = V - (R(2) * dot(V, m)) * m; /* reflectR, src/materialBase.h:36-38 */
+  if (L.z < R(0)) L = -L;
+  R roughg = sqr(ax * ay * R(0.5) + R(0.5));
+  R dlm = dot(L, m), dvm = dot(V, m);
+  R tl = std::sqrt(R(1) - dlm * dlm) / dlm;
+  R tv = std::sqrt(R(1) - dvm * dvm) / dvm;
+  R rg2 = roughg * roughg;
+  auto g1 = [&](V3<R> w, R tn) {
+    R xi = (dot(w, m) / w.z) > R(0) ? R(1) : R(0);
+    return xi * R(2) / (R(1) + std::sqrt(R(1) + rg2 * tn * tn));
+  };
+  R gs = g1(L, tl) * g1(V, tv);
+  R lw = (std::fabs(dot(L, m)) * gs) / (std::fabs(L.z) * std::fabs(m.z));
+  if (lw < R(0.01)) lw = R(0.01);
+  return lw;
+}
+
+/* sampleGlass, src/principledMaterial.h:86-118 (ior 1.9, :358). */
+template <class R> int sample_glass(R u, V3<R> V, V3<R> &L) {
+  const R ior = R(1.9);
+  R c = V.z;
+  R fr = schlick(c);
+  if (u > fr) { L = -V; return T_SPEC_REFL; }
+  R ti = R(1) + ior * (c * c - R(1));
+  if (ti <= R(0)) {
+    L = V - (R(2) * V.z) * mk<R>(0, 0, 1);
+    return T_SPEC_REFL;
+  }
+  R sg = V.z > R(0) ? R(1) : R(-1);
+  L = (ior * c - sg * std::sqrt(ti)) * mk<R>(0, 0, 1) - ior * V;
+  return T_SPEC_TRANS;
+}
+
+/* materialBRDF::sample (src/material-wrapper.h:30-33):
+ * LambertianMaterial::sample (src/simpleMaterial.h:103-106) or
+ * PrincipledMaterial::sample (src/principledMaterial.h:354-390). */
+template <class R> int brdf_sample(const MatP<R> &p, V3<R> wo, V3<R> xi, V3<R> &wi, R &pdf) {
+  if (!p.principled) {
+    wi = cosine_sample(xi.x, xi.y);
+    pdf = std::fabs(wi.z) * K<R>::inv_pi;
+    return T_DIFFUSE;
+  }
+  if (p.opacity < R(1)) {
+    pdf = R(1);
+    return sample_glass(xi.x, wo, wi);
+  }
+  if (xi.z < p.metallic) {
+    R aspect = std::sqrt(R(1) - p.aniso * R(0.9));
+    R ax = std::max(R(0.001), sqr(p.roughness) / aspect);
+    R ay = std::max(R(0.001), sqr(p.roughness) * aspect);
+    pdf = sample_gtr2(xi.x, xi.y, ax, ay, wo, wi);
+    if (pdf == R(0) || std::isnan(pdf)) {
+      wi = cosine_sample(xi.x, xi.y);
+      pdf = clamp01(wi.z) / K<R>::pi;
+      return T_DIFFUSE;
+    }
+    return T_DISNEY_REFL;
+  }
+  wi = cosine_sample(xi.x, xi.y);
+  pdf = clamp01(wi.z) * K<R>::inv_pi;
+  return T_DIFFUSE;
+}
+
+/* ------------------------------------------------------------------ camera */
+struct Cam {
+  float U[3], V[3], W[3], pos[3], vd[3];
+  float width, height, fw, fh, aperture, focus;
+};
+/* Basis and image-plane extent of src/raygen.cu:17-39. */
+Cam make_cam(const gi_view &v) {
+  Cam c;
+  float half = v.fovy / 2.0f;
+  c.height = (float)std::tan(3.14159265358979323846 * (double)half / 180.0);
+  c.width = ((float)v.w / (float)v.h) * c.height;
+  V3<float> vd = ld<float>(v.dir), up = ld<float>(v.up);
+  V3<float> W = divs(vd, length(vd));
+  V3<float> txw = cross(up, W);
+  V3<float> U = divs(txw, length(txw));
+  V3<float> V = cross(W, U);
+  std::memcpy(c.U, &U, 12); std::memcpy(c.V, &V, 12); std::memcpy(c.W, &W, 12);
+  std::memcpy(c.pos, v.pos, 12); std::memcpy(c.vd, v.dir, 12);
+  c.fw = (float)v.w; c.fh = (float)v.h;
+  c.aperture = v.aperture; c.focus = v.focus_distance;
+  return c;
+}
+/* One jittered camera ray: src/raygen.cu:22-58; thin lens :99-117 (the
+ * lens jitter is uniform on the square [-1/2,1/2]^2 -- the reference's
+ * rejection test against the unit disk never rejects such a point). */
+void cam_ray(const Cam &c, int32_t x, int32_t y, float jx, float jy, float lx, float ly, V3<float> &o,
+             V3<float> &d) {
+  float u_s = ((float(x) + jx) / c.fw) * 2.0f - 1.0f;
+  float v_s = ((float(y) + jy) / c.fh) * 2.0f - 1.0f;
+  u_s = c.width * u_s;
+  v_s = c.height * v_s;
+  V3<float> U = ld<float>(c.U), V = ld<float>(c.V), W = ld<float>(c.W);
+  d = normalize((U * u_s + V * v_s) + W);
+  o = ld<float>(c.pos);
+  if (c.aperture > 0.0f) {
+    V3<float> vd = ld<float>(c.vd);
+    V3<float> focal = o + d * ((1.0f / dot(d, vd)) * c.focus);
+    float ox = lx - 0.5f, oy = ly - 0.5f;
+    V3<float> jp = (o + U * (ox * c.aperture)) + V * (oy * c.aperture);
+    d = normalize(focal - jp);
+    o = jp;
+  }
+}
+
+/* ----------------------------------------------------------------- render -- */
+template <class R> struct Tracer {
+  const or_scene &s;
+  bool closest(V3<R> o, V3<R> d, R tmax, R &t, R &u, R &v, uint32_t &id, uint64_t &nb, uint64_t &nt) const {
+    return traverse<R, false>(s, o, d, tmax, t, u, v, id, nb, nt);
+  }
+  bool any(V3<R> o, V3<R> d, R tmax, uint64_t &nb, uint64_t &nt) const {
+    R t, u, v;
+    uint32_t id;
+    return traverse<R, true>(s, o, d, tmax, t, u, v, id, nb, nt);
+  }
+};
+
+struct Options {
+  int32_t depth, rr_start;
+  float off_b, off_s;
+  float bg[3];
+};
+
+Options resolve(const or_scene &s, const gi_render_opts &o) {
+  Options r;
+  r.depth = std::max(1, std::min(o.max_depth, 32));
+  r.rr_start = o.rr_start;
+  float diag = or_scene_diagonal(&s);
+  r.off_b = o.offset_bounce > 0.f ? o.offset_bounce : 1e-4f * diag;
+  r.off_s = o.offset_shadow > 0.f ? o.offset_shadow : 1e-4f * diag;
+  for (int k = 0; k < 3; ++k) r.bg[k] = o.background[k];
+  return r;
+}
+
+/* One sample of one pixel: the hybrid_pt state machine (closest hit, NEE,
+ * any hit, bounce; src/hybrid-pt.h:151-249) run to completion for a single
+ * path.  Returns the path's accumulated colour (path_accum_color). */
+template <class R>
+V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H, const Options &op,
+                   uint64_t n, V3<R> o, V3<R> d, or_stats &st) {
+  V3<R> L = mk<R>(0, 0, 0), TP = mk<R>(1, 1, 1);
+  R tmax = R(FLT_MAX);
+  const int nl = (int)s.lights.size();
+  for (int32_t b = 0; b < op.depth; ++b) {
+    const int32_t dim = DIM_VTX0 + DIMS_PER_VTX * b;
+    R t, bu = R(0), bv = R(0);
+    uint32_t id;
+    ++st.rays_closest;
+    bool hit = tr.closest(o, d, tmax, t, bu, bv, id, st.box_closest, st.tri_closest);
+    V3<R> org_dir = normalize(d);
+    if (!hit) {
+      /* handle_no_hit with NO_BACKGROUND (src/hybrid-pt.cpp:196-206,567-574):
+       * a path whose throughput is exactly white gets the background. */
+      if (TP.x == R(1) && TP.y == R(1) && TP.z == R(1))
+        L = L + mul(TP, mk<R>(R(op.bg[0]), R(op.bg[1]), R(op.bg[2])));
+      break;
+    }
+    ++st.vertices;
+    const gi_tri &T = s.tris[id];
+    V3<R> bc = mk<R>((R(1) - bu) - bv, bu, bv);
+    V3<R> P = (ld<R>(T.a) * bc.x + ld<R>(T.b) * bc.y) + ld<R>(T.c) * bc.z;
+    V3<R> Ni = (ld<R>(T.na) * bc.x + ld<R>(T.nb) * bc.y) + ld<R>(T.nc) * bc.z;
+    MatP<R> mp(s.mats[T.material_index]);
+
+    /* NEE: pixel_generate_arealight_sample_cpu, src/arealight-sampler-cpu.cpp:25-119.
+     * The normal is the unnormalised interpolation, as there. */
+    V3<R> xl = mk<R>(R(H.sample(n, dim + 0)), R(H.sample(n, dim + 1)), R(H.sample(n, dim + 2)));
+    R choice = xl.z * R(s.power_total);
+    R acc = R(0);
+    int k = 0;
+    while (k < nl && choice > acc + R(s.lights[k].power)) { acc += R(s.lights[k].power); ++k; }
+    if (k >= nl) k = nl - 1;
+    const gi_light &lt = s.lights[k];
+    V3<R> ldir = ld<R>(lt.dir);
+    V3<R> right = tangential(mk<R>(1, 0, 0), ldir), up = tangential(mk<R>(0, 1, 0), ldir);
+    R offx = (xl.x - R(0.5)) * R(lt.wh[0]), offy = (xl.y - R(0.5)) * R(lt.wh[1]);
+    V3<R> ls = (ld<R>(lt.center) + right * offx) + up * offy;
+    V3<R> sd = ls - P;
+    R len = length(sd);
+    sd = divs(sd, len);
+    R ndotl = std::max(dot(Ni, sd), R(0));
+    R lcos = std::max(dot(ldir, -sd), R(0));
+    R factor = (((R(lt.wh[0]) * R(lt.wh[1])) * ndotl) * lcos) / (len * len);
+    V3<R> contrib = ld<R>(lt.col) * factor;
+    sd = normalize(sd);
+    V3<R> so = P + sd * R(op.off_s);
+    V3<R> psc = contrib * (R(s.power_total) / R(lt.power));
+
+    /* bounce: getHitDataTriangle + sample_material + evaluate_material,
+     * src/hybrid-pt.cpp:330-371,421-448,519-563. */
+    V3<R> N = normalize(Ni);
+    V3<R> Tt, Bt;
+    tangent_frame(N, Tt, Bt);
+    if (dot(org_dir, N) > R(0)) N = -N;
+    V3<R> inv = -org_dir;
+    V3<R> wo_ts = mk<R>(dot(inv, Tt), dot(inv, Bt), dot(inv, N));
+    V3<R> xp = mk<R>(R(H.sample(n, dim + 3)), R(H.sample(n, dim + 4)), R(H.sample(n, dim + 5)));
+    V3<R> wi_ts;
+    R pdf = R(1);
+    int type = brdf_sample(mp, wo_ts, xp, wi_ts, pdf);
+    V3<R> wi = normalize((Tt * wi_ts.x + Bt * wi_ts.y) + N * wi_ts.z);
+    if (!mp.glass()) {
+      V3<R> ldn = normalize(sd);
+      V3<R> C = mul(brdf_eval(mp, type, inv, ldn, N), mul(TP, psc));
+      if (C.x > R(0) || C.y > R(0) || C.z > R(0)) {
+        ++st.rays_shadow;
+        if (!tr.any(so, sd, len, st.box_shadow, st.tri_shadow)) L = L + C;
+      }
+    }
+    if (b + 1 >= op.depth) break;
+    V3<R> f = brdf_eval(mp, type, inv, wi, N);
+    V3<R> np = mp.glass() ? P + org_dir * R(op.off_b) : P + N * R(op.off_b);
+    V3<R> tp = mk<R>((TP.x * f.x) / pdf, (TP.y * f.y) / pdf, (TP.z * f.z) / pdf);
+    if (tp.x > R(1)) tp.x = R(1);
+    if (tp.y > R(1)) tp.y = R(1);
+    if (tp.z > R(1)) tp.z = R(1);
+    if (op.rr_start > 0 && b + 1 >= op.rr_start) {
+      R q = maxc(tp);
+      R xr = R(H.sample(n, dim + 6));
+      if (!(xr < q)) break;
+      tp = divs(tp, q);
+    }
+    TP = tp;
+    o = np;
+    d = wi;
+    tmax = R(FLT_MAX);
+  }
+  return L;
+}
+
+template <class R, class Img>
+void render_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_render_opts &ro, Img *img,
+                 or_stats *st_out, int32_t threads, int64_t pb, int64_t pe) {
+  Options op = resolve(s, ro);
+  HaltonTable H(ro.seed, dims_for_depth(op.depth));
+  Tracer<R> tr{s};
+  or_stats total;
+  std::memset(&total, 0, sizeof(total));
+  uint64_t base = 0;
+  int64_t img_off = 0;
+#ifdef _OPENMP
+  int nth = threads > 0 ? threads : omp_get_max_threads();
+#else
+  int nth = 1;
+  (void)threads;
+#endif
+  for (int32_t vi = 0; vi < nv; ++vi) {
+    const gi_view &v = views[vi];
+    Cam cam = make_cam(v);
+    const int64_t npix = (int64_t)v.w * v.h;
+    int64_t b0 = std::max<int64_t>(0, pb), e0 = pe < 0 ? npix : std::min<int64_t>(npix, pe);
+    std::vector<or_stats> per(nth);
+    for (auto &q : per) std::memset(&q, 0, sizeof(q));
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nth)
+    for (int64_t p = b0; p < e0; ++p) {
+#ifdef _OPENMP
+      or_stats &st = per[omp_get_thread_num()];
+#else
+      or_stats &st = per[0];
+#endif
+      int32_t x = (int32_t)(p % v.w), y = (int32_t)(p / v.w);
+      V3<R> acc = mk<R>(0, 0, 0);
+      for (int32_t sidx = 0; sidx < v.spp; ++sidx) {
+        uint64_t n = base + (uint64_t)p * (uint64_t)v.spp + (uint64_t)sidx;
+        V3<float> of, df;
+        cam_ray(cam, x, y, H.sample(n, DIM_JX), H.sample(n, DIM_JY), H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
+        V3<R> Ls = trace_sample<R>(s, tr, H, op, n, mk<R>(of.x, of.y, of.z), mk<R>(df.x, df.y, df.z), st);
+        ++st.samples;
+        /* combine_color_samples, src/util-kernels.cu:17-25 */
+        acc = divs(R(sidx) * acc + Ls, R(sidx + 1));
+      }
+      Img *dst = img + 3 * (img_off + p);
+      dst[0] = (Img)acc.x; dst[1] = (Img)acc.y; dst[2] = (Img)acc.z;
+    }
+    for (auto &q : per) {
+      total.rays_closest += q.rays_closest; total.rays_shadow += q.rays_shadow;
+      total.vertices += q.vertices; total.samples += q.samples;
+      total.box_closest += q.box_closest; total.tri_closest += q.tri_closest;
+      total.box_shadow += q.box_shadow; total.tri_shadow += q.tri_shadow;
+    }
+    base += (uint64_t)npix * (uint64_t)v.spp;
+    img_off += npix;
+  }
+  if (st_out) *st_out = total;
+}
+
+}  // namespace
+
+/* ================================================================ C API === */
+extern "C" {
+
+or_scene *or_scene_create(const gi_tri *tris, int64_t n, const gi_material *mats, int32_t nm,
+                          const gi_light *lights, int32_t nl, const float *nodes2f4, int64_t n_nodes,
+                          const uint32_t *indices, int64_t n_indices, int32_t max_leaf) {
+  or_scene *s = new or_scene;
+  s->tris.assign(tris, tris + n);
+  if (mats) s->mats.assign(mats, mats + nm);
+  if (lights) s->lights.assign(lights, lights + nl);
+  for (auto &l : s->lights) s->power_total += l.power;
+  for (int k = 0; k < 3; ++k) { s->lo[k] = FLT_MAX; s->hi[k] = -FLT_MAX; }
+  for (auto &t : s->tris)
+    for (int k = 0; k < 3; ++k) {
+      s->lo[k] = std::min(s->lo[k], std::min(t.a[k], std::min(t.b[k], t.c[k])));
+      s->hi[k] = std::max(s->hi[k], std::max(t.a[k], std::max(t.b[k], t.c[k])));
+    }
+  if (nodes2f4) {
+    s->nodes.assign(nodes2f4, nodes2f4 + 8 * n_nodes);
+    s->indices.assign(indices, indices + n_indices);
+  } else if (n > 0) {
+    s->indices.resize(n);
+    std::iota(s->indices.begin(), s->indices.end(), 0u);
+    s->nodes.assign(8, 0.f);
+    MedianBuilder mb{s->tris, s->indices, s->nodes, max_leaf > 0 ? max_leaf : 4};
+    mb.build(0, 0, (uint32_t)n);
+  }
+  return s;
+}
+
+void or_scene_free(or_scene *s) { delete s; }
+int64_t or_scene_bvh_nodes(const or_scene *s) { return (int64_t)s->nodes.size() / 8; }
+void or_scene_export_bvh(const or_scene *s, float *nodes, uint32_t *indices) {
+  std::memcpy(nodes, s->nodes.data(), s->nodes.size() * 4);
+  std::memcpy(indices, s->indices.data(), s->indices.size() * 4);
+}
+float or_scene_diagonal(const or_scene *s) {
+  float dx = s->hi[0] - s->lo[0], dy = s->hi[1] - s->lo[1], dz = s->hi[2] - s->lo[2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+void or_trace_closest(const or_scene *s, const float *o, const float *d, const float *tmax, int64_t n,
+                      gi_hit *out, uint64_t *n_box, uint64_t *n_tri) {
+  uint64_t nb = 0, nt = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nb, nt)
+  for (int64_t i = 0; i < n; ++i) {
+    float t, u = 0.f, v = 0.f;
+    uint32_t id;
+    uint64_t b = 0, c = 0;
+    traverse<float, false>(*s, ld<float>(o + 3 * i), ld<float>(d + 3 * i), tmax[i], t, u, v, id, b, c);
+    nb += b; nt += c;
+    out[i].t = t; out[i].beta = id == GI_INVALID_REF ? 0.f : u; out[i].gamma = id == GI_INVALID_REF ? 0.f : v;
+    out[i].ref = id;
+  }
+  if (n_box) *n_box = nb;
+  if (n_tri) *n_tri = nt;
+}
+
+void or_trace_any(const or_scene *s, const float *o, const float *d, const float *tmax, int64_t n, uint8_t *occ) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t b = 0, c = 0;
+    occ[i] = Tracer<float>{*s}.any(ld<float>(o + 3 * i), ld<float>(d + 3 * i), tmax[i], b, c) ? 1 : 0;
+  }
+}
+
+void or_brute_closest(const or_scene *s, const float *o, const float *d, const float *tmax, int64_t n, gi_hit *out) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i) {
+    float t, u = 0.f, v = 0.f;
+    uint32_t id;
+    brute<float>(*s, ld<float>(o + 3 * i), ld<float>(d + 3 * i), tmax[i], t, u, v, id);
+    out[i].t = t; out[i].beta = id == GI_INVALID_REF ? 0.f : u; out[i].gamma = id == GI_INVALID_REF ? 0.f : v;
+    out[i].ref = id;
+  }
+}
+
+void or_brute_closest_f64(const or_scene *s, const float *o, const float *d, const float *tmax, int64_t n,
+                          double *t, double *beta, double *gamma, uint32_t *ref) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i) {
+    double tt, u = 0, v = 0;
+    uint32_t id;
+    brute<double>(*s, ld<double>(o + 3 * i), ld<double>(d + 3 * i), (double)tmax[i], tt, u, v, id);
+    t[i] = tt; beta[i] = u; gamma[i] = v; ref[i] = id;
+  }
+}
+
+int or_intersect_tri(const float *a, const float *b, const float *c, const float *o, const float *d, float *t,
+                     float *beta, float *gamma) {
+  V3<float> A = ld<float>(a), B = ld<float>(b), C = ld<float>(c);
+  return intersect_tri<float>(A, B - A, C - A, ld<float>(o), ld<float>(d), *t, *beta, *gamma) ? 1 : 0;
+}
+int or_intersect_tri_f64(const double *a, const double *b, const double *c, const double *o, const double *d,
+                         double *t, double *beta, double *gamma) {
+  V3<double> A{a[0], a[1], a[2]}, B{b[0], b[1], b[2]}, C{c[0], c[1], c[2]};
+  V3<double> O{o[0], o[1], o[2]}, D{d[0], d[1], d[2]};
+  return intersect_tri<double>(A, B - A, C - A, O, D, *t, *beta, *gamma) ? 1 : 0;
+}
+
+/* gi::halton, src/util.cpp:22-32 (f = f/base in float). */
+float or_radical_inverse(uint64_t index, int32_t base) {
+  float result = 0.0f;
+  float f = 1.0f / float(base);
+  uint64_t i = index;
+  while (i > 0) {
+    result += f * (float)(i % (uint64_t)base);
+    i = i / (uint64_t)base;
+    f = f / float(base);
+  }
+  return result;
+}
+float or_halton_sample(uint64_t n, int32_t dim, uint32_t seed) {
+  std::vector<uint16_t> perm(primes().p[dim]);
+  halton_perm(dim, seed, perm.data());
+  return scrambled_ri(n, primes().p[dim], perm.data());
+}
+int32_t or_prime(int32_t dim) { return primes().p[dim]; }
+void or_halton_perm(int32_t dim, uint32_t seed, uint16_t *perm) { halton_perm(dim, seed, perm); }
+
+void or_camera_rays(const gi_view *v, int32_t sample, uint32_t seed, uint64_t index_base, float *o, float *d,
+                    float *tmax) {
+  Cam cam = make_cam(*v);
+  HaltonTable H(seed, DIM_VTX0);
+  int64_t npix = (int64_t)v->w * v->h;
+  for (int64_t p = 0; p < npix; ++p) {
+    uint64_t n = index_base + (uint64_t)p * (uint64_t)v->spp + (uint64_t)sample;
+    V3<float> of, df;
+    cam_ray(cam, (int32_t)(p % v->w), (int32_t)(p / v->w), H.sample(n, DIM_JX), H.sample(n, DIM_JY),
+            H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
+    std::memcpy(o + 3 * p, &of, 12);
+    std::memcpy(d + 3 * p, &df, 12);
+    tmax[p] = FLT_MAX;
+  }
+}
+
+void or_tangent_frame(const float *n, float *t, float *b) {
+  V3<float> T, B;
+  tangent_frame(ld<float>(n), T, B);
+  std::memcpy(t, &T, 12);
+  std::memcpy(b, &B, 12);
+}
+
+void or_nee(const or_scene *s, const float *Pp, const float *Np, const float *xi, float *dir, float *len,
+            float *contrib, int32_t *light) {
+  const int nl = (int)s->lights.size();
+  float choice = xi[2] * s->power_total, acc = 0.f;
+  int k = 0;
+  while (k < nl && choice > acc + s->lights[k].power) { acc += s->lights[k].power; ++k; }
+  if (k >= nl) k = nl - 1;
+  const gi_light &lt = s->lights[k];
+  V3<float> ldir = ld<float>(lt.dir), P = ld<float>(Pp), N = ld<float>(Np);
+  V3<float> right = tangential(mk<float>(1, 0, 0), ldir), up = tangential(mk<float>(0, 1, 0), ldir);
+  float offx = (xi[0] - 0.5f) * lt.wh[0], offy = (xi[1] - 0.5f) * lt.wh[1];
+  V3<float> ls = (ld<float>(lt.center) + right * offx) + up * offy;
+  V3<float> sd = ls - P;
+  float l = length(sd);
+  sd = divs(sd, l);
+  float ndotl = std::max(dot(N, sd), 0.f), lcos = std::max(dot(ldir, -sd), 0.f);
+  float factor = (((lt.wh[0] * lt.wh[1]) * ndotl) * lcos) / (l * l);
+  V3<float> c = ld<float>(lt.col) * factor;
+  c = c * (s->power_total / lt.power);
+  sd = normalize(sd);
+  std::memcpy(dir, &sd, 12);
+  *len = l;
+  std::memcpy(contrib, &c, 12);
+  *light = k;
+}
+
+int or_brdf_sample(const gi_material *m, const float *wo_ts, const float *xi, float *wi_ts, float *pdf) {
+  MatP<float> p(*m);
+  V3<float> wi;
+  float pd = 1.f;
+  int t = brdf_sample(p, ld<float>(wo_ts), ld<float>(xi), wi, pd);
+  std::memcpy(wi_ts, &wi, 12);
+  *pdf = pd;
+  return t;
+}
+void or_brdf_eval(const gi_material *m, int type, const float *wo, const float *wi, const float *N, float *f) {
+  MatP<float> p(*m);
+  V3<float> r = brdf_eval(p, type, ld<float>(wo), ld<float>(wi), ld<float>(N));
+  std::memcpy(f, &r, 12);
+}
+int or_brdf_sample_f64(const gi_material *m, const double *wo_ts, const double *xi, double *wi_ts, double *pdf) {
+  MatP<double> p(*m);
+  V3<double> wi, wo{wo_ts[0], wo_ts[1], wo_ts[2]}, x{xi[0], xi[1], xi[2]};
+  double pd = 1.0;
+  int t = brdf_sample(p, wo, x, wi, pd);
+  wi_ts[0] = wi.x; wi_ts[1] = wi.y; wi_ts[2] = wi.z;
+  *pdf = pd;
+  return t;
+}
+void or_brdf_eval_f64(const gi_material *m, int type, const double *wo, const double *wi, const double *N, double *f) {
+  MatP<double> p(*m);
+  V3<double> r = brdf_eval(p, type, V3<double>{wo[0], wo[1], wo[2]}, V3<double>{wi[0], wi[1], wi[2]},
+                           V3<double>{N[0], N[1], N[2]});
+  f[0] = r.x; f[1] = r.y; f[2] = r.z;
+}
+
+void or_render(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts, float *images,
+               or_stats *st, int32_t threads, int64_t p_begin, int64_t p_end) {
+  render_impl<float, float>(*s, views, n_views, *opts, images, st, threads, p_begin, p_end);
+}
+void or_render_f64(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
+                   double *images, int32_t threads) {
+  render_impl<double, double>(*s, views, n_views, *opts, images, nullptr, threads, 0, -1);
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+/*
+ * ORACLE -- test infrastructure.  A CPU restatement of globillum's CPU path
+ * tracer (hybrid_pt / cpu_pt) used ONLY as the checker by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  Nothing in the
+ * product (globillum_amd/) includes, links or calls it.
+ *
+ * The reference cannot be built here (it needs the unvendored rta / bbvh /
+ * libhyb libraries, guile and nvcc), so there is no oracle/_ref; this file is
+ * the restatement, and each function cites the reference file:line it follows.
+ */
+#ifndef GI_ORACLE_H
+#define GI_ORACLE_H
+
+#include <stdint.h>
+#include "../include/gi_hip.h" /* data-type layouts only */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  uint64_t rays_closest, rays_shadow, vertices, samples;
+  uint64_t box_closest, tri_closest; /* child boxes / triangles tested */
+  uint64_t box_shadow, tri_shadow;
+} or_stats;
+
+typedef struct or_scene or_scene;
+
+/* nodes == NULL: the oracle builds its own object-median binary BVH. */
+or_scene *or_scene_create(const gi_tri *tris, int64_t n, const gi_material *mats, int32_t nm,
+                          const gi_light *lights, int32_t nl, const float *nodes2f4,
+                          int64_t n_nodes, const uint32_t *indices, int64_t n_indices,
+                          int32_t max_leaf);
+void or_scene_free(or_scene *s);
+int64_t or_scene_bvh_nodes(const or_scene *s);
+void or_scene_export_bvh(const or_scene *s, float *nodes, uint32_t *indices);
+float or_scene_diagonal(const or_scene *s);
+
+/* Tracing in FP32 through the binary BVH (reference traversal rules). */
+void or_trace_closest(const or_scene *s, const float *o, const float *d, const float *tmax,
+                      int64_t n, gi_hit *out, uint64_t *n_box, uint64_t *n_tri);
+void or_trace_any(const or_scene *s, const float *o, const float *d, const float *tmax,
+                  int64_t n, uint8_t *occ);
+/* Brute-force linear scan, FP32 (same arithmetic as the BVH path). */
+void or_brute_closest(const or_scene *s, const float *o, const float *d, const float *tmax,
+                      int64_t n, gi_hit *out);
+/* FP64 brute force: the tie adjudicator. */
+void or_brute_closest_f64(const or_scene *s, const float *o, const float *d, const float *tmax,
+                          int64_t n, double *t, double *beta, double *gamma, uint32_t *ref);
+/* Single triangle test on explicit vertices (FP32 / FP64). */
+int or_intersect_tri(const float *a, const float *b, const float *c, const float *o,
+                     const float *d, float *t, float *beta, float *gamma);
+int or_intersect_tri_f64(const double *a, const double *b, const double *c, const double *o,
+                         const double *d, double *t, double *beta, double *gamma);
+
+/* Halton (src/util.cpp:22-32 and the scrambled extension). */
+float or_radical_inverse(uint64_t n, int32_t base);
+float or_halton_sample(uint64_t n, int32_t dim, uint32_t seed);
+int32_t or_prime(int32_t dim);
+void or_halton_perm(int32_t dim, uint32_t seed, uint16_t *perm);
+
+/* Camera rays for one sample of one view (src/raygen.cu:10-118). */
+void or_camera_rays(const gi_view *v, int32_t sample, uint32_t seed, uint64_t index_base,
+                    float *o, float *d, float *tmax);
+
+/* Tangent frame (stand-in for libhyb make_tangent_frame). */
+void or_tangent_frame(const float *n, float *t, float *b);
+
+/* NEE for one shading point (src/arealight-sampler-cpu.cpp:25-119). */
+void or_nee(const or_scene *s, const float *P, const float *N, const float *xi, float *dir,
+            float *len, float *contrib, int32_t *light);
+
+/* BRDF sample + eval in tangent space (src/material-wrapper.h:15-51).
+ * Returns the lobe type; f_sample = eval(wo, wi), f_light = eval(wo, l). */
+int or_brdf_sample(const gi_material *m, const float *wo_ts, const float *xi, float *wi_ts,
+                   float *pdf);
+void or_brdf_eval(const gi_material *m, int type, const float *wo, const float *wi,
+                  const float *N, float *f);
+int or_brdf_sample_f64(const gi_material *m, const double *wo_ts, const double *xi,
+                       double *wi_ts, double *pdf);
+void or_brdf_eval_f64(const gi_material *m, int type, const double *wo, const double *wi,
+                      const double *N, double *f);
+
+/* Full renderer.  images: views concatenated, 3 floats per pixel.
+ * Renders pixels [p_begin, p_end) of every view (clamped per view; p_end<0 =
+ * all) and samples [0, spp).  threads <= 0: all OpenMP threads. */
+void or_render(const or_scene *s, const gi_view *views, int32_t n_views,
+               const gi_render_opts *opts, float *images, or_stats *st, int32_t threads,
+               int64_t p_begin, int64_t p_end);
+/* Same in FP64 (closest hits by the FP64 BVH path); images in double. */
+void or_render_f64(const or_scene *s, const gi_view *views, int32_t n_views,
+                   const gi_render_opts *opts, double *images, int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
