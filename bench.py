@@ -1,0 +1,145 @@
+"""Benchmark: path-tracing throughput on one MI355X.
+
+Workload (one step): a full render of the ~300k-triangle configuration --
+256x256-cell fBm heightfield (131,072 triangles) + 1000 UV spheres of 160
+triangles, 32 principled materials, 4 rect lights, 1280x720, 64 spp, depth
+8, FP32 -- with scene, BVH and camera already resident in HBM.
+
+value = (closest-hit + shadow rays traced) / wall time of the K timed
+renders, in Mrays/s.  roofline: the closest-hit traversal kernel's
+algorithmic bytes (per ray 32 B in + 16 B out + 32 B per child box + 40 B
+per triangle test, with the box / triangle test counts of the reference's
+binary-BVH traversal rules measured by the oracle on the same BVH) over its
+HIP-event time, against 8 TB/s HBM.  cpu_baseline: the oracle (a C++
+restatement of the reference's CPU path tracer, OpenMP) on a bounded sample
+of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s incl. shadow rays + adjoint grads/s at fixed RMSE, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--graph", action="store_true", help="replay the render as one hipGraph")
+    return p.parse_args()
+
+
+def cpu_baseline(scene, bvh2):
+    """Oracle on 1/4 of the samples: same scene, camera, depth and BVH at
+    640x360, 32 spp.  Returns the baseline record and the per-ray traversal
+    statistics (box / triangle tests per closest and shadow ray)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    threads = min(16, os.cpu_count() or 1)
+    osc = pyoracle.OracleScene(scene.tris, scene.mats, scene.lights, bvh2=bvh2)
+    v = scene.views.copy()
+    v["w"], v["h"], v["spp"] = 640, 360, 32
+    t0 = time.perf_counter()
+    _, st = osc.render(v, scene.opts, threads=threads)
+    dt = time.perf_counter() - t0
+    rays = st["rays_closest"] + st["rays_shadow"]
+    rec = {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"same scene/camera/depth/BVH at 640x360, 32 spp (1/4 of the samples): {rays} rays "
+                     f"in {dt:.2f} s on {threads} threads",
+           "samples_per_s": round(st["samples"] / dt, 1)}
+    per_ray = {"box_closest": st["box_closest"] / max(1, st["rays_closest"]),
+               "tri_closest": st["tri_closest"] / max(1, st["rays_closest"]),
+               "box_shadow": st["box_shadow"] / max(1, st["rays_shadow"]),
+               "tri_shadow": st["tri_shadow"] / max(1, st["rays_shadow"])}
+    return rec, per_ray
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the path shards by view / image "
+              f"tile / spp range, which this build does not run", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    from globillum_amd import scenes
+    from globillum_amd._lib import FLAG_BATCHED, FLAG_GRAPH
+    from globillum_amd.context import Context, build_bvh2
+
+    scene = scenes.config_scene(2)
+    bvh2 = build_bvh2(scene.tris)
+    flags = FLAG_BATCHED | (FLAG_GRAPH if a.graph else 0)
+    ctx = Context(0)
+    ctx.upload(scene, bvh2=bvh2)
+    print(f"[bench] scene {len(scene.tris)} triangles uploaded; warmup {a.warmup}", file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        ctx.render(flags=flags)
+    tot = {"rays_closest": 0, "rays_shadow": 0, "samples": 0, "ms_trace_closest": 0.0, "ms_trace_any": 0.0,
+           "ms_shade": 0.0, "launches_closest": 0}
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ctx.render(flags=flags)  # returns after the stream synchronised
+        c = ctx.counters()
+        for k in tot:
+            tot[k] += c[k]
+    dt = time.perf_counter() - t0
+    print(f"[bench] {a.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
+    img = ctx.image(0)
+    assert np.isfinite(img).all()
+    rays = tot["rays_closest"] + tot["rays_shadow"]
+    value = rays / dt / 1e6
+
+    base, per_ray = (None, None) if a.no_cpu else cpu_baseline(scene, bvh2)
+    roof = None
+    if per_ray is not None and tot["launches_closest"] > 0 and tot["ms_trace_closest"] > 0:
+        b_ray = 32 + 16 + 32 * per_ray["box_closest"] + 40 * per_ray["tri_closest"]
+        bytes_per_launch = tot["rays_closest"] * b_ray / tot["launches_closest"]
+        avg_launch_s = tot["ms_trace_closest"] / tot["launches_closest"] / 1e3
+        achieved = bytes_per_launch / avg_launch_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_trace_closest", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                "algorithmic_bytes_per_ray": round(b_ray, 1)}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(dt / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "terrain_spheres: 256x256-cell fBm heightfield + 1000 UV spheres "
+                               "(291072 triangles), 32 principled materials, 4 rect lights, 1280x720, "
+                               "64 spp, depth 8 (the ~300k-triangle plain configuration)",
+                   "mode": "batched wavefront" + (" + hipGraph" if a.graph else ""),
+                   "samples_per_step": tot["samples"] // max(1, a.steps)},
+        "samples_per_s": round(tot["samples"] / dt, 1),
+        "rays_per_step": {"closest": tot["rays_closest"] // a.steps, "shadow": tot["rays_shadow"] // a.steps},
+        "kernel_ms_per_step": {"trace_closest": round(tot["ms_trace_closest"] / a.steps, 3),
+                               "trace_shadow": round(tot["ms_trace_any"] / a.steps, 3),
+                               "shade": round(tot["ms_shade"] / a.steps, 3)},
+        "roofline": roof,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

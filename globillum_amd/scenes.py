@@ -266,7 +266,7 @@ def terrain_spheres(seed: int = 2, cells=(256, 256), n_spheres: int = 1000, n_ma
         lx = (li % g + 0.5) / g * size
         lz = (li // g + 0.5) / g * size
         lights.append(make_light((lx, 40.0, lz), (0.0, -1.0, 0.0), (40.0, 40.0, 40.0), 8.0, 8.0))
-    view = make_view((0.5 * size, 35.0, -0.3 * size), (0.5 * size, 0.0, 0.5 * size), (0, 1, 0), 50.0, w, h, spp)
+    view = make_view((0.5 * size, 40.0, 0.0), (0.5 * size, 0.0, 0.5 * size), (0, 1, 0), 50.0, w, h, spp)
     return Scene(tris, mats, np.concatenate(lights), view, make_opts(depth), name="terrain_spheres")
 
 

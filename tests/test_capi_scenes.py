@@ -1,0 +1,90 @@
+"""The C-ABI library loads and exports every symbol include/gi_hip.h
+declares; host-only entry points behave; scene generators are
+deterministic and have the configured sizes."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from globillum_amd import _lib, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    src = open(os.path.join(ROOT, "include", "gi_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gi_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = _lib.load()
+    names = _declared_symbols()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_version_and_status_strings():
+    lib = _lib.load()
+    assert lib.gi_abi_version() == 1
+    assert lib.gi_status_string(0) == b"ok"
+    assert lib.gi_status_string(4) == b"invalid call order"
+    assert lib.gi_status_string(99) == b"unknown status"
+
+
+def test_create_without_device_reports_status():
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.gi_create(None, 0) == 1
+    rc = lib.gi_create(C.byref(h), 0)
+    if rc == 0:
+        lib.gi_destroy(h)  # a GPU is present
+    else:
+        assert rc == 2 and not h.value
+
+
+def test_null_context_calls_fail_cleanly():
+    lib = _lib.load()
+    assert lib.gi_upload_triangles(None, None, 0) == 1
+    assert lib.gi_render(None, None, 0, None) != 0
+    assert lib.gi_get_counters(None, None) == 1
+    assert lib.gi_last_error(None) == b"null context"
+    lib.gi_destroy(None)
+
+
+def test_rect_light_power_matches_reference_formula():
+    lib = _lib.load()
+    l = scenes.make_light((0, 1, 0), (0, -1, 0), (10, 20, 30), 0.5, 0.25)
+    # add-rectlight: w*h*(r+g+b)*.333 (src/lights.cpp:92)
+    assert lib.gi_rect_light_power(_lib.ptr(l)) == pytest.approx(0.5 * 0.25 * 60 * 0.333, rel=1e-6)
+    assert l["power"][0] == pytest.approx(0.5 * 0.25 * 60 * 0.333, rel=1e-6)
+
+
+def test_config_scenes():
+    c1 = scenes.cornell_boxes()
+    assert len(c1.tris) == 34 and len(c1.mats) == 7 and len(c1.lights) == 1
+    assert (c1.mats["diffuse_color"] >= 0.2).all() and (c1.mats["diffuse_color"] <= 0.8).all()
+    assert np.array_equal(c1.tris, scenes.cornell_boxes().tris)
+    assert not np.array_equal(c1.tris, scenes.cornell_boxes(seed=2).tris)
+    c2 = scenes.config_scene(2)
+    assert len(c2.tris) == 256 * 256 * 2 + 1000 * 160
+    assert len(c2.mats) == 32 and len(c2.lights) == 4
+    assert int(c2.views["w"][0]) == 1280 and int(c2.views["spp"][0]) == 64 and int(c2.opts["max_depth"][0]) == 8
+    m = c2.mats
+    assert set(np.unique(m["metallic"]).tolist()) <= {0.0, 1.0}
+    assert (m["roughness"] >= 0.05).all() and (m["roughness"] <= 1).all()
+    c4 = scenes.config_scene(4)
+    assert len(c4.tris) == 200 * 250 * 2 + 50 * 160 and len(c4.mats) == 64
+
+
+def test_sphere_normals_are_radial():
+    s = scenes.uv_sphere((1.0, 2.0, 3.0), 0.5)
+    assert len(s) == 160
+    for k in ("a", "b", "c"):
+        r = s[k] - np.array([1.0, 2.0, 3.0], np.float32)
+        assert np.allclose(np.linalg.norm(r, axis=1), 0.5, atol=1e-6)
+        assert np.allclose(s["n" + k], r / 0.5, atol=1e-5)

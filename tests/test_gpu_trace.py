@@ -1,0 +1,132 @@
+"""GPU traversal parity through the C-ABI: gi_trace_closest / gi_trace_any
+against the oracle's binary-BVH traversal on the same rays -- triangle id,
+t and barycentrics bit-exact, any mismatch adjudicated by the FP64 brute
+force -- plus the edge cases (empty input, axis-parallel directions,
+max_t <= 0, odd sizes, external BVH upload, call-order errors)."""
+import numpy as np
+import pytest
+
+from globillum_amd import scenes
+from globillum_amd._lib import GiError
+from globillum_amd.context import build_bvh2
+
+pytestmark = pytest.mark.gpu
+FMAX = np.finfo(np.float32).max
+
+
+def _rays(rng, n, lo, hi, ylo=None, yhi=None):
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    if ylo is not None:
+        o[:, 1] = rng.uniform(ylo, yhi, n)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return o, d
+
+
+def _assert_hits_equal(gpu, ref, osc, o, d, tmax):
+    bad = np.nonzero(gpu != ref)[0]
+    if len(bad) == 0:
+        return
+    # adjudicate: both candidates must be FP64 ties (same exact t within 1e-6)
+    t64, _, _, r64 = osc.brute_closest_f64(o[bad], d[bad], tmax[bad])
+    for k, i in enumerate(bad):
+        assert gpu["ref"][i] != 0xFFFFFFFF and ref["ref"][i] != 0xFFFFFFFF, (i, gpu[i], ref[i])
+        assert abs(float(gpu["t"][i]) - t64[k]) <= 1e-6 * t64[k] and abs(float(ref["t"][i]) - t64[k]) <= 1e-6 * t64[k]
+    assert len(bad) <= 2, f"{len(bad)} FP64-tie mismatches"
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return scenes.cornell_boxes()
+
+
+@pytest.fixture(scope="module")
+def terrain():
+    return scenes.terrain_spheres(cells=(96, 96), n_spheres=300, w=96, h=54, spp=4)
+
+
+@pytest.mark.parametrize("which", ["c1", "terrain"])
+def test_closest_hit_parity_random_rays(which, c1, terrain, oracle, gpu_ctx_factory):
+    sc = c1 if which == "c1" else terrain
+    rng = np.random.default_rng(11)
+    if which == "c1":
+        o, d = _rays(rng, 250_000, -0.2, 1.2)
+    else:
+        o, d = _rays(rng, 250_000, -5, 105, 0, 20)
+    tmax = np.where(rng.random(len(o)) < 0.25, rng.uniform(0, 30, len(o)), FMAX).astype(np.float32)
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights)
+    ref = osc.trace_closest(o, d, tmax)
+    with gpu_ctx_factory(sc) as ctx:
+        _assert_hits_equal(ctx.trace_closest(o, d, tmax), ref, osc, o, d, tmax)
+    # the GPU traversing the oracle's own (median-split) BVH gives the same
+    with gpu_ctx_factory(sc, bvh2=osc.bvh2()) as ctx:
+        _assert_hits_equal(ctx.trace_closest(o, d, tmax), ref, osc, o, d, tmax)
+
+
+def test_closest_hit_parity_camera_and_secondary(terrain, oracle, gpu_ctx_factory):
+    osc = oracle.OracleScene(terrain.tris, terrain.mats, terrain.lights, bvh2=build_bvh2(terrain.tris))
+    o, d, t = oracle.camera_rays(terrain.views, 0, 0)
+    ref = osc.trace_closest(o, d, t)
+    with gpu_ctx_factory(terrain) as ctx:
+        g = ctx.trace_closest(o, d, t)
+        _assert_hits_equal(g, ref, osc, o, d, t)
+        # secondary rays: from the primary hit points (offset along the ray
+        # back-side) in random directions
+        hit = ref["ref"] != 0xFFFFFFFF
+        P = o[hit] + (ref["t"][hit, None] * 0.9999) * d[hit]
+        rng = np.random.default_rng(2)
+        d2 = rng.normal(size=P.shape).astype(np.float32)
+        d2[:, 1] = np.abs(d2[:, 1])
+        d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+        t2 = np.full(len(P), FMAX, np.float32)
+        _assert_hits_equal(ctx.trace_closest(P, d2, t2), osc.trace_closest(P, d2, t2), osc, P, d2, t2)
+
+
+def test_any_hit_parity(terrain, oracle, gpu_ctx_factory):
+    rng = np.random.default_rng(5)
+    o, d = _rays(rng, 200_000, 0, 100, 0, 20)
+    tmax = rng.uniform(0, 50, len(o)).astype(np.float32)
+    osc = oracle.OracleScene(terrain.tris)
+    ref = osc.trace_any(o, d, tmax)
+    with gpu_ctx_factory(terrain) as ctx:
+        occ = ctx.trace_any(o, d, tmax)
+        assert np.array_equal(occ, ref)
+        h = ctx.trace_closest(o, d, np.full(len(o), FMAX, np.float32))
+        assert np.array_equal(occ.astype(bool), (h["ref"] != 0xFFFFFFFF) & (h["t"] <= tmax))
+
+
+def test_edge_cases(c1, oracle, gpu_ctx_factory):
+    osc = oracle.OracleScene(c1.tris)
+    with gpu_ctx_factory(c1) as ctx:
+        assert len(ctx.trace_closest(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0))) == 0
+        # axis-parallel directions (zero components -> clamped reciprocal)
+        o = np.array([[0.5, 0.5, 0.5]] * 6, np.float32)
+        d = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+        t = np.full(6, FMAX, np.float32)
+        g = ctx.trace_closest(o, d, t)
+        assert (g == osc.trace_closest(o, d, t)).all()
+        assert (g["ref"][[0, 1, 2, 3, 4]] != 0xFFFFFFFF).all() and g["ref"][5] == 0xFFFFFFFF  # front open
+        # max_t <= 0: nothing can be hit
+        g = ctx.trace_closest(o, d, np.array([0, -1, 0, -5, 0, 0], np.float32))
+        assert (g["ref"] == 0xFFFFFFFF).all() and (g["t"] == FMAX).all()
+        assert not ctx.trace_any(o, d, np.zeros(6, np.float32)).any()
+        # sizes around the wave width
+        rng = np.random.default_rng(1)
+        for n in (1, 63, 64, 65, 1000, 1_000_003):
+            o, d = _rays(rng, n, -0.2, 1.2)
+            t = np.full(n, FMAX, np.float32)
+            assert (ctx.trace_closest(o, d, t) == osc.trace_closest(o, d, t)).all()
+
+
+def test_call_order_and_argument_errors(c1, gpu_ctx_factory):
+    with gpu_ctx_factory() as ctx:
+        with pytest.raises(GiError) as e:
+            ctx.trace_closest(np.zeros((1, 3)), np.ones((1, 3)), np.ones(1))
+        assert e.value.status == 4
+    with gpu_ctx_factory() as ctx:
+        ctx.upload(c1)
+        bad_nodes = np.zeros(8, np.float32)
+        bad_nodes.view(np.int32)[0] = 3  # inner node pointing at child 1 of a 1-node tree
+        with pytest.raises(GiError) as e:
+            ctx.upload(c1, bvh2=(bad_nodes, np.zeros(0, np.uint32)))
+        assert e.value.status == 1
