@@ -38,8 +38,30 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm toolchain required to build libgi_hip.so)")
 
 
+HOST_DIR = os.path.join(PKG, "host")
+HOST_BINS = {"gi_driver": ["gi_driver.cpp", "mi355x_pt.cpp"], "registry_test": ["registry_test.cpp", "mi355x_pt.cpp"]}
+
+
+def build_host(verbose: bool = False) -> None:
+    """C++ host side: the thin driver and the registry test, linked against
+    libgi_hip.so (rpath $ORIGIN)."""
+    cxx = shutil.which("g++") or "g++"
+    for name, srcs in HOST_BINS.items():
+        out = os.path.join(PKG, name)
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", out + ".tmp"] + [os.path.join(HOST_DIR, s) for s in srcs]
+        cmd += ["-L" + PKG, "-lgi_hip", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed building {name}")
+        os.replace(out + ".tmp", out)
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every source into libgi_hip.so (always recompiles)."""
+    """Compile every source into libgi_hip.so and the host binaries (always
+    recompiles)."""
     tmp = LIB + ".tmp"
     cmd = [hipcc(), *HIP_FLAGS, "-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
     if verbose:
@@ -51,6 +73,7 @@ def build(verbose: bool = False) -> str:
     if verbose and r.stderr.strip():
         sys.stderr.write(r.stderr)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -572,6 +572,17 @@ namespace {
     }                                                                              \
   } while (0)
 
+
+// Every copy goes through the context stream and is synchronised: the
+// stream is non-blocking (not ordered after the null stream), and a pageable
+// host-to-device hipMemcpy may return before its DMA has landed.
+hipError_t copy_sync(gi_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  if (bytes == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->stream);
+}
+
 int fail(gi_ctx *ctx, int code, const std::string &msg) {
   if (ctx) ctx->err = msg;
   return code;
@@ -647,8 +658,8 @@ int ensure_halton(gi_ctx *ctx, uint32_t seed, int32_t ndims) {
   halton_tables(seed, ndims, dims, perms);
   GI_HIP(ctx, ctx->d_hdims.ensure(dims.size() * sizeof(HaltonDim)));
   GI_HIP(ctx, ctx->d_hperm.ensure(perms.size() * sizeof(uint16_t)));
-  GI_HIP(ctx, hipMemcpy(ctx->d_hdims.p, dims.data(), dims.size() * sizeof(HaltonDim), hipMemcpyHostToDevice));
-  GI_HIP(ctx, hipMemcpy(ctx->d_hperm.p, perms.data(), perms.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_hdims.p, dims.data(), dims.size() * sizeof(HaltonDim), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_hperm.p, perms.data(), perms.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   ctx->hal_seed = seed;
   ctx->hal_dims = ndims;
   return GI_OK;
@@ -663,9 +674,9 @@ int upload_wide(gi_ctx *ctx) {
   collapse_wide(*ctx->bvh, ctx->tris.data(), wn, it, wd);
   GI_HIP(ctx, ctx->d_nodes.ensure(wn.size() * sizeof(WideNode)));
   GI_HIP(ctx, ctx->d_itris.ensure(std::max<size_t>(1, it.size()) * sizeof(IsectTri)));
-  GI_HIP(ctx, hipMemcpy(ctx->d_nodes.p, wn.data(), wn.size() * sizeof(WideNode), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_nodes.p, wn.data(), wn.size() * sizeof(WideNode), hipMemcpyHostToDevice));
   if (!it.empty())
-    GI_HIP(ctx, hipMemcpy(ctx->d_itris.p, it.data(), it.size() * sizeof(IsectTri), hipMemcpyHostToDevice));
+    GI_HIP(ctx, copy_sync(ctx, ctx->d_itris.p, it.data(), it.size() * sizeof(IsectTri), hipMemcpyHostToDevice));
   ctx->nnodes = (int64_t)wn.size();
   ctx->wide_depth = wd;
   // at most three pushes per wide level on the current path
@@ -681,7 +692,7 @@ int upload_wide(gi_ctx *ctx) {
 
 uint32_t read_overflow(gi_ctx *ctx) {
   uint32_t v = 0;
-  (void)hipMemcpy(&v, ctx->d_overflow.p, 4, hipMemcpyDeviceToHost);
+  (void)copy_sync(ctx, &v, ctx->d_overflow.p, 4, hipMemcpyDeviceToHost);
   return v;
 }
 
@@ -757,8 +768,11 @@ struct PassPlan {
 };
 
 // Pack (view, sample range) chunks into passes of at most `target` slots.
-// batched: chunks of different views share a pass; otherwise every pass
-// holds one view (the per-view wavefront).
+// A pass holds at most one chunk per view (the accumulate kernel updates
+// each pixel once per pass, in sample order).  batched: views advance
+// round-robin and chunks of different views share passes; otherwise every
+// pass holds one chunk and views are rendered one after the other (the
+// per-view wavefront).
 void plan_passes(const std::vector<gi_view> &views, const std::vector<int32_t> &s_begin,
                  const std::vector<int32_t> &s_end, bool batched, size_t target, int32_t spp_per_pass,
                  PassPlan &plan) {
@@ -772,29 +786,43 @@ void plan_passes(const std::vector<gi_view> &views, const std::vector<int32_t> &
     plan.passes.push_back(cur);
     std::memset(&cur, 0, sizeof(cur));
   };
-  for (size_t v = 0; v < views.size(); ++v) {
+  auto holds = [&](int32_t v) {
+    for (int c = 0; c < cur.nchunks; ++c)
+      if (cur.chunk[c].view == v) return true;
+    return false;
+  };
+  std::vector<int32_t> s(s_begin);
+  auto add = [&](size_t v) {
     const size_t npix = (size_t)views[v].w * (size_t)views[v].h;
-    int32_t s = s_begin[v];
-    while (s < s_end[v]) {
-      int32_t k;
-      if (spp_per_pass > 0) k = spp_per_pass;
-      else k = (int32_t)std::max<size_t>(1, target / std::max<size_t>(1, npix));
-      k = std::min(k, s_end[v] - s);
-      size_t slots = npix * (size_t)k;
-      if (cur.nchunks > 0 && (!batched || cur.nchunks == kMaxChunks || cur.total_slots + slots > std::max(target, npix)))
-        flush();
-      PassChunk &c = cur.chunk[cur.nchunks++];
-      c.view = (int32_t)v;
-      c.s0 = s;
-      c.k = k;
-      c.npix = (int32_t)npix;
-      c.slot_base = cur.total_slots;
-      c.pix_base = cur.total_pix;
-      cur.total_slots += (uint32_t)slots;
-      cur.total_pix += (uint32_t)npix;
-      s += k;
+    int32_t k = spp_per_pass > 0 ? spp_per_pass : (int32_t)std::max<size_t>(1, target / std::max<size_t>(1, npix));
+    k = std::min(k, s_end[v] - s[v]);
+    const size_t slots = npix * (size_t)k;
+    if (cur.nchunks > 0 && (!batched || holds((int32_t)v) || cur.nchunks == kMaxChunks ||
+                            cur.total_slots + slots > std::max(target, npix)))
+      flush();
+    PassChunk &c = cur.chunk[cur.nchunks++];
+    c.view = (int32_t)v;
+    c.s0 = s[v];
+    c.k = k;
+    c.npix = (int32_t)npix;
+    c.slot_base = cur.total_slots;
+    c.pix_base = cur.total_pix;
+    cur.total_slots += (uint32_t)slots;
+    cur.total_pix += (uint32_t)npix;
+    s[v] += k;
+  };
+  if (batched) {
+    for (bool more = true; more;) {
+      more = false;
+      for (size_t v = 0; v < views.size(); ++v)
+        if (s[v] < s_end[v]) {
+          add(v);
+          more = true;
+        }
     }
-    if (!batched) flush();
+  } else {
+    for (size_t v = 0; v < views.size(); ++v)
+      while (s[v] < s_end[v]) add(v);
   }
   flush();
 }
@@ -949,7 +977,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
   }
   // telemetry
   std::vector<uint32_t> hc(npass * ncnt);
-  GI_HIP(ctx, hipMemcpy(hc.data(), ctx->d_counters.p, hc.size() * 4, hipMemcpyDeviceToHost));
+  GI_HIP(ctx, copy_sync(ctx, hc.data(), ctx->d_counters.p, hc.size() * 4, hipMemcpyDeviceToHost));
   for (size_t pi = 0; pi < npass; ++pi) {
     for (int32_t b = 0; b < depth; ++b) {
       ctx->counters.rays_closest += hc[pi * ncnt + b];
@@ -1068,7 +1096,9 @@ int gi_create(gi_ctx **out, int device) {
   ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return GI_ERR_NO_DEVICE;
   if (ctx->d_overflow.ensure(256) != hipSuccess) return GI_ERR_OUT_OF_MEMORY;
-  if (hipMemset(ctx->d_overflow.p, 0, 4) != hipSuccess) return GI_ERR_HIP;
+  if (hipMemsetAsync(ctx->d_overflow.p, 0, 4, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return GI_ERR_HIP;
   *out = ctx.release();
   return GI_OK;
 }
@@ -1159,7 +1189,7 @@ int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n) {
   float dx = ctx->hi[0] - ctx->lo[0], dy = ctx->hi[1] - ctx->lo[1], dz = ctx->hi[2] - ctx->lo[2];
   ctx->diag = std::sqrt(dx * dx + dy * dy + dz * dz);
   GI_HIP(ctx, ctx->d_stris.ensure(n * sizeof(ShadeTri)));
-  GI_HIP(ctx, hipMemcpy(ctx->d_stris.p, st.data(), n * sizeof(ShadeTri), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_stris.p, st.data(), n * sizeof(ShadeTri), hipMemcpyHostToDevice));
   return GI_OK;
 }
 
@@ -1170,7 +1200,7 @@ int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n) {
   std::vector<DevMat> dm(n);
   for (int32_t i = 0; i < n; ++i) dm[i] = make_devmat(mats[i]);
   GI_HIP(ctx, ctx->d_mats.ensure(n * sizeof(DevMat)));
-  GI_HIP(ctx, hipMemcpy(ctx->d_mats.p, dm.data(), n * sizeof(DevMat), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_mats.p, dm.data(), n * sizeof(DevMat), hipMemcpyHostToDevice));
   return GI_OK;
 }
 
@@ -1201,7 +1231,7 @@ int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n) {
   ctx->lights.assign(lights, lights + n);
   ctx->power_total = total;
   GI_HIP(ctx, ctx->d_lights.ensure(n * sizeof(DevLight)));
-  GI_HIP(ctx, hipMemcpy(ctx->d_lights.p, dl.data(), n * sizeof(DevLight), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_lights.p, dl.data(), n * sizeof(DevLight), hipMemcpyHostToDevice));
   return GI_OK;
 }
 
@@ -1281,13 +1311,13 @@ int gi_trace_closest(gi_ctx *ctx, const float *orig, const float *dir, const flo
   GI_HIP(ctx, ctx->p_dir.ensure(n * 12));
   GI_HIP(ctx, ctx->p_maxt.ensure(n * 4));
   GI_HIP(ctx, ctx->p_hit.ensure(n * sizeof(gi_hit)));
-  GI_HIP(ctx, hipMemcpy(ctx->p_orig.p, orig, n * 12, hipMemcpyHostToDevice));
-  GI_HIP(ctx, hipMemcpy(ctx->p_dir.p, dir, n * 12, hipMemcpyHostToDevice));
-  GI_HIP(ctx, hipMemcpy(ctx->p_maxt.p, max_t, n * 4, hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->p_orig.p, orig, n * 12, hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->p_dir.p, dir, n * 12, hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->p_maxt.p, max_t, n * 4, hipMemcpyHostToDevice));
   rc = gi_trace_closest_device(ctx, ctx->p_orig.as<float>(), ctx->p_dir.as<float>(), ctx->p_maxt.as<float>(), n,
                                ctx->p_hit.as<gi_hit>());
   if (rc) return rc;
-  GI_HIP(ctx, hipMemcpy(out, ctx->p_hit.p, n * sizeof(gi_hit), hipMemcpyDeviceToHost));
+  GI_HIP(ctx, copy_sync(ctx, out, ctx->p_hit.p, n * sizeof(gi_hit), hipMemcpyDeviceToHost));
   return GI_OK;
 }
 
@@ -1301,13 +1331,13 @@ int gi_trace_any(gi_ctx *ctx, const float *orig, const float *dir, const float *
   GI_HIP(ctx, ctx->p_dir.ensure(n * 12));
   GI_HIP(ctx, ctx->p_maxt.ensure(n * 4));
   GI_HIP(ctx, ctx->p_occ.ensure(n));
-  GI_HIP(ctx, hipMemcpy(ctx->p_orig.p, orig, n * 12, hipMemcpyHostToDevice));
-  GI_HIP(ctx, hipMemcpy(ctx->p_dir.p, dir, n * 12, hipMemcpyHostToDevice));
-  GI_HIP(ctx, hipMemcpy(ctx->p_maxt.p, max_t, n * 4, hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->p_orig.p, orig, n * 12, hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->p_dir.p, dir, n * 12, hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->p_maxt.p, max_t, n * 4, hipMemcpyHostToDevice));
   rc = gi_trace_any_device(ctx, ctx->p_orig.as<float>(), ctx->p_dir.as<float>(), ctx->p_maxt.as<float>(), n,
                            ctx->p_occ.as<uint8_t>());
   if (rc) return rc;
-  GI_HIP(ctx, hipMemcpy(occluded, ctx->p_occ.p, n, hipMemcpyDeviceToHost));
+  GI_HIP(ctx, copy_sync(ctx, occluded, ctx->p_occ.p, n, hipMemcpyDeviceToHost));
   return GI_OK;
 }
 
@@ -1329,7 +1359,7 @@ int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint3
   pd.total_pix = npix;
   pd.chunk[0] = PassChunk{0, sample, 1, (int32_t)npix, 0, 0, {0, 0}};
   GI_HIP(ctx, ctx->d_passes.ensure(sizeof(PassDesc)));
-  GI_HIP(ctx, hipMemcpy(ctx->d_passes.p, &pd, sizeof(pd), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_passes.p, &pd, sizeof(pd), hipMemcpyHostToDevice));
   rc = ensure_workspace(ctx, npix);
   if (rc) return rc;
   HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hperm.as<uint16_t>()};
@@ -1418,7 +1448,7 @@ int gi_get_image(gi_ctx *ctx, int32_t view, float *rgb) {
   GI_HIP(ctx, hipSetDevice(ctx->device));
   const size_t npix = (size_t)ctx->views[view].w * ctx->views[view].h;
   std::vector<float4> tmp(npix);
-  GI_HIP(ctx, hipMemcpy(tmp.data(), ctx->images[view].p, npix * sizeof(float4), hipMemcpyDeviceToHost));
+  GI_HIP(ctx, copy_sync(ctx, tmp.data(), ctx->images[view].p, npix * sizeof(float4), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < npix; ++i) {
     rgb[3 * i] = tmp[i].x;
     rgb[3 * i + 1] = tmp[i].y;

@@ -270,6 +270,19 @@ def terrain_spheres(seed: int = 2, cells=(256, 256), n_spheres: int = 1000, n_ma
     return Scene(tris, mats, np.concatenate(lights), view, make_opts(depth), name="terrain_spheres")
 
 
+def save_scene(scene: Scene, path: str) -> None:
+    """Binary dump read by the C++ driver (host/gi_driver.cpp): "GISC", u32
+    version 1, u64 n_tris, u32 n_mats, n_lights, n_views, then the records."""
+    with open(path, "wb") as f:
+        f.write(b"GISC")
+        np.array([1], "<u4").tofile(f)
+        np.array([len(scene.tris)], "<u8").tofile(f)
+        np.array([len(scene.mats), len(scene.lights), len(scene.views)], "<u4").tofile(f)
+        for a, dt in ((scene.tris, TRI), (scene.mats, MATERIAL), (scene.lights, LIGHT), (scene.views, VIEW),
+                      (scene.opts, OPTS)):
+            np.ascontiguousarray(a, dt).tofile(f)
+
+
 def config_scene(idx: int, **kw) -> Scene:
     """Scenes of the benchmark configurations by number (1-based)."""
     if idx == 1:

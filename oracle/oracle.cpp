@@ -250,10 +250,17 @@ struct MedianBuilder {
       }
     }
   }
+  /* Boxes grow by 2^-20 of their largest coordinate magnitude / extent so
+   * the float slab test stays conservative for points on the triangles
+   * (rays grazing an axis-aligned wall's zero-thickness box). */
   void set_box(int64_t n, const float *lo, const float *hi) {
+    float m = 0.f;
+    for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+    for (int k = 0; k < 3; ++k) m = std::max(m, hi[k] - lo[k]);
+    const float pad = m * 0x1p-20f + 1e-30f;
     float *p = &nodes[8 * n];
-    p[2] = lo[0]; p[3] = lo[1]; p[4] = lo[2];
-    p[5] = hi[0]; p[6] = hi[1]; p[7] = hi[2];
+    p[2] = lo[0] - pad; p[3] = lo[1] - pad; p[4] = lo[2] - pad;
+    p[5] = hi[0] + pad; p[6] = hi[1] + pad; p[7] = hi[2] + pad;
   }
   void build(int64_t node, uint32_t b, uint32_t e) {
     float lo[3], hi[3];

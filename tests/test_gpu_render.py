@@ -13,6 +13,14 @@ from globillum_amd._lib import FLAG_BATCHED, FLAG_GRAPH, GiError
 pytestmark = pytest.mark.gpu
 
 
+def _same(a, b):
+    a = np.asarray(a).reshape(-1, 3)
+    b = np.asarray(b).reshape(-1, 3)
+    bad = np.nonzero((a != b).any(1))[0]
+    assert len(bad) == 0, (f"{len(bad)} of {len(a)} pixels differ, first {bad[:5].tolist()}, "
+                           f"max |d| {np.abs(a - b).max():.3g}")
+
+
 def _rel_rmse(a, b):
     return float(np.sqrt(((a - b) ** 2).mean()) / max(np.sqrt((b ** 2).mean()), 1e-12))
 
@@ -29,7 +37,7 @@ def test_c1_full_config_bit_exact(oracle, gpu_ctx_factory):
         img = ctx.render().image(0).reshape(-1, 3)
         c = ctx.counters()
     assert _rel_rmse(img, ref) <= 0.01
-    assert np.array_equal(img, ref)
+    _same(img, ref)
     assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
     assert c["samples"] == 256 * 256 * 16
 
@@ -41,7 +49,8 @@ def test_plain_batched_graph_identical(flags, gpu_ctx_factory):
         a = ctx.render().image(0)
         b = ctx.render(flags=flags).image(0)
         c2 = ctx.render(flags=flags, samples_per_pass=3).image(0)
-    assert np.array_equal(a, b) and np.array_equal(a, c2)
+    _same(a, b)
+    _same(a, c2)
 
 
 def test_progressive_equals_one_shot(gpu_ctx_factory):
@@ -65,7 +74,7 @@ def test_russian_roulette_bit_exact(oracle, gpu_ctx_factory):
     with gpu_ctx_factory(sc) as ctx:
         img = ctx.render().image(0).reshape(-1, 3)
         c = ctx.counters()
-    assert np.array_equal(img, ref)
+    _same(img, ref)
     assert c["rays_closest"] == st["rays_closest"]
     assert st["rays_closest"] < 0.9 * 8 * 96 * 96 * 16
 
@@ -77,7 +86,7 @@ def test_principled_scene_bit_exact(oracle, gpu_ctx_factory):
         img = ctx.render().image(0).reshape(-1, 3)
         c = ctx.counters()
     assert _rel_rmse(img, ref) <= 0.01
-    assert np.array_equal(img, ref)
+    _same(img, ref)
     assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
 
 
@@ -94,7 +103,7 @@ def test_multi_view_batched_matches_per_view_oracle(oracle, gpu_ctx_factory):
     with gpu_ctx_factory(sc) as ctx:
         ctx.render(views, opts)
         got = np.concatenate([im.reshape(-1, 3) for im in ctx.images()])
-    assert np.array_equal(got, ref)
+    _same(got, ref)
 
 
 def test_thin_lens_camera_rays_bit_exact(oracle, gpu_ctx_factory):
@@ -104,7 +113,7 @@ def test_thin_lens_camera_rays_bit_exact(oracle, gpu_ctx_factory):
         for s in (0, 3):
             go, gd, _ = ctx.camera_rays(v, s, 5)
             oo, od, _ = oracle.camera_rays(v, s, 5)
-            assert np.array_equal(go, oo) and np.array_equal(gd, od)
+            assert np.array_equal(go, oo) and np.array_equal(gd, od), s
 
 
 def test_converged_1024spp_within_one_percent(oracle, gpu_ctx_factory):
@@ -113,7 +122,7 @@ def test_converged_1024spp_within_one_percent(oracle, gpu_ctx_factory):
     with gpu_ctx_factory(sc) as ctx:
         img = ctx.render().image(0).reshape(-1, 3)
     assert _rel_rmse(img, ref) <= 0.01
-    assert np.array_equal(img, ref)
+    _same(img, ref)
 
 
 def test_full_size_c2_properties_and_crop_parity(oracle, gpu_ctx_factory):
@@ -127,7 +136,7 @@ def test_full_size_c2_properties_and_crop_parity(oracle, gpu_ctx_factory):
     # oracle on a pixel range of the same view (same Halton indices)
     p0, p1 = 360 * 1280 + 400, 360 * 1280 + 1400
     ref, _ = _oracle(oracle, sc, p_begin=p0, p_end=p1)
-    assert np.array_equal(img[p0:p1], ref[p0:p1])
+    _same(img[p0:p1], ref[p0:p1])
 
 
 def test_render_argument_errors(gpu_ctx_factory):

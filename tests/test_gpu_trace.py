@@ -115,7 +115,8 @@ def test_edge_cases(c1, oracle, gpu_ctx_factory):
         for n in (1, 63, 64, 65, 1000, 1_000_003):
             o, d = _rays(rng, n, -0.2, 1.2)
             t = np.full(n, FMAX, np.float32)
-            assert (ctx.trace_closest(o, d, t) == osc.trace_closest(o, d, t)).all()
+            _assert_hits_equal(ctx.trace_closest(o, d, t), osc.brute_closest(o, d, t), osc, o, d, t)
+            _assert_hits_equal(ctx.trace_closest(o, d, t), osc.trace_closest(o, d, t), osc, o, d, t)
 
 
 def test_call_order_and_argument_errors(c1, gpu_ctx_factory):
