@@ -227,6 +227,7 @@ bool import_bvh2(const float *nodes, int64_t n_nodes, const uint32_t *indices, i
     Bvh2Node &o = out.nodes[i];
     o.lo[0] = p[2]; o.lo[1] = p[3]; o.lo[2] = p[4];
     o.hi[0] = p[5]; o.hi[1] = p[6]; o.hi[2] = p[7];
+    pad_box(o.lo, o.hi);  // keep the device slab test conservative
     int32_t x = fbits(p[0]), y = fbits(p[1]);
     if (x & 1) {
       o.left = x >> 1;
@@ -398,39 +399,53 @@ void halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm) {
   for (int32_t i = p - 1; i >= 1; --i) std::swap(perm[i], perm[r.bounded((uint32_t)(i + 1))]);
 }
 
-void halton_tables(uint32_t seed, int32_t ndims, std::vector<HaltonDim> &dims, std::vector<uint16_t> &perms) {
+void halton_tables(uint32_t seed, int32_t ndims, std::vector<HaltonDim> &dims, std::vector<uint16_t> &chunks) {
   dims.resize(ndims);
-  perms.clear();
+  chunks.clear();
+  std::vector<uint16_t> perm;
   for (int32_t d = 0; d < ndims; ++d) {
     const uint32_t p = (uint32_t)prime_of(d);
+    perm.resize(p);
+    halton_permutation(d, seed, perm.data());
+    uint32_t m = 1, P = p;
+    while (P * p <= 1024u) {
+      P *= p;
+      ++m;
+    }
     HaltonDim &h = dims[d];
     h.prime = p;
-    h.perm_off = (uint32_t)perms.size();
-    perms.resize(perms.size() + p);
-    halton_permutation(d, seed, perms.data() + h.perm_off);
-    // floor(2^64 / p) + 1: exact floor(n / p) = mulhi(n, magic) for n < 2^32
-    h.magic = (uint64_t)(~0ULL / p) + 1ULL;
-    h.inv = 1.0 / (double)p;
-    h.tail = (double)perms[h.perm_off] / (double)(p - 1);
+    h.chunk_off = (uint32_t)chunks.size();
+    h.P = P;
+    h.perm0 = perm[0];
+    h.magic = (uint64_t)(~0ULL / P) + 1ULL;  // exact floor(n / P) = mulhi(n, magic) for n < 2^32
+    h.pad = 0;
+    chunks.resize(chunks.size() + P);
+    for (uint32_t c = 0; c < P; ++c) {
+      uint32_t v = 0, x = c;
+      for (uint32_t i = 0; i < m; ++i) {  // least significant digit first
+        v = v * p + perm[x % p];
+        x /= p;
+      }
+      chunks[h.chunk_off + c] = (uint16_t)v;
+    }
   }
 }
 
+// Scrambled radical inverse as the exact rational A/B (see gi_device.h).
 float halton_host(uint64_t n, int32_t dim, uint32_t seed) {
   const int32_t p = prime_of(dim);
   std::vector<uint16_t> perm(p);
   halton_permutation(dim, seed, perm.data());
-  const double inv = 1.0 / (double)p;
-  uint64_t rev = 0;
-  double inv_n = 1.0;
+  uint64_t rev = 0, pk = 1;
   while (n > 0) {
     uint64_t q = n / (uint64_t)p;
-    uint64_t dg = n - q * (uint64_t)p;
-    rev = rev * (uint64_t)p + perm[dg];
-    inv_n *= inv;
+    rev = rev * (uint64_t)p + perm[n - q * (uint64_t)p];
+    pk *= (uint64_t)p;
     n = q;
   }
-  double r = inv_n * ((double)rev + (double)perm[0] / (double)(p - 1));
-  float f = (float)r;
+  const uint64_t A = rev * (uint64_t)(p - 1) + perm[0];
+  const uint64_t B = pk * (uint64_t)(p - 1);
+  float f = (float)((double)A / (double)B);
   return f >= 1.0f ? 0x1.fffffep-1f : f;
 }
 

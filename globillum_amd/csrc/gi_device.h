@@ -78,26 +78,31 @@ __device__ __forceinline__ void sincos2pi(float v, float &s, float &c) {
 // ---------------------------------------------------------------- Halton --
 struct HaltonDev {
   const HaltonDim *dims;
-  const uint16_t *perm;
+  const uint16_t *chunks;
 };
-// Scrambled radical inverse of n in the prime base of dimension dim:
-// digits d_k map to perm[d_k]; the infinitely many leading zero digits add
-// perm[0]/(p-1) * p^-K.  FP64, then one rounding to FP32.
+// Scrambled radical inverse of n in the prime base p of dimension dim: the
+// digits d_k (K of them) map to perm[d_k] and the infinitely many leading
+// zeros to perm[0], which is the exact rational
+//   A / B,  A = rev*(p-1) + perm[0],  B = p^K (p-1),  rev = sum perm[d_k] p^(K-1-k).
+// Digits are consumed m at a time through the chunk table (P = p^m <= 1024);
+// a partial last chunk adds leading zeros, which scales A and B by the same
+// power of p.  A, B < 2^53 for n < 2^32, so the value is one correctly
+// rounded FP64 division and one rounding to FP32.
 __device__ __forceinline__ float halton(uint32_t n, int dim, HaltonDev H) {
   const HaltonDim hd = H.dims[dim];
-  const uint16_t *perm = H.perm + hd.perm_off;
-  uint64_t rev = 0;
-  double inv_n = 1.0;
+  const uint16_t *chunk = H.chunks + hd.chunk_off;
+  uint64_t rev = 0, pk = 1;
   uint32_t m = n;
   while (m > 0) {
-    uint32_t q = (uint32_t)__umul64hi((uint64_t)m, hd.magic);
-    uint32_t dg = m - q * hd.prime;
-    rev = rev * hd.prime + perm[dg];
-    inv_n *= hd.inv;
+    const uint32_t q = (uint32_t)__umul64hi((uint64_t)m, hd.magic);
+    const uint32_t c = m - q * hd.P;
+    rev = rev * hd.P + chunk[c];
+    pk *= hd.P;
     m = q;
   }
-  double r = inv_n * ((double)rev + hd.tail);
-  float f = (float)r;
+  const uint64_t A = rev * (uint64_t)(hd.prime - 1) + hd.perm0;
+  const uint64_t B = pk * (uint64_t)(hd.prime - 1);
+  const float f = (float)((double)A / (double)B);
   return f >= 1.0f ? 0x1.fffffep-1f : f;
 }
 

@@ -68,21 +68,23 @@ void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &n
 
 // ------------------------------------------------------------- Halton ----
 // Per-dimension constants of the scrambled radical inverse, laid out for
-// the device: prime, offset of its digit permutation, floor(2^64/p)+1 for
-// the division, 1/p and perm[0]/(p-1) in FP64.
+// the device.  Digits are consumed m at a time (P = p^m <= 1024): chunk[c]
+// for c < P holds the scrambled, reversed m-digit value
+// sum_i perm[digit_i(c)] p^(m-1-i); magic = floor(2^64/P)+1 divides by P.
 struct HaltonDim {
-  uint32_t prime;
-  uint32_t perm_off;
-  uint64_t magic;
-  double inv;
-  double tail;
+  uint32_t prime;       // p
+  uint32_t chunk_off;   // offset of the chunk table in the uint16 pool
+  uint32_t P;           // p^m
+  uint32_t perm0;       // perm[0]
+  uint64_t magic;       // floor(2^64 / P) + 1
+  uint64_t pad;
 };
 static_assert(sizeof(HaltonDim) == 32, "halton dim layout");
 
 int32_t prime_of(int32_t dim);
 void halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm);
 void halton_tables(uint32_t seed, int32_t ndims, std::vector<HaltonDim> &dims,
-                   std::vector<uint16_t> &perms);
+                   std::vector<uint16_t> &chunks);
 float halton_host(uint64_t n, int32_t dim, uint32_t seed);
 
 // Sample dimension map: jitter x/y, lens x/y, then per path vertex b:

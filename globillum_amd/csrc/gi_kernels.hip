@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__rest
         const f3 bc = f3{(1.0f - beta) - gamma, beta, gamma};
         const f3 Pp = add(add(scl(xyz(T.a), bc.x), scl(xyz(T.b), bc.y)), scl(xyz(T.c), bc.z));
         const f3 Ni = add(add(scl(xyz(T.na), bc.x), scl(xyz(T.nb), bc.y)), scl(xyz(T.nc), bc.z));
-        const DevMat m = P.mats[fbits(T.a.w)];
+        const DevMat &m = P.mats[fbits(T.a.w)];
 
         // ---- NEE: power-proportional light choice, point on the rect
         const f3 xl = f3{halton(n, dim + 0, P.H), halton(n, dim + 1, P.H), halton(n, dim + 2, P.H)};
@@ -539,7 +539,7 @@ struct gi_ctx {
   // Halton tables
   uint32_t hal_seed = 0;
   int32_t hal_dims = 0;
-  DBuf d_hdims, d_hperm;
+  DBuf d_hdims, d_hchunks;
   // traversal stack + overflow flag
   DBuf d_gstack, d_overflow;
   uint32_t stack_cap = 0;
@@ -654,12 +654,12 @@ DevMat make_devmat(const gi_material &m) {
 int ensure_halton(gi_ctx *ctx, uint32_t seed, int32_t ndims) {
   if (ctx->hal_dims >= ndims && ctx->hal_seed == seed && ctx->d_hdims.p) return GI_OK;
   std::vector<HaltonDim> dims;
-  std::vector<uint16_t> perms;
+  std::vector<uint16_t> perms;  // chunk tables
   halton_tables(seed, ndims, dims, perms);
   GI_HIP(ctx, ctx->d_hdims.ensure(dims.size() * sizeof(HaltonDim)));
-  GI_HIP(ctx, ctx->d_hperm.ensure(perms.size() * sizeof(uint16_t)));
+  GI_HIP(ctx, ctx->d_hchunks.ensure(perms.size() * sizeof(uint16_t)));
   GI_HIP(ctx, copy_sync(ctx, ctx->d_hdims.p, dims.data(), dims.size() * sizeof(HaltonDim), hipMemcpyHostToDevice));
-  GI_HIP(ctx, copy_sync(ctx, ctx->d_hperm.p, perms.data(), perms.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_hchunks.p, perms.data(), perms.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   ctx->hal_seed = seed;
   ctx->hal_dims = ndims;
   return GI_OK;
@@ -851,7 +851,7 @@ ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
   P.lights = ctx->d_lights.as<DevLight>();
   P.nlights = (int32_t)ctx->lights.size();
   P.power_total = ctx->power_total;
-  P.H = HaltonDev{ctx->d_hdims.as<HaltonDim>(), ctx->d_hperm.as<uint16_t>()};
+  P.H = HaltonDev{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>()};
   P.off_b = ctx->opts.offset_bounce > 0.f ? ctx->opts.offset_bounce : 1e-4f * ctx->diag;
   P.off_s = ctx->opts.offset_shadow > 0.f ? ctx->opts.offset_shadow : 1e-4f * ctx->diag;
   for (int k = 0; k < 3; ++k) P.bg[k] = ctx->opts.background[k];
@@ -1362,7 +1362,7 @@ int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint3
   GI_HIP(ctx, copy_sync(ctx, ctx->d_passes.p, &pd, sizeof(pd), hipMemcpyHostToDevice));
   rc = ensure_workspace(ctx, npix);
   if (rc) return rc;
-  HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hperm.as<uint16_t>()};
+  HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>()};
   hipLaunchKernelGGL(k_raygen, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_passes.as<PassDesc>(),
                      ctx->d_views.as<DevView>(), H, ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(),
                      ctx->q_tp[0].as<float4>(), ctx->path_l.as<float4>());

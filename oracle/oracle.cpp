@@ -113,22 +113,25 @@ void halton_perm(int32_t dim, uint32_t seed, uint16_t *perm) {
   }
 }
 
-/* Scrambled radical inverse: digits d_k of n in base p map to perm[d_k]; the
- * infinitely many leading zero digits contribute perm[0]/(p-1) * p^-K.
- * Evaluated in FP64 and rounded once to FP32. */
+/* Scrambled radical inverse: digits d_k of n (K of them) map to perm[d_k]
+ * and the infinitely many leading zero digits to perm[0], i.e. the exact
+ * rational  A / B  with  A = rev*(p-1) + perm[0],  B = p^K * (p-1),
+ * rev = sum_k perm[d_k] p^(K-1-k).  A and B are integers below 2^53, so the
+ * value is one correctly rounded FP64 division, then one rounding to FP32.
+ * (Grouping digits differently scales A and B by the same power of p and
+ * leaves the value unchanged.) */
 float scrambled_ri(uint64_t n, int32_t p, const uint16_t *perm) {
-  const double inv = 1.0 / (double)p;
-  uint64_t rev = 0;
-  double inv_n = 1.0;
+  uint64_t rev = 0, pk = 1;
   while (n > 0) {
     uint64_t q = n / (uint64_t)p;
     uint64_t d = n - q * (uint64_t)p;
     rev = rev * (uint64_t)p + perm[d];
-    inv_n *= inv;
+    pk *= (uint64_t)p;
     n = q;
   }
-  double r = inv_n * ((double)rev + (double)perm[0] / (double)(p - 1));
-  float f = (float)r;
+  const uint64_t A = rev * (uint64_t)(p - 1) + perm[0];
+  const uint64_t B = pk * (uint64_t)(p - 1);
+  float f = (float)((double)A / (double)B);
   if (f >= 1.0f) f = 0x1.fffffep-1f;
   return f;
 }
