@@ -36,7 +36,7 @@ COUNTERS = np.dtype([("rays_closest", "<u8"), ("rays_shadow", "<u8"), ("samples"
                      ("ms_trace_any", "<f8"), ("ms_shade", "<f8"), ("ms_other", "<f8"),
                      ("launches_closest", "<i4"), ("passes", "<i4")])
 TUNING = np.dtype([("trace_block", "<i4"), ("trace_blocks_per_cu", "<i4"), ("shade_block", "<i4"),
-                   ("samples_per_pass", "<i4")])
+                   ("samples_per_pass", "<i4"), ("stack_lds", "<i4"), ("target_slots", "<i4")])
 MATGRAD = np.dtype([("d_color", "<f4", 3), ("d_roughness", "<f4"), ("d_metallic", "<f4")])
 
 assert TRI.itemsize == 100 and MATERIAL.itemsize == 96 and LIGHT.itemsize == 64

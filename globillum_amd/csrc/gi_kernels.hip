@@ -195,12 +195,12 @@ __device__ __forceinline__ bool traverse(const WideNode *__restrict__ nodes, con
   return bid != GI_INVALID_REF;
 }
 
-constexpr int kLdsDepth = 16;
+constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the global part
 
 // Closest-hit kernel over a ray queue.  kHasTmax: max_t in ray_o.w (the
 // primitive entry point); otherwise ray_o.w carries the Halton index and
 // max_t is FLT_MAX (camera and bounce rays, src/hybrid-pt.cpp:558).
-template <bool kHasTmax>
+template <bool kHasTmax, int kLdsDepth>
 __global__ __launch_bounds__(256) void k_trace_closest(const uint32_t *__restrict__ count_ptr, uint32_t count_host,
                                                        const float4 *__restrict__ ray_o,
                                                        const float4 *__restrict__ ray_d, float4 *__restrict__ hits,
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(const uint32_t *__restric
 // ray adds its precomputed NEE contribution to its path's colour (the
 // !is.valid() branch of src/hybrid-pt.cpp:355-365).  Otherwise writes the
 // occlusion byte (primitive entry point).
-template <bool kDeposit>
+template <bool kDeposit, int kLdsDepth>
 __global__ __launch_bounds__(256) void k_trace_any(const uint32_t *__restrict__ count_ptr, uint32_t count_host,
                                                    const float4 *__restrict__ ray_o, const float4 *__restrict__ ray_d,
                                                    const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
@@ -557,7 +557,7 @@ struct gi_ctx {
   int32_t prog_sample = 0, prog_max = 0;
   bool prog_active = false;
   gi_counters counters{};
-  gi_tuning tuning{256, 8, 256, 0};
+  gi_tuning tuning{256, 8, 256, 0, 16, 0};
   std::vector<hipEvent_t> events;
 };
 
@@ -681,8 +681,7 @@ int upload_wide(gi_ctx *ctx) {
   ctx->wide_depth = wd;
   // at most three pushes per wide level on the current path
   ctx->stack_cap = (uint32_t)(3 * (wd + 1) + 4);
-  const uint32_t gdepth = ctx->stack_cap > (uint32_t)kLdsDepth ? ctx->stack_cap - kLdsDepth : 1;
-  (void)gdepth;
+
   ctx->trace_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
   int rc = ensure_gstack(ctx);
   if (rc) return rc;
@@ -702,7 +701,7 @@ uint32_t read_overflow(gi_ctx *ctx) {
 int ensure_gstack(gi_ctx *ctx) {
   const int block = std::max(64, (std::min(1024, ctx->tuning.trace_block) / 64) * 64);
   const size_t threads = (size_t)ctx->num_cus * (size_t)std::max(1, ctx->tuning.trace_blocks_per_cu) * block;
-  const uint32_t gdepth = ctx->stack_cap > (uint32_t)kLdsDepth ? ctx->stack_cap - kLdsDepth : 1;
+  const uint32_t gdepth = ctx->stack_cap > (uint32_t)kMinLdsDepth ? ctx->stack_cap - kMinLdsDepth : 1;
   GI_HIP(ctx, ctx->d_gstack.ensure(threads * gdepth * sizeof(uint32_t)));
   return GI_OK;
 }
@@ -725,20 +724,44 @@ int check_scene(gi_ctx *ctx, bool need_shading) {
   return GI_OK;
 }
 
+template <bool kHasTmax, int kLds>
+void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
+                      const float4 *ro, const float4 *rd, float4 *hits) {
+  const size_t lds = (size_t)kLds * block * sizeof(uint32_t);
+  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kLds>), dim3(grid), dim3(block), lds, ctx->stream, count_ptr,
+                     count_host, ro, rd, hits, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
+                     ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+}
+
+template <bool kDeposit, int kLds>
+void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
+                  const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ) {
+  const size_t lds = (size_t)kLds * block * sizeof(uint32_t);
+  hipLaunchKernelGGL((k_trace_any<kDeposit, kLds>), dim3(grid), dim3(block), lds, ctx->stream, count_ptr,
+                     count_host, ro, rd, contrib, pl, occ, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
+                     ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+}
+
+int stack_lds_of(const gi_ctx *ctx) {
+  const int s = ctx->tuning.stack_lds;
+  return s == 8 || s == 32 ? s : 16;
+}
+
 int launch_closest(gi_ctx *ctx, bool has_tmax, const uint32_t *count_ptr, uint32_t count_host, uint64_t n_bound,
                    const float4 *ro, const float4 *rd, float4 *hits) {
   int grid, block;
   int rc = trace_grid_for(ctx, n_bound, grid, block);
   if (rc) return rc;
-  const size_t lds = (size_t)kLdsDepth * block * sizeof(uint32_t);
-  if (has_tmax)
-    hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(block), lds, ctx->stream, count_ptr, count_host, ro,
-                       rd, hits, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
-                       ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
-  else
-    hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(block), lds, ctx->stream, count_ptr, count_host, ro,
-                       rd, hits, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
-                       ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+  const int L = stack_lds_of(ctx);
+  if (has_tmax) {
+    if (L == 8) launch_closest_t<true, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
+    else if (L == 32) launch_closest_t<true, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
+    else launch_closest_t<true, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
+  } else {
+    if (L == 8) launch_closest_t<false, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
+    else if (L == 32) launch_closest_t<false, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
+    else launch_closest_t<false, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
+  }
   GI_HIP(ctx, hipGetLastError());
   return GI_OK;
 }
@@ -748,15 +771,16 @@ int launch_any(gi_ctx *ctx, bool deposit, const uint32_t *count_ptr, uint32_t co
   int grid, block;
   int rc = trace_grid_for(ctx, n_bound, grid, block);
   if (rc) return rc;
-  const size_t lds = (size_t)kLdsDepth * block * sizeof(uint32_t);
-  if (deposit)
-    hipLaunchKernelGGL(k_trace_any<true>, dim3(grid), dim3(block), lds, ctx->stream, count_ptr, count_host, ro, rd,
-                       contrib, pl, occ, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
-                       ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
-  else
-    hipLaunchKernelGGL(k_trace_any<false>, dim3(grid), dim3(block), lds, ctx->stream, count_ptr, count_host, ro, rd,
-                       contrib, pl, occ, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
-                       ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+  const int L = stack_lds_of(ctx);
+  if (deposit) {
+    if (L == 8) launch_any_t<true, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
+    else if (L == 32) launch_any_t<true, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
+    else launch_any_t<true, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
+  } else {
+    if (L == 8) launch_any_t<false, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
+    else if (L == 32) launch_any_t<false, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
+    else launch_any_t<false, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
+  }
   GI_HIP(ctx, hipGetLastError());
   return GI_OK;
 }
@@ -1057,8 +1081,8 @@ int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_ren
 }
 
 size_t target_slots(gi_ctx *ctx) {
-  (void)ctx;
-  return (size_t)4 << 20;
+  const int t = ctx->tuning.target_slots;
+  return (size_t)(t > 0 ? std::min(t, 64) : 4) << 20;
 }
 
 }  // namespace
@@ -1476,8 +1500,13 @@ int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t) {
   if (n.trace_block <= 0) n.trace_block = 256;
   if (n.trace_blocks_per_cu <= 0) n.trace_blocks_per_cu = 8;
   if (n.shade_block <= 0) n.shade_block = 256;
-  if (n.trace_block % 64 || n.trace_block > 1024 || n.shade_block % 64 || n.shade_block > 1024)
-    return fail(ctx, GI_ERR_INVALID_ARG, "block sizes must be multiples of 64, at most 1024");
+  if (n.stack_lds <= 0) n.stack_lds = 16;
+  if (n.trace_block % 64 || n.trace_block > 256 || n.shade_block % 64 || n.shade_block > 1024)
+    return fail(ctx, GI_ERR_INVALID_ARG, "block sizes must be multiples of 64 (traversal at most 256, shading 1024)");
+  if (n.stack_lds != 8 && n.stack_lds != 16 && n.stack_lds != 32)
+    return fail(ctx, GI_ERR_INVALID_ARG, "stack_lds must be 8, 16 or 32");
+  if (n.target_slots < 0 || n.target_slots > 64 || n.trace_blocks_per_cu > 64 || n.samples_per_pass < 0)
+    return fail(ctx, GI_ERR_INVALID_ARG, "target_slots in [0,64], trace_blocks_per_cu <= 64");
   ctx->tuning = n;
   return GI_OK;
 }
