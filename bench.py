@@ -36,7 +36,57 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--graph", action="store_true", help="replay the render as one hipGraph")
+    p.add_argument("--no-batching", action="store_true", help="skip the multi-view batching leg")
+    p.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batching leg")
     return p.parse_args()
+
+
+def batching_leg(steps: int, tune: bool):
+    """16 views of the cornell scene (resolutions from 256x256 to 1920x1080,
+    4/16/64 spp, depth 8, Russian roulette from 3): the per-view wavefront
+    (host reads each bounce's queue length and sizes the launches, one view
+    after the other) against the batched renderer (all views in shared
+    queues, device-side lengths, the whole frame replayed as one hipGraph,
+    autotuned launch parameters).  Both produce the same image bits."""
+    import numpy as np
+
+    from globillum_amd import autotune, scenes
+    from globillum_amd._lib import FLAG_BATCHED, FLAG_GRAPH
+    from globillum_amd.context import Context
+
+    sc = scenes.config_scene(3)
+    plain = sc.opts.copy()
+    plain["flags"] = 0
+    batched = sc.opts.copy()
+    batched["flags"] = FLAG_BATCHED | FLAG_GRAPH
+    out = {"workload": "16 views of the cornell scene, 256x256..1920x1080, spp 4/16/64, depth 8, RR from 3",
+           "samples": int(sum(int(v["w"]) * int(v["h"]) * int(v["spp"]) for v in sc.views))}
+    with Context(0) as ctx:
+        ctx.upload(sc)
+        res = {}
+        for name, o in (("plain", plain), ("batched", batched)):
+            if name == "batched" and tune:
+                t = autotune.tune(ctx, sc.views, o, repeats=1,
+                                  log=lambda m: print(m, file=sys.stderr, flush=True))
+                out["tuned"] = t["params"]
+            ctx.render(sc.views, o)
+            rays = 0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.render(sc.views, o)
+                c = ctx.counters()
+                rays += c["rays_closest"] + c["rays_shadow"]
+            dt = time.perf_counter() - t0
+            res[name] = {"mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt / steps * 1e3, 2),
+                         "image": [im.copy() for im in ctx.images()[:3]]}
+            print(f"[bench] batching leg {name}: {res[name]['mrays_per_s']} Mrays/s", file=sys.stderr, flush=True)
+        same = all(np.array_equal(a, b) for a, b in zip(res["plain"]["image"], res["batched"]["image"]))
+    for k in res:
+        res[k].pop("image")
+    out.update(res)
+    out["speedup"] = round(res["batched"]["mrays_per_s"] / res["plain"]["mrays_per_s"], 3)
+    out["images_identical"] = bool(same)
+    return out
 
 
 def cpu_baseline(scene, bvh2):
@@ -137,8 +187,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": base,
     }
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if not a.no_batching:
+        out["batching"] = batching_leg(max(1, min(a.steps, 3)), not a.no_tune)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

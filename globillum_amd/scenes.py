@@ -283,12 +283,33 @@ def save_scene(scene: Scene, path: str) -> None:
             np.ascontiguousarray(a, dt).tofile(f)
 
 
+def cornell_batch(seed: int = 3, n_views: int = 16, depth: int = 8, rr_start: int = 3,
+                  resolutions=((256, 256), (640, 480), (1280, 720), (1920, 1080)), spps=(4, 16, 64)) -> Scene:
+    """The cornell_boxes scene seen by n_views random cameras (position in front
+    of the opening, look-at inside the box, fovy U[30,50]), each with a
+    resolution and spp drawn from the given sets; depth 8 with Russian
+    roulette from 3 segments on."""
+    base = cornell_boxes()
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n_views):
+        w, h = resolutions[int(rng.integers(0, len(resolutions)))]
+        spp = int(spps[int(rng.integers(0, len(spps)))])
+        pos = (rng.uniform(0.25, 0.75), rng.uniform(0.25, 0.75), rng.uniform(-1.6, -0.9))
+        at = (rng.uniform(0.35, 0.65), rng.uniform(0.25, 0.6), rng.uniform(0.4, 0.8))
+        views.append(make_view(pos, at, (0, 1, 0), rng.uniform(30.0, 50.0), w, h, spp))
+    return Scene(base.tris, base.mats, base.lights, np.concatenate(views), make_opts(depth, rr_start=rr_start),
+                 name="cornell_batch")
+
+
 def config_scene(idx: int, **kw) -> Scene:
     """Scenes of the benchmark configurations by number (1-based)."""
     if idx == 1:
         return cornell_boxes(**kw)
     if idx == 2:
         return terrain_spheres(**kw)
+    if idx == 3:
+        return cornell_batch(**kw)
     if idx == 4:
         args = dict(seed=4, cells=(200, 250), n_spheres=50, n_mats=64, n_lights=4, w=512, h=512, spp=16, depth=4)
         args.update(kw)
