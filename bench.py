@@ -38,6 +38,7 @@ def parse():
     p.add_argument("--graph", action="store_true", help="replay the render as one hipGraph")
     p.add_argument("--no-batching", action="store_true", help="skip the multi-view batching leg")
     p.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batching leg")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     return p.parse_args()
 
 
@@ -86,6 +87,46 @@ def batching_leg(steps: int, tune: bool):
     out.update(res)
     out["speedup"] = round(res["batched"]["mrays_per_s"] / res["plain"]["mrays_per_s"], 3)
     out["images_identical"] = bool(same)
+    return out
+
+
+def adjoint_leg(steps: int):
+    """Differentiation workload: heightfield + spheres (108k triangles), 64
+    principled materials, 4 lights, 512x512, 16 spp, depth 4; loss = image L2
+    against a target rendered with parameters perturbed by N(0, 0.1^2).  One
+    gradient = the full vector (64 x (rgb, roughness, metallic) + 4 x rgb
+    emission) of one loss evaluation."""
+    import numpy as np
+
+    from globillum_amd import scenes
+    from globillum_amd.context import Context
+
+    sc = scenes.config_scene(4)
+    tgt = scenes.perturbed(sc, seed=0)
+    with Context(0) as ctx:
+        ctx.upload(tgt)
+        timg = ctx.render().image(0).reshape(-1, 3)
+        ctx.set_materials(sc.mats)
+        ctx.set_lights(sc.lights)
+        img = ctx.render().image(0).reshape(-1, 3)
+        dL = (2.0 * (img - timg) / img.size).astype(np.float32)
+        ctx.adjoint(dL)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.render()
+        t_fwd = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            gm, ge = ctx.adjoint(dL)
+        t_adj = (time.perf_counter() - t0) / steps
+    n_params = gm.size + ge.size
+    out = {"workload": "diff_scene 108k triangles, 64 principled materials, 4 lights, 512x512, 16 spp, depth 4",
+           "params_per_gradient": int(n_params), "ms_forward": round(t_fwd * 1e3, 2),
+           "ms_adjoint": round(t_adj * 1e3, 2), "adjoint_over_forward": round(t_adj / t_fwd, 3),
+           "grads_per_s": round(1.0 / t_adj, 2), "loss": float(((img - timg) ** 2).mean()),
+           "grad_finite": bool(np.isfinite(gm).all() and np.isfinite(ge).all())}
+    print(f"[bench] adjoint leg: fwd {out['ms_forward']} ms, adj {out['ms_adjoint']} ms", file=sys.stderr,
+          flush=True)
     return out
 
 
@@ -190,6 +231,8 @@ def main():
     ctx.close()
     if not a.no_batching:
         out["batching"] = batching_leg(max(1, min(a.steps, 3)), not a.no_tune)
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_leg(max(1, min(a.steps, 3)))
     print(json.dumps(out), flush=True)
 
 

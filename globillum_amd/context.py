@@ -121,6 +121,30 @@ class Context:
         self._ck(self.lib.gi_progress_step(self.h, C.byref(done)))
         return bool(done.value)
 
+    def adjoint(self, dL_dimage):
+        """Gradient of sum(dL_dimage * image) over the views of the last
+        render (concatenated, 3 floats per pixel): (d_mats (n_mats, 5) as
+        color rgb / roughness / metallic, d_emission (n_lights, 3))."""
+        from ._lib import MATGRAD
+        dL = np.ascontiguousarray(dL_dimage, np.float32).reshape(-1)
+        nm = len(self.scene.mats)
+        nl = len(self.scene.lights)
+        dm = np.zeros(nm, MATGRAD)
+        de = np.zeros((nl, 3), np.float32)
+        self._ck(self.lib.gi_adjoint(self.h, ptr(dL), ptr(dm), ptr(de)))
+        g = np.concatenate([dm["d_color"], dm["d_roughness"][:, None], dm["d_metallic"][:, None]], 1)
+        return g, de
+
+    def set_materials(self, mats):
+        mats = np.ascontiguousarray(mats, MATERIAL)
+        self._ck(self.lib.gi_upload_materials(self.h, ptr(mats), len(mats)))
+        self.scene.mats = mats
+
+    def set_lights(self, lights):
+        lights = np.ascontiguousarray(lights, LIGHT)
+        self._ck(self.lib.gi_upload_lights(self.h, ptr(lights), len(lights)))
+        self.scene.lights = lights
+
     def counters(self) -> dict:
         c = np.zeros(1, COUNTERS)
         self._ck(self.lib.gi_get_counters(self.h, ptr(c)))

@@ -44,7 +44,8 @@ struct alignas(16) DevView {
   int32_t w, h, spp;
   uint32_t index_base;
   float4 *image;
-  int32_t pad[2];
+  uint32_t pix_base;  // offset of the view's pixels in views-concatenated images
+  int32_t pad;
 };
 
 constexpr int kMaxChunks = 64;
@@ -231,7 +232,8 @@ __global__ __launch_bounds__(256) void k_trace_any(const uint32_t *__restrict__ 
                                                    const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
                                                    uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes,
                                                    const float4 *__restrict__ itris, uint32_t *__restrict__ gstack,
-                                                   uint32_t stack_cap, uint32_t *overflow) {
+                                                   uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
+                                                   uint32_t rec_stride, int32_t b) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
@@ -250,6 +252,10 @@ __global__ __launch_bounds__(256) void k_trace_any(const uint32_t *__restrict__ 
         l.y = l.y + c.y;
         l.z = l.z + c.z;
         path_l[slot] = l;
+        if (rec) {
+          float4 &r = rec[(uint32_t)b * rec_stride + slot];
+          r = make_float4(c.x, c.y, c.z, r.w);
+        }
       }
     } else {
       occluded[i] = occ ? 1 : 0;
@@ -298,7 +304,8 @@ __device__ __forceinline__ int find_chunk(const PassDesc *pass, uint32_t slot) {
 // the throughput / colour reset, src/gpu-pt.h:110-118).
 __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pass, const DevView *__restrict__ views,
                                                 HaltonDev H, float4 *__restrict__ ray_o, float4 *__restrict__ ray_d,
-                                                float4 *__restrict__ tp, float4 *__restrict__ path_l) {
+                                                float4 *__restrict__ tp, float4 *__restrict__ path_l,
+                                                float4 *__restrict__ rec, uint32_t rec_stride, int32_t depth) {
   const uint32_t total = pass->total_slots;
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
     const PassChunk &ck = pass->chunk[find_chunk(pass, slot)];
@@ -314,15 +321,103 @@ __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pas
     ray_d[slot] = make_float4(d.x, d.y, d.z, bitsf(slot));
     tp[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
     path_l[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (rec)
+      for (int32_t b = 0; b < depth; ++b) rec[(uint32_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
-// Shading of path vertex b: NEE (pixel_generate_arealight_sample_cpu,
-// src/arealight-sampler-cpu.cpp:25-119) and bounce
-// (compute_path_contribution_and_bounce, src/hybrid-pt.cpp:450-577) fused.
-// The NEE contribution is evaluated here with the lobe the bounce sample
-// selects (as evaluate_material does after sample_material) and travels
-// with the shadow ray; the shadow kernel adds it when unoccluded.
+// One path vertex: NEE (pixel_generate_arealight_sample_cpu,
+// src/arealight-sampler-cpu.cpp:25-119) and the BRDF sample of the bounce
+// (compute_path_contribution_and_bounce, src/hybrid-pt.cpp:450-577).  The
+// NEE contribution is evaluated with the lobe the bounce sample selects (as
+// evaluate_material does after sample_material).  Shared by the forward
+// shading kernel and the adjoint replay, so both see identical values.
+struct VertexEval {
+  f3 P, N, org_dir, wi, wo_ts, wi_ts, xp;
+  f3 sd, so, psc, fnee;
+  float len, gk, pdf;
+  int type, light;
+  uint32_t mat;
+  bool glass;
+};
+
+__device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, int dim, f3 org_dir, float4 h,
+                                            VertexEval &ve) {
+  const ShadeTri &T = P.tris[fbits(h.w)];
+  const float beta = h.y, gamma = h.z;
+  const f3 bc = f3{(1.0f - beta) - gamma, beta, gamma};
+  const f3 Pp = add(add(scl(xyz(T.a), bc.x), scl(xyz(T.b), bc.y)), scl(xyz(T.c), bc.z));
+  const f3 Ni = add(add(scl(xyz(T.na), bc.x), scl(xyz(T.nb), bc.y)), scl(xyz(T.nc), bc.z));
+  ve.mat = fbits(T.a.w);
+  const DevMat &m = P.mats[ve.mat];
+  ve.org_dir = org_dir;
+  ve.P = Pp;
+
+  // NEE: power-proportional light choice, uniform point on the rect
+  const f3 xl = f3{halton(n, dim + 0, P.H), halton(n, dim + 1, P.H), halton(n, dim + 2, P.H)};
+  const float choice = xl.z * P.power_total;
+  float acc = 0.0f;
+  int k = 0;
+  while (k < P.nlights && choice > acc + P.lights[k].power) {
+    acc += P.lights[k].power;
+    ++k;
+  }
+  if (k >= P.nlights) k = P.nlights - 1;
+  ve.light = k;
+  const DevLight &lt = P.lights[k];
+  const f3 ldir = ld3(lt.dir);
+  const float offx = (xl.x - 0.5f) * lt.w, offy = (xl.y - 0.5f) * lt.h;
+  const f3 ls = add(add(ld3(lt.center), scl(ld3(lt.right), offx)), scl(ld3(lt.up), offy));
+  f3 sd = sub(ls, Pp);
+  const float len = length(sd);
+  sd = divs(sd, len);
+  const float ndotl = smax(dot(Ni, sd), 0.0f);
+  const float lcos = smax(dot(ldir, neg(sd)), 0.0f);
+  const float factor = (((lt.w * lt.h) * ndotl) * lcos) / (len * len);
+  const f3 contrib = scl(ld3(lt.col), factor);
+  sd = normalize(sd);
+  ve.sd = sd;
+  ve.len = len;
+  ve.so = add(Pp, scl(sd, P.off_s));
+  const float sel = P.power_total / lt.power;
+  ve.psc = scl(contrib, sel);
+  ve.gk = factor * sel;
+
+  // bounce: tangent frame, normal flip, BRDF sample
+  f3 N = normalize(Ni);
+  f3 Tt, Bt;
+  tangent_frame(N, Tt, Bt);
+  if (dot(org_dir, N) > 0.0f) N = neg(N);
+  ve.N = N;
+  const f3 inv = neg(org_dir);
+  ve.wo_ts = f3{dot(inv, Tt), dot(inv, Bt), dot(inv, N)};
+  ve.xp = f3{halton(n, dim + 3, P.H), halton(n, dim + 4, P.H), halton(n, dim + 5, P.H)};
+  ve.pdf = 1.0f;
+  ve.type = brdf_sample(m, ve.wo_ts, ve.xp, ve.wi_ts, ve.pdf);
+  ve.wi = normalize(add(add(scl(Tt, ve.wi_ts.x), scl(Bt, ve.wi_ts.y)), scl(N, ve.wi_ts.z)));
+  ve.glass = m.principled && m.opacity < 1.0f;
+  ve.fnee = ve.glass ? f3{0.f, 0.f, 0.f} : brdf_eval(m, ve.type, inv, normalize(sd), N);
+}
+
+// Next throughput: min(1, TP * f / pdf) per channel (src/hybrid-pt.cpp:559-562);
+// clamp bit c set where channel c was clamped.
+__device__ __forceinline__ f3 next_throughput(f3 TP, f3 f, float pdf, uint32_t &clamp) {
+  f3 t = f3{(TP.x * f.x) / pdf, (TP.y * f.y) / pdf, (TP.z * f.z) / pdf};
+  clamp = 0;
+  if (t.x > 1.0f) { t.x = 1.0f; clamp |= 1u; }
+  if (t.y > 1.0f) { t.y = 1.0f; clamp |= 2u; }
+  if (t.z > 1.0f) { t.z = 1.0f; clamp |= 4u; }
+  return t;
+}
+
+// Per-vertex record of the adjoint's forward pass: the radiance the vertex
+// adds to its path (NEE if unoccluded, background on a white miss) and flag
+// bits (0-2: throughput clamp per channel, 3: background term).
+constexpr uint32_t kRecBackground = 8u;
+
+// Shading of path vertex b.  The NEE contribution travels with the shadow
+// ray; the shadow kernel adds it when unoccluded.  rec != nullptr: also
+// record the vertex for the adjoint (see kRecBackground).
 __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__restrict__ count_ptr, ShadeParams P,
                                                const float4 *__restrict__ ray_o, const float4 *__restrict__ ray_d,
                                                const float4 *__restrict__ tp_in, const float4 *__restrict__ hits,
@@ -330,7 +425,8 @@ __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__rest
                                                float4 *__restrict__ nrd, float4 *__restrict__ ntp,
                                                uint32_t *__restrict__ next_count, float4 *__restrict__ sho,
                                                float4 *__restrict__ shd, float4 *__restrict__ shc,
-                                               uint32_t *__restrict__ shadow_count) {
+                                               uint32_t *__restrict__ shadow_count, float4 *__restrict__ rec,
+                                               uint32_t rec_stride) {
   const uint32_t count = *count_ptr;
   const int dim = kDimVertex0 + kDimsPerVertex * b;
   const bool last = b + 1 >= P.depth;
@@ -347,73 +443,31 @@ __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__rest
         // handle_no_hit with NO_BACKGROUND (src/hybrid-pt.cpp:196-206,567-574)
         if (TP.x == 1.0f && TP.y == 1.0f && TP.z == 1.0f) {
           float4 l = path_l[slot];
-          l.x = l.x + TP.x * P.bg[0];
-          l.y = l.y + TP.y * P.bg[1];
-          l.z = l.z + TP.z * P.bg[2];
+          const float bx = TP.x * P.bg[0], by = TP.y * P.bg[1], bz = TP.z * P.bg[2];
+          l.x = l.x + bx;
+          l.y = l.y + by;
+          l.z = l.z + bz;
           path_l[slot] = l;
+          if (rec) rec[(uint32_t)b * rec_stride + slot] = make_float4(bx, by, bz, bitsf(kRecBackground));
         }
       } else {
-        const ShadeTri &T = P.tris[id];
-        const float beta = h.y, gamma = h.z;
-        const f3 bc = f3{(1.0f - beta) - gamma, beta, gamma};
-        const f3 Pp = add(add(scl(xyz(T.a), bc.x), scl(xyz(T.b), bc.y)), scl(xyz(T.c), bc.z));
-        const f3 Ni = add(add(scl(xyz(T.na), bc.x), scl(xyz(T.nb), bc.y)), scl(xyz(T.nc), bc.z));
-        const DevMat &m = P.mats[fbits(T.a.w)];
-
-        // ---- NEE: power-proportional light choice, point on the rect
-        const f3 xl = f3{halton(n, dim + 0, P.H), halton(n, dim + 1, P.H), halton(n, dim + 2, P.H)};
-        const float choice = xl.z * P.power_total;
-        float acc = 0.0f;
-        int k = 0;
-        while (k < P.nlights && choice > acc + P.lights[k].power) {
-          acc += P.lights[k].power;
-          ++k;
-        }
-        if (k >= P.nlights) k = P.nlights - 1;
-        const DevLight &lt = P.lights[k];
-        const f3 ldir = ld3(lt.dir);
-        const float offx = (xl.x - 0.5f) * lt.w, offy = (xl.y - 0.5f) * lt.h;
-        const f3 ls = add(add(ld3(lt.center), scl(ld3(lt.right), offx)), scl(ld3(lt.up), offy));
-        f3 sd = sub(ls, Pp);
-        const float len = length(sd);
-        sd = divs(sd, len);
-        const float ndotl = smax(dot(Ni, sd), 0.0f);
-        const float lcos = smax(dot(ldir, neg(sd)), 0.0f);
-        const float factor = (((lt.w * lt.h) * ndotl) * lcos) / (len * len);
-        const f3 contrib = scl(ld3(lt.col), factor);
-        sd = normalize(sd);
-        const f3 so = add(Pp, scl(sd, P.off_s));
-        const f3 psc = scl(contrib, P.power_total / lt.power);
-
-        // ---- bounce: tangent frame, normal flip, BRDF sample
-        f3 N = normalize(Ni);
-        f3 Tt, Bt;
-        tangent_frame(N, Tt, Bt);
-        if (dot(org_dir, N) > 0.0f) N = neg(N);
-        const f3 inv = neg(org_dir);
-        const f3 wo_ts = f3{dot(inv, Tt), dot(inv, Bt), dot(inv, N)};
-        const f3 xp = f3{halton(n, dim + 3, P.H), halton(n, dim + 4, P.H), halton(n, dim + 5, P.H)};
-        f3 wi_ts;
-        float pdf = 1.0f;
-        const int type = brdf_sample(m, wo_ts, xp, wi_ts, pdf);
-        const f3 wi = normalize(add(add(scl(Tt, wi_ts.x), scl(Bt, wi_ts.y)), scl(N, wi_ts.z)));
-        const bool glass = m.principled && m.opacity < 1.0f;
-        if (!glass) {
-          const f3 C = mul(brdf_eval(m, type, inv, normalize(sd), N), mul(TP, psc));
+        VertexEval ve;
+        eval_vertex(P, n, dim, org_dir, h, ve);
+        const DevMat &m = P.mats[ve.mat];
+        if (!ve.glass) {
+          const f3 C = mul(ve.fnee, mul(TP, ve.psc));
           if (C.x > 0.0f || C.y > 0.0f || C.z > 0.0f) {
             emit_shadow = true;
-            s_o = make_float4(so.x, so.y, so.z, len);
-            s_d = make_float4(sd.x, sd.y, sd.z, bitsf(slot));
+            s_o = make_float4(ve.so.x, ve.so.y, ve.so.z, ve.len);
+            s_d = make_float4(ve.sd.x, ve.sd.y, ve.sd.z, bitsf(slot));
             s_c = make_float4(C.x, C.y, C.z, 0.0f);
           }
         }
+        uint32_t clamp = 0;
         if (!last) {
-          const f3 f = brdf_eval(m, type, inv, wi, N);
-          const f3 np = glass ? add(Pp, scl(org_dir, P.off_b)) : add(Pp, scl(N, P.off_b));
-          f3 t = f3{(TP.x * f.x) / pdf, (TP.y * f.y) / pdf, (TP.z * f.z) / pdf};
-          if (t.x > 1.0f) t.x = 1.0f;
-          if (t.y > 1.0f) t.y = 1.0f;
-          if (t.z > 1.0f) t.z = 1.0f;
+          const f3 f = brdf_eval(m, ve.type, neg(org_dir), ve.wi, ve.N);
+          const f3 np = ve.glass ? add(ve.P, scl(org_dir, P.off_b)) : add(ve.P, scl(ve.N, P.off_b));
+          f3 t = next_throughput(TP, f, ve.pdf, clamp);
           bool alive = true;
           if (P.rr_start > 0 && b + 1 >= P.rr_start) {
             const float q = smax(t.x, smax(t.y, t.z));
@@ -424,10 +478,11 @@ __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__rest
           if (alive) {
             emit_next = true;
             o_next = make_float4(np.x, np.y, np.z, bitsf(n));
-            d_next = make_float4(wi.x, wi.y, wi.z, bitsf(slot));
+            d_next = make_float4(ve.wi.x, ve.wi.y, ve.wi.z, bitsf(slot));
             tp_next = make_float4(t.x, t.y, t.z, 0.0f);
           }
         }
+        if (rec) rec[(uint32_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, bitsf(clamp));
       }
     }
     const uint32_t qn = wave_append(emit_next, next_count);
@@ -441,6 +496,252 @@ __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__rest
       sho[qs] = s_o;
       shd[qs] = s_d;
       shc[qs] = s_c;
+    }
+  }
+}
+
+// ------------------------------------------------------------- adjoint ----
+// Derivatives d f / d (color r, g, b, roughness, metallic) of the BRDF value
+// for the lobe `type`, for the frozen directions wo, wi.  Lambert: d/d rho.
+struct alignas(16) DevMatGrad {
+  float dcd[3];   // d Cdlin / d color = 2.2 color^1.2
+  float lum;      // .3 Cd.r + .6 Cd.g + .1 Cd.b
+  float tint[3];  // Cdlin / lum
+  float spec_tint;
+  float sheen_tint, aspect, pad0, pad1;
+};
+
+__device__ __forceinline__ float f3c(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ __forceinline__ float dtint(const DevMatGrad &g, int c, int i) {
+  if (!(g.lum > 0.0f)) return 0.0f;
+  const float w = i == 0 ? 0.3f : (i == 1 ? 0.6f : 0.1f);
+  return ((c == i ? 1.0f : 0.0f) - g.tint[c] * w) / g.lum;
+}
+
+__device__ __forceinline__ void brdf_grad(const DevMat &m, const DevMatGrad &g, int type, f3 wo, f3 wi, f3 N,
+                                          float df[5][3]) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) df[k][0] = df[k][1] = df[k][2] = 0.0f;
+  if (!m.principled) {
+    const float a = fabsf(dot(wi, N)) * kInvPi;
+    df[0][0] = a;
+    df[1][1] = a;
+    df[2][2] = a;
+    return;
+  }
+  const float ndl = dot(N, wi), ndv = dot(N, wo);
+  if (ndl < 0.0f || ndv < 0.0f) return;
+  const f3 H = normalize(add(wi, wo));
+  const float ldh = dot(wi, H);
+  if (type == kTypeDisneyRefl) {
+    const float ndh = dot(N, H);
+    const float fh = schlick(ldh), mm = m.metallic, sp08 = m.specular * 0.08f, st = g.spec_tint;
+    float cg = 0.0f;
+    if (m.clearcoat > 0.0f) {
+      const float dr = gtr1(ndh, m.cc_a, m.cc_log_a2);
+      const float fr = mixf(0.04f, 1.0f, fh);
+      const float gr = smith_ggx(ndl, 0.25f) * smith_ggx(ndv, 0.25f);
+      cg = 0.25f * m.clearcoat * gr * fr * dr;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float mt = (1.0f - st) + st * g.tint[c];
+      const float cs0 = (1.0f - mm) * sp08 * mt + mm * m.cdlin[c];
+      const float fs = (1.0f - fh) * cs0 + fh;
+      df[4][c] = (fs + cg) * ndl + mm * ndl * (1.0f - fh) * (m.cdlin[c] - sp08 * mt);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float dcs0 = mm * (c == i ? 1.0f : 0.0f) + (1.0f - mm) * sp08 * st * dtint(g, c, i);
+        df[i][c] = mm * ndl * (1.0f - fh) * dcs0 * g.dcd[i];
+      }
+    }
+    return;
+  }
+  if (type != kTypeDiffuse) return;
+  const float fl = schlick(ndl), fv = schlick(ndv), r = m.roughness, mm = m.metallic, sub = m.subsurface;
+  const float fd90 = 0.5f + 2.0f * ldh * ldh * r;
+  const float fd = mixf(1.0f, fd90, fl) * mixf(1.0f, fd90, fv);
+  const float fh = schlick(ldh);
+  const float fss90 = ldh * ldh * r;
+  const float fss = mixf(1.0f, fss90, fl) * mixf(1.0f, fss90, fv);
+  const float q = 1.0f / (ndl + ndv) - 0.5f;
+  const float ss = 1.25f * (fss * q + 0.5f);
+  const float k = kInvPi * mixf(fd, ss, sub);
+  const float dfd = 2.0f * ldh * ldh * (fl * mixf(1.0f, fd90, fv) + mixf(1.0f, fd90, fl) * fv);
+  const float dss = 1.25f * q * ldh * ldh * (fl * mixf(1.0f, fss90, fv) + mixf(1.0f, fss90, fl) * fv);
+  const float dk = kInvPi * ((1.0f - sub) * dfd + sub * dss);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float csh = (1.0f - g.sheen_tint) + g.sheen_tint * g.tint[c];
+    df[3][c] = dk * m.cdlin[c] * (1.0f - mm) * ndl;
+    df[4][c] = -(k * m.cdlin[c] + fh * m.sheen * csh) * ndl;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      df[i][c] = (1.0f - mm) * ndl * (k * (c == i ? 1.0f : 0.0f) + fh * m.sheen * g.sheen_tint * dtint(g, c, i)) *
+                 g.dcd[i];
+  }
+}
+
+// GGX roughness derivatives of a sampled specular lobe: d log(lweight)/dr
+// (the reference's specular "pdf", src/principledMaterial.h:158-175) and
+// the score d log D(h)/dr of the half-vector density D(h) cos(theta_h).
+__device__ __forceinline__ void gtr2_grads(const DevMat &m, const DevMatGrad &g, f3 V, f3 L, f3 h, float &dlogw,
+                                           float &score) {
+  const float r = m.roughness, r2 = r * r, aspect = g.aspect;
+  const float ax = m.ax, ay = m.ay;
+  const float dax = (r2 / aspect > 0.001f) ? 2.0f * r / aspect : 0.0f;
+  const float day = (r2 * aspect > 0.001f) ? 2.0f * r * aspect : 0.0f;
+  const float qq = ax * ay * 0.5f + 0.5f;
+  const float drg2 = 4.0f * qq * qq * qq * 0.5f * (dax * ay + ax * day);
+  const float dlm = dot(L, h), dvm = dot(V, h);
+  const float tl = sqrtf(1.0f - dlm * dlm) / dlm, tv = sqrtf(1.0f - dvm * dvm) / dvm;
+  const float xl = (dot(L, h) / L.z) > 0.0f ? 1.0f : 0.0f, xv = (dot(V, h) / V.z) > 0.0f ? 1.0f : 0.0f;
+  const float sl = sqrtf(1.0f + m.roughg2 * tl * tl), sv = sqrtf(1.0f + m.roughg2 * tv * tv);
+  const float g1l = xl * 2.0f / (1.0f + sl), g1v = xv * 2.0f / (1.0f + sv);
+  const float lw = (fabsf(dot(L, h)) * g1l * g1v) / (fabsf(L.z) * fabsf(h.z));
+  dlogw = 0.0f;
+  if (lw >= 0.01f && g1l > 0.0f && g1v > 0.0f) {
+    const float dg1l = -(1.0f / (1.0f + sl)) * (tl * tl / (2.0f * sl));
+    const float dg1v = -(1.0f / (1.0f + sv)) * (tv * tv / (2.0f * sv));
+    dlogw = (dg1l + dg1v) * drg2;
+  }
+  const float Q = h.x * h.x / (ax * ax) + h.y * h.y / (ay * ay) + h.z * h.z;
+  score = (-1.0f / ax + 4.0f * h.x * h.x / (ax * ax * ax * Q)) * dax +
+          (-1.0f / ay + 4.0f * h.y * h.y / (ay * ay * ay * Q)) * day;
+}
+
+struct AdjParams {
+  const float *dL;          // adjoint image, 3 floats per pixel, views concatenated
+  const float4 *rec;        // per-vertex records of the forward pass
+  uint32_t rec_stride;
+  const DevMatGrad *mg;
+  float *g_mats;            // 5 per material
+  float *g_em;              // 3 per light
+  int32_t nmats, nlights;
+  const PassDesc *pass;
+  const DevView *views;
+};
+
+// Adjoint replay of path vertex b: the same hit, NEE point and BRDF sample
+// as the forward pass (same Halton indices), the visibility read off the
+// recorded contribution, and the gradient terms of the frozen-sample
+// estimator (NEE, throughput chain up to the next clamp, score of the lobe
+// choice and of the GGX half-vector, emission).  Gradients gather in LDS per
+// block and go to global memory with one atomic per parameter per block.
+__global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t *__restrict__ count_ptr,
+                                                       ShadeParams P, AdjParams A, const float4 *__restrict__ ray_o,
+                                                       const float4 *__restrict__ ray_d,
+                                                       const float4 *__restrict__ tp_in,
+                                                       const float4 *__restrict__ hits, float4 *__restrict__ nro,
+                                                       float4 *__restrict__ nrd, float4 *__restrict__ ntp,
+                                                       uint32_t *__restrict__ next_count) {
+  extern __shared__ float sg[];  // 5*nmats + 3*nlights
+  const int ng = 5 * A.nmats + 3 * A.nlights;
+  for (int k = threadIdx.x; k < ng; k += blockDim.x) sg[k] = 0.0f;
+  __syncthreads();
+  const uint32_t count = *count_ptr;
+  const int dim = kDimVertex0 + kDimsPerVertex * b;
+  const bool last = b + 1 >= P.depth;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    bool emit_next = false;
+    float4 o_next, d_next, tp_next;
+    if (i < count) {
+      const float4 ro = ray_o[i], rd = ray_d[i], tpv = tp_in[i], h = hits[i];
+      const uint32_t n = fbits(ro.w), slot = fbits(rd.w), id = fbits(h.w);
+      const f3 TP = xyz(tpv);
+      const f3 org_dir = normalize(xyz(rd));
+      if (id != GI_INVALID_REF) {
+        VertexEval ve;
+        eval_vertex(P, n, dim, org_dir, h, ve);
+        const DevMat &m = P.mats[ve.mat];
+        const DevMatGrad &mg = A.mg[ve.mat];
+        const f3 inv = neg(org_dir);
+        // adjoint weight of the path's pixel: dL/dI / spp
+        const PassChunk &ck = A.pass->chunk[find_chunk(A.pass, slot)];
+        const DevView &v = A.views[ck.view];
+        const uint32_t p = (slot - ck.slot_base) % (uint32_t)ck.npix;
+        const float *w = A.dL + 3 * ((size_t)v.pix_base + p);
+        const float inv_spp = 1.0f / (float)v.spp;
+        const f3 delta = f3{w[0] * inv_spp, w[1] * inv_spp, w[2] * inv_spp};
+        // recorded contributions from this vertex on: S (all) and R (after
+        // this vertex, up to the next throughput clamp per channel)
+        const float4 r0 = A.rec[(uint32_t)b * A.rec_stride + slot];
+        f3 S = xyz(r0);
+        f3 R = f3{0.f, 0.f, 0.f};
+        uint32_t open = 7u & ~fbits(r0.w);  // channels whose chain continues past b
+        for (int bb = b + 1; bb < P.depth; ++bb) {
+          const float4 r = A.rec[(uint32_t)bb * A.rec_stride + slot];
+          S = add(S, xyz(r));
+          const uint32_t fl = fbits(r.w);
+          if (!(fl & kRecBackground)) {
+            if (open & 1u) R.x += r.x;
+            if (open & 2u) R.y += r.y;
+            if (open & 4u) R.z += r.z;
+          }
+          open &= ~fl;
+        }
+        const bool vis = r0.x != 0.0f || r0.y != 0.0f || r0.z != 0.0f;
+        float dfn[5][3], dfb[5][3];
+        float score[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, dlogw = 0.0f;
+        if (m.principled && !ve.glass) {
+          score[4] = ve.xp.z < m.metallic ? 1.0f / m.metallic : -1.0f / (1.0f - m.metallic);
+          if (ve.type == kTypeDisneyRefl) {
+            float sn, cs;
+            sincos2pi(ve.xp.x, sn, cs);
+            const float fac = sqrtf(ve.xp.y / (1.0f - ve.xp.y));
+            const f3 hv = normalize(f3{fac * (m.ax * cs), fac * (m.ay * sn), 1.0f});
+            gtr2_grads(m, mg, ve.wo_ts, ve.wi_ts, hv, dlogw, score[3]);
+          }
+        }
+        if (!ve.glass) brdf_grad(m, mg, ve.type, inv, normalize(ve.sd), ve.N, dfn);
+        f3 f = f3{0.f, 0.f, 0.f};
+        uint32_t clamp = 7u;
+        if (!last) {
+          f = brdf_eval(m, ve.type, inv, ve.wi, ve.N);
+          brdf_grad(m, mg, ve.type, inv, ve.wi, ve.N, dfb);
+          const f3 t = next_throughput(TP, f, ve.pdf, clamp);
+          const f3 np = ve.glass ? add(ve.P, scl(org_dir, P.off_b)) : add(ve.P, scl(ve.N, P.off_b));
+          emit_next = true;
+          o_next = make_float4(np.x, np.y, np.z, bitsf(n));
+          d_next = make_float4(ve.wi.x, ve.wi.y, ve.wi.z, bitsf(slot));
+          tp_next = make_float4(t.x, t.y, t.z, 0.0f);
+        }
+        const float sdot = dot(delta, S);
+        const float dl[3] = {delta.x, delta.y, delta.z};
+        const float Tc[3] = {TP.x, TP.y, TP.z}, Pc[3] = {ve.psc.x, ve.psc.y, ve.psc.z};
+        const float Rc[3] = {R.x, R.y, R.z}, Fc[3] = {f.x, f.y, f.z};
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          float gq = sdot * score[q];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            if (vis && !ve.glass) gq += dl[c] * dfn[q][c] * Tc[c] * Pc[c];
+            if (!last && Fc[c] != 0.0f && Rc[c] != 0.0f)
+              gq += dl[c] * Rc[c] * (dfb[q][c] / Fc[c] - (q == 3 ? dlogw : 0.0f));
+          }
+          if (gq != 0.0f) atomicAdd(&sg[5 * ve.mat + q], gq);
+        }
+        if (vis && !ve.glass) {
+          atomicAdd(&sg[5 * A.nmats + 3 * ve.light + 0], dl[0] * ve.fnee.x * Tc[0] * ve.gk);
+          atomicAdd(&sg[5 * A.nmats + 3 * ve.light + 1], dl[1] * ve.fnee.y * Tc[1] * ve.gk);
+          atomicAdd(&sg[5 * A.nmats + 3 * ve.light + 2], dl[2] * ve.fnee.z * Tc[2] * ve.gk);
+        }
+      }
+    }
+    const uint32_t qn = wave_append(emit_next, next_count);
+    if (emit_next) {
+      nro[qn] = o_next;
+      nrd[qn] = d_next;
+      ntp[qn] = tp_next;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < ng; k += blockDim.x) {
+    const float v = sg[k];
+    if (v != 0.0f) {
+      if (k < 5 * A.nmats) atomicAdd(&A.g_mats[k], v);
+      else atomicAdd(&A.g_em[k - 5 * A.nmats], v);
     }
   }
 }
@@ -549,6 +850,7 @@ struct gi_ctx {
   gi_render_opts opts{};
   std::vector<DBuf> images;
   DBuf d_views, d_passes, d_counters;
+  DBuf d_matgrad, d_rec, d_dl, d_gmats, d_gem;
   DBuf q_o[2], q_d[2], q_tp[2], q_hit, s_o, s_d, s_c, path_l;
   size_t slot_cap = 0;
   // primitives
@@ -651,6 +953,25 @@ DevMat make_devmat(const gi_material &m) {
   return d;
 }
 
+// Host-side quantities of the adjoint's material derivatives.
+DevMatGrad make_devmatgrad(const gi_material &m) {
+  DevMatGrad g;
+  std::memset(&g, 0, sizeof(g));
+  float cd[3];
+  for (int k = 0; k < 3; ++k) {
+    cd[k] = std::pow(m.color[k], 2.2f);
+    g.dcd[k] = m.color[k] > 0.f ? 2.2f * std::pow(m.color[k], 1.2f) : 0.f;
+  }
+  if (!m.principled)
+    for (int k = 0; k < 3; ++k) g.dcd[k] = 1.0f;  // Lambert: the parameter is rho itself
+  g.lum = 0.3f * cd[0] + 0.6f * cd[1] + 0.1f * cd[2];
+  for (int k = 0; k < 3; ++k) g.tint[k] = g.lum > 0.f ? cd[k] / g.lum : 1.f;
+  g.spec_tint = m.specular_tint;
+  g.sheen_tint = m.sheen_tint;
+  g.aspect = std::sqrt(1.0f - m.anisotropic * 0.9f);
+  return g;
+}
+
 int ensure_halton(gi_ctx *ctx, uint32_t seed, int32_t ndims) {
   if (ctx->hal_dims >= ndims && ctx->hal_seed == seed && ctx->d_hdims.p) return GI_OK;
   std::vector<HaltonDim> dims;
@@ -735,11 +1056,13 @@ void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_pt
 
 template <bool kDeposit, int kLds>
 void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
-                  const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ) {
+                  const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ, float4 *rec,
+                  uint32_t rec_stride, int32_t b) {
   const size_t lds = (size_t)kLds * block * sizeof(uint32_t);
   hipLaunchKernelGGL((k_trace_any<kDeposit, kLds>), dim3(grid), dim3(block), lds, ctx->stream, count_ptr,
                      count_host, ro, rd, contrib, pl, occ, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
-                     ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+                     ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), rec, rec_stride,
+                     b);
 }
 
 int stack_lds_of(const gi_ctx *ctx) {
@@ -767,19 +1090,20 @@ int launch_closest(gi_ctx *ctx, bool has_tmax, const uint32_t *count_ptr, uint32
 }
 
 int launch_any(gi_ctx *ctx, bool deposit, const uint32_t *count_ptr, uint32_t count_host, uint64_t n_bound,
-               const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ) {
+               const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ,
+               float4 *rec = nullptr, uint32_t rec_stride = 0, int32_t b = 0) {
   int grid, block;
   int rc = trace_grid_for(ctx, n_bound, grid, block);
   if (rc) return rc;
   const int L = stack_lds_of(ctx);
   if (deposit) {
-    if (L == 8) launch_any_t<true, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
-    else if (L == 32) launch_any_t<true, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
-    else launch_any_t<true, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
+    if (L == 8) launch_any_t<true, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
+    else if (L == 32) launch_any_t<true, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
+    else launch_any_t<true, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
   } else {
-    if (L == 8) launch_any_t<false, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
-    else if (L == 32) launch_any_t<false, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
-    else launch_any_t<false, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ);
+    if (L == 8) launch_any_t<false, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
+    else if (L == 32) launch_any_t<false, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
+    else launch_any_t<false, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
   }
   GI_HIP(ctx, hipGetLastError());
   return GI_OK;
@@ -938,7 +1262,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
     const int rg = (int)std::min<uint64_t>((pd.total_slots + 255) / 256, (uint64_t)persist);
     hipLaunchKernelGGL(k_raygen, dim3(std::max(1, rg)), dim3(256), 0, ctx->stream, dp, dviews, SP.H,
                        ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->q_tp[0].as<float4>(),
-                       ctx->path_l.as<float4>());
+                       ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, depth);
     uint32_t host_count = pd.total_slots;
     for (int32_t b = 0; b < depth; ++b) {
       const int cur = b & 1, nxt = cur ^ 1;
@@ -959,7 +1283,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
                          ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(), ctx->q_hit.as<float4>(),
                          ctx->path_l.as<float4>(), ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(),
                          ctx->q_tp[nxt].as<float4>(), cnt + b + 1, ctx->s_o.as<float4>(), ctx->s_d.as<float4>(),
-                         ctx->s_c.as<float4>(), cnt + depth + 1 + b);
+                         ctx->s_c.as<float4>(), cnt + depth + 1 + b, (float4 *)nullptr, 0u);
       GI_HIP(ctx, hipGetLastError());
       if (timing && (rc = mark(2))) return rc;
       uint32_t shadow_host = 0;
@@ -1017,6 +1341,86 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
   return GI_OK;
 }
 
+// Adjoint: per pass, a forward pass in record mode (the per-vertex
+// contributions and clamp bits of every path) and the replay of the same
+// paths -- same Halton indices, closest-hit rays re-traced, no shadow rays --
+// that accumulates the gradients.  Device-side queue lengths, persistent
+// grids, no host synchronisation until the end.
+int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjParams A) {
+  const int32_t ncnt = 2 * depth + 1;
+  const size_t npass = plan.passes.size();
+  GI_HIP(ctx, ctx->d_passes.ensure(npass * sizeof(PassDesc)));
+  GI_HIP(ctx, hipMemcpyAsync(ctx->d_passes.p, plan.passes.data(), npass * sizeof(PassDesc), hipMemcpyHostToDevice,
+                             ctx->stream));
+  GI_HIP(ctx, ctx->d_counters.ensure(2 * npass * ncnt * sizeof(uint32_t)));
+  int rc = ensure_workspace(ctx, plan.max_slots);
+  if (rc) return rc;
+  if ((rc = ensure_gstack(ctx))) return rc;
+  GI_HIP(ctx, ctx->d_rec.ensure((size_t)depth * plan.max_slots * sizeof(float4)));
+  ShadeParams SP = shade_params(ctx, depth);
+  const PassDesc *dpass = ctx->d_passes.as<PassDesc>();
+  const DevView *dviews = ctx->d_views.as<DevView>();
+  float4 *rec = ctx->d_rec.as<float4>();
+  const uint32_t stride = (uint32_t)plan.max_slots;
+  A.rec = rec;
+  A.rec_stride = stride;
+  A.views = dviews;
+  const int sblock = std::max(64, (ctx->tuning.shade_block / 64) * 64);
+  const int persist = ctx->num_cus * 8;
+  const size_t lds_adj = (size_t)(5 * A.nmats + 3 * A.nlights) * sizeof(float);
+  for (size_t pi = 0; pi < npass; ++pi) {
+    const PassDesc &pd = plan.passes[pi];
+    const PassDesc *dp = dpass + pi;
+    A.pass = dp;
+    const int rg = (int)std::min<uint64_t>((pd.total_slots + 255) / 256, (uint64_t)persist);
+    const int sg = (int)std::max<uint64_t>(1, std::min<uint64_t>((pd.total_slots + sblock - 1) / sblock,
+                                                                 (uint64_t)persist));
+    for (int phase = 0; phase < 2; ++phase) {
+      uint32_t *cnt = ctx->d_counters.as<uint32_t>() + (2 * pi + phase) * ncnt;
+      hipLaunchKernelGGL(k_pass_setup, dim3(1), dim3(256), 0, ctx->stream, dp, cnt, ncnt);
+      hipLaunchKernelGGL(k_raygen, dim3(std::max(1, rg)), dim3(256), 0, ctx->stream, dp, dviews, SP.H,
+                         ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->q_tp[0].as<float4>(),
+                         ctx->path_l.as<float4>(), phase == 0 ? rec : (float4 *)nullptr, stride, depth);
+      for (int32_t b = 0; b < depth; ++b) {
+        const int cur = b & 1, nxt = cur ^ 1;
+        rc = launch_closest(ctx, false, cnt + b, 0, pd.total_slots, ctx->q_o[cur].as<float4>(),
+                            ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
+        if (rc) return rc;
+        if (phase == 0) {
+          hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b, cnt + b, SP,
+                             ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
+                             ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(), ctx->q_o[nxt].as<float4>(),
+                             ctx->q_d[nxt].as<float4>(), ctx->q_tp[nxt].as<float4>(), cnt + b + 1,
+                             ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>(),
+                             cnt + depth + 1 + b, rec, stride);
+          GI_HIP(ctx, hipGetLastError());
+          rc = launch_any(ctx, true, cnt + depth + 1 + b, 0, pd.total_slots, ctx->s_o.as<float4>(),
+                          ctx->s_d.as<float4>(), ctx->s_c.as<float4>(), ctx->path_l.as<float4>(), nullptr, rec,
+                          stride, b);
+          if (rc) return rc;
+        } else {
+          hipLaunchKernelGGL(k_shade_adjoint, dim3(sg), dim3(sblock), lds_adj, ctx->stream, b, cnt + b, SP, A,
+                             ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
+                             ctx->q_hit.as<float4>(), ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(),
+                             ctx->q_tp[nxt].as<float4>(), cnt + b + 1);
+          GI_HIP(ctx, hipGetLastError());
+        }
+      }
+    }
+  }
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<uint32_t> hc(2 * npass * ncnt);
+  GI_HIP(ctx, copy_sync(ctx, hc.data(), ctx->d_counters.p, hc.size() * 4, hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < 2 * npass; ++k)
+    for (int32_t b = 0; b < depth; ++b) {
+      ctx->counters.rays_closest += hc[k * ncnt + b];
+      ctx->counters.rays_shadow += hc[k * ncnt + depth + 1 + b];
+    }
+  ctx->counters.passes += (int32_t)npass;
+  if (read_overflow(ctx) != 0) return fail(ctx, GI_ERR_LIMIT, "traversal stack overflow");
+  return GI_OK;
+}
+
 int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_render_opts *opts) {
   if (!views || n_views <= 0 || !opts) return fail(ctx, GI_ERR_INVALID_ARG, "views/opts");
   if (opts->max_depth < 1 || opts->max_depth > kMaxDepth)
@@ -1037,7 +1441,7 @@ int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_ren
   ctx->opts = *opts;
   ctx->images.resize(n_views);
   std::vector<DevView> dv(n_views);
-  uint32_t base = 0;
+  uint32_t base = 0, pix_base = 0;
   for (int32_t v = 0; v < n_views; ++v) {
     const gi_view &w = views[v];
     DevView &d = dv[v];
@@ -1064,6 +1468,8 @@ int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_ren
     d.w = w.w; d.h = w.h; d.spp = w.spp;
     d.index_base = base;
     base += (uint32_t)((uint64_t)w.w * w.h * w.spp);
+    d.pix_base = pix_base;
+    pix_base += (uint32_t)((uint64_t)w.w * w.h);
     const size_t npix = (size_t)w.w * w.h;
     GI_HIP(ctx, ctx->images[v].ensure(npix * sizeof(float4)));
     d.image = ctx->images[v].as<float4>();
@@ -1222,9 +1628,15 @@ int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n) {
   GI_HIP(ctx, hipSetDevice(ctx->device));
   ctx->mats.assign(mats, mats + n);
   std::vector<DevMat> dm(n);
-  for (int32_t i = 0; i < n; ++i) dm[i] = make_devmat(mats[i]);
+  std::vector<DevMatGrad> dg(n);
+  for (int32_t i = 0; i < n; ++i) {
+    dm[i] = make_devmat(mats[i]);
+    dg[i] = make_devmatgrad(mats[i]);
+  }
   GI_HIP(ctx, ctx->d_mats.ensure(n * sizeof(DevMat)));
   GI_HIP(ctx, copy_sync(ctx, ctx->d_mats.p, dm.data(), n * sizeof(DevMat), hipMemcpyHostToDevice));
+  GI_HIP(ctx, ctx->d_matgrad.ensure(n * sizeof(DevMatGrad)));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_matgrad.p, dg.data(), n * sizeof(DevMatGrad), hipMemcpyHostToDevice));
   return GI_OK;
 }
 
@@ -1389,7 +1801,7 @@ int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint3
   HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>()};
   hipLaunchKernelGGL(k_raygen, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_passes.as<PassDesc>(),
                      ctx->d_views.as<DevView>(), H, ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(),
-                     ctx->q_tp[0].as<float4>(), ctx->path_l.as<float4>());
+                     ctx->q_tp[0].as<float4>(), ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, 1);
   GI_HIP(ctx, hipGetLastError());
   std::vector<float4> ro(npix), rd(npix);
   GI_HIP(ctx, hipMemcpyAsync(ro.data(), ctx->q_o[0].p, npix * 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -1518,10 +1930,56 @@ int gi_get_tuning(const gi_ctx *ctx, gi_tuning *t) {
 }
 
 int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, float *d_emission) {
-  (void)dL_dimage;
-  (void)d_mats;
-  (void)d_emission;
-  return fail(ctx, GI_ERR_STATE, "adjoint not available in this build");
+  int rc = check_scene(ctx, true);
+  if (rc) return rc;
+  if (!dL_dimage || !d_mats || !d_emission) return fail(ctx, GI_ERR_INVALID_ARG, "adjoint buffers");
+  if (ctx->views.empty()) return fail(ctx, GI_ERR_STATE, "gi_render first");
+  if (ctx->opts.rr_start > 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support Russian roulette");
+  const int32_t nm = (int32_t)ctx->mats.size(), nl = (int32_t)ctx->lights.size();
+  if ((size_t)(5 * nm + 3 * nl) * sizeof(float) > 48 * 1024)
+    return fail(ctx, GI_ERR_LIMIT, "adjoint: 5*materials + 3*lights must fit 48 KiB of LDS");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  auto t0 = std::chrono::steady_clock::now();
+  std::memset(&ctx->counters, 0, sizeof(ctx->counters));
+  const int32_t nv = (int32_t)ctx->views.size();
+  size_t total_pix = 0;
+  std::vector<int32_t> sb(nv, 0), se(nv);
+  for (int32_t v = 0; v < nv; ++v) {
+    total_pix += (size_t)ctx->views[v].w * ctx->views[v].h;
+    se[v] = ctx->views[v].spp;
+    ctx->counters.samples += (uint64_t)ctx->views[v].w * ctx->views[v].h * ctx->views[v].spp;
+  }
+  GI_HIP(ctx, ctx->d_dl.ensure(total_pix * 3 * sizeof(float)));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_dl.p, dL_dimage, total_pix * 3 * sizeof(float), hipMemcpyHostToDevice));
+  GI_HIP(ctx, ctx->d_gmats.ensure((size_t)5 * nm * sizeof(float)));
+  GI_HIP(ctx, ctx->d_gem.ensure((size_t)3 * nl * sizeof(float)));
+  GI_HIP(ctx, hipMemsetAsync(ctx->d_gmats.p, 0, (size_t)5 * nm * sizeof(float), ctx->stream));
+  GI_HIP(ctx, hipMemsetAsync(ctx->d_gem.p, 0, (size_t)3 * nl * sizeof(float), ctx->stream));
+  const int32_t spp_pp = ctx->opts.samples_per_pass > 0 ? ctx->opts.samples_per_pass : ctx->tuning.samples_per_pass;
+  PassPlan plan;
+  plan_passes(ctx->views, sb, se, true, target_slots(ctx), spp_pp, plan);
+  AdjParams A;
+  std::memset(&A, 0, sizeof(A));
+  A.dL = ctx->d_dl.as<float>();
+  A.mg = ctx->d_matgrad.as<DevMatGrad>();
+  A.g_mats = ctx->d_gmats.as<float>();
+  A.g_em = ctx->d_gem.as<float>();
+  A.nmats = nm;
+  A.nlights = nl;
+  rc = run_adjoint_passes(ctx, plan, ctx->opts.max_depth, A);
+  if (rc) return rc;
+  std::vector<float> gm((size_t)5 * nm), ge((size_t)3 * nl);
+  GI_HIP(ctx, copy_sync(ctx, gm.data(), ctx->d_gmats.p, gm.size() * sizeof(float), hipMemcpyDeviceToHost));
+  GI_HIP(ctx, copy_sync(ctx, ge.data(), ctx->d_gem.p, ge.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int32_t i = 0; i < nm; ++i) {
+    for (int k = 0; k < 3; ++k) d_mats[i].d_color[k] = gm[5 * i + k];
+    d_mats[i].d_roughness = gm[5 * i + 3];
+    d_mats[i].d_metallic = gm[5 * i + 4];
+  }
+  std::memcpy(d_emission, ge.data(), ge.size() * sizeof(float));
+  ctx->counters.ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return GI_OK;
 }
 
 float gi_halton_sample(uint64_t n, int32_t dim, uint32_t seed) {

@@ -302,6 +302,38 @@ def cornell_batch(seed: int = 3, n_views: int = 16, depth: int = 8, rr_start: in
                  name="cornell_batch")
 
 
+def diff_scene(seed: int = 4, cells=(200, 250), n_spheres: int = 50, n_mats: int = 64, n_lights: int = 4,
+               w: int = 512, h: int = 512, spp: int = 16, depth: int = 4, radius=(0.5, 4.0)) -> Scene:
+    """The differentiation workload: heightfield + spheres (200x250 cells + 50
+    spheres = 108k triangles), 64 principled materials with metallic and
+    roughness drawn continuously (U[0.1,0.9], U[0.2,0.9]) so both have
+    gradients, 4 lights, 512x512, 16 spp, depth 4, black background (the
+    image is then smooth in every parameter except at visibility changes)."""
+    sc = terrain_spheres(seed=seed, cells=cells, n_spheres=n_spheres, n_mats=n_mats, n_lights=n_lights, w=w, h=h,
+                         spp=spp, depth=depth, radius=radius)
+    rng = np.random.default_rng(seed + 100)
+    sc.mats["metallic"] = rng.uniform(0.1, 0.9, n_mats).astype(np.float32)
+    sc.mats["roughness"] = rng.uniform(0.2, 0.9, n_mats).astype(np.float32)
+    sc.opts["background"] = (0.0, 0.0, 0.0)
+    sc.name = "diff_scene"
+    return sc
+
+
+def perturbed(scene: Scene, sigma: float = 0.1, seed: int = 0) -> Scene:
+    """Copy of the scene with baseColor, roughness, metallic and emission
+    perturbed by N(0, sigma^2) (relative for emission) and clamped."""
+    rng = np.random.default_rng(seed)
+    m = scene.mats.copy()
+    m["color"] = np.clip(m["color"] + rng.normal(0, sigma, m["color"].shape), 0.0, 1.0)
+    lam = m["principled"] == 0
+    m["diffuse_color"][lam] = m["color"][lam]
+    m["roughness"] = np.clip(m["roughness"] + rng.normal(0, sigma, len(m)), 0.05, 1.0)
+    m["metallic"] = np.clip(m["metallic"] + rng.normal(0, sigma, len(m)), 0.0, 1.0)
+    l = scene.lights.copy()
+    l["col"] = np.clip(l["col"] * (1 + rng.normal(0, sigma, l["col"].shape)), 0.0, None)
+    return Scene(scene.tris, m, l, scene.views, scene.opts, name=scene.name + "_target")
+
+
 def config_scene(idx: int, **kw) -> Scene:
     """Scenes of the benchmark configurations by number (1-based)."""
     if idx == 1:
@@ -311,7 +343,5 @@ def config_scene(idx: int, **kw) -> Scene:
     if idx == 3:
         return cornell_batch(**kw)
     if idx == 4:
-        args = dict(seed=4, cells=(200, 250), n_spheres=50, n_mats=64, n_lights=4, w=512, h=512, spp=16, depth=4)
-        args.update(kw)
-        return terrain_spheres(**args)
+        return diff_scene(**kw)
     raise ValueError(f"no scene generator for configuration {idx}")

@@ -240,9 +240,15 @@ int gi_progress_begin(gi_ctx *ctx, const gi_view *views, int32_t n_views,
 int gi_progress_step(gi_ctx *ctx, int32_t *done);
 
 /* ---- adjoint (path-replay backprop; no counterpart in the reference) -------
- * Gradient of sum_i dL_dimage[i] * image[i] of view 0 with respect to the
- * material parameters and the light emissions, replaying the same Halton
- * samples as the last gi_render.  d_emission[3*n_lights]. */
+ * Gradient of sum_i dL_dimage[i] * image[i] over the views of the last
+ * gi_render (dL_dimage: 3 floats per pixel, views concatenated) with respect
+ * to every material's (color rgb, roughness, metallic) -- for a Lambert
+ * material d_color is d/d diffuse_color -- and every light's emission rgb
+ * (d_emission[3*n_lights]; the light-selection weight gi_light.power is held
+ * fixed).  The paths are replayed with the same Halton indices; samples
+ * (lobe choice, cosine direction or GGX half-vector, light choice) are held
+ * fixed and the sampling's dependence on metallic and roughness enters as a
+ * score-function term.  Not defined with Russian roulette (rr_start > 0). */
 int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats,
                float *d_emission);
 

@@ -807,6 +807,334 @@ void render_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_r
   if (st_out) *st_out = total;
 }
 
+
+/* ---------------------------------------------------------------- adjoint --
+ * Path-replay gradient of an image-weighted loss sum_p dL/dI_p I_p with
+ * respect to (baseColor rgb, roughness, metallic) of every material and the
+ * emission (rgb) of every light, for the estimator above with its samples
+ * frozen: the lobe choice, the cosine direction or the GGX half-vector and
+ * the light choice are fixed functions of the Halton point, so the path
+ * does not move when a parameter changes.  Terms per vertex j:
+ *   NEE      dC_j = V_j * d f_nee_j * T_j * psc_j
+ *   chain    R_{j,c} * d log(f_j / pdf_j)_c, R = radiance after j up to the
+ *            next throughput clamp in channel c (the clamp zeroes the chain)
+ *   score    S_j * d log p(sample_j): lobe choice P(spec) = metallic, GGX
+ *            half-vector density D(h) cos(theta_h) w.r.t. roughness; S_j =
+ *            everything the path gathers from j on
+ *   emission dC_j / d col = V_j * f_nee_j * T_j * factor * P_tot / P_k with
+ *            the light-selection weight (gi_light.power) held fixed.
+ * This restatement records the whole path and back-propagates afterwards;
+ * the device replays the path instead.  Russian roulette is not supported. */
+template <class R> struct MatD {
+  V3<R> cd, dcd, tint;
+  R lum, w[3];
+  explicit MatD(const MatP<R> &p) {
+    cd = mon2lin(p.color);
+    dcd = mk<R>(p.color.x > R(0) ? R(2.2) * std::pow(p.color.x, R(1.2)) : R(0),
+                p.color.y > R(0) ? R(2.2) * std::pow(p.color.y, R(1.2)) : R(0),
+                p.color.z > R(0) ? R(2.2) * std::pow(p.color.z, R(1.2)) : R(0));
+    lum = R(0.3) * cd.x + R(0.6) * cd.y + R(0.1) * cd.z;
+    tint = lum > R(0) ? divs(cd, lum) : mk<R>(1, 1, 1);
+    w[0] = R(0.3); w[1] = R(0.6); w[2] = R(0.1);
+  }
+  R c(const V3<R> &v, int i) const { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+  /* d tint_c / d Cd_i */
+  R dtint(int c, int i) const {
+    if (!(lum > R(0))) return R(0);
+    return ((c == i ? R(1) : R(0)) - c3(tint, c) * w[i]) / lum;
+  }
+  static R c3(const V3<R> &v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+};
+template <class R> inline R &at(V3<R> &v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+template <class R> inline R cat(const V3<R> &v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+/* d f / d (color r, g, b, roughness, metallic) for the lobe `type`. */
+template <class R>
+void brdf_grad(const MatP<R> &p, const MatD<R> &d, int type, V3<R> wo, V3<R> wi, V3<R> N, V3<R> df[5]) {
+  for (int k = 0; k < 5; ++k) df[k] = mk<R>(0, 0, 0);
+  if (!p.principled) {
+    const R a = std::fabs(dot(wi, N)) * K<R>::inv_pi;
+    df[0].x = a; df[1].y = a; df[2].z = a;
+    return;
+  }
+  const V3<R> V = wo, L = wi;
+  const R ndl = dot(N, L), ndv = dot(N, V);
+  if (ndl < R(0) || ndv < R(0)) return;
+  if (type == T_DISNEY_REFL) {
+    V3<R> H = normalize(L + V);
+    const R ldh = dot(L, H), ndh = dot(N, H);
+    const R fh = schlick(ldh), m = p.metallic, sp08 = p.specular * R(0.08), st = p.spec_tint;
+    R cg = R(0);
+    if (p.clearcoat > R(0)) {
+      R dr = gtr1(ndh, mixf(R(0.1), R(0.001), p.cc_gloss));
+      R fr = mixf(R(0.04), R(1), fh);
+      R gr = smith_ggx(ndl, R(0.25)) * smith_ggx(ndv, R(0.25));
+      cg = R(0.25) * p.clearcoat * gr * fr * dr;
+    }
+    for (int c = 0; c < 3; ++c) {
+      const R mt = (R(1) - st) + st * cat(d.tint, c);
+      const R cs0 = (R(1) - m) * sp08 * mt + m * cat(d.cd, c);
+      const R fs = (R(1) - fh) * cs0 + fh;
+      at(df[4], c) = (fs + cg) * ndl + m * ndl * (R(1) - fh) * (cat(d.cd, c) - sp08 * mt);
+      for (int i = 0; i < 3; ++i) {
+        const R dcs0 = m * (c == i ? R(1) : R(0)) + (R(1) - m) * sp08 * st * d.dtint(c, i);
+        at(df[i], c) = m * ndl * (R(1) - fh) * dcs0 * cat(d.dcd, i);
+      }
+    }
+    return;
+  }
+  if (type != T_DIFFUSE) return;  /* glass lobes have no parameter */
+  V3<R> H = normalize(L + V);
+  const R ldh = dot(L, H);
+  const R fl = schlick(ndl), fv = schlick(ndv), r = p.roughness, m = p.metallic, sub = p.subsurface;
+  const R fd90 = R(0.5) + R(2) * ldh * ldh * r;
+  const R fd = mixf(R(1), fd90, fl) * mixf(R(1), fd90, fv);
+  const R fh = schlick(ldh);
+  const R fss90 = ldh * ldh * r;
+  const R fss = mixf(R(1), fss90, fl) * mixf(R(1), fss90, fv);
+  const R q = R(1) / (ndl + ndv) - R(0.5);
+  const R ss = R(1.25) * (fss * q + R(0.5));
+  const R k = K<R>::inv_pi * mixf(fd, ss, sub);
+  const R dfd = R(2) * ldh * ldh * (fl * mixf(R(1), fd90, fv) + mixf(R(1), fd90, fl) * fv);
+  const R dss = R(1.25) * q * ldh * ldh * (fl * mixf(R(1), fss90, fv) + mixf(R(1), fss90, fl) * fv);
+  const R dk = K<R>::inv_pi * ((R(1) - sub) * dfd + sub * dss);
+  for (int c = 0; c < 3; ++c) {
+    const R csh = (R(1) - p.sheen_tint) + p.sheen_tint * cat(d.tint, c);
+    at(df[3], c) = dk * cat(d.cd, c) * (R(1) - m) * ndl;
+    at(df[4], c) = -(k * cat(d.cd, c) + fh * p.sheen * csh) * ndl;
+    for (int i = 0; i < 3; ++i)
+      at(df[i], c) = (R(1) - m) * ndl * (k * (c == i ? R(1) : R(0)) + fh * p.sheen * p.sheen_tint * d.dtint(c, i)) *
+                     cat(d.dcd, i);
+  }
+}
+
+/* GGX roughness derivatives of the sampled specular lobe: d log(lweight)/dr
+ * (the reference's specular "pdf", src/principledMaterial.h:158-175) and the
+ * score d log D(h)/dr of the half-vector density. */
+template <class R> void gtr2_grads(const MatP<R> &p, V3<R> V, V3<R> L, V3<R> h, R &dlogw, R &score) {
+  const R aspect = std::sqrt(R(1) - p.aniso * R(0.9));
+  const R r = p.roughness, r2 = r * r;
+  const R ax = std::max(R(0.001), r2 / aspect), ay = std::max(R(0.001), r2 * aspect);
+  const R dax = (r2 / aspect > R(0.001)) ? R(2) * r / aspect : R(0);
+  const R day = (r2 * aspect > R(0.001)) ? R(2) * r * aspect : R(0);
+  const R qq = ax * ay * R(0.5) + R(0.5);
+  const R roughg = qq * qq, rg2 = roughg * roughg;
+  const R drg2 = R(4) * qq * qq * qq * R(0.5) * (dax * ay + ax * day);
+  const R dlm = dot(L, h), dvm = dot(V, h);
+  const R tl = std::sqrt(R(1) - dlm * dlm) / dlm, tv = std::sqrt(R(1) - dvm * dvm) / dvm;
+  const R xl = (dot(L, h) / L.z) > R(0) ? R(1) : R(0), xv = (dot(V, h) / V.z) > R(0) ? R(1) : R(0);
+  const R sl = std::sqrt(R(1) + rg2 * tl * tl), sv = std::sqrt(R(1) + rg2 * tv * tv);
+  const R g1l = xl * R(2) / (R(1) + sl), g1v = xv * R(2) / (R(1) + sv);
+  const R lw = (std::fabs(dot(L, h)) * g1l * g1v) / (std::fabs(L.z) * std::fabs(h.z));
+  dlogw = R(0);
+  if (lw >= R(0.01) && g1l > R(0) && g1v > R(0)) {
+    const R dg1l = -(R(1) / (R(1) + sl)) * (tl * tl / (R(2) * sl));
+    const R dg1v = -(R(1) / (R(1) + sv)) * (tv * tv / (R(2) * sv));
+    dlogw = (dg1l + dg1v) * drg2;
+  }
+  const R Q = h.x * h.x / (ax * ax) + h.y * h.y / (ay * ay) + h.z * h.z;
+  const R dlx = -R(1) / ax + R(4) * h.x * h.x / (ax * ax * ax * Q);
+  const R dly = -R(1) / ay + R(4) * h.y * h.y / (ay * ay * ay * Q);
+  score = dlx * dax + dly * day;
+}
+
+template <class R> struct VRec {
+  bool hit = false, bg = false, vis = false, chain = false;
+  int mat = 0, light = 0, clamp = 0;
+  V3<R> rec{0, 0, 0}, T{0, 0, 0}, psc{0, 0, 0}, fnee{0, 0, 0};
+  V3<R> dfn[5] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  V3<R> dlr[5] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  R gk = 0, score[5] = {0, 0, 0, 0, 0};
+};
+
+template <class R>
+void adjoint_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H, const Options &op, uint64_t n,
+                    V3<R> o, V3<R> d, V3<R> delta, std::vector<double> &gm, std::vector<double> &ge) {
+  VRec<R> vr[32];
+  int nv = 0;
+  V3<R> TP = mk<R>(1, 1, 1);
+  R tmax = R(FLT_MAX);
+  const int nl = (int)s.lights.size();
+  uint64_t nb = 0, nt = 0;
+  for (int32_t b = 0; b < op.depth; ++b) {
+    const int32_t dim = DIM_VTX0 + DIMS_PER_VTX * b;
+    R t, bu = R(0), bv = R(0);
+    uint32_t id;
+    bool hit = tr.closest(o, d, tmax, t, bu, bv, id, nb, nt);
+    V3<R> org_dir = normalize(d);
+    VRec<R> &v = vr[nv++];
+    v.T = TP;
+    if (!hit) {
+      if (TP.x == R(1) && TP.y == R(1) && TP.z == R(1)) {
+        v.rec = mul(TP, mk<R>(R(op.bg[0]), R(op.bg[1]), R(op.bg[2])));
+        v.bg = true;
+      }
+      break;
+    }
+    v.hit = true;
+    const gi_tri &T = s.tris[id];
+    V3<R> bc = mk<R>((R(1) - bu) - bv, bu, bv);
+    V3<R> P = (ld<R>(T.a) * bc.x + ld<R>(T.b) * bc.y) + ld<R>(T.c) * bc.z;
+    V3<R> Ni = (ld<R>(T.na) * bc.x + ld<R>(T.nb) * bc.y) + ld<R>(T.nc) * bc.z;
+    MatP<R> mp(s.mats[T.material_index]);
+    MatD<R> md(mp);
+    v.mat = T.material_index;
+    V3<R> xl = mk<R>(R(H.sample(n, dim + 0)), R(H.sample(n, dim + 1)), R(H.sample(n, dim + 2)));
+    R choice = xl.z * R(s.power_total), acc = R(0);
+    int k = 0;
+    while (k < nl && choice > acc + R(s.lights[k].power)) { acc += R(s.lights[k].power); ++k; }
+    if (k >= nl) k = nl - 1;
+    const gi_light &lt = s.lights[k];
+    v.light = k;
+    V3<R> ldir = ld<R>(lt.dir);
+    V3<R> right = tangential(mk<R>(1, 0, 0), ldir), up = tangential(mk<R>(0, 1, 0), ldir);
+    R offx = (xl.x - R(0.5)) * R(lt.wh[0]), offy = (xl.y - R(0.5)) * R(lt.wh[1]);
+    V3<R> ls = (ld<R>(lt.center) + right * offx) + up * offy;
+    V3<R> sd = ls - P;
+    R len = length(sd);
+    sd = divs(sd, len);
+    R ndotl = std::max(dot(Ni, sd), R(0)), lcos = std::max(dot(ldir, -sd), R(0));
+    R factor = (((R(lt.wh[0]) * R(lt.wh[1])) * ndotl) * lcos) / (len * len);
+    v.gk = factor * (R(s.power_total) / R(lt.power));
+    V3<R> psc = ld<R>(lt.col) * factor * (R(s.power_total) / R(lt.power));
+    v.psc = psc;
+    sd = normalize(sd);
+    V3<R> so = P + sd * R(op.off_s);
+    V3<R> N = normalize(Ni), Tt, Bt;
+    tangent_frame(N, Tt, Bt);
+    if (dot(org_dir, N) > R(0)) N = -N;
+    V3<R> inv = -org_dir;
+    V3<R> wo_ts = mk<R>(dot(inv, Tt), dot(inv, Bt), dot(inv, N));
+    V3<R> xp = mk<R>(R(H.sample(n, dim + 3)), R(H.sample(n, dim + 4)), R(H.sample(n, dim + 5)));
+    V3<R> wi_ts;
+    R pdf = R(1);
+    int type = brdf_sample(mp, wo_ts, xp, wi_ts, pdf);
+    V3<R> wi = normalize((Tt * wi_ts.x + Bt * wi_ts.y) + N * wi_ts.z);
+    const bool glass = mp.glass();
+    /* scores of the frozen sample */
+    if (mp.principled && !glass) {
+      const bool spec_chosen = xp.z < mp.metallic;
+      v.score[4] = spec_chosen ? R(1) / mp.metallic : -R(1) / (R(1) - mp.metallic);
+      if (type == T_DISNEY_REFL) {
+        R sn, cs;
+        sincos2pi(xp.x, sn, cs);
+        R aspect = std::sqrt(R(1) - mp.aniso * R(0.9));
+        R ax = std::max(R(0.001), sqr(mp.roughness) / aspect), ay = std::max(R(0.001), sqr(mp.roughness) * aspect);
+        R fac = std::sqrt(xp.y / (R(1) - xp.y));
+        V3<R> h = normalize(mk<R>(fac * (ax * cs), fac * (ay * sn), R(1)));
+        R dlogw, sc;
+        gtr2_grads(mp, wo_ts, wi_ts, h, dlogw, sc);
+        v.score[3] = sc;
+        v.dlr[3] = mk<R>(-dlogw, -dlogw, -dlogw);
+      }
+    }
+    if (!glass) {
+      V3<R> ldn = normalize(sd);
+      V3<R> fn = brdf_eval(mp, type, inv, ldn, N);
+      V3<R> C = mul(fn, mul(TP, psc));
+      v.fnee = fn;
+      brdf_grad(mp, md, type, inv, ldn, N, v.dfn);
+      if (C.x > R(0) || C.y > R(0) || C.z > R(0)) {
+        if (!tr.any(so, sd, len, nb, nt)) {
+          v.vis = true;
+          v.rec = C;
+        }
+      }
+    }
+    if (b + 1 >= op.depth) break;
+    V3<R> f = brdf_eval(mp, type, inv, wi, N);
+    V3<R> df[5];
+    brdf_grad(mp, md, type, inv, wi, N, df);
+    V3<R> np = glass ? P + org_dir * R(op.off_b) : P + N * R(op.off_b);
+    V3<R> tp = mk<R>((TP.x * f.x) / pdf, (TP.y * f.y) / pdf, (TP.z * f.z) / pdf);
+    for (int c = 0; c < 3; ++c)
+      if (cat(tp, c) > R(1)) { at(tp, c) = R(1); v.clamp |= 1 << c; }
+    v.chain = true;
+    for (int q = 0; q < 5; ++q)
+      for (int c = 0; c < 3; ++c)
+        at(v.dlr[q], c) = cat(f, c) != R(0) ? cat(df[q], c) / cat(f, c) + cat(v.dlr[q], c) : R(0);
+    TP = tp;
+    o = np;
+    d = wi;
+    tmax = R(FLT_MAX);
+  }
+  for (int j = 0; j < nv; ++j) {
+    const VRec<R> &v = vr[j];
+    if (!v.hit) continue;
+    V3<R> Rj = mk<R>(0, 0, 0), Sj = v.rec;
+    for (int b = j + 1; b < nv; ++b) Sj = Sj + vr[b].rec;
+    if (v.chain)
+      for (int c = 0; c < 3; ++c)
+        for (int b = j + 1; b < nv; ++b) {
+          if (vr[b - 1].clamp & (1 << c)) break;
+          if (!vr[b].bg) at(Rj, c) += cat(vr[b].rec, c);
+        }
+    const R sdot = dot(delta, Sj);
+    for (int q = 0; q < 5; ++q) {
+      R g = sdot * v.score[q];
+      for (int c = 0; c < 3; ++c) {
+        if (v.vis) g += cat(delta, c) * cat(v.dfn[q], c) * cat(v.T, c) * cat(v.psc, c);
+        if (v.chain) g += cat(delta, c) * cat(Rj, c) * cat(v.dlr[q], c);
+      }
+      gm[5 * v.mat + q] += (double)g;
+    }
+    if (v.vis)
+      for (int c = 0; c < 3; ++c)
+        ge[3 * v.light + c] += (double)(cat(delta, c) * cat(v.fnee, c) * cat(v.T, c) * v.gk);
+  }
+}
+
+template <class R>
+void adjoint_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_render_opts &ro, const float *dL,
+                  double *d_mats, double *d_em, int32_t threads) {
+  Options op = resolve(s, ro);
+  HaltonTable H(ro.seed, dims_for_depth(op.depth));
+  Tracer<R> tr{s};
+#ifdef _OPENMP
+  int nth = threads > 0 ? threads : omp_get_max_threads();
+#else
+  int nth = 1;
+  (void)threads;
+#endif
+  const size_t nm = s.mats.size(), nl = s.lights.size();
+  std::vector<std::vector<double>> gm(nth, std::vector<double>(5 * nm, 0.0)), ge(nth, std::vector<double>(3 * nl, 0.0));
+  uint64_t base = 0;
+  int64_t img_off = 0;
+  for (int32_t vi = 0; vi < nv; ++vi) {
+    const gi_view &v = views[vi];
+    Cam cam = make_cam(v);
+    const int64_t npix = (int64_t)v.w * v.h;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nth)
+    for (int64_t p = 0; p < npix; ++p) {
+#ifdef _OPENMP
+      const int th = omp_get_thread_num();
+#else
+      const int th = 0;
+#endif
+      const float *w = dL + 3 * (img_off + p);
+      V3<R> delta = mk<R>(R(w[0]) / R(v.spp), R(w[1]) / R(v.spp), R(w[2]) / R(v.spp));
+      if (delta.x == R(0) && delta.y == R(0) && delta.z == R(0)) continue;
+      int32_t x = (int32_t)(p % v.w), y = (int32_t)(p / v.w);
+      for (int32_t si = 0; si < v.spp; ++si) {
+        uint64_t n = base + (uint64_t)p * (uint64_t)v.spp + (uint64_t)si;
+        V3<float> of, df;
+        cam_ray(cam, x, y, H.sample(n, DIM_JX), H.sample(n, DIM_JY), H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
+        adjoint_sample<R>(s, tr, H, op, n, mk<R>(of.x, of.y, of.z), mk<R>(df.x, df.y, df.z), delta, gm[th], ge[th]);
+      }
+    }
+    base += (uint64_t)npix * (uint64_t)v.spp;
+    img_off += npix;
+  }
+  for (size_t i = 0; i < 5 * nm; ++i) {
+    d_mats[i] = 0;
+    for (int t = 0; t < nth; ++t) d_mats[i] += gm[t][i];
+  }
+  for (size_t i = 0; i < 3 * nl; ++i) {
+    d_em[i] = 0;
+    for (int t = 0; t < nth; ++t) d_em[i] += ge[t][i];
+  }
+}
+
 }  // namespace
 
 /* ================================================================ C API === */
@@ -1015,6 +1343,14 @@ void or_render(const or_scene *s, const gi_view *views, int32_t n_views, const g
 void or_render_f64(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
                    double *images, int32_t threads) {
   render_impl<double, double>(*s, views, n_views, *opts, images, nullptr, threads, 0, -1);
+}
+
+int or_adjoint(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
+               const float *dL_dimage, double *d_mats, double *d_emission, int32_t threads, int32_t f64) {
+  if (opts->rr_start > 0 || opts->max_depth > 32) return 1;
+  if (f64) adjoint_impl<double>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, threads);
+  else adjoint_impl<float>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, threads);
+  return 0;
 }
 
 }  // extern "C"

@@ -91,6 +91,14 @@ void or_render(const or_scene *s, const gi_view *views, int32_t n_views,
 void or_render_f64(const or_scene *s, const gi_view *views, int32_t n_views,
                    const gi_render_opts *opts, double *images, int32_t threads);
 
+/* Path-replay gradient of sum(dL_dimage * image) (views concatenated) with
+ * respect to (color rgb, roughness, metallic) per material -> d_mats[5*nm]
+ * and the emission (rgb) per light -> d_emission[3*nl].  f64: arithmetic in
+ * FP64 (the reference gradient), else FP32.  Returns 1 for unsupported
+ * options (Russian roulette). */
+int or_adjoint(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
+               const float *dL_dimage, double *d_mats, double *d_emission, int32_t threads, int32_t f64);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,7 @@ def load():
             "or_brdf_eval_f64": (None, [P, C.c_int, P, P, P, P]),
             "or_render": (None, [P, P, I32, P, P, P, I32, I64, I64]),
             "or_render_f64": (None, [P, P, I32, P, P, I32]),
+            "or_adjoint": (C.c_int, [P, P, I32, P, P, P, P, I32, I32]),
         }
         for k, (res, args) in sig.items():
             f = getattr(lib, k)
@@ -163,6 +164,21 @@ class OracleScene:
         img = np.zeros((npix, 3), np.float64)
         self.lib.or_render_f64(self.h, _p(views), len(views), _p(opts), _p(img), threads)
         return img
+
+
+def adjoint(osc: OracleScene, views, opts, dL, threads=0, f64=True):
+    """(d_mats (n_mats, 5): color rgb, roughness, metallic; d_emission (n_lights, 3))."""
+    views = np.ascontiguousarray(views).reshape(-1)
+    opts = np.ascontiguousarray(opts).reshape(1)
+    dL = np.ascontiguousarray(dL, np.float32).reshape(-1, 3)
+    nm = len(osc._keep[1]) if osc._keep[1] is not None else 0
+    nl = len(osc._keep[2]) if osc._keep[2] is not None else 0
+    dm = np.zeros((nm, 5))
+    de = np.zeros((nl, 3))
+    rc = osc.lib.or_adjoint(osc.h, _p(views), len(views), _p(opts), _p(dL), _p(dm), _p(de), threads, int(f64))
+    if rc != 0:
+        raise ValueError("or_adjoint: unsupported options")
+    return dm, de
 
 
 def radical_inverse(n, base):
