@@ -15,7 +15,7 @@ SPACE = {
     "trace_block": [64, 128, 256],
     "trace_blocks_per_cu": [4, 8, 16],
     "stack_lds": [8, 16, 32],
-    "shade_block": [128, 256, 512],
+    "shade_block": [64, 128, 256],
     "target_slots": [2, 4, 8, 16],
 }
 DEFAULT = {"trace_block": 256, "trace_blocks_per_cu": 8, "stack_lds": 16, "shade_block": 256, "target_slots": 4}

@@ -3,6 +3,7 @@ scene, the BVH and the render buffers (a gi_ctx)."""
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 
 import numpy as np
 
@@ -62,7 +63,9 @@ class Context:
             nodes = np.ascontiguousarray(nodes, np.float32)
             idx = np.ascontiguousarray(idx, np.uint32)
             self._ck(self.lib.gi_upload_bvh2(self.h, ptr(nodes), len(nodes) // 8, ptr(idx), len(idx)))
-        self.scene = scene
+        # own copy: set_materials / set_lights must not rewrite the caller's scene
+        self.scene = dataclasses.replace(scene, mats=None if scene.mats is None else scene.mats.copy(),
+                                         lights=None if scene.lights is None else scene.lights.copy())
         return self
 
     # ---- tracing primitives ---------------------------------------------

