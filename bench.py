@@ -39,6 +39,9 @@ def parse():
     p.add_argument("--no-batching", action="store_true", help="skip the multi-view batching leg")
     p.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batching leg")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="first run one device-to-device copy of this many MiB (untimed): the known-size "
+                        "kernel that scales FETCH_SIZE / WRITE_SIZE counter runs")
     return p.parse_args()
 
 
@@ -168,6 +171,14 @@ def main():
     from globillum_amd._lib import FLAG_BATCHED, FLAG_GRAPH
     from globillum_amd.context import Context, build_bvh2
 
+    if a.copy_mib > 0:
+        import torch
+        src = torch.empty(a.copy_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(f"[bench] copied {a.copy_mib} MiB device to device", file=sys.stderr, flush=True)
     scene = scenes.config_scene(2)
     bvh2 = build_bvh2(scene.tris)
     flags = FLAG_BATCHED | (FLAG_GRAPH if a.graph else 0)

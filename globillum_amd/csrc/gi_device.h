@@ -88,17 +88,32 @@ struct HaltonDev {
 // a partial last chunk adds leading zeros, which scales A and B by the same
 // power of p.  A, B < 2^53 for n < 2^32, so the value is one correctly
 // rounded FP64 division and one rounding to FP32.
+// Every dimension has P >= 37, so n < 2^32 has at most 7 chunks: all 7 are
+// split off and looked up at once (independent loads), and the ones above
+// the leading chunk are then dropped.
+constexpr int kHaltonMaxChunks = 7;
 __device__ __forceinline__ float halton(uint32_t n, int dim, HaltonDev H) {
   const HaltonDim hd = H.dims[dim];
   const uint16_t *chunk = H.chunks + hd.chunk_off;
-  uint64_t rev = 0, pk = 1;
-  uint32_t m = n;
-  while (m > 0) {
+  uint32_t c[kHaltonMaxChunks];
+  uint32_t m = n, K = 0;
+#pragma unroll
+  for (int k = 0; k < kHaltonMaxChunks; ++k) {
     const uint32_t q = (uint32_t)__umul64hi((uint64_t)m, hd.magic);
-    const uint32_t c = m - q * hd.P;
-    rev = rev * hd.P + chunk[c];
-    pk *= hd.P;
+    c[k] = m - q * hd.P;
+    K += m > 0u ? 1u : 0u;
     m = q;
+  }
+  uint32_t v[kHaltonMaxChunks];
+#pragma unroll
+  for (int k = 0; k < kHaltonMaxChunks; ++k) v[k] = chunk[c[k]];
+  uint64_t rev = 0, pk = 1;
+#pragma unroll
+  for (int k = 0; k < kHaltonMaxChunks; ++k) {
+    if ((uint32_t)k < K) {
+      rev = rev * hd.P + v[k];
+      pk *= hd.P;
+    }
   }
   const uint64_t A = rev * (uint64_t)(hd.prime - 1) + hd.perm0;
   const uint64_t B = pk * (uint64_t)(hd.prime - 1);
@@ -129,18 +144,18 @@ __device__ __forceinline__ f3 tangential(f3 v, f3 n) {
 // t > 0.  e1 = b - a and e2 = c - a come precomputed (same float rounding).
 __device__ __forceinline__ bool intersect_tri(f3 a, f3 e1, f3 e2, f3 o, f3 d, float &t, float &u,
                                               float &v) {
-  f3 pvec = cross(d, e2);
-  float det = dot(e1, pvec);
-  if (det == 0.0f) return false;
-  float inv_det = 1.0f / det;
-  f3 tvec = sub(o, a);
+  // evaluated without early exits (one wave runs every lane's test anyway);
+  // same operations and roundings as the early-exit form, so a hit has
+  // bit-identical t, u, v
+  const f3 pvec = cross(d, e2);
+  const float det = dot(e1, pvec);
+  const f3 tvec = sub(o, a);
+  const f3 qvec = cross(tvec, e1);
+  const float inv_det = 1.0f / det;
   u = dot(tvec, pvec) * inv_det;
-  if (!(u >= 0.0f) || u > 1.0f) return false;
-  f3 qvec = cross(tvec, e1);
   v = dot(d, qvec) * inv_det;
-  if (!(v >= 0.0f) || u + v > 1.0f) return false;
   t = dot(e2, qvec) * inv_det;
-  return t > 0.0f;
+  return (det != 0.0f) & (u >= 0.0f) & (u <= 1.0f) & (v >= 0.0f) & (u + v <= 1.0f) & (t > 0.0f);
 }
 
 __device__ __forceinline__ float rcp_clamped(float d) {

@@ -85,115 +85,250 @@ struct ShadeParams {
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
 
-// Append under a wave-wide ballot: one atomic per wave, lane offsets from
-// the mask prefix.  Called uniformly by every lane of the wave.
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
-  const uint64_t mask = __ballot(pred);
-  if (mask == 0) return 0;
-  const uint32_t lane = __lane_id();
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = (uint32_t)__shfl((int)base, leader);
-  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+// Append of two predicates per thread across the whole block: one atomic
+// per counter per block (device-wide atomics on one address serialise, so
+// they are kept few).  Called uniformly by every thread of the block;
+// blockDim <= 256.
+__device__ __forceinline__ void block_append2(bool p0, uint32_t *c0, bool p1, uint32_t *c1, uint32_t &q0,
+                                              uint32_t &q1) {
+  __shared__ uint32_t s_n[2][4];
+  __shared__ uint32_t s_base[2];
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+  const uint64_t below = (1ull << lane) - 1ull;
+  if (lane == 0) {
+    s_n[0][wave] = (uint32_t)__popcll(m0);
+    s_n[1][wave] = (uint32_t)__popcll(m1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const uint32_t k = threadIdx.x, nw = blockDim.x >> 6;
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < nw; ++w) tot += s_n[k][w];
+    s_base[k] = tot ? atomicAdd(k ? c1 : c0, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t o0 = s_base[0], o1 = s_base[1];
+  for (uint32_t w = 0; w < wave; ++w) {
+    o0 += s_n[0][w];
+    o1 += s_n[1][w];
+  }
+  q0 = o0 + (uint32_t)__popcll(m0 & below);
+  q1 = o1 + (uint32_t)__popcll(m1 & below);
+  __syncthreads();  // s_n / s_base are rewritten by the next call
 }
 
 // -------------------------------------------------------------- traversal --
-// Ordered 4-wide BVH traversal, one ray per lane.  The traversal stack lives
-// in LDS (kLdsDepth entries per lane, interleaved by lane so every access is
-// bank-conflict free) and continues in a per-thread global-memory region for
-// deep trees.  Closest hit = minimum (t, id); any-hit returns on the first
-// triangle with 0 < t <= tmax.
+// Ordered 4-wide BVH traversal, one ray per lane, persistent.  The queue is
+// cut into chunks of kChunk consecutive rays; wave w of W owns chunks w,
+// w + W, w + 2W, ... (no atomics).  A wave keeps its lanes busy: whenever
+// at least kChunk lanes have finished their rays, they take the wave's next
+// whole chunks, one ray each.  Interior nodes are processed in an inner
+// loop in which a lane that reaches a leaf parks it and goes on with its
+// stack (speculative traversal, Aila & Laine 2009), until every lane with
+// interior work left holds a parked leaf; then the parked leaves are
+// tested, and the wave refills.
+//
+// Stack: kLdsDepth entries per lane in LDS, interleaved by lane (entry k of
+// lane l at (k << shift) + l, with 1 << shift >= blockDim: conflict free),
+// continued in a per-thread global column.  The capacity covers 3 entries
+// per level of the tree; an overflow (impossible by construction) is
+// counted and turns into GI_ERR_LIMIT on the host.
+//
+// Box test: slab distances fma(bound, 1/d, -o/d), entry <= exit * (1 +
+// 2^-21).  The relative slack covers the rounding of the reciprocal, of
+// o/d and of the fma relative to the distance; the boxes' padding (2^-20
+// of their largest coordinate magnitude) covers the part proportional to
+// the box's own coordinates.  Box culling is therefore conservative and
+// the result is the closest hit = minimum (t, id) over all triangles with
+// 0 < t <= max_t; any-hit stops at the first such triangle.
+// Occupancy target of the traversal kernels (waves per SIMD): 7 caps them
+// at 72 VGPRs, which they fit without spilling.
+#ifndef GI_TRACE_WAVES_PER_EU
+#define GI_TRACE_WAVES_PER_EU 7
+#endif
+
+constexpr uint32_t kNoRay = 0xffffffffu;
+constexpr uint32_t kDoneRef = kLeafBit | kMaxLeafFirst;  // a leaf of 0 triangles: traversal over
+#ifndef GI_TRACE_CHUNK
+#define GI_TRACE_CHUNK 32
+#endif
+constexpr uint32_t kChunk = GI_TRACE_CHUNK;
+constexpr float kSlabSlack = 1.0f + 0x1p-21f;
+
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 template <int kLdsDepth>
 struct Stack {
-  uint32_t *lds;
-  uint32_t *glb;
-  uint32_t lds_stride, glb_stride, cap;
-  int sp;
-  __device__ __forceinline__ bool push(uint32_t v) {
-    if ((uint32_t)sp >= cap) return false;
-    if (sp < kLdsDepth) lds[sp * lds_stride] = v;
-    else glb[(sp - kLdsDepth) * glb_stride] = v;
-    ++sp;
-    return true;
+  lds_u32 *lds;    // this lane's LDS column: entry k at lds[k << shift]
+  uint32_t *glb;   // this thread's global column: entry k at glb[k * glb_stride]
+  uint32_t shift, glb_stride;
+  int cap, sp;
+  __device__ __forceinline__ void put(int pos, uint32_t v) {
+    if (pos < kLdsDepth) lds[(uint32_t)pos << shift] = v;
+    else glb[(uint32_t)(pos - kLdsDepth) * glb_stride] = v;
   }
-  __device__ __forceinline__ uint32_t pop() {
+  __device__ __forceinline__ uint32_t take() {
     --sp;
-    return sp < kLdsDepth ? lds[sp * lds_stride] : glb[(sp - kLdsDepth) * glb_stride];
+    return sp < kLdsDepth ? lds[(uint32_t)sp << shift] : glb[(uint32_t)(sp - kLdsDepth) * glb_stride];
   }
 };
 
-template <bool kAny, int kLdsDepth>
-__device__ __forceinline__ bool traverse(const WideNode *__restrict__ nodes, const float4 *__restrict__ itris, f3 o,
-                                         f3 d, float tmax, float &bt, float &bu, float &bv, uint32_t &bid,
-                                         Stack<kLdsDepth> &st, uint32_t *overflow) {
-  const f3 idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
-  bt = FLT_MAX;
-  bu = bv = 0.0f;
-  bid = GI_INVALID_REF;
-  uint32_t ref = 0;
-  st.sp = 0;
+// LDS column stride of a block: the next power of two >= blockDim.
+__host__ __device__ __forceinline__ uint32_t lds_shift_for(uint32_t block) {
+  uint32_t s = 0;
+  while ((1u << s) < block) ++s;
+  return s;
+}
+
+template <int kLdsDepth>
+__device__ __forceinline__ Stack<kLdsDepth> make_stack(uint32_t *smem, uint32_t *gstack, uint32_t cap) {
+  const uint32_t shift = blockDim.x <= 1 ? 0u : 32u - (uint32_t)__clz((int)(blockDim.x - 1));
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  return Stack<kLdsDepth>{(lds_u32 *)(smem + threadIdx.x), gstack + gtid, shift, gridDim.x * blockDim.x,
+                          (int)cap, 0};
+}
+
+// The persistent traversal loop over queue entries [0, count).
+//   start(i, o, d, tmax) loads ray i;
+//   finish(i, t, u, v, id) stores its result (id == GI_INVALID_REF: no hit).
+template <bool kAny, int kLdsDepth, class Start, class Finish>
+__device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode *__restrict__ nodes,
+                                                 const float4 *__restrict__ itris, Stack<kLdsDepth> &st,
+                                                 uint32_t *overflow, Start start, Finish finish) {
+  const uint32_t lane = __lane_id();
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t nchunks = (count + kChunk - 1) / kChunk;
+  uint32_t chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // wave-uniform: next chunk
+  uint32_t ray = kNoRay, ref = kDoneRef, parked = kDoneRef;
+  f3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f}, idir{0.f, 0.f, 0.f}, oid{0.f, 0.f, 0.f};
+  float tmax = 0.f, bt = FLT_MAX, bu = 0.f, bv = 0.f;
+  uint32_t bid = GI_INVALID_REF;
   for (;;) {
-    if (!(ref & kLeafBit)) {
-      const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
-      const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-      const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
-      const float lim = kAny ? tmax : smin(bt, tmax);
-      float dist[4];
-      uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
-      const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-      const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-      const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        float x0 = (lx[k] - o.x) * idir.x, x1 = (hx[k] - o.x) * idir.x;
-        float y0 = (ly[k] - o.y) * idir.y, y1 = (hy[k] - o.y) * idir.y;
-        float z0 = (lz[k] - o.z) * idir.z, z1 = (hz[k] - o.z) * idir.z;
-        float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-        float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), lim));
-        bool hit = tn <= tf && cref[k] != kEmptyRef;
-        dist[k] = hit ? tn : INFINITY;
+    // ---- refill: the idle lanes, in lane order, take whole chunks
+    const uint64_t idle = __ballot(ray == kNoRay);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= kChunk && chunk < nchunks) {
+      if (ray == kNoRay) {
+        const uint32_t r = (uint32_t)__popcll(idle & below);
+        const uint32_t c = chunk + (r / kChunk) * waves;
+        const uint32_t i = c * kChunk + r % kChunk;
+        if (r < (n_idle / kChunk) * kChunk && c < nchunks && i < count) {
+          ray = i;
+          start(ray, o, d, tmax);
+          idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
+          oid = f3{o.x * idir.x, o.y * idir.y, o.z * idir.z};
+          bt = FLT_MAX;
+          bu = bv = 0.0f;
+          bid = GI_INVALID_REF;
+          ref = 0;
+          st.sp = 0;
+        }
       }
-      // sort (dist, ref) ascending: network (0,1)(2,3)(0,2)(1,3)(1,2)
+      chunk += (n_idle / kChunk) * waves;
+    }
+    if (__ballot(ray != kNoRay) == 0) {
+      if (chunk >= nchunks) break;
+      continue;
+    }
+    if (ray == kNoRay) continue;
+    // ---- interior nodes.  A lane that reaches a leaf parks it (one slot)
+    // and goes on with its stack, so it keeps doing interior work while the
+    // other lanes look for their leaves; the loop ends once every lane with
+    // interior work left has a parked leaf (or none has interior work).
+    for (;;) {
+      if (!(ref & kLeafBit)) {
+        const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
+        const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+        const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
+        const float lim = kAny ? tmax : smin(bt, tmax);
+        float dist[4];
+        uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+        const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+        const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+        const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float x0 = fmaf(lx[k], idir.x, -oid.x), x1 = fmaf(hx[k], idir.x, -oid.x);
+          const float y0 = fmaf(ly[k], idir.y, -oid.y), y1 = fmaf(hy[k], idir.y, -oid.y);
+          const float z0 = fmaf(lz[k], idir.z, -oid.z), z1 = fmaf(hz[k], idir.z, -oid.z);
+          const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+          const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), lim));
+          const bool hit = tn <= tf * kSlabSlack && cref[k] != kEmptyRef;
+          dist[k] = hit ? tn : INFINITY;
+        }
+        // sort (dist, ref) ascending: network (0,1)(2,3)(0,2)(1,3)(1,2)
 #define GI_CSWAP(i, j)                                      \
   if (dist[j] < dist[i]) {                                  \
     float td = dist[i]; dist[i] = dist[j]; dist[j] = td;    \
     uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
   }
-      GI_CSWAP(0, 1) GI_CSWAP(2, 3) GI_CSWAP(0, 2) GI_CSWAP(1, 3) GI_CSWAP(1, 2)
+        GI_CSWAP(0, 1) GI_CSWAP(2, 3) GI_CSWAP(0, 2) GI_CSWAP(1, 3) GI_CSWAP(1, 2)
 #undef GI_CSWAP
-      if (dist[0] != INFINITY) {
-        bool ok = true;
-        if (dist[3] != INFINITY) ok &= st.push(cref[3]);
-        if (dist[2] != INFINITY) ok &= st.push(cref[2]);
-        if (dist[1] != INFINITY) ok &= st.push(cref[1]);
-        if (!ok) atomicAdd(overflow, 1u);
-        ref = cref[0];
-        continue;
+        if (dist[0] != INFINITY) {
+          // the hit children beyond the nearest, cref[1..nh], go on the stack
+          // farthest first, so the nearer ones are popped first
+          const int nh = (dist[1] != INFINITY ? 1 : 0) + (dist[2] != INFINITY ? 1 : 0) +
+                         (dist[3] != INFINITY ? 1 : 0);
+          const int sp = st.sp;
+          if (sp + nh > st.cap) {
+            atomicAdd(overflow, 1u);
+          } else {
+            if (nh >= 1) st.put(sp + nh - 1, cref[1]);
+            if (nh >= 2) st.put(sp + nh - 2, cref[2]);
+            if (nh >= 3) st.put(sp, cref[3]);
+            st.sp = sp + nh;
+          }
+          ref = cref[0];
+        } else {
+          ref = st.sp > 0 ? st.take() : kDoneRef;
+        }
       }
-    } else {
-      const uint32_t cnt = (ref >> 27) & 15u, first = ref & kMaxLeafFirst;
+      if (parked == kDoneRef && ref != kDoneRef && (ref & kLeafBit)) {
+        parked = ref;
+        ref = st.sp > 0 ? st.take() : kDoneRef;
+      }
+      if (__ballot(!(ref & kLeafBit) && parked == kDoneRef) == 0) break;
+    }
+    // ---- the parked leaf
+    if (parked != kDoneRef) {
+      // triangle j+1 is loaded while triangle j is tested
+      const uint32_t cnt = (parked >> 27) & 15u;
+      const float4 *tp = itris + 3 * (parked & kMaxLeafFirst);
+      float4 na = make_float4(0.f, 0.f, 0.f, 0.f), n1 = na, n2 = na;
+      if (cnt > 0) {
+        na = tp[0];
+        n1 = tp[1];
+        n2 = tp[2];
+      }
       for (uint32_t j = 0; j < cnt; ++j) {
-        const float4 ta = itris[3 * (first + j) + 0];
-        const float4 t1 = itris[3 * (first + j) + 1];
-        const float4 t2 = itris[3 * (first + j) + 2];
+        const float4 ta = na, t1 = n1, t2 = n2;
+        if (j + 1 < cnt) {
+          na = tp[3 * j + 3];
+          n1 = tp[3 * j + 4];
+          n2 = tp[3 * j + 5];
+        }
         float t, u, v;
         if (intersect_tri(xyz(ta), xyz(t1), xyz(t2), o, d, t, u, v) && t <= tmax) {
           const uint32_t id = fbits(ta.w);
           if (kAny) {
             bt = t; bu = u; bv = v; bid = id;
-            return true;
+            ref = kDoneRef;  // any hit ends the ray
+            break;
           }
           if (t < bt || (t == bt && id < bid)) {
             bt = t; bu = u; bv = v; bid = id;
           }
         }
       }
+      parked = kDoneRef;
     }
-    if (st.sp == 0) break;
-    ref = st.pop();
+    if (ref == kDoneRef) {
+      finish(ray, bt, bu, bv, bid);
+      ray = kNoRay;
+    }
   }
-  return bid != GI_INVALID_REF;
 }
 
 constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the global part
@@ -202,24 +337,22 @@ constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the g
 // primitive entry point); otherwise ray_o.w carries the Halton index and
 // max_t is FLT_MAX (camera and bounce rays, src/hybrid-pt.cpp:558).
 template <bool kHasTmax, int kLdsDepth>
-__global__ __launch_bounds__(256) void k_trace_closest(const uint32_t *__restrict__ count_ptr, uint32_t count_host,
-                                                       const float4 *__restrict__ ray_o,
-                                                       const float4 *__restrict__ ray_d, float4 *__restrict__ hits,
-                                                       const WideNode *__restrict__ nodes,
-                                                       const float4 *__restrict__ itris, uint32_t *__restrict__ gstack,
-                                                       uint32_t stack_cap, uint32_t *overflow) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_closest(
+    const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
+    const float4 *__restrict__ ray_d, float4 *__restrict__ hits, const WideNode *__restrict__ nodes,
+    const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
-  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
-  Stack<kLdsDepth> st{smem + threadIdx.x, gstack + gtid, blockDim.x, gstride, stack_cap, 0};
-  for (uint32_t i = gtid; i < count; i += gstride) {
-    const float4 ro = ray_o[i], rd = ray_d[i];
-    const float tmax = kHasTmax ? ro.w : FLT_MAX;
-    float t, u, v;
-    uint32_t id;
-    traverse<false>(nodes, itris, xyz(ro), xyz(rd), tmax, t, u, v, id, st, overflow);
-    hits[i] = make_float4(t, u, v, bitsf(id));
-  }
+  Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
+  trace_persistent<false, kLdsDepth>(
+      count, nodes, itris, st, overflow,
+      [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
+        const float4 ro = ray_o[i], rd = ray_d[i];
+        o = xyz(ro);
+        d = xyz(rd);
+        tmax = kHasTmax ? ro.w : FLT_MAX;
+      },
+      [&](uint32_t i, float t, float u, float v, uint32_t id) { hits[i] = make_float4(t, u, v, bitsf(id)); });
 }
 
 // Any-hit kernel.  kDeposit: shadow rays of the renderer -- an unoccluded
@@ -227,40 +360,43 @@ __global__ __launch_bounds__(256) void k_trace_closest(const uint32_t *__restric
 // !is.valid() branch of src/hybrid-pt.cpp:355-365).  Otherwise writes the
 // occlusion byte (primitive entry point).
 template <bool kDeposit, int kLdsDepth>
-__global__ __launch_bounds__(256) void k_trace_any(const uint32_t *__restrict__ count_ptr, uint32_t count_host,
-                                                   const float4 *__restrict__ ray_o, const float4 *__restrict__ ray_d,
-                                                   const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
-                                                   uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes,
-                                                   const float4 *__restrict__ itris, uint32_t *__restrict__ gstack,
-                                                   uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
-                                                   uint32_t rec_stride, int32_t b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_any(
+    const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
+    const float4 *__restrict__ ray_d, const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
+    uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes, const float4 *__restrict__ itris,
+    uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
+    uint32_t rec_stride, int32_t b) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
-  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
-  Stack<kLdsDepth> st{smem + threadIdx.x, gstack + gtid, blockDim.x, gstride, stack_cap, 0};
-  for (uint32_t i = gtid; i < count; i += gstride) {
-    const float4 ro = ray_o[i], rd = ray_d[i];
-    float t, u, v;
-    uint32_t id;
-    const bool occ = traverse<true>(nodes, itris, xyz(ro), xyz(rd), ro.w, t, u, v, id, st, overflow);
-    if (kDeposit) {
-      if (!occ) {
-        const uint32_t slot = fbits(rd.w);
-        const float4 c = contrib[i];
-        float4 l = path_l[slot];
-        l.x = l.x + c.x;
-        l.y = l.y + c.y;
-        l.z = l.z + c.z;
-        path_l[slot] = l;
-        if (rec) {
-          float4 &r = rec[(uint32_t)b * rec_stride + slot];
-          r = make_float4(c.x, c.y, c.z, r.w);
+  Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
+  trace_persistent<true, kLdsDepth>(
+      count, nodes, itris, st, overflow,
+      [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
+        const float4 ro = ray_o[i], rd = ray_d[i];
+        o = xyz(ro);
+        d = xyz(rd);
+        tmax = ro.w;
+      },
+      [&](uint32_t i, float, float, float, uint32_t id) {
+        const bool occ = id != GI_INVALID_REF;
+        if (kDeposit) {
+          if (!occ) {
+            const uint32_t slot = fbits(ray_d[i].w);
+            const float4 c = contrib[i];
+            float4 l = path_l[slot];
+            l.x = l.x + c.x;
+            l.y = l.y + c.y;
+            l.z = l.z + c.z;
+            path_l[slot] = l;
+            if (rec) {
+              float4 &r = rec[(uint32_t)b * rec_stride + slot];
+              r = make_float4(c.x, c.y, c.z, r.w);
+            }
+          }
+        } else {
+          occluded[i] = occ ? 1 : 0;
         }
-      }
-    } else {
-      occluded[i] = occ ? 1 : 0;
-    }
-  }
+      });
 }
 
 __global__ void k_pack_rays(const float *__restrict__ orig, const float *__restrict__ dir,
@@ -485,13 +621,13 @@ __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__rest
         if (rec) rec[(uint32_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, bitsf(clamp));
       }
     }
-    const uint32_t qn = wave_append(emit_next, next_count);
+    uint32_t qn, qs;
+    block_append2(emit_next, next_count, emit_shadow, shadow_count, qn, qs);
     if (emit_next) {
       nro[qn] = o_next;
       nrd[qn] = d_next;
       ntp[qn] = tp_next;
     }
-    const uint32_t qs = wave_append(emit_shadow, shadow_count);
     if (emit_shadow) {
       sho[qs] = s_o;
       shd[qs] = s_d;
@@ -729,7 +865,8 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
         }
       }
     }
-    const uint32_t qn = wave_append(emit_next, next_count);
+    uint32_t qn, unused;
+    block_append2(emit_next, next_count, false, next_count, qn, unused);
     if (emit_next) {
       nro[qn] = o_next;
       nrd[qn] = d_next;
@@ -777,6 +914,11 @@ __global__ void k_pass_setup(const PassDesc *__restrict__ pass, uint32_t *__rest
 }
 
 __global__ void k_fill4(float4 *__restrict__ p, uint32_t n, float4 v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -1045,24 +1187,29 @@ int check_scene(gi_ctx *ctx, bool need_shading) {
   return GI_OK;
 }
 
+// LDS bytes of a traversal block: kLds entries per lane, lane stride the
+// next power of two >= block.
+size_t trace_lds_bytes(int lds_depth, int block) {
+  return (size_t)lds_depth * ((size_t)1 << lds_shift_for((uint32_t)block)) * sizeof(uint32_t);
+}
+
 template <bool kHasTmax, int kLds>
 void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
                       const float4 *ro, const float4 *rd, float4 *hits) {
-  const size_t lds = (size_t)kLds * block * sizeof(uint32_t);
-  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kLds>), dim3(grid), dim3(block), lds, ctx->stream, count_ptr,
-                     count_host, ro, rd, hits, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
-                     ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>());
+  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kLds>), dim3(grid), dim3(block), trace_lds_bytes(kLds, block),
+                     ctx->stream, count_ptr, count_host, ro, rd, hits, ctx->d_nodes.as<WideNode>(),
+                     ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
+                     ctx->d_overflow.as<uint32_t>());
 }
 
 template <bool kDeposit, int kLds>
 void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
                   const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ, float4 *rec,
                   uint32_t rec_stride, int32_t b) {
-  const size_t lds = (size_t)kLds * block * sizeof(uint32_t);
-  hipLaunchKernelGGL((k_trace_any<kDeposit, kLds>), dim3(grid), dim3(block), lds, ctx->stream, count_ptr,
-                     count_host, ro, rd, contrib, pl, occ, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
-                     ctx->d_gstack.as<uint32_t>(), ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), rec, rec_stride,
-                     b);
+  hipLaunchKernelGGL((k_trace_any<kDeposit, kLds>), dim3(grid), dim3(block), trace_lds_bytes(kLds, block),
+                     ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ, ctx->d_nodes.as<WideNode>(),
+                     ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
+                     ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b);
 }
 
 int stack_lds_of(const gi_ctx *ctx) {
@@ -1096,15 +1243,18 @@ int launch_any(gi_ctx *ctx, bool deposit, const uint32_t *count_ptr, uint32_t co
   int rc = trace_grid_for(ctx, n_bound, grid, block);
   if (rc) return rc;
   const int L = stack_lds_of(ctx);
+#define GI_ANY(D, LL)                                                                                               \
+  launch_any_t<D, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b)
   if (deposit) {
-    if (L == 8) launch_any_t<true, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
-    else if (L == 32) launch_any_t<true, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
-    else launch_any_t<true, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
+    if (L == 8) GI_ANY(true, 8);
+    else if (L == 32) GI_ANY(true, 32);
+    else GI_ANY(true, 16);
   } else {
-    if (L == 8) launch_any_t<false, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
-    else if (L == 32) launch_any_t<false, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
-    else launch_any_t<false, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b);
+    if (L == 8) GI_ANY(false, 8);
+    else if (L == 32) GI_ANY(false, 32);
+    else GI_ANY(false, 16);
   }
+#undef GI_ANY
   GI_HIP(ctx, hipGetLastError());
   return GI_OK;
 }
@@ -1273,8 +1423,8 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
       }
       const uint64_t bound = sync_counts ? host_count : pd.total_slots;
       if (timing && (rc = mark(-1))) return rc;
-      rc = launch_closest(ctx, false, cnt + b, 0, bound, ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(),
-                          ctx->q_hit.as<float4>());
+      rc = launch_closest(ctx, false, cnt + b, 0, bound, ctx->q_o[cur].as<float4>(),
+                          ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
       if (rc) return rc;
       ++launches;
       if (timing && (rc = mark(0))) return rc;

@@ -88,3 +88,16 @@ def test_product_matches_oracle_bitwise(oracle):
     assert lib.gi_halton_permutation(50, 17, _lib.ptr(perm)) == 0
     assert np.array_equal(perm, oracle.halton_perm(50, 17))
     assert lib.gi_halton_prime(-1) == -1 and lib.gi_halton_prime(227) == 1439
+
+
+def test_every_dimension_needs_at_most_seven_chunks():
+    # the device splits an index n < 2^32 into at most 7 chunks of m digits,
+    # P = p^m <= 1024; that requires P^7 >= 2^32 for every dimension used
+    # (4 camera dimensions + 7 per vertex, 32 vertices)
+    lib = _lib.load()
+    for dim in range(4 + 7 * 32):
+        p = lib.gi_halton_prime(dim)
+        P = p
+        while P * p <= 1024:
+            P *= p
+        assert P ** 7 >= 2 ** 32, (dim, p, P)
