@@ -27,6 +27,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s incl. shadow rays + adjoint grads/s at fixed RMSE, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+# Memory traffic of one k_trace_closest launch of this workload at the
+# default tuning (one pass, 8 launches per frame, 18.7 M rays per launch),
+# from two rocprofv3 counter runs of `bench.py --steps 1 --warmup 0 --no-cpu
+# --no-batching --no-adjoint --copy-mib 512`: FETCH_SIZE 3,083,962 KiB and
+# WRITE_SIZE 507,188 KiB per dispatch, each scaled by what the counter
+# reads for the 512 MiB device copy (FETCH 262,168 KiB -> x2.000, WRITE
+# 524,288 KiB -> x1.000).  FETCH counts L2 misses to the fabric, so
+# Infinity Cache hits are included: an upper bound of the HBM bytes.
+TRACE_CLOSEST_TRAFFIC_BYTES = (3083962.2 * 1.99982 + 507187.8 * 1.0) * 1024
 
 
 def parse():
@@ -48,10 +57,12 @@ def parse():
 def batching_leg(steps: int, tune: bool):
     """16 views of the cornell scene (resolutions from 256x256 to 1920x1080,
     4/16/64 spp, depth 8, Russian roulette from 3): the per-view wavefront
-    (host reads each bounce's queue length and sizes the launches, one view
-    after the other) against the batched renderer (all views in shared
-    queues, device-side lengths, the whole frame replayed as one hipGraph,
-    autotuned launch parameters).  Both produce the same image bits."""
+    scheduled as the reference schedules it (one view after the other, one
+    sample per pixel per wavefront, the host reading each bounce's queue
+    length; src/gpu-pt.h:102-226) against the batched renderer (all views
+    and samples in shared queues, device-side lengths, the whole frame
+    replayed as one hipGraph, autotuned launch parameters).  Both run the
+    same kernels and produce the same image bits."""
     import numpy as np
 
     from globillum_amd import autotune, scenes
@@ -61,9 +72,11 @@ def batching_leg(steps: int, tune: bool):
     sc = scenes.config_scene(3)
     plain = sc.opts.copy()
     plain["flags"] = 0
+    plain["samples_per_pass"] = 1
     batched = sc.opts.copy()
     batched["flags"] = FLAG_BATCHED | FLAG_GRAPH
-    out = {"workload": "16 views of the cornell scene, 256x256..1920x1080, spp 4/16/64, depth 8, RR from 3",
+    out = {"workload": "16 views of the cornell scene, 256x256..1920x1080, spp 4/16/64, depth 8, RR from 3; "
+                       "plain = per view, 1 sample per pixel per wavefront, host-synchronised bounces",
            "samples": int(sum(int(v["w"]) * int(v["h"]) * int(v["spp"]) for v in sc.views))}
     with Context(0) as ctx:
         ctx.upload(sc)
@@ -210,7 +223,8 @@ def main():
         avg_launch_s = tot["ms_trace_closest"] / tot["launches_closest"] / 1e3
         achieved = bytes_per_launch / avg_launch_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(TRACE_CLOSEST_TRAFFIC_BYTES) if tot["launches_closest"] == 8 * a.steps else None,
                 "kernel": "k_trace_closest", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                 "algorithmic_bytes_per_ray": round(b_ray, 1)}
     out = {

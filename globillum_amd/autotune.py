@@ -13,12 +13,12 @@ import time
 
 SPACE = {
     "trace_block": [64, 128, 256],
-    "trace_blocks_per_cu": [4, 8, 16],
+    "trace_blocks_per_cu": [4, 5, 6, 8],
     "stack_lds": [8, 16, 32],
     "shade_block": [64, 128, 256],
-    "target_slots": [2, 4, 8, 16],
+    "target_slots": [16, 32, 64],
 }
-DEFAULT = {"trace_block": 256, "trace_blocks_per_cu": 8, "stack_lds": 16, "shade_block": 256, "target_slots": 4}
+DEFAULT = {"trace_block": 256, "trace_blocks_per_cu": 5, "stack_lds": 16, "shade_block": 256, "target_slots": 64}
 
 
 def _time(ctx, views, opts, repeats: int) -> float:

@@ -1001,7 +1001,7 @@ struct gi_ctx {
   int32_t prog_sample = 0, prog_max = 0;
   bool prog_active = false;
   gi_counters counters{};
-  gi_tuning tuning{256, 8, 256, 0, 16, 0};
+  gi_tuning tuning{256, 5, 256, 0, 16, 0};
   std::vector<hipEvent_t> events;
 };
 
@@ -1638,7 +1638,9 @@ int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_ren
 
 size_t target_slots(gi_ctx *ctx) {
   const int t = ctx->tuning.target_slots;
-  return (size_t)(t > 0 ? std::min(t, 64) : 4) << 20;
+  // default: 64 Mi paths in flight (10.7 GB of path state), so a
+  // 1280x720x64 render is one pass and every launch is as wide as it gets
+  return (size_t)(t > 0 ? std::min(t, 64) : 64) << 20;
 }
 
 }  // namespace
@@ -2060,7 +2062,7 @@ int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t) {
   if (!ctx || !t) return fail(ctx, GI_ERR_INVALID_ARG, "tuning");
   gi_tuning n = *t;
   if (n.trace_block <= 0) n.trace_block = 256;
-  if (n.trace_blocks_per_cu <= 0) n.trace_blocks_per_cu = 8;
+  if (n.trace_blocks_per_cu <= 0) n.trace_blocks_per_cu = 5;
   if (n.shade_block <= 0) n.shade_block = 256;
   if (n.stack_lds <= 0) n.stack_lds = 16;
   // every kernel is compiled with __launch_bounds__(256)

@@ -157,7 +157,7 @@ typedef struct {
   int32_t shade_block;
   int32_t samples_per_pass;
   int32_t stack_lds;        /* traversal stack entries per lane in LDS: 8, 16 or 32 */
-  int32_t target_slots;     /* paths in flight per pass, in units of 2^20; 0 = 4 */
+  int32_t target_slots;     /* paths in flight per pass, in units of 2^20; 0 = 64 */
 } gi_tuning;
 
 /* Per-material gradient of the adjoint pass. */
