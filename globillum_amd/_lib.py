@@ -91,6 +91,15 @@ SIGNATURES = {
     "gi_halton_prime": (_I32, [_I32]),
     "gi_halton_permutation": (_I, [_I32, _U32, _P]),
     "gi_rect_light_power": (_F, [_P]),
+    "gi_obj_open": (_I, [C.POINTER(_P)]),
+    "gi_obj_append": (_I, [_P, C.c_char_p, _P]),
+    "gi_obj_apply_pbrdf": (_I, [_P, C.c_char_p]),
+    "gi_obj_sizes": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "gi_obj_export": (_I, [_P, _P, _P]),
+    "gi_obj_material_name": (C.c_char_p, [_P, _I32]),
+    "gi_obj_error": (C.c_char_p, [_P]),
+    "gi_obj_free": (None, [_P]),
+    "gi_pbrdf_read": (_I, [C.c_char_p, _P]),
 }
 
 _lib = None

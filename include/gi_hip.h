@@ -266,6 +266,36 @@ int gi_halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm);
  * (src/lights.cpp:92). */
 float gi_rect_light_power(const gi_light *l);
 
+/* ---- Scene ingestion (host functions, no device needed) -------------------
+ * Wavefront OBJ / MTL into the triangle and material tables, as
+ * add_objfile_to_flat_tri_list builds them (src/hl-main.cpp:277-350):
+ * triangles in file order; material 0 is the fallback "gi/fallback"
+ * (diffuse = specular = (1,0,0)) registered with the first file, used by
+ * faces without a (defined) material; other materials are registered once
+ * per name on first use with diffuse Kd, specular Ks and alpha d; texture
+ * coordinates are kept only when the material names a diffuse texture.
+ * Several files append to one table (the reference's multi-file ftl).
+ * trafo (optional, 16 floats row-major) maps positions as M (x,y,z,1) and
+ * normals by the inverse transpose of its 3x3 part.  Faces without normals
+ * get their geometric normal; polygons are split into fans. */
+typedef struct gi_scene_file gi_scene_file;
+int gi_obj_open(gi_scene_file **out);
+int gi_obj_append(gi_scene_file *f, const char *obj_path, const float *trafo);
+/* Principled parameters for every material that has <dir>/<name>.pbrdf
+ * (convert_and_upload_materials, src/material-conversion.cpp:59-151). */
+int gi_obj_apply_pbrdf(gi_scene_file *f, const char *dir);
+int gi_obj_sizes(const gi_scene_file *f, int64_t *n_tris, int32_t *n_mats);
+int gi_obj_export(const gi_scene_file *f, gi_tri *tris, gi_material *mats);
+const char *gi_obj_material_name(const gi_scene_file *f, int32_t i);
+const char *gi_obj_error(const gi_scene_file *f);
+void gi_obj_free(gi_scene_file *f);
+/* One .pbrdf file (PrincipledBRDFParameters::readFrom,
+ * src/principledParameters.h:64-125) into *m: principled = 1, the named
+ * parameters set, the others at the default constructor's values (opacity
+ * 1, transmission 1, others 0; color = m's diffuse colour).  A malformed
+ * number is GI_ERR_INVALID_ARG. */
+int gi_pbrdf_read(const char *path, gi_material *m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,3 +150,41 @@ def test_render_argument_errors(gpu_ctx_factory):
         v["spp"] = 0
         with pytest.raises(GiError):
             ctx.render(v)
+
+
+def _obj_room(path):
+    """A room of quads (open front), a box and a pyramid with its own MTL;
+    some faces without normals."""
+    path.joinpath("room.mtl").write_text(
+        "newmtl wall\nKd 0.7 0.7 0.7\nnewmtl redwall\nKd 0.7 0.1 0.1\nnewmtl gold\nKd 0.8 0.6 0.2\n")
+    v = ["0 0 0", "4 0 0", "4 3 0", "0 3 0", "0 0 4", "4 0 4", "4 3 4", "0 3 4",   # room 1-8
+         "1 0 2", "2 0 2", "2 1 2", "1 1 2", "1 0 3", "2 0 3", "2 1 3", "1 1 3",   # box 9-16
+         "3 0 1", "3.8 0 1", "3.4 0 1.8", "3.4 1.5 1.4"]                           # pyramid 17-20
+    # windings give the geometric normals inside the room and outside the solids
+    f = ["usemtl wall", "f 1 5 6 2", "f 4 3 7 8", "f 5 8 7 6",
+         "usemtl redwall", "f 1 4 8 5",
+         "usemtl wall", "f 2 6 7 3",
+         "usemtl gold", "f 9 12 11 10", "f 13 14 15 16", "f 9 13 16 12", "f 10 11 15 14", "f 12 16 15 11",
+         "usemtl nosuchmaterial", "f 17 20 18", "f 18 20 19", "f 19 20 17"]
+    path.joinpath("room.obj").write_text("mtllib room.mtl\n" + "".join(f"v {x}\n" for x in v) + "\n".join(f) + "\n")
+    return str(path / "room.obj")
+
+
+def test_obj_scene_bit_exact(oracle, gpu_ctx_factory, tmp_path):
+    from globillum_amd import scene_io
+
+    mats_dir = tmp_path / "materials"
+    mats_dir.mkdir()
+    (mats_dir / "gold.pbrdf").write_text("color 0.9 0.7 0.3 roughness 0.3 metallic 1.0 specular 0.5\n")
+    view = scenes.make_view((2.0, 1.5, -3.0), (2.0, 1.2, 2.0), (0, 1, 0), 50.0, 64, 48, 4)
+    light = scenes.make_light((2.0, 2.95, 2.0), (0, -1, 0), (8, 8, 8), 1.0, 1.0)
+    opts = scenes.make_opts(4)
+    sc = scene_io.obj_scene(_obj_room(tmp_path), view, light, opts, pbrdf_dir=str(mats_dir))
+    assert sc.mats["principled"].tolist() == [0, 0, 0, 1]
+    ref, st = _oracle(oracle, sc)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render().image(0).reshape(-1, 3)
+        c = ctx.counters()
+    assert img.max() > 0
+    _same(img, ref)
+    assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
