@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--no-batching", action="store_true", help="skip the multi-view batching leg")
     p.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batching leg")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
+    p.add_argument("--no-bvh", action="store_true", help="skip the BVH construction leg")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (untimed): the known-size "
                         "kernel that scales FETCH_SIZE / WRITE_SIZE counter runs")
@@ -143,6 +144,52 @@ def adjoint_leg(steps: int):
            "grad_finite": bool(np.isfinite(gm).all() and np.isfinite(ge).all())}
     print(f"[bench] adjoint leg: fwd {out['ms_forward']} ms, adj {out['ms_adjoint']} ms", file=sys.stderr,
           flush=True)
+    return out
+
+
+def bvh_leg(steps: int):
+    """BVH construction: the host binned-SAH builder against the device
+    linear-BVH builder on the headline scene (291k triangles) and on a
+    2048x1024-cell heightfield (4.19M triangles, the large configuration's
+    terrain), plus the headline frame time traversing each tree."""
+    import numpy as np
+
+    from globillum_amd import scenes
+    from globillum_amd._lib import FLAG_BATCHED
+    from globillum_amd.context import Context
+
+    out = {}
+    sc = scenes.config_scene(2)
+    with Context(0) as ctx:
+        t0 = time.perf_counter()
+        ctx.upload(sc)  # host SAH (includes the upload of the tables)
+        host_ms = (time.perf_counter() - t0) * 1e3
+        frames = {}
+        for name in ("sah", "lbvh"):
+            if name == "lbvh":
+                ctx.build_bvh_device()  # warm (allocations, code objects)
+                t0 = time.perf_counter()
+                ctx.build_bvh_device()
+                dev_ms = (time.perf_counter() - t0) * 1e3
+            ctx.render(flags=FLAG_BATCHED)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.render(flags=FLAG_BATCHED)
+            frames[name] = (time.perf_counter() - t0) / steps * 1e3
+    out["terrain_spheres_291k"] = {"host_sah_ms_incl_upload": round(host_ms, 1), "device_lbvh_ms": round(dev_ms, 2),
+                                   "frame_ms_sah": round(frames["sah"], 2), "frame_ms_lbvh": round(frames["lbvh"], 2)}
+    rng = np.random.default_rng(5)
+    hf, _ = scenes.heightfield(rng, 2048, 1024, 200.0, 100.0, 10.0, 8)
+    big = scenes.Scene(hf, sc.mats, sc.lights, sc.views, sc.opts)
+    with Context(0) as ctx:
+        t0 = time.perf_counter()
+        ctx.upload(big)
+        host_ms = (time.perf_counter() - t0) * 1e3
+        ctx.build_bvh_device()
+        t0 = time.perf_counter()
+        ctx.build_bvh_device()
+        dev_ms = (time.perf_counter() - t0) * 1e3
+    out["heightfield_4194k"] = {"host_sah_ms_incl_upload": round(host_ms, 1), "device_lbvh_ms": round(dev_ms, 2)}
     return out
 
 
@@ -258,6 +305,8 @@ def main():
         out["batching"] = batching_leg(max(1, min(a.steps, 3)), not a.no_tune)
     if not a.no_adjoint:
         out["adjoint"] = adjoint_leg(max(1, min(a.steps, 3)))
+    if not a.no_bvh:
+        out["bvh_build"] = bvh_leg(max(1, min(a.steps, 3)))
     print(json.dumps(out), flush=True)
 
 

@@ -91,6 +91,7 @@ SIGNATURES = {
     "gi_halton_prime": (_I32, [_I32]),
     "gi_halton_permutation": (_I, [_I32, _U32, _P]),
     "gi_rect_light_power": (_F, [_P]),
+    "gi_build_bvh_device": (_I, [_P, _I32]),
     "gi_obj_open": (_I, [C.POINTER(_P)]),
     "gi_obj_append": (_I, [_P, C.c_char_p, _P]),
     "gi_obj_apply_pbrdf": (_I, [_P, C.c_char_p]),

@@ -13,7 +13,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libgi_hip.so")
-SOURCES = ["gi_kernels.hip", "bvh_build.cpp", "scene_io.cpp"]
+SOURCES = ["gi_kernels.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene_io.cpp"]
 
 # -ffp-contract=off: hit ids, sample directions and image values are
 # reproduced bit-for-bit on the host, so no a*b+c may be fused.

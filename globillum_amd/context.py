@@ -47,7 +47,11 @@ class Context:
         _check(self.lib, self.h, rc)
 
     # ---- scene ---------------------------------------------------------
-    def upload(self, scene, max_leaf: int = 4, bvh2: tuple | None = None):
+    def upload(self, scene, max_leaf: int = 4, bvh2: tuple | None = None, builder: str = "host"):
+        """Triangles, materials and lights of `scene`, and its BVH: the
+        host binned-SAH builder (builder="host"), the device LBVH builder
+        (builder="device"), or an external tree in the reference's 2 x
+        float4 layout (bvh2=(nodes, indices))."""
         tris = np.ascontiguousarray(scene.tris, TRI)
         self._ck(self.lib.gi_upload_triangles(self.h, ptr(tris), len(tris)))
         if scene.mats is not None and len(scene.mats):
@@ -56,7 +60,11 @@ class Context:
         if scene.lights is not None and len(scene.lights):
             lights = np.ascontiguousarray(scene.lights, LIGHT)
             self._ck(self.lib.gi_upload_lights(self.h, ptr(lights), len(lights)))
-        if bvh2 is None:
+        if bvh2 is None and builder == "device":
+            self._ck(self.lib.gi_build_bvh_device(self.h, max_leaf))
+        elif bvh2 is None:
+            if builder != "host":
+                raise ValueError(f"builder must be 'host' or 'device', not {builder!r}")
             self._ck(self.lib.gi_build_bvh(self.h, max_leaf))
         else:
             nodes, idx = bvh2
@@ -67,6 +75,10 @@ class Context:
         self.scene = dataclasses.replace(scene, mats=None if scene.mats is None else scene.mats.copy(),
                                          lights=None if scene.lights is None else scene.lights.copy())
         return self
+
+    def build_bvh_device(self, max_leaf: int = 4):
+        """Rebuild the BVH of the uploaded triangles on the device."""
+        self._ck(self.lib.gi_build_bvh_device(self.h, max_leaf))
 
     # ---- tracing primitives ---------------------------------------------
     def trace_closest(self, orig, dirs, max_t) -> np.ndarray:

@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "bvh_gpu.h"
 #include "gi_device.h"
 #include "gi_internal.h"
 
@@ -978,6 +979,7 @@ struct gi_ctx {
   int32_t wide_depth = 0;
   bool have_bvh = false;
   float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, diag = 0.f, power_total = 0.f;
+  float clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};  // bounds of the triangle centroids
   DBuf d_stris, d_itris, d_nodes, d_mats, d_lights;
   // Halton tables
   uint32_t hal_seed = 0;
@@ -1130,6 +1132,19 @@ int ensure_halton(gi_ctx *ctx, uint32_t seed, int32_t ndims) {
 
 int ensure_gstack(gi_ctx *ctx);
 
+// State that follows a new wide tree on the device.
+int finish_bvh(gi_ctx *ctx, int64_t nnodes, int32_t wide_depth) {
+  ctx->nnodes = nnodes;
+  ctx->wide_depth = wide_depth;
+  // at most three pushes per wide level on the current path
+  ctx->stack_cap = (uint32_t)(3 * (wide_depth + 1) + 4);
+  ctx->trace_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
+  int rc = ensure_gstack(ctx);
+  if (rc) return rc;
+  ctx->have_bvh = true;
+  return GI_OK;
+}
+
 int upload_wide(gi_ctx *ctx) {
   std::vector<WideNode> wn;
   std::vector<IsectTri> it;
@@ -1140,16 +1155,7 @@ int upload_wide(gi_ctx *ctx) {
   GI_HIP(ctx, copy_sync(ctx, ctx->d_nodes.p, wn.data(), wn.size() * sizeof(WideNode), hipMemcpyHostToDevice));
   if (!it.empty())
     GI_HIP(ctx, copy_sync(ctx, ctx->d_itris.p, it.data(), it.size() * sizeof(IsectTri), hipMemcpyHostToDevice));
-  ctx->nnodes = (int64_t)wn.size();
-  ctx->wide_depth = wd;
-  // at most three pushes per wide level on the current path
-  ctx->stack_cap = (uint32_t)(3 * (wd + 1) + 4);
-
-  ctx->trace_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
-  int rc = ensure_gstack(ctx);
-  if (rc) return rc;
-  ctx->have_bvh = true;
-  return GI_OK;
+  return finish_bvh(ctx, (int64_t)wn.size(), wd);
 }
 
 uint32_t read_overflow(gi_ctx *ctx) {
@@ -1745,8 +1751,8 @@ int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n) {
   ctx->have_bvh = false;
   ctx->bvh.reset();
   for (int k = 0; k < 3; ++k) {
-    ctx->lo[k] = FLT_MAX;
-    ctx->hi[k] = -FLT_MAX;
+    ctx->lo[k] = ctx->clo[k] = FLT_MAX;
+    ctx->hi[k] = ctx->chi[k] = -FLT_MAX;
   }
   std::vector<ShadeTri> st(n);
   for (int64_t i = 0; i < n; ++i) {
@@ -1754,6 +1760,9 @@ int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n) {
     for (int k = 0; k < 3; ++k) {
       ctx->lo[k] = std::min(ctx->lo[k], std::min(t.a[k], std::min(t.b[k], t.c[k])));
       ctx->hi[k] = std::max(ctx->hi[k], std::max(t.a[k], std::max(t.b[k], t.c[k])));
+      const float cen = (t.a[k] + t.b[k] + t.c[k]) * (1.0f / 3.0f);
+      ctx->clo[k] = std::min(ctx->clo[k], cen);
+      ctx->chi[k] = std::max(ctx->chi[k], cen);
     }
     ShadeTri &s = st[i];
     int32_t mi = t.material_index;
@@ -1829,6 +1838,24 @@ int gi_build_bvh(gi_ctx *ctx, int32_t max_leaf) {
   ctx->bvh = std::make_unique<gi_bvh>();
   build_bvh2(ctx->tris.data(), (int64_t)ctx->tris.size(), max_leaf > 0 ? max_leaf : 4, *ctx->bvh);
   return upload_wide(ctx);
+}
+
+int gi_build_bvh_device(gi_ctx *ctx, int32_t max_leaf) {
+  if (!ctx || ctx->tris.empty()) return fail(ctx, GI_ERR_STATE, "upload triangles first");
+  const int ml = max_leaf > 0 ? max_leaf : 4;
+  if (ml > kMaxLeafTris) return fail(ctx, GI_ERR_INVALID_ARG, "max_leaf must be at most 15");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  const uint32_t n = (uint32_t)ctx->ntri;
+  GI_HIP(ctx, ctx->d_nodes.ensure(std::max<size_t>(1, n) * sizeof(WideNode)));
+  GI_HIP(ctx, ctx->d_itris.ensure(std::max<size_t>(1, n) * sizeof(IsectTri)));
+  ctx->bvh.reset();
+  ctx->have_bvh = false;
+  LbvhResult r;
+  std::string err;
+  if (!lbvh_build(ctx->stream, ctx->d_stris.as<float4>(), (uint32_t)(sizeof(ShadeTri) / sizeof(float4)), n, ctx->clo,
+                  ctx->chi, ml, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), r, err))
+    return fail(ctx, GI_ERR_HIP, err);
+  return finish_bvh(ctx, r.nnodes, r.depth);
 }
 
 int gi_upload_bvh2(gi_ctx *ctx, const float *nodes, int64_t n_nodes, const uint32_t *indices, int64_t n_indices) {

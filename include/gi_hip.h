@@ -195,6 +195,12 @@ int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n);
 int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n);
 /* Build the binned-SAH BVH on the host and upload it. */
 int gi_build_bvh(gi_ctx *ctx, int32_t max_leaf);
+/* The same tree format built on the device: a linear BVH over 30-bit
+ * Morton codes of the triangle centroids (the reference's GPU LBVH,
+ * src/main.cpp:218-225), refit bottom-up, collapsed into 4-wide nodes with
+ * leaves of at most max_leaf (<= 15) triangles.  Boxes are padded as the host
+ * builder pads them, so closest-hit results are the same with either tree. */
+int gi_build_bvh_device(gi_ctx *ctx, int32_t max_leaf);
 /* Upload an externally built binary BVH in the reference's 2 x float4 layout
  * (src/tracers.cu:26-36): nodes[8*n_nodes], indices[n_indices]. */
 int gi_upload_bvh2(gi_ctx *ctx, const float *nodes, int64_t n_nodes,

@@ -188,3 +188,13 @@ def test_obj_scene_bit_exact(oracle, gpu_ctx_factory, tmp_path):
     assert img.max() > 0
     _same(img, ref)
     assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
+
+
+def test_device_bvh_render_bit_exact(oracle, gpu_ctx_factory):
+    sc = scenes.terrain_spheres(cells=(64, 64), n_spheres=100, w=80, h=45, spp=4)
+    ref, st = _oracle(oracle, sc)
+    with gpu_ctx_factory(sc, builder="device") as ctx:
+        img = ctx.render().image(0).reshape(-1, 3)
+        c = ctx.counters()
+    _same(img, ref)
+    assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]

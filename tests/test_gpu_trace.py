@@ -131,3 +131,43 @@ def test_call_order_and_argument_errors(c1, gpu_ctx_factory):
         with pytest.raises(GiError) as e:
             ctx.upload(c1, bvh2=(bad_nodes, np.zeros(0, np.uint32)))
         assert e.value.status == 1
+
+
+# ---- device-built BVH (linear BVH, refit, 4-wide collapse) -----------------
+# Closest hit is the minimum (t, id) over all triangles, so the hits do not
+# depend on the tree: the device-built tree must give the oracle's answers.
+
+@pytest.mark.parametrize("max_leaf", [1, 4, 15])
+def test_device_bvh_closest_and_any(max_leaf, terrain, oracle, gpu_ctx_factory):
+    sc = terrain
+    rng = np.random.default_rng(5 + max_leaf)
+    o, d = _rays(rng, 200_000, -5, 105, 0, 20)
+    tmax = np.where(rng.random(len(o)) < 0.3, rng.uniform(0, 30, len(o)), FMAX).astype(np.float32)
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights)
+    ref = osc.trace_closest(o, d, tmax)
+    occ = osc.trace_any(o, d, tmax)
+    with gpu_ctx_factory(sc, builder="device", max_leaf=max_leaf) as ctx:
+        _assert_hits_equal(ctx.trace_closest(o, d, tmax), ref, osc, o, d, tmax)
+        assert np.array_equal(ctx.trace_any(o, d, tmax), occ)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17])
+def test_device_bvh_tiny_and_degenerate(n, oracle, gpu_ctx_factory):
+    rng = np.random.default_rng(n)
+    sc = scenes.cornell_boxes(w=8, h=8, spp=1)
+    tris = sc.tris[:n].copy() if n <= len(sc.tris) else np.concatenate([sc.tris] * 2)[:n].copy()
+    if n == 17:
+        # many triangles with one centroid: equal Morton keys, ties broken by index
+        base = tris[0].copy()
+        for i in range(n):
+            tris[i] = base
+            rot = rng.permutation(3)
+            for k, src in zip("abc", rot):
+                tris[i][k] = base["abc"[src]]
+    o, d = _rays(rng, 20_000, -0.2, 1.2)
+    t = np.full(len(o), FMAX, np.float32)
+    osc = oracle.OracleScene(tris, sc.mats, sc.lights)
+    ref = osc.trace_closest(o, d, t)
+    one = scenes.Scene(tris, sc.mats, sc.lights, sc.views, sc.opts)
+    with gpu_ctx_factory(one, builder="device") as ctx:
+        _assert_hits_equal(ctx.trace_closest(o, d, t), ref, osc, o, d, t)
