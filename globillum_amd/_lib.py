@@ -101,6 +101,8 @@ SIGNATURES = {
     "gi_obj_error": (C.c_char_p, [_P]),
     "gi_obj_free": (None, [_P]),
     "gi_pbrdf_read": (_I, [C.c_char_p, _P]),
+    "gi_write_png": (_I, [C.c_char_p, _P, _I32, _I32]),
+    "gi_write_exr": (_I, [C.c_char_p, _P, _I32, _I32]),
 }
 
 _lib = None

@@ -13,7 +13,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libgi_hip.so")
-SOURCES = ["gi_kernels.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene_io.cpp"]
+SOURCES = ["gi_kernels.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene_io.cpp", "image_io.cpp"]
 
 # -ffp-contract=off: hit ids, sample directions and image values are
 # reproduced bit-for-bit on the host, so no a*b+c may be fused.
@@ -63,7 +63,7 @@ def build(verbose: bool = False) -> str:
     """Compile every source into libgi_hip.so and the host binaries (always
     recompiles)."""
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIP_FLAGS, "-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+    cmd = [hipcc(), *HIP_FLAGS, "-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

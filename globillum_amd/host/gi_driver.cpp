@@ -6,10 +6,11 @@
 //   "GISC" u32 version=1, u64 n_tris, u32 n_mats, u32 n_lights, u32 n_views,
 //   gi_tri[n_tris], gi_material[n_mats], gi_light[n_lights],
 //   gi_view[n_views], gi_render_opts
-// Output: PFM (3 floats per pixel, bottom row first, as the renderer's
-// row order y*w+x already is).
+// Output by extension: .png / .exr as the reference's save_image writes them
+// (gi_write_png / gi_write_exr), anything else PFM (3 floats per pixel,
+// bottom row first, as the renderer's row order y*w+x already is).
 //
-//   gi_driver scene.gisc out.pfm [--view i] [--spp n] [--depth d] [--list]
+//   gi_driver scene.gisc out.{pfm,png,exr} [--view i] [--spp n] [--depth d]
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -118,7 +119,18 @@ int main(int argc, char **argv) {
       ++steps;
     }
     double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    write_pfm(argv[2], pt.result(), v.w, v.h);
+    const std::string out = argv[2];
+    auto ends = [&](const char *ext) {
+      const size_t n = std::strlen(ext);
+      return out.size() >= n && out.compare(out.size() - n, n, ext) == 0;
+    };
+    if (ends(".png") || ends(".exr")) {
+      const int rc = ends(".png") ? gi_write_png(out.c_str(), pt.result().data(), v.w, v.h)
+                                  : gi_write_exr(out.c_str(), pt.result().data(), v.w, v.h);
+      if (rc != GI_OK) throw std::runtime_error("cannot write " + out);
+    } else {
+      write_pfm(out, pt.result(), v.w, v.h);
+    }
     gi_counters c = pt.counters();
     std::printf("{\"steps\": %d, \"ms\": %.3f, \"rays_closest\": %llu, \"rays_shadow\": %llu, \"samples\": %llu}\n",
                 steps, ms, (unsigned long long)c.rays_closest, (unsigned long long)c.rays_shadow,

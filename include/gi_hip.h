@@ -302,6 +302,15 @@ void gi_obj_free(gi_scene_file *f);
  * number is GI_ERR_INVALID_ARG. */
 int gi_pbrdf_read(const char *path, gi_material *m);
 
+/* ---- Image output (save_image, src/util.cpp:111-150; host functions) -------
+ * rgb: w*h*3 floats, row-major, row 0 first as gi_get_image returns them.
+ * Pixel (x, y) is written at (w-1-x, h-1-y), as the reference's writer
+ * flips both axes.  PNG: 8-bit RGB, clamp(int(255*v), 0, 255) per channel
+ * (no gamma).  EXR: RGBA half, A = 1, uncompressed scanlines, float to half
+ * rounded to nearest even. */
+int gi_write_png(const char *path, const float *rgb, int32_t w, int32_t h);
+int gi_write_exr(const char *path, const float *rgb, int32_t w, int32_t h);
+
 #ifdef __cplusplus
 }
 #endif

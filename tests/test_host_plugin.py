@@ -4,6 +4,7 @@ image must equal the oracle's bit for bit."""
 import json
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -50,3 +51,26 @@ def test_driver_progressive_render_matches_oracle(tmp_path, oracle):
     ref, st = oracle.OracleScene(sc.tris, sc.mats, sc.lights).render(sc.views, sc.opts)
     assert np.array_equal(img, ref)
     assert info["rays_closest"] == st["rays_closest"] and info["rays_shadow"] == st["rays_shadow"]
+
+
+@pytest.mark.gpu
+def test_driver_writes_png_and_exr(tmp_path, oracle):
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_image_io import _read_exr, _read_png
+
+    sc = scenes.cornell_boxes(w=40, h=30, spp=2, depth=3)
+    path = tmp_path / "c1.gisc"
+    scenes.save_scene(sc, str(path))
+    ref, _ = oracle.OracleScene(sc.tris, sc.mats, sc.lights).render(sc.views, sc.opts)
+    ref = ref.reshape(30, 40, 3)
+    for ext in ("png", "exr"):
+        out = tmp_path / f"c1.{ext}"
+        r = subprocess.run([os.path.join(PKG, "gi_driver"), str(path), str(out)], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        if ext == "png":
+            want = np.where(ref > 0, np.minimum(np.trunc(255.0 * ref), 255), 0).astype(np.uint8)[::-1, ::-1]
+            assert np.array_equal(_read_png(out), want)
+        else:
+            got = _read_exr(out)
+            assert np.array_equal(got[..., 3], ref.astype(np.float16)[::-1, ::-1, 0])
