@@ -91,6 +91,8 @@ SIGNATURES = {
     "gi_halton_prime": (_I32, [_I32]),
     "gi_halton_permutation": (_I, [_I32, _U32, _P]),
     "gi_rect_light_power": (_F, [_P]),
+    "gi_sky_light_power": (_F, [_F, _F]),
+    "gi_upload_sky": (_I, [_P, _P, _I32, _I32]),
     "gi_build_bvh_device": (_I, [_P, _I32]),
     "gi_obj_open": (_I, [C.POINTER(_P)]),
     "gi_obj_append": (_I, [_P, C.c_char_p, _P]),

@@ -60,6 +60,10 @@ class Context:
         if scene.lights is not None and len(scene.lights):
             lights = np.ascontiguousarray(scene.lights, LIGHT)
             self._ck(self.lib.gi_upload_lights(self.h, ptr(lights), len(lights)))
+        sky = getattr(scene, "sky", None)
+        if sky is not None:
+            m = np.ascontiguousarray(sky, np.float32)
+            self._ck(self.lib.gi_upload_sky(self.h, ptr(m), m.shape[1], m.shape[0]))
         if bvh2 is None and builder == "device":
             self._ck(self.lib.gi_build_bvh_device(self.h, max_leaf))
         elif bvh2 is None:

@@ -75,6 +75,50 @@ __device__ __forceinline__ void sincos2pi(float v, float &s, float &c) {
   else { s = -pc; c = ps; }
 }
 
+// ----------------------------------------------------------- atan2, acos --
+// atan(t) / (2 pi) for t in [0, 1] as t * P(t^2), P of degree 8: a
+// least-squares fit (relative error < 2e-7) evaluated with basic operations
+// only, so the host restatement gives the same bits.
+__host__ __device__ __forceinline__ float atan_turns01(float t) {
+  const float x2 = t * t;
+  float p = 0.000454627268f;
+  p = p * x2 + -0.00256048539f;
+  p = p * x2 + 0.00679757446f;
+  p = p * x2 + -0.0119441384f;
+  p = p * x2 + 0.0169349238f;
+  p = p * x2 + -0.0226053596f;
+  p = p * x2 + 0.0318191461f;
+  p = p * x2 + -0.0530512258f;
+  p = p * x2 + 0.159154937f;
+  return t * p;
+}
+// atan2(y, x) / (2 pi) in [-1/2, 1/2]; 0 for (0, 0).
+__host__ __device__ __forceinline__ float atan2_turns(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  if (ax == 0.0f && ay == 0.0f) return 0.0f;
+  float r = ay > ax ? 0.25f - atan_turns01(ax / ay) : atan_turns01(ay / ax);
+  if (x < 0.0f) r = 0.5f - r;
+  return y < 0.0f ? -r : r;
+}
+
+// Distribution1D::sample (src/lights.h:50-57): upper_bound over cdf[0,size),
+// index clamped to [0,size-1], in-cell position t = (cdf[i+1]-u) /
+// (cdf[i+1]-cdf[i]), x = (1.0-t)*i + t*(i+1) (first product in double, the
+// second in float, as the reference's expression evaluates).
+__device__ __forceinline__ float cdf_sample(const float *__restrict__ cdf, const float *__restrict__ pdf, int size,
+                                            float u, float &p) {
+  int lo = 0, hi = size;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] <= u) lo = mid + 1;
+    else hi = mid;
+  }
+  const int idx = min(max(lo - 1, 0), size - 1);
+  const float t = (cdf[idx + 1] - u) / (cdf[idx + 1] - cdf[idx]);
+  p = pdf[idx];
+  return (float)((1.0 - (double)t) * (double)idx + (double)(t * (float)(idx + 1)));
+}
+
 // ---------------------------------------------------------------- Halton --
 struct HaltonDev {
   const HaltonDim *dims;

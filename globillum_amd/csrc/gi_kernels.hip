@@ -67,8 +67,18 @@ struct alignas(16) DevLight {
   float center[3], power;
   float dir[3], w;
   float right[3], h;
-  float up[3], pad;
+  float up[3];
+  int32_t type;  // GI_LIGHT_RECT / GI_LIGHT_SKY (scale in col[0])
   float col[3], pad2;
+};
+
+// Sky map of the scene's sky light and the tables of its Distribution2D.
+struct SkyDev {
+  const float4 *rgb;                  // w*h texels (r, g, b, 0)
+  const float *ycdf, *ypdf;           // h+1, h
+  const float *xcdf, *xpdf;           // h*(w+1), h*w
+  int32_t w, h, light;                // light: index of the sky light, -1 = none
+  float scale;
 };
 
 struct ShadeParams {
@@ -81,6 +91,7 @@ struct ShadeParams {
   float off_b, off_s;
   float bg[3];
   int32_t depth, rr_start;
+  SkyDev sky;
 };
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -469,6 +480,48 @@ __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pas
 // NEE contribution is evaluated with the lobe the bounce sample selects (as
 // evaluate_material does after sample_material).  Shared by the forward
 // shading kernel and the adjoint replay, so both see identical values.
+// Sky light sample for NEE (pixel_generate_arealight_sample_cpu with
+// USE_SKY_SAMPLING, src/arealight-sampler-cpu.cpp:87-96; sky_light::sample,
+// src/lights.h:121-139): (u, v) from the map's Distribution2D; direction
+// (sin t cos p, sin t sin p, cos t) with t = u pi / h, p = v 2 pi / w, turned
+// into the frame around the normal (make_tangential); pdf = pdf_uv w h /
+// (2 pi^2 sin t); contribution = scale * L / pdf * |dir . N| with the
+// unnormalised NEE normal, as for rect lights.
+__device__ __forceinline__ void sky_sample(const SkyDev &S, float x0, float x1, f3 Ni, f3 &dir, f3 &contrib) {
+  float yp, xp;
+  const float u = cdf_sample(S.ycdf, S.ypdf, S.h, x0, yp);
+  const int row = min(max((int)u, 0), S.h - 1);
+  const float v = cdf_sample(S.xcdf + (size_t)row * (S.w + 1), S.xpdf + (size_t)row * S.w, S.w, x1, xp);
+  const float pdf = yp * xp;
+  float sth, cth, sph, cph;
+  sincos2pi(u / (2.0f * (float)S.h), sth, cth);
+  sincos2pi(v / (float)S.w, sph, cph);
+  const f3 dm = f3{sth * cph, sth * sph, cth};
+  const float outpdf = (float)((double)(pdf * (float)(S.w * S.h)) / (2.0 * M_PI * M_PI * (double)sth));
+  const int ix = min(max((int)v, 0), S.w - 1);
+  const float4 L = S.rgb[(size_t)row * S.w + ix];
+  dir = tangential(dm, normalize(Ni));
+  const float a = 1.0f / outpdf;
+  const float cosn = fabsf(dot(dir, Ni));
+  contrib = f3{((S.scale * L.x) * a) * cosn, ((S.scale * L.y) * a) * cosn, ((S.scale * L.z) * a) * cosn};
+}
+
+// Sky radiance seen by a ray that leaves the scene (evaluateSkyLight,
+// src/hybrid-pt.cpp:175-189): theta = acos(d.y), phi = atan2(d.z, d.x) in
+// [0, 2 pi), texel (int(t h), int(s w)); row h (d.y = -1) is black as the
+// reference's range check makes it; the column is clamped to w-1.  Unscaled.
+__device__ __forceinline__ f3 sky_eval(const SkyDev &S, f3 d) {
+  const float yc = smin(smax(d.y, -1.0f), 1.0f);
+  const float st = sqrtf((1.0f - yc) * (1.0f + yc));
+  const float t = 2.0f * atan2_turns(st, yc);
+  float sx = atan2_turns(d.z, d.x);
+  if (sx < 0.0f) sx = sx + 1.0f;
+  const int iy = (int)(t * (float)S.h), ix = min((int)(sx * (float)S.w), S.w - 1);
+  if (iy < 0 || iy >= S.h || ix < 0) return f3{0.f, 0.f, 0.f};
+  const float4 L = S.rgb[(size_t)iy * S.w + ix];
+  return f3{L.x, L.y, L.z};
+}
+
 struct VertexEval {
   f3 P, N, org_dir, wi, wo_ts, wi_ts, xp;
   f3 sd, so, psc, fnee;
@@ -502,17 +555,25 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
   if (k >= P.nlights) k = P.nlights - 1;
   ve.light = k;
   const DevLight &lt = P.lights[k];
-  const f3 ldir = ld3(lt.dir);
-  const float offx = (xl.x - 0.5f) * lt.w, offy = (xl.y - 0.5f) * lt.h;
-  const f3 ls = add(add(ld3(lt.center), scl(ld3(lt.right), offx)), scl(ld3(lt.up), offy));
-  f3 sd = sub(ls, Pp);
-  const float len = length(sd);
-  sd = divs(sd, len);
-  const float ndotl = smax(dot(Ni, sd), 0.0f);
-  const float lcos = smax(dot(ldir, neg(sd)), 0.0f);
-  const float factor = (((lt.w * lt.h) * ndotl) * lcos) / (len * len);
-  const f3 contrib = scl(ld3(lt.col), factor);
-  sd = normalize(sd);
+  f3 sd, contrib;
+  float len, factor = 0.0f;
+  if (lt.type == GI_LIGHT_SKY) {
+    sky_sample(P.sky, xl.x, xl.y, Ni, sd, contrib);
+    len = FLT_MAX;
+    sd = normalize(sd);
+  } else {
+    const f3 ldir = ld3(lt.dir);
+    const float offx = (xl.x - 0.5f) * lt.w, offy = (xl.y - 0.5f) * lt.h;
+    const f3 ls = add(add(ld3(lt.center), scl(ld3(lt.right), offx)), scl(ld3(lt.up), offy));
+    sd = sub(ls, Pp);
+    len = length(sd);
+    sd = divs(sd, len);
+    const float ndotl = smax(dot(Ni, sd), 0.0f);
+    const float lcos = smax(dot(ldir, neg(sd)), 0.0f);
+    factor = (((lt.w * lt.h) * ndotl) * lcos) / (len * len);
+    contrib = scl(ld3(lt.col), factor);
+    sd = normalize(sd);
+  }
   ve.sd = sd;
   ve.len = len;
   ve.so = add(Pp, scl(sd, P.off_s));
@@ -586,6 +647,13 @@ __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__rest
           l.z = l.z + bz;
           path_l[slot] = l;
           if (rec) rec[(uint32_t)b * rec_stride + slot] = make_float4(bx, by, bz, bitsf(kRecBackground));
+        } else if (P.sky.light >= 0) {
+          const f3 Ls = sky_eval(P.sky, org_dir);
+          float4 l = path_l[slot];
+          l.x = l.x + TP.x * Ls.x;
+          l.y = l.y + TP.y * Ls.y;
+          l.z = l.z + TP.z * Ls.z;
+          path_l[slot] = l;
         }
       } else {
         VertexEval ve;
@@ -987,6 +1055,8 @@ struct gi_ctx {
   DBuf d_hdims, d_hchunks;
   // traversal stack + overflow flag
   DBuf d_gstack, d_overflow;
+  DBuf d_sky_rgb, d_sky_x, d_sky_y;  // sky map, per-row pdf|cdf, row pdf|cdf
+  int32_t sky_light = -1, sky_w = 0, sky_h = 0;
   uint32_t stack_cap = 0;
   int32_t trace_grid = 0;
   // render state
@@ -1189,6 +1259,7 @@ int check_scene(gi_ctx *ctx, bool need_shading) {
   if (need_shading) {
     if (ctx->mats.empty()) return fail(ctx, GI_ERR_STATE, "no materials uploaded");
     if (ctx->lights.empty()) return fail(ctx, GI_ERR_STATE, "no lights uploaded");
+    if (ctx->sky_light >= 0 && ctx->sky_w <= 0) return fail(ctx, GI_ERR_STATE, "sky light without a map (gi_upload_sky)");
   }
   return GI_OK;
 }
@@ -1361,6 +1432,10 @@ ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
   for (int k = 0; k < 3; ++k) P.bg[k] = ctx->opts.background[k];
   P.depth = depth;
   P.rr_start = ctx->opts.rr_start;
+  P.sky = SkyDev{ctx->d_sky_rgb.as<float4>(), ctx->d_sky_y.as<float>() + ctx->sky_h, ctx->d_sky_y.as<float>(),
+                 ctx->d_sky_x.as<float>() + (size_t)ctx->sky_w * ctx->sky_h, ctx->d_sky_x.as<float>(),
+                 ctx->sky_w, ctx->sky_h, ctx->sky_light,
+                 ctx->sky_light >= 0 ? ctx->lights[(size_t)ctx->sky_light].col[0] : 0.0f};
   return P;
 }
 
@@ -1806,12 +1881,19 @@ int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n) {
   GI_HIP(ctx, hipSetDevice(ctx->device));
   std::vector<DevLight> dl(n);
   float total = 0.f;
+  int32_t sky = -1;
   for (int32_t i = 0; i < n; ++i) {
     const gi_light &l = lights[i];
-    if (l.type != GI_LIGHT_RECT) return fail(ctx, GI_ERR_INVALID_ARG, "only rect lights are supported");
+    if (l.type != GI_LIGHT_RECT && l.type != GI_LIGHT_SKY)
+      return fail(ctx, GI_ERR_INVALID_ARG, "light type must be GI_LIGHT_RECT or GI_LIGHT_SKY");
+    if (l.type == GI_LIGHT_SKY) {
+      if (sky >= 0) return fail(ctx, GI_ERR_INVALID_ARG, "at most one sky light");
+      sky = i;
+    }
     if (!(l.power > 0.f)) return fail(ctx, GI_ERR_INVALID_ARG, "light power must be > 0");
     DevLight &d = dl[i];
     std::memset(&d, 0, sizeof(d));
+    d.type = l.type;
     for (int k = 0; k < 3; ++k) {
       d.center[k] = l.center[k];
       d.dir[k] = l.dir[k];
@@ -1827,8 +1909,56 @@ int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n) {
   }
   ctx->lights.assign(lights, lights + n);
   ctx->power_total = total;
+  if (sky != ctx->sky_light) ctx->sky_w = ctx->sky_h = 0;  // a new sky light needs its map
+  ctx->sky_light = sky;
   GI_HIP(ctx, ctx->d_lights.ensure(n * sizeof(DevLight)));
   GI_HIP(ctx, copy_sync(ctx, ctx->d_lights.p, dl.data(), n * sizeof(DevLight), hipMemcpyHostToDevice));
+  return GI_OK;
+}
+
+int gi_upload_sky(gi_ctx *ctx, const float *rgb, int32_t w, int32_t h) {
+  if (!ctx || !rgb || w <= 0 || h <= 0) return fail(ctx, GI_ERR_INVALID_ARG, "sky map");
+  if (ctx->sky_light < 0) return fail(ctx, GI_ERR_STATE, "upload a GI_LIGHT_SKY light first");
+  if ((int64_t)w * h > (1 << 26)) return fail(ctx, GI_ERR_LIMIT, "sky map larger than 2^26 texels");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  // initSkylight (src/lights.h:107-116) and Distribution1D::init
+  // (src/lights.h:27-44), in float, in order
+  auto init = [](const float *data, int size, float *pdf, float *cdf) {
+    float sum = 0.0f;
+    for (int i = 0; i < size; ++i) sum += data[i];
+    float scale = 0.0f;
+    if (sum > 0.0f) scale = 1.0f / sum;
+    cdf[0] = 0.0f;
+    for (int i = 0; i < size; ++i) {
+      pdf[i] = data[i] * scale;
+      if (i > 0) cdf[i] = cdf[i - 1] + pdf[i - 1];
+    }
+    cdf[size] = 1.0f;
+    return sum;
+  };
+  const size_t n = (size_t)w * h;
+  std::vector<float> imp(n), rows(h), xpdf(n), xcdf((size_t)(w + 1) * h), ypdf(h), ycdf(h + 1);
+  std::vector<float4> tex(n);
+  for (int32_t y = 0; y < h; ++y)
+    for (int32_t x = 0; x < w; ++x) {
+      const float *c = rgb + 3 * ((size_t)y * w + x);
+      imp[(size_t)y * w + x] = std::sin((float)(M_PI * (y + 0.5f) / h)) * ((c[0] + c[1]) + c[2]);
+      tex[(size_t)y * w + x] = make_float4(c[0], c[1], c[2], 0.f);
+    }
+  for (int32_t y = 0; y < h; ++y) rows[y] = init(&imp[(size_t)y * w], w, &xpdf[(size_t)y * w], &xcdf[(size_t)y * (w + 1)]);
+  init(rows.data(), h, ypdf.data(), ycdf.data());
+  GI_HIP(ctx, ctx->d_sky_rgb.ensure(n * sizeof(float4)));
+  GI_HIP(ctx, ctx->d_sky_x.ensure((n + (size_t)(w + 1) * h) * sizeof(float)));
+  GI_HIP(ctx, ctx->d_sky_y.ensure((size_t)(2 * h + 1) * sizeof(float)));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_sky_rgb.p, tex.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_sky_x.p, xpdf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_sky_x.as<float>() + n, xcdf.data(), xcdf.size() * sizeof(float),
+                        hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_sky_y.p, ypdf.data(), h * sizeof(float), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_sky_y.as<float>() + h, ycdf.data(), (h + 1) * sizeof(float),
+                        hipMemcpyHostToDevice));
+  ctx->sky_w = w;
+  ctx->sky_h = h;
   return GI_OK;
 }
 
@@ -2115,6 +2245,7 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
   if (!dL_dimage || !d_mats || !d_emission) return fail(ctx, GI_ERR_INVALID_ARG, "adjoint buffers");
   if (ctx->views.empty()) return fail(ctx, GI_ERR_STATE, "gi_render first");
   if (ctx->opts.rr_start > 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support Russian roulette");
+  if (ctx->sky_light >= 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support sky lights");
   const int32_t nm = (int32_t)ctx->mats.size(), nl = (int32_t)ctx->lights.size();
   if ((size_t)(5 * nm + 3 * nl) * sizeof(float) > 48 * 1024)
     return fail(ctx, GI_ERR_LIMIT, "adjoint: 5*materials + 3*lights must fit 48 KiB of LDS");
@@ -2190,6 +2321,11 @@ int gi_halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm) {
   if (dim < 0 || dim >= 1024 || !perm) return GI_ERR_INVALID_ARG;
   halton_permutation(dim, seed, perm);
   return GI_OK;
+}
+
+float gi_sky_light_power(float scale, float diameter) {
+  // l.power = s*4*M_PI*(d*.5)*(d*.5) (src/lights.cpp:104), evaluated in double
+  return (float)((double)(scale * 4.0f) * M_PI * ((double)diameter * 0.5) * ((double)diameter * 0.5));
 }
 
 float gi_rect_light_power(const gi_light *l) { return l ? rect_power(*l) : 0.0f; }

@@ -26,6 +26,7 @@ class Scene:
     opts: np.ndarray
     name: str = ""
     meta: dict = field(default_factory=dict)
+    sky: np.ndarray | None = None  # (h, w, 3) float32 map of the scene's sky light
 
 
 def rect_light_power(w: float, h: float, col) -> np.float32:
@@ -43,6 +44,23 @@ def make_light(center, direction, col, w, h) -> np.ndarray:
     l["wh"] = (w, h)
     l["power"] = rect_light_power(w, h, col)
     l["type"] = 0
+    return l
+
+
+def sky_light_power(scale: float, diameter: float) -> np.float32:
+    """add-skylight's selection weight s*4*pi*(d*.5)^2 (src/lights.cpp:104),
+    evaluated in double as there."""
+    s4 = np.float32(np.float32(scale) * np.float32(4.0))
+    d = float(np.float32(diameter))
+    return np.float32(float(s4) * np.pi * (d * 0.5) * (d * 0.5))
+
+
+def make_sky_light(scale: float, diameter: float) -> np.ndarray:
+    """A GI_LIGHT_SKY record: scale in col[0]; its map goes in Scene.sky."""
+    l = np.zeros(1, LIGHT)
+    l["col"] = (scale, 0.0, 0.0)
+    l["power"] = sky_light_power(scale, diameter)
+    l["type"] = 1
     return l
 
 
@@ -268,6 +286,40 @@ def terrain_spheres(seed: int = 2, cells=(256, 256), n_spheres: int = 1000, n_ma
         lights.append(make_light((lx, 40.0, lz), (0.0, -1.0, 0.0), (40.0, 40.0, 40.0), 8.0, 8.0))
     view = make_view((0.5 * size, 40.0, 0.0), (0.5 * size, 0.0, 0.5 * size), (0, 1, 0), 50.0, w, h, spp)
     return Scene(tris, mats, np.concatenate(lights), view, make_opts(depth), name="terrain_spheres")
+
+
+def sky_map(w: int = 64, h: int = 32, seed: int = 3) -> np.ndarray:
+    """A procedural sky: bright zenith-to-horizon gradient, a small sun and
+    some noise, (h, w, 3) float32, row 0 at theta = 0."""
+    rng = np.random.default_rng(seed)
+    th = (np.arange(h) + 0.5) / h * np.pi
+    ph = (np.arange(w) + 0.5) / w * 2 * np.pi
+    T, Pp = np.meshgrid(th, ph, indexing="ij")
+    base = np.stack([0.4 + 0.6 * np.cos(T / 2) ** 2, 0.6 + 0.4 * np.cos(T / 2) ** 2, np.ones_like(T)], -1)
+    sun = np.exp(-(((T - 0.6) / 0.08) ** 2 + ((Pp - 1.0) / 0.15) ** 2))[..., None] * np.array([40.0, 36.0, 30.0])
+    return (base * (0.8 + 0.4 * rng.random((h, w, 1))) + sun).astype(np.float32)
+
+
+def sky_scene(w=96, h=64, spp=4, depth=4, with_rect=True, seed=4) -> Scene:
+    """Ground plane, a few boxes and a UV sphere under a sky light (and
+    optionally a rect light)."""
+    rng = np.random.default_rng(seed)
+    ground = np.array([[(-10, 0, -10), (-10, 0, 10), (10, 0, 10), (10, 0, -10)]], np.float32)
+    tris = [_quads_to_tris(ground, np.array([(0, 1, 0)], np.float32), np.zeros(1, np.int32))]
+    for i in range(4):
+        lo = np.array([rng.uniform(-4, 3), 0.0, rng.uniform(-2, 5)], np.float32)
+        ext = rng.uniform(0.5, 2.0, 3).astype(np.float32)
+        bq, bn = _box_quads(lo, lo + ext, inward=False)
+        tris.append(_quads_to_tris(bq, bn, np.full(6, 1 + i % 3, np.int32)))
+    tris.append(uv_sphere((1.0, 1.2, 0.0), 1.2, material=3))
+    mats = np.concatenate([lambert((0.6, 0.6, 0.6)), lambert((0.7, 0.3, 0.2)), lambert((0.2, 0.5, 0.7)),
+                           principled((0.8, 0.7, 0.5), roughness=0.4, metallic=1.0)])
+    lights = [make_sky_light(0.5, 30.0)]
+    if with_rect:
+        lights.append(make_light((0.0, 6.0, 2.0), (0, -1, 0), (30, 30, 30), 1.0, 1.0))
+    view = make_view((0.0, 3.0, -9.0), (0.0, 1.0, 2.0), (0, 1, 0), 55.0, w, h, spp)
+    return Scene(np.concatenate(tris), mats, np.concatenate(lights), view, make_opts(depth),
+                 name="sky_scene", sky=sky_map())
 
 
 def save_scene(scene: Scene, path: str) -> None:

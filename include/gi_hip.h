@@ -91,16 +91,21 @@ typedef struct {
   int32_t texture;          /* reserved: -1 (textures are not on this path) */
 } gi_material;
 
-/* Light: gi::light with the rect_light member (src/lights.h:94-154).
- * power is the selection weight; gi_rect_light_power() gives the value the
- * reference computes in add-rectlight (src/lights.cpp:92). */
+/* Light: gi::light with the rect_light or sky_light member
+ * (src/lights.h:94-154).  power is the selection weight; gi_rect_light_power()
+ * gives the value the reference computes in add-rectlight
+ * (src/lights.cpp:92), gi_sky_light_power() the add-skylight value
+ * (src/lights.cpp:98-110).  A GI_LIGHT_SKY light keeps its scale in col[0]
+ * (the other fields are unused); its map comes with gi_upload_sky.  At most
+ * one sky light per scene. */
 typedef struct {
   float center[3], dir[3], up[3], col[3];
   float wh[2];
   float power;
-  int32_t type;             /* GI_LIGHT_RECT */
+  int32_t type;             /* GI_LIGHT_RECT or GI_LIGHT_SKY */
 } gi_light;
 #define GI_LIGHT_RECT 0
+#define GI_LIGHT_SKY 1
 
 /* One view of a render: camera as given to camera_ray_generator_shirley::
  * setup(pos, dir, up, fovy) (src/gpu-pt.cpp:67) plus the per-view image
@@ -193,6 +198,13 @@ int gi_bvh_export2(const gi_bvh *bvh, float *nodes, uint32_t *indices);
 int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n);
 int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n);
 int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n);
+/* Map of the sky light (rgb: w*h*3 floats, row y = 0 at theta = 0), as
+ * load_skylight_data reads it (src/lights.cpp:58-65).  Builds the sampling
+ * tables of initSkylight (src/lights.h:107-116).  NEE samples it by
+ * importance (sky_light::sample, src/lights.h:121-139); a path that leaves
+ * the scene after a bounce picks up the map's radiance (evaluateSkyLight,
+ * src/hybrid-pt.cpp:175-189, unscaled as there). */
+int gi_upload_sky(gi_ctx *ctx, const float *rgb, int32_t w, int32_t h);
 /* Build the binned-SAH BVH on the host and upload it. */
 int gi_build_bvh(gi_ctx *ctx, int32_t max_leaf);
 /* The same tree format built on the device: a linear BVH over 30-bit
@@ -271,6 +283,8 @@ int gi_halton_permutation(int32_t dim, uint32_t seed, uint16_t *perm);
 /* Light selection weight of a rect light, as add-rectlight computes it
  * (src/lights.cpp:92). */
 float gi_rect_light_power(const gi_light *l);
+/* scale * 4 pi (diameter/2)^2, as add-skylight computes it. */
+float gi_sky_light_power(float scale, float diameter);
 
 /* ---- Scene ingestion (host functions, no device needed) -------------------
  * Wavefront OBJ / MTL into the triangle and material tables, as

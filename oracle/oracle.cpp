@@ -228,6 +228,10 @@ struct or_scene {
   std::vector<uint32_t> indices;
   float lo[3], hi[3];
   float power_total = 0.f;
+  /* sky map of the scene's GI_LIGHT_SKY light (gi::sky_light, src/lights.h:99-142):
+   * rgb w*h, and the tables of its Distribution2D (row CDF/PDF, per-row CDF/PDF) */
+  int32_t sky_w = 0, sky_h = 0, sky_light = -1;
+  std::vector<float> sky_rgb, sky_ycdf, sky_ypdf, sky_xcdf, sky_xpdf;
 };
 
 namespace {
@@ -415,6 +419,52 @@ inline void sincos2pi(float v, float &s, float &c) {
 inline void sincos2pi(double v, double &s, double &c) {
   s = std::sin(K<double>::two_pi * v);
   c = std::cos(K<double>::two_pi * v);
+}
+
+/* atan(t) / (2 pi) for t in [0, 1] as t * P(t^2), P of degree 8: a
+ * least-squares fit (relative error < 2e-7) -- basic operations only, so the
+ * device evaluates the same bits.  atan2 and acos below are built on it. */
+inline float atan_turns01(float t) {
+  const float c0 = 0.159154937f, c1 = -0.0530512258f, c2 = 0.0318191461f, c3 = -0.0226053596f,
+              c4 = 0.0169349238f, c5 = -0.0119441384f, c6 = 0.00679757446f, c7 = -0.00256048539f,
+              c8 = 0.000454627268f;
+  const float x2 = t * t;
+  float p = c8;
+  p = p * x2 + c7;
+  p = p * x2 + c6;
+  p = p * x2 + c5;
+  p = p * x2 + c4;
+  p = p * x2 + c3;
+  p = p * x2 + c2;
+  p = p * x2 + c1;
+  p = p * x2 + c0;
+  return t * p;
+}
+inline double atan_turns01(double t) { return std::atan(t) / K<double>::two_pi; }
+/* atan2(y, x) / (2 pi), in [-1/2, 1/2]; 0 for (0, 0). */
+template <class R> inline R atan2_turns(R y, R x) {
+  const R ax = std::fabs(x), ay = std::fabs(y);
+  if (ax == R(0) && ay == R(0)) return R(0);
+  R r = ay > ax ? R(0.25) - atan_turns01(ax / ay) : atan_turns01(ay / ax);
+  if (x < R(0)) r = R(0.5) - r;
+  return y < R(0) ? -r : r;
+}
+
+/* Distribution1D::sample (src/lights.h:50-57): upper_bound over CDF[0,size),
+ * index clamped to [0,size-1], then the in-cell position as the reference
+ * writes it, t = (CDF[i+1]-u)/(CDF[i+1]-CDF[i]), x = (1.0-t)*i + t*(i+1)
+ * (the first product in double, the second in float). */
+inline float cdf_sample(const float *cdf, const float *pdf, int size, float u, float &p) {
+  int lo = 0, hi = size;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (cdf[mid] <= u) lo = mid + 1;
+    else hi = mid;
+  }
+  const int idx = std::min(std::max(lo - 1, 0), size - 1);
+  const float t = (cdf[idx + 1] - u) / (cdf[idx + 1] - cdf[idx]);
+  p = pdf[idx];
+  return (float)((1.0 - (double)t) * (double)idx + (double)(t * (float)(idx + 1)));
 }
 
 /* cosineSampleHemisphere, src/materialBase.h:16-24 */
@@ -655,6 +705,55 @@ Options resolve(const or_scene &s, const gi_render_opts &o) {
   return r;
 }
 
+/* Sky light sample for NEE (pixel_generate_arealight_sample_cpu with
+ * USE_SKY_SAMPLING, src/arealight-sampler-cpu.cpp:87-96; sky_light::sample,
+ * src/lights.h:121-139): (u, v) from the map's Distribution2D; direction
+ * (sin t cos p, sin t sin p, cos t) with t = u pi / h, p = v 2 pi / w, turned
+ * into the frame around the normal (make_tangential); pdf = pdf_uv w h /
+ * (2 pi^2 sin t); contribution = scale * L / pdf * |dir . N| with the
+ * unnormalised NEE normal, as for rect lights.  scale is col[0]. */
+template <class R>
+void sky_sample(const or_scene &s, R x0, R x1, V3<R> Ni, V3<R> &dir, V3<R> &contrib) {
+  const int w = s.sky_w, h = s.sky_h;
+  float yp, xp;
+  const float u = cdf_sample(s.sky_ycdf.data(), s.sky_ypdf.data(), h, (float)x0, yp);
+  const int row = std::min(std::max((int)u, 0), h - 1);
+  const float v =
+      cdf_sample(s.sky_xcdf.data() + (size_t)row * (w + 1), s.sky_xpdf.data() + (size_t)row * w, w, (float)x1, xp);
+  const float pdf = yp * xp;
+  R sth, cth, sph, cph;
+  sincos2pi(R(u) / (R(2) * R(h)), sth, cth);
+  sincos2pi(R(v) / R(w), sph, cph);
+  const V3<R> dm = mk<R>(sth * cph, sth * sph, cth);
+  const R outpdf = R((double)(pdf * (float)(w * h)) / (2.0 * M_PI * M_PI * (double)sth));
+  const int ix = std::min(std::max((int)v, 0), w - 1);
+  const float *L = &s.sky_rgb[3 * ((size_t)row * w + ix)];
+  dir = tangential(dm, normalize(Ni));
+  const R a = R(1) / outpdf;
+  const R scale = R(s.lights[s.sky_light].col[0]);
+  const R cosn = std::fabs(dot(dir, Ni));
+  contrib = mk<R>(((scale * R(L[0])) * a) * cosn, ((scale * R(L[1])) * a) * cosn, ((scale * R(L[2])) * a) * cosn);
+}
+
+/* Sky radiance seen by a ray that leaves the scene (evaluateSkyLight,
+ * src/hybrid-pt.cpp:175-189): theta = acos(d.y), phi = atan2(d.z, d.x) in
+ * [0, 2 pi), texel (int(t h), int(s w)); a row index of h (d.y = -1) gives
+ * black, as the reference's range check does; the column is clamped to w-1.
+ * Unscaled, as there. */
+template <class R> V3<R> sky_eval(const or_scene &s, V3<R> d) {
+  if (s.sky_light < 0) return mk<R>(0, 0, 0);
+  const int w = s.sky_w, h = s.sky_h;
+  const R yc = std::min(std::max(d.y, R(-1)), R(1));
+  const R st = std::sqrt((R(1) - yc) * (R(1) + yc));
+  const R t = R(2) * atan2_turns(st, yc);
+  R sx = atan2_turns(d.z, d.x);
+  if (sx < R(0)) sx = sx + R(1);
+  const int iy = (int)(t * R(h)), ix = std::min((int)(sx * R(w)), w - 1);
+  if (iy < 0 || iy >= h || ix < 0) return mk<R>(0, 0, 0);
+  const float *L = &s.sky_rgb[3 * ((size_t)iy * w + ix)];
+  return mk<R>(R(L[0]), R(L[1]), R(L[2]));
+}
+
 /* One sample of one pixel: the hybrid_pt state machine (closest hit, NEE,
  * any hit, bounce; src/hybrid-pt.h:151-249) run to completion for a single
  * path.  Returns the path's accumulated colour (path_accum_color). */
@@ -676,6 +775,8 @@ V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H,
        * a path whose throughput is exactly white gets the background. */
       if (TP.x == R(1) && TP.y == R(1) && TP.z == R(1))
         L = L + mul(TP, mk<R>(R(op.bg[0]), R(op.bg[1]), R(op.bg[2])));
+      else if (s.sky_light >= 0)
+        L = L + mul(TP, sky_eval(s, org_dir));
       break;
     }
     ++st.vertices;
@@ -694,18 +795,26 @@ V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H,
     while (k < nl && choice > acc + R(s.lights[k].power)) { acc += R(s.lights[k].power); ++k; }
     if (k >= nl) k = nl - 1;
     const gi_light &lt = s.lights[k];
-    V3<R> ldir = ld<R>(lt.dir);
-    V3<R> right = tangential(mk<R>(1, 0, 0), ldir), up = tangential(mk<R>(0, 1, 0), ldir);
-    R offx = (xl.x - R(0.5)) * R(lt.wh[0]), offy = (xl.y - R(0.5)) * R(lt.wh[1]);
-    V3<R> ls = (ld<R>(lt.center) + right * offx) + up * offy;
-    V3<R> sd = ls - P;
-    R len = length(sd);
-    sd = divs(sd, len);
-    R ndotl = std::max(dot(Ni, sd), R(0));
-    R lcos = std::max(dot(ldir, -sd), R(0));
-    R factor = (((R(lt.wh[0]) * R(lt.wh[1])) * ndotl) * lcos) / (len * len);
-    V3<R> contrib = ld<R>(lt.col) * factor;
-    sd = normalize(sd);
+    V3<R> sd, contrib;
+    R len;
+    if (lt.type == GI_LIGHT_SKY) {
+      sky_sample(s, xl.x, xl.y, Ni, sd, contrib);
+      len = R(FLT_MAX);
+      sd = normalize(sd);
+    } else {
+      V3<R> ldir = ld<R>(lt.dir);
+      V3<R> right = tangential(mk<R>(1, 0, 0), ldir), up = tangential(mk<R>(0, 1, 0), ldir);
+      R offx = (xl.x - R(0.5)) * R(lt.wh[0]), offy = (xl.y - R(0.5)) * R(lt.wh[1]);
+      V3<R> ls = (ld<R>(lt.center) + right * offx) + up * offy;
+      sd = ls - P;
+      len = length(sd);
+      sd = divs(sd, len);
+      R ndotl = std::max(dot(Ni, sd), R(0));
+      R lcos = std::max(dot(ldir, -sd), R(0));
+      R factor = (((R(lt.wh[0]) * R(lt.wh[1])) * ndotl) * lcos) / (len * len);
+      contrib = ld<R>(lt.col) * factor;
+      sd = normalize(sd);
+    }
     V3<R> so = P + sd * R(op.off_s);
     V3<R> psc = contrib * (R(s.power_total) / R(lt.power));
 
@@ -1345,9 +1454,69 @@ void or_render_f64(const or_scene *s, const gi_view *views, int32_t n_views, con
   render_impl<double, double>(*s, views, n_views, *opts, images, nullptr, threads, 0, -1);
 }
 
+/* The sky map of the scene's (single) GI_LIGHT_SKY light, and its sampling
+ * tables: gi::sky_light::initSkylight (src/lights.h:107-116) -- importance
+ * sin(pi (y + 0.5) / h) * (r + g + b) -- and Distribution1D::init
+ * (src/lights.h:27-44), in float, in order. */
+int or_scene_set_sky(or_scene *s, const float *rgb, int32_t w, int32_t h) {
+  int sky = -1, count = 0;
+  for (size_t i = 0; i < s->lights.size(); ++i)
+    if (s->lights[i].type == GI_LIGHT_SKY) { sky = (int)i; ++count; }
+  if (count != 1 || !rgb || w <= 0 || h <= 0) return 1;
+  s->sky_light = sky;
+  s->sky_w = w;
+  s->sky_h = h;
+  s->sky_rgb.assign(rgb, rgb + 3 * (size_t)w * h);
+  auto init = [](const float *data, int size, float *pdf, float *cdf) {
+    float sum = 0.0f;
+    for (int i = 0; i < size; ++i) sum += data[i];
+    float scale = 0.0f;
+    if (sum > 0.0f) scale = 1.0f / sum;
+    cdf[0] = 0.0f;
+    for (int i = 0; i < size; ++i) {
+      pdf[i] = data[i] * scale;
+      if (i > 0) cdf[i] = cdf[i - 1] + pdf[i - 1];
+    }
+    cdf[size] = 1.0f;
+    return sum;
+  };
+  std::vector<float> imp((size_t)w * h), rows(h);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const float *c = &rgb[3 * ((size_t)y * w + x)];
+      imp[(size_t)y * w + x] = std::sin((float)(M_PI * (y + 0.5f) / h)) * ((c[0] + c[1]) + c[2]);
+    }
+  s->sky_xpdf.assign((size_t)w * h, 0.f);
+  s->sky_xcdf.assign((size_t)(w + 1) * h, 0.f);
+  for (int y = 0; y < h; ++y)
+    rows[y] = init(&imp[(size_t)y * w], w, &s->sky_xpdf[(size_t)y * w], &s->sky_xcdf[(size_t)y * (w + 1)]);
+  s->sky_ypdf.assign(h, 0.f);
+  s->sky_ycdf.assign(h + 1, 0.f);
+  init(rows.data(), h, s->sky_ypdf.data(), s->sky_ycdf.data());
+  return 0;
+}
+
+void or_sky_eval(const or_scene *s, const float *d, float *rgb) {
+  V3<float> c = sky_eval(*s, ld<float>(d));
+  std::memcpy(rgb, &c, 12);
+}
+
+void or_sky_sample(const or_scene *s, const float *xi, const float *N, float *dir, float *contrib) {
+  V3<float> sd, c;
+  sky_sample(*s, xi[0], xi[1], ld<float>(N), sd, c);
+  std::memcpy(dir, &sd, 12);
+  std::memcpy(contrib, &c, 12);
+}
+
+float or_atan2_turns(float y, float x) { return atan2_turns(y, x); }
+
+float or_cdf_sample(const float *cdf, const float *pdf, int32_t size, float u, float *p) {
+  return cdf_sample(cdf, pdf, size, u, *p);
+}
+
 int or_adjoint(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
                const float *dL_dimage, double *d_mats, double *d_emission, int32_t threads, int32_t f64) {
-  if (opts->rr_start > 0 || opts->max_depth > 32) return 1;
+  if (opts->rr_start > 0 || opts->max_depth > 32 || s->sky_light >= 0) return 1;
   if (f64) adjoint_impl<double>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, threads);
   else adjoint_impl<float>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, threads);
   return 0;

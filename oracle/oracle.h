@@ -99,6 +99,14 @@ void or_render_f64(const or_scene *s, const gi_view *views, int32_t n_views,
 int or_adjoint(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
                const float *dL_dimage, double *d_mats, double *d_emission, int32_t threads, int32_t f64);
 
+/* Sky light (gi::sky_light): the map of the scene's single GI_LIGHT_SKY
+ * light (scale in col[0]); returns nonzero unless exactly one sky light. */
+int or_scene_set_sky(or_scene *s, const float *rgb, int32_t w, int32_t h);
+void or_sky_eval(const or_scene *s, const float *d, float *rgb);
+void or_sky_sample(const or_scene *s, const float *xi, const float *N, float *dir, float *contrib);
+float or_atan2_turns(float y, float x);
+float or_cdf_sample(const float *cdf, const float *pdf, int32_t size, float u, float *p);
+
 #ifdef __cplusplus
 }
 #endif

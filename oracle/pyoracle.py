@@ -65,6 +65,11 @@ def load():
             "or_render": (None, [P, P, I32, P, P, P, I32, I64, I64]),
             "or_render_f64": (None, [P, P, I32, P, P, I32]),
             "or_adjoint": (C.c_int, [P, P, I32, P, P, P, P, I32, I32]),
+            "or_scene_set_sky": (C.c_int, [P, P, I32, I32]),
+            "or_sky_eval": (None, [P, P, P]),
+            "or_sky_sample": (None, [P, P, P, P, P]),
+            "or_atan2_turns": (C.c_float, [C.c_float, C.c_float]),
+            "or_cdf_sample": (C.c_float, [P, P, I32, C.c_float, P]),
         }
         for k, (res, args) in sig.items():
             f = getattr(lib, k)
@@ -84,7 +89,7 @@ def _p(a):
 class OracleScene:
     """The oracle's scene: its own object-median BVH unless bvh2 is given."""
 
-    def __init__(self, tris, mats=None, lights=None, bvh2=None, max_leaf=4):
+    def __init__(self, tris, mats=None, lights=None, bvh2=None, max_leaf=4, sky=None):
         self.lib = load()
         self._keep = [np.ascontiguousarray(tris)]
         mats = None if mats is None else np.ascontiguousarray(mats)
@@ -100,6 +105,11 @@ class OracleScene:
         self.h = self.lib.or_scene_create(_p(self._keep[0]), len(tris), _p(mats), 0 if mats is None else len(mats),
                                           _p(lights), 0 if lights is None else len(lights), _p(nodes), nn, _p(idx),
                                           ni, max_leaf)
+        if sky is not None:
+            m = np.ascontiguousarray(sky, np.float32)
+            self._keep.append(m)
+            if self.lib.or_scene_set_sky(self.h, _p(m), m.shape[1], m.shape[0]) != 0:
+                raise ValueError("or_scene_set_sky: the scene needs exactly one sky light")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -179,6 +189,31 @@ def adjoint(osc: OracleScene, views, opts, dL, threads=0, f64=True):
     if rc != 0:
         raise ValueError("or_adjoint: unsupported options")
     return dm, de
+
+
+def sky_eval(osc: OracleScene, d):
+    d = np.ascontiguousarray(d, np.float32)
+    c = np.zeros(3, np.float32)
+    osc.lib.or_sky_eval(osc.h, _p(d), _p(c))
+    return c
+
+
+def sky_sample(osc: OracleScene, xi, N):
+    xi, N = np.ascontiguousarray(xi, np.float32), np.ascontiguousarray(N, np.float32)
+    d, c = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    osc.lib.or_sky_sample(osc.h, _p(xi), _p(N), _p(d), _p(c))
+    return d, c
+
+
+def atan2_turns(y, x):
+    return float(load().or_atan2_turns(y, x))
+
+
+def cdf_sample(cdf, pdf, u):
+    cdf, pdf = np.ascontiguousarray(cdf, np.float32), np.ascontiguousarray(pdf, np.float32)
+    p = C.c_float()
+    x = load().or_cdf_sample(_p(cdf), _p(pdf), len(pdf), u, C.byref(p))
+    return float(x), p.value
 
 
 def radical_inverse(n, base):

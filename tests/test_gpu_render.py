@@ -198,3 +198,31 @@ def test_device_bvh_render_bit_exact(oracle, gpu_ctx_factory):
         c = ctx.counters()
     _same(img, ref)
     assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
+
+
+@pytest.mark.parametrize("with_rect", [True, False])
+def test_sky_light_bit_exact(with_rect, oracle, gpu_ctx_factory):
+    sc = scenes.sky_scene(w=96, h=64, spp=4, depth=4, with_rect=with_rect)
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, sky=sc.sky)
+    ref, st = osc.render(sc.views, sc.opts)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render(flags=FLAG_BATCHED).image(0).reshape(-1, 3)
+        c = ctx.counters()
+    _same(img, ref)
+    assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
+
+
+def test_sky_light_errors(gpu_ctx_factory):
+    import dataclasses
+
+    sc = scenes.sky_scene(w=16, h=16, spp=1)
+    with gpu_ctx_factory(dataclasses.replace(sc, sky=None)) as ctx:
+        with pytest.raises(GiError, match="sky light without a map"):
+            ctx.render()
+    two = dataclasses.replace(sc, lights=np.concatenate([sc.lights, scenes.make_sky_light(1.0, 1.0)]))
+    with pytest.raises(GiError, match="at most one sky light"):
+        gpu_ctx_factory(two)
+    with gpu_ctx_factory(sc) as ctx:
+        ctx.render()
+        with pytest.raises(GiError, match="sky"):
+            ctx.adjoint(np.ones((16, 16, 3), np.float32))
