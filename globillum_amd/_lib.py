@@ -37,6 +37,7 @@ COUNTERS = np.dtype([("rays_closest", "<u8"), ("rays_shadow", "<u8"), ("samples"
                      ("launches_closest", "<i4"), ("passes", "<i4")])
 TUNING = np.dtype([("trace_block", "<i4"), ("trace_blocks_per_cu", "<i4"), ("shade_block", "<i4"),
                    ("samples_per_pass", "<i4"), ("stack_lds", "<i4"), ("target_slots", "<i4")])
+INSTANCE = np.dtype([("xform", "<f4", 12), ("mesh", "<i4"), ("material", "<i4")])
 MATGRAD = np.dtype([("d_color", "<f4", 3), ("d_roughness", "<f4"), ("d_metallic", "<f4")])
 
 assert TRI.itemsize == 100 and MATERIAL.itemsize == 96 and LIGHT.itemsize == 64
@@ -94,6 +95,8 @@ SIGNATURES = {
     "gi_sky_light_power": (_F, [_F, _F]),
     "gi_upload_sky": (_I, [_P, _P, _I32, _I32]),
     "gi_build_bvh_device": (_I, [_P, _I32]),
+    "gi_upload_mesh": (_I, [_P, _P, _I64, C.POINTER(C.c_int32)]),
+    "gi_upload_instances": (_I, [_P, _P, _I32]),
     "gi_obj_open": (_I, [C.POINTER(_P)]),
     "gi_obj_append": (_I, [_P, C.c_char_p, _P]),
     "gi_obj_apply_pbrdf": (_I, [_P, C.c_char_p]),

@@ -8,7 +8,7 @@ import dataclasses
 import numpy as np
 
 from . import _lib
-from ._lib import COUNTERS, HIT, LIGHT, MATERIAL, OPTS, TRI, TUNING, VIEW, GiError, ptr
+from ._lib import COUNTERS, HIT, INSTANCE, LIGHT, MATERIAL, OPTS, TRI, TUNING, VIEW, GiError, ptr
 
 
 def _check(lib, ctx, rc):
@@ -75,14 +75,37 @@ class Context:
             nodes = np.ascontiguousarray(nodes, np.float32)
             idx = np.ascontiguousarray(idx, np.uint32)
             self._ck(self.lib.gi_upload_bvh2(self.h, ptr(nodes), len(nodes) // 8, ptr(idx), len(idx)))
+        if getattr(scene, "instances", None) is not None and len(scene.instances):
+            self.upload_instances(scene.meshes, scene.instances)
         # own copy: set_materials / set_lights must not rewrite the caller's scene
         self.scene = dataclasses.replace(scene, mats=None if scene.mats is None else scene.mats.copy(),
                                          lights=None if scene.lights is None else scene.lights.copy())
         return self
 
     def build_bvh_device(self, max_leaf: int = 4):
-        """Rebuild the BVH of the uploaded triangles on the device."""
+        """Rebuild the BVH of the uploaded triangles on the device (a new
+        world tree drops the instances; the scene's are placed again)."""
         self._ck(self.lib.gi_build_bvh_device(self.h, max_leaf))
+        sc = self.scene
+        if sc is not None and getattr(sc, "instances", None) is not None and len(sc.instances):
+            self._ck(self.lib.gi_upload_instances(self.h, ptr(self._inst), len(self._inst)))
+
+    def upload_instances(self, meshes, instances):
+        """Meshes (object-space TRI arrays) and INSTANCE records placing
+        them over the world geometry (two-level BVH).  Call after the world
+        BVH; meshes uploaded before are kept, so the ids in `instances`
+        count from the first mesh of this scene."""
+        base = None
+        for m in meshes:
+            m = np.ascontiguousarray(m, TRI)
+            mid = C.c_int32()
+            self._ck(self.lib.gi_upload_mesh(self.h, ptr(m), len(m), C.byref(mid)))
+            base = mid.value if base is None else base
+        inst = np.array(instances, INSTANCE).reshape(-1)
+        if base:
+            inst["mesh"] += base
+        self._inst = inst
+        self._ck(self.lib.gi_upload_instances(self.h, ptr(inst), len(inst)))
 
     # ---- tracing primitives ---------------------------------------------
     def trace_closest(self, orig, dirs, max_t) -> np.ndarray:

@@ -6,6 +6,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <numeric>
 
 #include "gi_internal.h"
@@ -36,6 +37,8 @@ struct Box {
   }
 };
 
+}  // namespace
+
 // Relative padding that keeps float slab tests conservative for points on
 // the triangles: 2^-20 of the largest coordinate magnitude / extent.
 void pad_box(float *lo, float *hi) {
@@ -49,6 +52,8 @@ void pad_box(float *lo, float *hi) {
   }
 }
 
+namespace {
+
 constexpr int kBins = 32;
 constexpr int kSahDepth = 48;  // beyond this depth, split at the object median
 
@@ -60,6 +65,7 @@ struct Builder {
   std::vector<Bvh2Node> &nodes;
   int32_t max_leaf;
   int32_t max_depth = 0;
+  bool sah_leaves = true;  // false: split down to max_leaf primitives per leaf
 
   Builder(const gi_tri *t, int64_t n, int32_t ml, std::vector<uint32_t> &p, std::vector<Bvh2Node> &nd)
       : tris(t), tbox(n), cen(3 * n), perm(p), nodes(nd), max_leaf(ml) {
@@ -67,6 +73,15 @@ struct Builder {
       tbox[i].grow(t[i].a);
       tbox[i].grow(t[i].b);
       tbox[i].grow(t[i].c);
+      for (int k = 0; k < 3; ++k) cen[3 * i + k] = 0.5f * (tbox[i].lo[k] + tbox[i].hi[k]);
+    }
+  }
+  // over arbitrary boxes (lo xyz, hi xyz per primitive)
+  Builder(const float *boxes, int64_t n, int32_t ml, std::vector<uint32_t> &p, std::vector<Bvh2Node> &nd)
+      : tris(nullptr), tbox(n), cen(3 * n), perm(p), nodes(nd), max_leaf(ml) {
+    for (int64_t i = 0; i < n; ++i) {
+      tbox[i].grow(boxes + 6 * i);
+      tbox[i].grow(boxes + 6 * i + 3);
       for (int k = 0; k < 3; ++k) cen[3 * i + k] = 0.5f * (tbox[i].lo[k] + tbox[i].hi[k]);
     }
   }
@@ -137,7 +152,7 @@ struct Builder {
       float parent_area = bb.area();
       if (axis >= 0) {
         float sah = 1.0f + best / std::max(parent_area, 1e-30f);
-        if (sah >= leaf_cost && n <= (uint32_t)kMaxLeafTris) {
+        if (sah_leaves && sah >= leaf_cost && n <= (uint32_t)kMaxLeafTris) {
           make_leaf(node, b, e);
           return;
         }
@@ -202,6 +217,24 @@ void build_bvh2(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh &out) {
   out.max_depth = bld.max_depth;
 }
 
+void build_bvh2_boxes(const float *boxes, int64_t n, int32_t max_leaf, gi_bvh &out) {
+  out.nodes.clear();
+  out.perm.resize(n);
+  std::iota(out.perm.begin(), out.perm.end(), 0u);
+  max_leaf = std::max(1, std::min(max_leaf, kMaxLeafTris));
+  out.nodes.reserve(2 * n + 1);
+  out.nodes.resize(1);
+  if (n == 0) {
+    out.nodes[0] = Bvh2Node{{0, 0, 0}, {0, 0, 0}, -1, -1, 0, 0};
+    out.max_depth = 0;
+    return;
+  }
+  Builder bld(boxes, n, max_leaf, out.perm, out.nodes);
+  bld.sah_leaves = false;
+  bld.build(0, 0, (uint32_t)n, 0);
+  out.max_depth = bld.max_depth;
+}
+
 bool import_bvh2(const float *nodes, int64_t n_nodes, const uint32_t *indices, int64_t n_indices,
                  int64_t n_tris, gi_bvh &out, std::string &err) {
   // Reference layout (src/tracers.cu:26-36): x bit0 = inner flag,
@@ -260,10 +293,10 @@ bool import_bvh2(const float *nodes, int64_t n_nodes, const uint32_t *indices, i
   return true;
 }
 
-void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &wnodes,
-                   std::vector<IsectTri> &leaf_tris, int32_t &wide_depth) {
+void leaf_order_tris(const gi_bvh &b, const gi_tri *tris, std::vector<IsectTri> &leaf_tris) {
   // Leaf-ordered intersection triangles: e1 = b - a, e2 = c - a in float,
-  // the same rounding the oracle's per-test subtraction produces.
+  // the same rounding the oracle's per-test subtraction produces; a.w holds
+  // the triangle's index in tris.
   leaf_tris.resize(b.perm.size());
   for (size_t i = 0; i < b.perm.size(); ++i) {
     const gi_tri &t = tris[b.perm[i]];
@@ -277,16 +310,16 @@ void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &w
     o.e1[3] = 0.f;
     o.e2[3] = 0.f;
   }
+}
+
+void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh2Node &)> &leaf_ref,
+                         uint32_t node_base, std::vector<WideNode> &wnodes, int32_t &wide_depth) {
   wnodes.clear();
   wide_depth = 0;
   const auto &N = b.nodes;
   auto area = [&](int32_t i) {
     float dx = N[i].hi[0] - N[i].lo[0], dy = N[i].hi[1] - N[i].lo[1], dz = N[i].hi[2] - N[i].lo[2];
     return dx * dy + dy * dz + dz * dx;
-  };
-  auto leaf_ref = [&](int32_t i) -> uint32_t {
-    if (N[i].count == 0) return kEmptyRef;
-    return kLeafBit | ((uint32_t)N[i].count << 27) | (uint32_t)N[i].first;
   };
   // BFS over binary nodes that become wide nodes.
   std::vector<std::pair<int32_t, int32_t>> work;  // (binary node, wide slot)
@@ -328,18 +361,26 @@ void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &w
       nd.lox[k] = N[c].lo[0]; nd.loy[k] = N[c].lo[1]; nd.loz[k] = N[c].lo[2];
       nd.hix[k] = N[c].hi[0]; nd.hiy[k] = N[c].hi[1]; nd.hiz[k] = N[c].hi[2];
       if (N[c].left < 0) {
-        nd.child[k] = leaf_ref(c);
+        nd.child[k] = N[c].count == 0 ? kEmptyRef : leaf_ref(N[c]);
       } else {
         int32_t slot = (int32_t)wnodes.size();
         wnodes.emplace_back();
         wdepth.push_back(wdepth[wi] + 1);
         wide_depth = std::max(wide_depth, wdepth.back());
-        nd.child[k] = (uint32_t)slot;
+        nd.child[k] = node_base + (uint32_t)slot;
         work.push_back({c, slot});
       }
     }
     wnodes[wi] = nd;
   }
+}
+
+void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &wnodes,
+                   std::vector<IsectTri> &leaf_tris, int32_t &wide_depth) {
+  leaf_order_tris(b, tris, leaf_tris);
+  collapse_wide_nodes(
+      b, [](const Bvh2Node &n) { return kLeafBit | ((uint32_t)n.count << 27) | (uint32_t)n.first; }, 0u, wnodes,
+      wide_depth);
 }
 
 // ------------------------------------------------------------- Halton ----

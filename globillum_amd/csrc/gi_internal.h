@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,26 @@ bool import_bvh2(const float *nodes, int64_t n_nodes, const uint32_t *indices, i
 // and emit the leaf-ordered intersection triangles.
 void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &nodes,
                    std::vector<IsectTri> &leaf_tris, int32_t &wide_depth);
+// The pieces of collapse_wide: leaf-ordered triangles (a.w = index in tris),
+// and the 4-wide nodes with interior references offset by node_base and
+// leaf references from leaf_ref (called for leaves with count > 0).
+void leaf_order_tris(const gi_bvh &b, const gi_tri *tris, std::vector<IsectTri> &leaf_tris);
+void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh2Node &)> &leaf_ref,
+                         uint32_t node_base, std::vector<WideNode> &nodes, int32_t &wide_depth);
+// Relative box padding of the builders (2^-20 of the largest coordinate
+// magnitude / extent): keeps the float slab tests conservative.
+void pad_box(float *lo, float *hi);
+// Binned-SAH build over boxes (lo xyz, hi xyz per primitive), split down to
+// max_leaf primitives per leaf.
+void build_bvh2_boxes(const float *boxes, int64_t n, int32_t max_leaf, gi_bvh &out);
+
+// Special references (count 0): an empty slot, the end of a traversal, the
+// return from an instance to world space, the world BVH as a leaf of the
+// instance tree, and instance i (kLeafBit | i + 1).
+constexpr uint32_t kDoneRef = kLeafBit | kMaxLeafFirst;
+constexpr uint32_t kReturnRef = kLeafBit | (kMaxLeafFirst - 1);
+constexpr uint32_t kWorldRef = kLeafBit | (kMaxLeafFirst - 2);
+constexpr uint32_t kMaxInstances = kMaxLeafFirst - 3;
 
 // ------------------------------------------------------------- Halton ----
 // Per-dimension constants of the scrambled radical inverse, laid out for

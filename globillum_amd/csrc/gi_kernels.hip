@@ -72,6 +72,27 @@ struct alignas(16) DevLight {
   float col[3], pad2;
 };
 
+// A placed mesh: world-to-object and object-to-world rows (3x4), the root
+// of the mesh's BVH, the virtual id of its triangle 0, the ShadeTri index of
+// its triangle 0 and the material override (-1 = the triangles' own).
+struct alignas(16) DevInstance {
+  float4 inv[3];
+  float4 fwd[3];
+  uint32_t root, vbase, stri;
+  int32_t material;
+  uint32_t pad[4];
+};
+static_assert(sizeof(DevInstance) == 128, "instance record");
+
+// The instance tree: root of the tree over the instances' world boxes
+// (leaves: kWorldRef or instance refs), the instances, and the number of
+// world triangles (ids below it are world triangles).
+struct InstScene {
+  const DevInstance *inst;
+  uint32_t tlas_root, n_world;
+  int32_t n_inst;
+};
+
 // Sky map of the scene's sky light and the tables of its Distribution2D.
 struct SkyDev {
   const float4 *rgb;                  // w*h texels (r, g, b, 0)
@@ -92,6 +113,7 @@ struct ShadeParams {
   float bg[3];
   int32_t depth, rr_start;
   SkyDev sky;
+  InstScene is;
 };
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -161,7 +183,6 @@ __device__ __forceinline__ void block_append2(bool p0, uint32_t *c0, bool p1, ui
 #endif
 
 constexpr uint32_t kNoRay = 0xffffffffu;
-constexpr uint32_t kDoneRef = kLeafBit | kMaxLeafFirst;  // a leaf of 0 triangles: traversal over
 #ifndef GI_TRACE_CHUNK
 #define GI_TRACE_CHUNK 32
 #endif
@@ -204,10 +225,20 @@ __device__ __forceinline__ Stack<kLdsDepth> make_stack(uint32_t *smem, uint32_t 
 // The persistent traversal loop over queue entries [0, count).
 //   start(i, o, d, tmax) loads ray i;
 //   finish(i, t, u, v, id) stores its result (id == GI_INVALID_REF: no hit).
-template <bool kAny, int kLdsDepth, class Start, class Finish>
+// kInst: two-level traversal from the instance tree's root.  Reaching an
+// instance (only with no leaf parked, so a parked leaf is always tested in
+// the space it was found in) pushes kReturnRef, moves the ray into the
+// mesh's object space and continues at the mesh's root; popping kReturnRef
+// reloads the world ray.  Triangle ids found in a mesh are offset by the
+// instance's virtual base.
+__device__ __forceinline__ bool special_ref(uint32_t ref) {
+  return (ref & kLeafBit) && ((ref >> 27) & 15u) == 0u && ref != kDoneRef;
+}
+
+template <bool kAny, bool kInst, int kLdsDepth, class Start, class Finish>
 __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode *__restrict__ nodes,
                                                  const float4 *__restrict__ itris, Stack<kLdsDepth> &st,
-                                                 uint32_t *overflow, Start start, Finish finish) {
+                                                 uint32_t *overflow, InstScene IS, Start start, Finish finish) {
   const uint32_t lane = __lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
@@ -217,6 +248,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
   f3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f}, idir{0.f, 0.f, 0.f}, oid{0.f, 0.f, 0.f};
   float tmax = 0.f, bt = FLT_MAX, bu = 0.f, bv = 0.f;
   uint32_t bid = GI_INVALID_REF;
+  uint32_t vbase = 0;  // virtual id base of the current space (0: world)
   for (;;) {
     // ---- refill: the idle lanes, in lane order, take whole chunks
     const uint64_t idle = __ballot(ray == kNoRay);
@@ -234,7 +266,8 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
           bt = FLT_MAX;
           bu = bv = 0.0f;
           bid = GI_INVALID_REF;
-          ref = 0;
+          ref = kInst ? IS.tlas_root : 0u;
+          vbase = 0;
           st.sp = 0;
         }
       }
@@ -297,11 +330,45 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
           ref = st.sp > 0 ? st.take() : kDoneRef;
         }
       }
-      if (parked == kDoneRef && ref != kDoneRef && (ref & kLeafBit)) {
+      if (kInst && special_ref(ref)) {
+        if (ref == kWorldRef) {
+          ref = 0;  // the world BVH, in world space
+        } else if (parked == kDoneRef) {
+          if (ref == kReturnRef) {
+            start(ray, o, d, tmax);  // back to the world ray
+            vbase = 0;
+            ref = st.sp > 0 ? st.take() : kDoneRef;
+          } else {
+            const DevInstance &I = IS.inst[(ref & kMaxLeafFirst) - 1u];
+            const int sp = st.sp;
+            if (sp + 1 > st.cap) {
+              atomicAdd(overflow, 1u);
+              ref = st.sp > 0 ? st.take() : kDoneRef;
+            } else {
+              st.put(sp, kReturnRef);
+              st.sp = sp + 1;
+              const float4 r0 = I.inv[0], r1 = I.inv[1], r2 = I.inv[2];
+              const f3 oo = f3{((r0.x * o.x + r0.y * o.y) + r0.z * o.z) + r0.w,
+                               ((r1.x * o.x + r1.y * o.y) + r1.z * o.z) + r1.w,
+                               ((r2.x * o.x + r2.y * o.y) + r2.z * o.z) + r2.w};
+              const f3 dd = f3{(r0.x * d.x + r0.y * d.y) + r0.z * d.z, (r1.x * d.x + r1.y * d.y) + r1.z * d.z,
+                               (r2.x * d.x + r2.y * d.y) + r2.z * d.z};
+              o = oo;
+              d = dd;
+              vbase = I.vbase;
+              ref = I.root;
+            }
+          }
+          idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
+          oid = f3{o.x * idir.x, o.y * idir.y, o.z * idir.z};
+        }
+      }
+      if (parked == kDoneRef && ref != kDoneRef && (ref & kLeafBit) && !(kInst && special_ref(ref))) {
         parked = ref;
         ref = st.sp > 0 ? st.take() : kDoneRef;
       }
-      if (__ballot(!(ref & kLeafBit) && parked == kDoneRef) == 0) break;
+      const bool busy = !(ref & kLeafBit) || (kInst && special_ref(ref));
+      if (__ballot(busy && parked == kDoneRef) == 0) break;
     }
     // ---- the parked leaf
     if (parked != kDoneRef) {
@@ -323,7 +390,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
         }
         float t, u, v;
         if (intersect_tri(xyz(ta), xyz(t1), xyz(t2), o, d, t, u, v) && t <= tmax) {
-          const uint32_t id = fbits(ta.w);
+          const uint32_t id = fbits(ta.w) + vbase;
           if (kAny) {
             bt = t; bu = u; bv = v; bid = id;
             ref = kDoneRef;  // any hit ends the ray
@@ -348,16 +415,17 @@ constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the g
 // Closest-hit kernel over a ray queue.  kHasTmax: max_t in ray_o.w (the
 // primitive entry point); otherwise ray_o.w carries the Halton index and
 // max_t is FLT_MAX (camera and bounce rays, src/hybrid-pt.cpp:558).
-template <bool kHasTmax, int kLdsDepth>
+template <bool kHasTmax, bool kInst, int kLdsDepth>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_closest(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, float4 *__restrict__ hits, const WideNode *__restrict__ nodes,
-    const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow) {
+    const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow,
+    InstScene IS) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
-  trace_persistent<false, kLdsDepth>(
-      count, nodes, itris, st, overflow,
+  trace_persistent<false, kInst, kLdsDepth>(
+      count, nodes, itris, st, overflow, IS,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
         const float4 ro = ray_o[i], rd = ray_d[i];
         o = xyz(ro);
@@ -371,18 +439,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WA
 // ray adds its precomputed NEE contribution to its path's colour (the
 // !is.valid() branch of src/hybrid-pt.cpp:355-365).  Otherwise writes the
 // occlusion byte (primitive entry point).
-template <bool kDeposit, int kLdsDepth>
+template <bool kDeposit, bool kInst, int kLdsDepth>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_any(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
     uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes, const float4 *__restrict__ itris,
     uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
-    uint32_t rec_stride, int32_t b) {
+    uint32_t rec_stride, int32_t b, InstScene IS) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
-  trace_persistent<true, kLdsDepth>(
-      count, nodes, itris, st, overflow,
+  trace_persistent<true, kInst, kLdsDepth>(
+      count, nodes, itris, st, overflow, IS,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
         const float4 ro = ray_o[i], rd = ray_d[i];
         o = xyz(ro);
@@ -533,12 +601,36 @@ struct VertexEval {
 
 __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, int dim, f3 org_dir, float4 h,
                                             VertexEval &ve) {
-  const ShadeTri &T = P.tris[fbits(h.w)];
+  uint32_t tid = fbits(h.w);
+  const DevInstance *I = nullptr;
+  if (tid >= P.is.n_world) {
+    // instance triangle: the last instance whose virtual base is <= tid
+    int lo = 0, hi = P.is.n_inst - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (P.is.inst[mid].vbase <= tid) lo = mid;
+      else hi = mid - 1;
+    }
+    I = &P.is.inst[lo];
+    tid = I->stri + (tid - I->vbase);
+  }
+  const ShadeTri &T = P.tris[tid];
   const float beta = h.y, gamma = h.z;
   const f3 bc = f3{(1.0f - beta) - gamma, beta, gamma};
-  const f3 Pp = add(add(scl(xyz(T.a), bc.x), scl(xyz(T.b), bc.y)), scl(xyz(T.c), bc.z));
-  const f3 Ni = add(add(scl(xyz(T.na), bc.x), scl(xyz(T.nb), bc.y)), scl(xyz(T.nc), bc.z));
+  f3 Pp = add(add(scl(xyz(T.a), bc.x), scl(xyz(T.b), bc.y)), scl(xyz(T.c), bc.z));
+  f3 Ni = add(add(scl(xyz(T.na), bc.x), scl(xyz(T.nb), bc.y)), scl(xyz(T.nc), bc.z));
   ve.mat = fbits(T.a.w);
+  if (I) {
+    // object to world: positions by the forward rows, normals by the
+    // inverse transpose (columns of the inverse rows)
+    const float4 f0 = I->fwd[0], f1 = I->fwd[1], f2 = I->fwd[2];
+    const float4 r0 = I->inv[0], r1 = I->inv[1], r2 = I->inv[2];
+    Pp = f3{((f0.x * Pp.x + f0.y * Pp.y) + f0.z * Pp.z) + f0.w, ((f1.x * Pp.x + f1.y * Pp.y) + f1.z * Pp.z) + f1.w,
+            ((f2.x * Pp.x + f2.y * Pp.y) + f2.z * Pp.z) + f2.w};
+    Ni = f3{(r0.x * Ni.x + r1.x * Ni.y) + r2.x * Ni.z, (r0.y * Ni.x + r1.y * Ni.y) + r2.y * Ni.z,
+            (r0.z * Ni.x + r1.z * Ni.y) + r2.z * Ni.z};
+    if (I->material >= 0) ve.mat = (uint32_t)I->material;
+  }
   const DevMat &m = P.mats[ve.mat];
   ve.org_dir = org_dir;
   ve.P = Pp;
@@ -1056,6 +1148,15 @@ struct gi_ctx {
   // traversal stack + overflow flag
   DBuf d_gstack, d_overflow;
   DBuf d_sky_rgb, d_sky_x, d_sky_y;  // sky map, per-row pdf|cdf, row pdf|cdf
+  // instancing: meshes (object space), instances, the instance tree
+  std::vector<std::vector<gi_tri>> meshes;
+  DBuf d_inst;
+  int64_t world_nnodes = 0;
+  int32_t world_depth = 0, n_inst = 0;
+  // range of the material indices shading can reach: world triangles, and
+  // with instances the instanced meshes' triangles and the overrides
+  int32_t mat_lo = 0, mat_hi = 0, inst_mat_lo = 0, inst_mat_hi = 0;
+  uint32_t tlas_root = 0;
   int32_t sky_light = -1, sky_w = 0, sky_h = 0;
   uint32_t stack_cap = 0;
   int32_t trace_grid = 0;
@@ -1202,10 +1303,32 @@ int ensure_halton(gi_ctx *ctx, uint32_t seed, int32_t ndims) {
 
 int ensure_gstack(gi_ctx *ctx);
 
+ShadeTri make_shadetri(const gi_tri &t) {
+  ShadeTri s;
+  int32_t mi = t.material_index;
+  float mf;
+  std::memcpy(&mf, &mi, 4);
+  s.a = make_float4(t.a[0], t.a[1], t.a[2], mf);
+  s.b = make_float4(t.b[0], t.b[1], t.b[2], 0.f);
+  s.c = make_float4(t.c[0], t.c[1], t.c[2], 0.f);
+  s.na = make_float4(t.na[0], t.na[1], t.na[2], 0.f);
+  s.nb = make_float4(t.nb[0], t.nb[1], t.nb[2], 0.f);
+  s.nc = make_float4(t.nc[0], t.nc[1], t.nc[2], 0.f);
+  s.tab = make_float4(t.ta[0], t.ta[1], t.tb[0], t.tb[1]);
+  s.tcm = make_float4(t.tc[0], t.tc[1], 0.f, 0.f);
+  return s;
+}
+
+InstScene inst_scene(const gi_ctx *ctx) {
+  return InstScene{ctx->d_inst.as<DevInstance>(), ctx->tlas_root, (uint32_t)ctx->ntri, ctx->n_inst};
+}
+bool instanced(const gi_ctx *ctx) { return ctx->n_inst > 0; }
+
 // State that follows a new wide tree on the device.
 int finish_bvh(gi_ctx *ctx, int64_t nnodes, int32_t wide_depth) {
-  ctx->nnodes = nnodes;
-  ctx->wide_depth = wide_depth;
+  ctx->nnodes = ctx->world_nnodes = nnodes;
+  ctx->wide_depth = ctx->world_depth = wide_depth;
+  ctx->n_inst = 0;  // a new world tree drops the instances (gi_upload_instances again)
   // at most three pushes per wide level on the current path
   ctx->stack_cap = (uint32_t)(3 * (wide_depth + 1) + 4);
   ctx->trace_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
@@ -1260,6 +1383,13 @@ int check_scene(gi_ctx *ctx, bool need_shading) {
     if (ctx->mats.empty()) return fail(ctx, GI_ERR_STATE, "no materials uploaded");
     if (ctx->lights.empty()) return fail(ctx, GI_ERR_STATE, "no lights uploaded");
     if (ctx->sky_light >= 0 && ctx->sky_w <= 0) return fail(ctx, GI_ERR_STATE, "sky light without a map (gi_upload_sky)");
+    int32_t lo = ctx->mat_lo, hi = ctx->mat_hi;
+    if (instanced(ctx)) {
+      lo = std::min(lo, ctx->inst_mat_lo);
+      hi = std::max(hi, ctx->inst_mat_hi);
+    }
+    if (lo < 0 || hi >= (int32_t)ctx->mats.size())
+      return fail(ctx, GI_ERR_STATE, "a triangle or instance material index is outside the uploaded materials");
   }
   return GI_OK;
 }
@@ -1270,23 +1400,25 @@ size_t trace_lds_bytes(int lds_depth, int block) {
   return (size_t)lds_depth * ((size_t)1 << lds_shift_for((uint32_t)block)) * sizeof(uint32_t);
 }
 
-template <bool kHasTmax, int kLds>
+InstScene inst_scene(const gi_ctx *ctx);
+
+template <bool kHasTmax, bool kInst, int kLds>
 void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
                       const float4 *ro, const float4 *rd, float4 *hits) {
-  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kLds>), dim3(grid), dim3(block), trace_lds_bytes(kLds, block),
+  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kInst, kLds>), dim3(grid), dim3(block), trace_lds_bytes(kLds, block),
                      ctx->stream, count_ptr, count_host, ro, rd, hits, ctx->d_nodes.as<WideNode>(),
                      ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
-                     ctx->d_overflow.as<uint32_t>());
+                     ctx->d_overflow.as<uint32_t>(), inst_scene(ctx));
 }
 
-template <bool kDeposit, int kLds>
+template <bool kDeposit, bool kInst, int kLds>
 void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
                   const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ, float4 *rec,
                   uint32_t rec_stride, int32_t b) {
-  hipLaunchKernelGGL((k_trace_any<kDeposit, kLds>), dim3(grid), dim3(block), trace_lds_bytes(kLds, block),
+  hipLaunchKernelGGL((k_trace_any<kDeposit, kInst, kLds>), dim3(grid), dim3(block), trace_lds_bytes(kLds, block),
                      ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ, ctx->d_nodes.as<WideNode>(),
                      ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
-                     ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b);
+                     ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b, inst_scene(ctx));
 }
 
 int stack_lds_of(const gi_ctx *ctx) {
@@ -1294,21 +1426,27 @@ int stack_lds_of(const gi_ctx *ctx) {
   return s == 8 || s == 32 ? s : 16;
 }
 
+bool instanced(const gi_ctx *ctx);
+
 int launch_closest(gi_ctx *ctx, bool has_tmax, const uint32_t *count_ptr, uint32_t count_host, uint64_t n_bound,
                    const float4 *ro, const float4 *rd, float4 *hits) {
   int grid, block;
   int rc = trace_grid_for(ctx, n_bound, grid, block);
   if (rc) return rc;
   const int L = stack_lds_of(ctx);
+  const bool I = instanced(ctx);
+#define GI_CL(T, II, LL) launch_closest_t<T, II, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, hits)
+#define GI_CL_L(T, II) \
+  if (L == 8) GI_CL(T, II, 8); \
+  else if (L == 32) GI_CL(T, II, 32); \
+  else GI_CL(T, II, 16)
   if (has_tmax) {
-    if (L == 8) launch_closest_t<true, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
-    else if (L == 32) launch_closest_t<true, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
-    else launch_closest_t<true, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
+    if (I) { GI_CL_L(true, true); } else { GI_CL_L(true, false); }
   } else {
-    if (L == 8) launch_closest_t<false, 8>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
-    else if (L == 32) launch_closest_t<false, 32>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
-    else launch_closest_t<false, 16>(ctx, grid, block, count_ptr, count_host, ro, rd, hits);
+    if (I) { GI_CL_L(false, true); } else { GI_CL_L(false, false); }
   }
+#undef GI_CL_L
+#undef GI_CL
   GI_HIP(ctx, hipGetLastError());
   return GI_OK;
 }
@@ -1320,17 +1458,19 @@ int launch_any(gi_ctx *ctx, bool deposit, const uint32_t *count_ptr, uint32_t co
   int rc = trace_grid_for(ctx, n_bound, grid, block);
   if (rc) return rc;
   const int L = stack_lds_of(ctx);
-#define GI_ANY(D, LL)                                                                                               \
-  launch_any_t<D, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b)
+  const bool I = instanced(ctx);
+#define GI_ANY(D, II, LL) \
+  launch_any_t<D, II, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b)
+#define GI_ANY_L(D, II) \
+  if (L == 8) GI_ANY(D, II, 8); \
+  else if (L == 32) GI_ANY(D, II, 32); \
+  else GI_ANY(D, II, 16)
   if (deposit) {
-    if (L == 8) GI_ANY(true, 8);
-    else if (L == 32) GI_ANY(true, 32);
-    else GI_ANY(true, 16);
+    if (I) { GI_ANY_L(true, true); } else { GI_ANY_L(true, false); }
   } else {
-    if (L == 8) GI_ANY(false, 8);
-    else if (L == 32) GI_ANY(false, 32);
-    else GI_ANY(false, 16);
+    if (I) { GI_ANY_L(false, true); } else { GI_ANY_L(false, false); }
   }
+#undef GI_ANY_L
 #undef GI_ANY
   GI_HIP(ctx, hipGetLastError());
   return GI_OK;
@@ -1436,6 +1576,7 @@ ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
                  ctx->d_sky_x.as<float>() + (size_t)ctx->sky_w * ctx->sky_h, ctx->d_sky_x.as<float>(),
                  ctx->sky_w, ctx->sky_h, ctx->sky_light,
                  ctx->sky_light >= 0 ? ctx->lights[(size_t)ctx->sky_light].col[0] : 0.0f};
+  P.is = inst_scene(ctx);
   return P;
 }
 
@@ -1825,6 +1966,14 @@ int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n) {
   ctx->ntri = n;
   ctx->have_bvh = false;
   ctx->bvh.reset();
+  ctx->meshes.clear();  // a new scene: meshes and instances are uploaded again
+  ctx->n_inst = 0;
+  ctx->mat_lo = INT32_MAX;
+  ctx->mat_hi = INT32_MIN;
+  for (int64_t i = 0; i < n; ++i) {
+    ctx->mat_lo = std::min(ctx->mat_lo, tris[i].material_index);
+    ctx->mat_hi = std::max(ctx->mat_hi, tris[i].material_index);
+  }
   for (int k = 0; k < 3; ++k) {
     ctx->lo[k] = ctx->clo[k] = FLT_MAX;
     ctx->hi[k] = ctx->chi[k] = -FLT_MAX;
@@ -1839,18 +1988,7 @@ int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n) {
       ctx->clo[k] = std::min(ctx->clo[k], cen);
       ctx->chi[k] = std::max(ctx->chi[k], cen);
     }
-    ShadeTri &s = st[i];
-    int32_t mi = t.material_index;
-    float mf;
-    std::memcpy(&mf, &mi, 4);
-    s.a = make_float4(t.a[0], t.a[1], t.a[2], mf);
-    s.b = make_float4(t.b[0], t.b[1], t.b[2], 0.f);
-    s.c = make_float4(t.c[0], t.c[1], t.c[2], 0.f);
-    s.na = make_float4(t.na[0], t.na[1], t.na[2], 0.f);
-    s.nb = make_float4(t.nb[0], t.nb[1], t.nb[2], 0.f);
-    s.nc = make_float4(t.nc[0], t.nc[1], t.nc[2], 0.f);
-    s.tab = make_float4(t.ta[0], t.ta[1], t.tb[0], t.tb[1]);
-    s.tcm = make_float4(t.tc[0], t.tc[1], 0.f, 0.f);
+    st[i] = make_shadetri(t);
   }
   float dx = ctx->hi[0] - ctx->lo[0], dy = ctx->hi[1] - ctx->lo[1], dz = ctx->hi[2] - ctx->lo[2];
   ctx->diag = std::sqrt(dx * dx + dy * dy + dz * dz);
@@ -1986,6 +2124,183 @@ int gi_build_bvh_device(gi_ctx *ctx, int32_t max_leaf) {
                   ctx->chi, ml, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), r, err))
     return fail(ctx, GI_ERR_HIP, err);
   return finish_bvh(ctx, r.nnodes, r.depth);
+}
+
+int gi_upload_mesh(gi_ctx *ctx, const gi_tri *tris, int64_t n, int32_t *mesh_id) {
+  if (!ctx || !tris || n <= 0 || !mesh_id) return fail(ctx, GI_ERR_INVALID_ARG, "mesh");
+  if (n > (int64_t)kMaxLeafFirst) return fail(ctx, GI_ERR_LIMIT, "mesh of more than 2^27 triangles");
+  ctx->meshes.emplace_back(tris, tris + n);
+  *mesh_id = (int32_t)ctx->meshes.size() - 1;
+  return GI_OK;
+}
+
+// Inverse of the affine map x -> A x + t (A = rows of m, t = column 3), in
+// double, as 3 rows of (A^-1 | -A^-1 t) rounded to float.  false if singular.
+bool invert_affine(const float *m, float *inv) {
+  const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+  const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+  if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return false;
+  const double r[9] = {(e * i - f * h) / det, (c * h - b * i) / det, (b * f - c * e) / det,
+                       (f * g - d * i) / det, (a * i - c * g) / det, (c * d - a * f) / det,
+                       (d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det};
+  const double t[3] = {m[3], m[7], m[11]};
+  for (int row = 0; row < 3; ++row) {
+    for (int col = 0; col < 3; ++col) inv[4 * row + col] = (float)r[3 * row + col];
+    inv[4 * row + 3] = (float)-(r[3 * row] * t[0] + r[3 * row + 1] * t[1] + r[3 * row + 2] * t[2]);
+  }
+  return true;
+}
+
+int gi_upload_instances(gi_ctx *ctx, const gi_instance *inst, int32_t n) {
+  if (!ctx || !inst || n <= 0) return fail(ctx, GI_ERR_INVALID_ARG, "instances");
+  if (!ctx->have_bvh) return fail(ctx, GI_ERR_STATE, "build the world BVH before gi_upload_instances");
+  if ((uint32_t)n > kMaxInstances) return fail(ctx, GI_ERR_LIMIT, "too many instances");
+  const int32_t nm = (int32_t)ctx->meshes.size();
+  uint64_t vcount = (uint64_t)ctx->ntri;
+  int32_t mlo = INT32_MAX, mhi = INT32_MIN;
+  for (int32_t i = 0; i < n; ++i) {
+    if (inst[i].mesh < 0 || inst[i].mesh >= nm) return fail(ctx, GI_ERR_INVALID_ARG, "instance mesh id");
+    if (inst[i].material < -1) return fail(ctx, GI_ERR_INVALID_ARG, "instance material");
+    const std::vector<gi_tri> &mt = ctx->meshes[(size_t)inst[i].mesh];
+    vcount += mt.size();
+    if (inst[i].material >= 0) {
+      mlo = std::min(mlo, inst[i].material);
+      mhi = std::max(mhi, inst[i].material);
+    } else {
+      for (const gi_tri &t : mt) {
+        mlo = std::min(mlo, t.material_index);
+        mhi = std::max(mhi, t.material_index);
+      }
+    }
+  }
+  if (vcount >= 0xffffffffull) return fail(ctx, GI_ERR_LIMIT, "more than 2^32 - 1 triangles with instances");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  const uint32_t n_world = (uint32_t)ctx->ntri;
+  // per mesh: its BVH (object space), nodes and triangles appended after the world's
+  std::vector<WideNode> xnodes;
+  std::vector<IsectTri> xit;
+  std::vector<ShadeTri> xst;
+  std::vector<uint32_t> root(nm), stri(nm);
+  std::vector<float> mbox(6 * (size_t)nm);
+  int32_t mesh_depth = 0;
+  uint32_t node_base = (uint32_t)ctx->world_nnodes, tri_base = n_world;
+  for (int32_t m = 0; m < nm; ++m) {
+    const std::vector<gi_tri> &mt = ctx->meshes[(size_t)m];
+    gi_bvh b;
+    build_bvh2(mt.data(), (int64_t)mt.size(), 4, b);
+    std::vector<IsectTri> it;
+    leaf_order_tris(b, mt.data(), it);
+    std::vector<WideNode> wn;
+    int32_t depth = 0;
+    const uint32_t tb = tri_base;
+    collapse_wide_nodes(
+        b, [tb](const Bvh2Node &nd) { return kLeafBit | ((uint32_t)nd.count << 27) | ((uint32_t)nd.first + tb); },
+        node_base, wn, depth);
+    root[m] = node_base;
+    stri[m] = tri_base;
+    std::memcpy(&mbox[6 * m], b.nodes[0].lo, 12);
+    std::memcpy(&mbox[6 * m + 3], b.nodes[0].hi, 12);
+    mesh_depth = std::max(mesh_depth, depth);
+    xnodes.insert(xnodes.end(), wn.begin(), wn.end());
+    xit.insert(xit.end(), it.begin(), it.end());
+    for (const gi_tri &t : mt) xst.push_back(make_shadetri(t));
+    node_base += (uint32_t)wn.size();
+    tri_base += (uint32_t)mt.size();
+  }
+  if (tri_base > kMaxLeafFirst) return fail(ctx, GI_ERR_LIMIT, "world + mesh triangles beyond 2^27");
+  // instances: transforms, virtual bases, world boxes (mesh box corners
+  // mapped in double, rounded outwards, then padded as the builder pads)
+  std::vector<DevInstance> di(n);
+  std::vector<float> boxes(6 * ((size_t)n + 1));
+  uint32_t vbase = n_world;
+  for (int32_t i = 0; i < n; ++i) {
+    const gi_instance &g = inst[i];
+    DevInstance &D = di[i];
+    std::memset(&D, 0, sizeof(D));
+    float inv[12];
+    if (!invert_affine(g.xform, inv)) return fail(ctx, GI_ERR_INVALID_ARG, "singular instance transform");
+    for (int r = 0; r < 3; ++r) {
+      D.fwd[r] = make_float4(g.xform[4 * r], g.xform[4 * r + 1], g.xform[4 * r + 2], g.xform[4 * r + 3]);
+      D.inv[r] = make_float4(inv[4 * r], inv[4 * r + 1], inv[4 * r + 2], inv[4 * r + 3]);
+    }
+    D.root = root[(size_t)g.mesh];
+    D.vbase = vbase;
+    D.stri = stri[(size_t)g.mesh];
+    D.material = g.material;
+    vbase += (uint32_t)ctx->meshes[(size_t)g.mesh].size();
+    const float *mb = &mbox[6 * (size_t)g.mesh];
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int c = 0; c < 8; ++c) {
+      const double p[3] = {mb[(c & 1) ? 3 : 0], mb[(c & 2) ? 4 : 1], mb[(c & 4) ? 5 : 2]};
+      for (int r = 0; r < 3; ++r) {
+        const double w = (double)g.xform[4 * r] * p[0] + (double)g.xform[4 * r + 1] * p[1] +
+                         (double)g.xform[4 * r + 2] * p[2] + (double)g.xform[4 * r + 3];
+        lo[r] = std::min(lo[r], w);
+        hi[r] = std::max(hi[r], w);
+      }
+    }
+    float *bx = &boxes[6 * (size_t)i];
+    for (int r = 0; r < 3; ++r) {
+      bx[r] = std::nextafter((float)lo[r], -FLT_MAX);
+      bx[3 + r] = std::nextafter((float)hi[r], FLT_MAX);
+    }
+    pad_box(bx, bx + 3);
+  }
+  {
+    float *bx = &boxes[6 * (size_t)n];  // the world geometry as the last primitive
+    for (int r = 0; r < 3; ++r) {
+      bx[r] = ctx->lo[r];
+      bx[3 + r] = ctx->hi[r];
+    }
+    pad_box(bx, bx + 3);
+  }
+  gi_bvh tb;
+  build_bvh2_boxes(boxes.data(), (int64_t)n + 1, 1, tb);
+  std::vector<WideNode> tn;
+  int32_t tlas_depth = 0;
+  const uint32_t tlas_base = node_base;
+  collapse_wide_nodes(
+      tb,
+      [&](const Bvh2Node &nd) {
+        const uint32_t prim = tb.perm[(size_t)nd.first];
+        return prim == (uint32_t)n ? kWorldRef : (kLeafBit | (prim + 1u));
+      },
+      tlas_base, tn, tlas_depth);
+  // combined device arrays: world part copied on the device, the rest uploaded
+  const size_t nw = (size_t)ctx->world_nnodes, nx = xnodes.size(), nt = tn.size();
+  DBuf nodes, itris, stris;
+  GI_HIP(ctx, nodes.ensure((nw + nx + nt) * sizeof(WideNode)));
+  GI_HIP(ctx, itris.ensure((size_t)tri_base * sizeof(IsectTri)));
+  GI_HIP(ctx, stris.ensure((size_t)tri_base * sizeof(ShadeTri)));
+  GI_HIP(ctx, hipMemcpyAsync(nodes.p, ctx->d_nodes.p, nw * sizeof(WideNode), hipMemcpyDeviceToDevice, ctx->stream));
+  GI_HIP(ctx, hipMemcpyAsync(itris.p, ctx->d_itris.p, (size_t)n_world * sizeof(IsectTri), hipMemcpyDeviceToDevice,
+                             ctx->stream));
+  GI_HIP(ctx, hipMemcpyAsync(stris.p, ctx->d_stris.p, (size_t)n_world * sizeof(ShadeTri), hipMemcpyDeviceToDevice,
+                             ctx->stream));
+  if (nx) GI_HIP(ctx, copy_sync(ctx, nodes.as<WideNode>() + nw, xnodes.data(), nx * sizeof(WideNode), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, nodes.as<WideNode>() + nw + nx, tn.data(), nt * sizeof(WideNode), hipMemcpyHostToDevice));
+  if (!xit.empty()) {
+    GI_HIP(ctx, copy_sync(ctx, itris.as<IsectTri>() + n_world, xit.data(), xit.size() * sizeof(IsectTri),
+                          hipMemcpyHostToDevice));
+    GI_HIP(ctx, copy_sync(ctx, stris.as<ShadeTri>() + n_world, xst.data(), xst.size() * sizeof(ShadeTri),
+                          hipMemcpyHostToDevice));
+  }
+  GI_HIP(ctx, ctx->d_inst.ensure((size_t)n * sizeof(DevInstance)));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_inst.p, di.data(), (size_t)n * sizeof(DevInstance), hipMemcpyHostToDevice));
+  std::swap(ctx->d_nodes.p, nodes.p);
+  std::swap(ctx->d_nodes.bytes, nodes.bytes);
+  std::swap(ctx->d_itris.p, itris.p);
+  std::swap(ctx->d_itris.bytes, itris.bytes);
+  std::swap(ctx->d_stris.p, stris.p);
+  std::swap(ctx->d_stris.bytes, stris.bytes);
+  ctx->nnodes = (int64_t)(nw + nx + nt);
+  ctx->tlas_root = tlas_base;
+  ctx->n_inst = n;
+  ctx->inst_mat_lo = mlo;
+  ctx->inst_mat_hi = mhi;
+  ctx->wide_depth = tlas_depth + 1 + std::max(ctx->world_depth, mesh_depth);
+  ctx->stack_cap = (uint32_t)(3 * (ctx->wide_depth + 1) + 4 + 1);  // + the return marker
+  return ensure_gstack(ctx);
 }
 
 int gi_upload_bvh2(gi_ctx *ctx, const float *nodes, int64_t n_nodes, const uint32_t *indices, int64_t n_indices) {

@@ -7,6 +7,7 @@ PrincipledBRDFParameters, gi::light rect lights) plus its views.
 
   cornell_boxes      closed box of 5 walls + 2 boxes, 1 rect light (34 tris)
   terrain_spheres    fBm heightfield + UV spheres, principled materials
+  instanced_terrain  fBm heightfield + instanced meshes (two-level BVH)
 """
 from __future__ import annotations
 
@@ -14,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import LIGHT, MATERIAL, OPTS, TRI, VIEW
+from ._lib import INSTANCE, LIGHT, MATERIAL, OPTS, TRI, VIEW
 
 
 @dataclass
@@ -27,6 +28,10 @@ class Scene:
     name: str = ""
     meta: dict = field(default_factory=dict)
     sky: np.ndarray | None = None  # (h, w, 3) float32 map of the scene's sky light
+    # two-level instancing: object-space meshes (TRI arrays) and INSTANCE
+    # records placing them; tris stays the world geometry
+    meshes: list | None = None
+    instances: np.ndarray | None = None
 
 
 def rect_light_power(w: float, h: float, col) -> np.float32:
@@ -320,6 +325,89 @@ def sky_scene(w=96, h=64, spp=4, depth=4, with_rect=True, seed=4) -> Scene:
     view = make_view((0.0, 3.0, -9.0), (0.0, 1.0, 2.0), (0, 1, 0), 55.0, w, h, spp)
     return Scene(np.concatenate(tris), mats, np.concatenate(lights), view, make_opts(depth),
                  name="sky_scene", sky=sky_map())
+
+
+def rotation(axis, angle: float) -> np.ndarray:
+    """3x3 rotation about `axis` by `angle` radians (Rodrigues)."""
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * (K @ K)
+
+
+def make_instance(A, t, mesh: int, material: int = -1) -> np.ndarray:
+    """An INSTANCE record: object-to-world x -> A x + t (rows of (A | t))."""
+    r = np.zeros(1, INSTANCE)
+    r["xform"] = np.concatenate([np.asarray(A, np.float64), np.asarray(t, np.float64).reshape(3, 1)], 1).ravel()
+    r["mesh"] = mesh
+    r["material"] = material
+    return r
+
+
+def unit_box(material=0) -> np.ndarray:
+    """[-1,1]^3 as 12 outward-facing triangles."""
+    q, n = _box_quads(np.float32([-1, -1, -1]), np.float32([1, 1, 1]), inward=False)
+    return _quads_to_tris(q, n, np.full(6, material, np.int32))
+
+
+def instanced_terrain(seed: int = 5, cells=(64, 64), n_instances: int = 200, n_mats: int = 16, n_lights: int = 4,
+                      w: int = 160, h: int = 96, spp: int = 4, depth: int = 4, size: float = 40.0,
+                      sphere=(10, 9)) -> Scene:
+    """fBm heightfield world plus n_instances placed copies of three meshes
+    (a UV sphere and a box of radius 1, a coarse sphere): random rotations,
+    uniform and non-uniform scales, translations above the terrain; about a
+    third of the instances override the mesh's material."""
+    rng = np.random.default_rng(seed)
+    nx, nz = cells
+    ter, H = heightfield(rng, nx, nz, size, size, 4.0, n_mats)
+    meshes = [uv_sphere((0.0, 0.0, 0.0), 1.0, *sphere, material=1), unit_box(material=2),
+              uv_sphere((0.0, 0.0, 0.0), 1.0, 6, 5, material=3)]
+    inst = []
+    for _ in range(n_instances):
+        cx, cz = rng.uniform(0.0, size, 2)
+        hx = min(int(cx / size * nx), nx)
+        hz = min(int(cz / size * nz), nz)
+        s = rng.uniform(0.2, 1.0)
+        S = np.diag(rng.uniform(0.5, 1.5, 3) * s) if rng.random() < 0.4 else np.eye(3) * s
+        A = rotation(rng.normal(size=3), rng.uniform(0.0, 2 * np.pi)) @ S
+        m = int(rng.integers(0, len(meshes)))
+        mat = int(rng.integers(0, n_mats)) if rng.random() < 0.35 else -1
+        inst.append(make_instance(A, (cx, H[hx, hz] + 0.6 * s, cz), m, mat))
+    mats = np.concatenate([
+        principled(rng.uniform(0.0, 1.0, 3).astype(np.float32), rng.uniform(0.05, 1.0),
+                   float(rng.random() < 0.5)) for _ in range(n_mats)])
+    g = int(np.ceil(np.sqrt(n_lights)))
+    lights = []
+    for li in range(n_lights):
+        lx = (li % g + 0.5) / g * size
+        lz = (li // g + 0.5) / g * size
+        lights.append(make_light((lx, 20.0, lz), (0.0, -1.0, 0.0), (40.0, 40.0, 40.0), 4.0, 4.0))
+    view = make_view((0.5 * size, 14.0, -2.0), (0.5 * size, 0.0, 0.5 * size), (0, 1, 0), 55.0, w, h, spp)
+    return Scene(ter, mats, np.concatenate(lights), view, make_opts(depth), name="instanced_terrain",
+                 meshes=meshes, instances=np.concatenate(inst))
+
+
+def flatten(scene: Scene) -> Scene:
+    """The same scene without instancing: every instance's mesh mapped to
+    world space in float64 (positions by the transform, normals by its
+    inverse transpose) and appended after the world triangles, so triangle
+    ids equal the instanced scene's virtual ids."""
+    if scene.instances is None or len(scene.instances) == 0:
+        return scene
+    parts = [scene.tris]
+    for r in scene.instances:
+        M = r["xform"].astype(np.float64).reshape(3, 4)
+        A, t = M[:, :3], M[:, 3]
+        Nt = np.linalg.inv(A).T
+        m = scene.meshes[int(r["mesh"])].copy()
+        for k in ("a", "b", "c"):
+            m[k] = m[k].astype(np.float64) @ A.T + t
+            m["n" + k] = m["n" + k].astype(np.float64) @ Nt.T
+        if r["material"] >= 0:
+            m["material_index"] = r["material"]
+        parts.append(m)
+    return Scene(np.concatenate(parts), scene.mats, scene.lights, scene.views, scene.opts, name=scene.name + "_flat",
+                 sky=scene.sky)
 
 
 def save_scene(scene: Scene, path: str) -> None:

@@ -125,7 +125,9 @@ typedef struct {
  * rr_start > 0 enables Russian roulette once rr_start segments were traced
  * (survival = max channel of the throughput).  offset_bounce / offset_shadow
  * replace the scene-scale constants 0.3 (src/hybrid-pt.cpp:551) and 0.9
- * (src/arealight-sampler-cpu.cpp:115); <= 0 selects 1e-4 * scene diagonal. */
+ * (src/arealight-sampler-cpu.cpp:115); <= 0 selects 1e-4 * scene diagonal
+ * (the bounding box of the gi_upload_triangles geometry; instances do not
+ * change it). */
 typedef struct {
   int32_t max_depth;
   int32_t rr_start;
@@ -194,7 +196,11 @@ int gi_bvh_sizes(const gi_bvh *bvh, int64_t *n_nodes, int64_t *n_indices);
 /* Export in the reference layout: nodes[8*n_nodes] floats, indices[n_indices]. */
 int gi_bvh_export2(const gi_bvh *bvh, float *nodes, uint32_t *indices);
 
-/* ---- scene upload (src/hl-main.cpp:426-428) -------------------------------- */
+/* ---- scene upload (src/hl-main.cpp:426-428) --------------------------------
+ * gi_upload_triangles starts a new scene: it also drops the meshes and
+ * instances.  Rendering checks that every material index shading can reach
+ * (triangles, instanced meshes, instance overrides) names an uploaded
+ * material. */
 int gi_upload_triangles(gi_ctx *ctx, const gi_tri *tris, int64_t n);
 int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n);
 int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n);
@@ -269,6 +275,27 @@ int gi_progress_step(gi_ctx *ctx, int32_t *done);
  * score-function term.  Not defined with Russian roulette (rr_start > 0). */
 int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats,
                float *d_emission);
+
+/* ---- Instancing (two-level BVH) --------------------------------------------
+ * No counterpart in the reference; needed by the large configuration
+ * (thousands of placed copies of one mesh).  A mesh is a triangle table in
+ * object space with its own BVH; an instance places a mesh with a 3x4
+ * row-major object-to-world transform (affine, invertible) and optionally
+ * one material for all its triangles.  The triangles of gi_upload_triangles
+ * stay the world geometry.  Rays hit the world and every instance: the
+ * closest hit is the minimum (t, id) over all of them, with t the world ray
+ * parameter.  Instance triangles have virtual ids n_world + base_i + k
+ * (k = index in the mesh; base_i = sum of the mesh sizes of instances
+ * before i), as gi_hit.ref and in rendering.  Call gi_upload_instances
+ * after the world BVH is built; it builds the meshes' BVHs and the instance
+ * tree. */
+typedef struct {
+  float xform[12];
+  int32_t mesh;
+  int32_t material;         /* -1: the mesh triangles' own material indices */
+} gi_instance;
+int gi_upload_mesh(gi_ctx *ctx, const gi_tri *tris, int64_t n, int32_t *mesh_id);
+int gi_upload_instances(gi_ctx *ctx, const gi_instance *inst, int32_t n);
 
 /* ---- Halton (gi::halton, src/util.cpp:22-32) ------------------------------
  * Scrambled radical inverse of index n in the prime base of dimension dim,

@@ -232,6 +232,19 @@ struct or_scene {
    * rgb w*h, and the tables of its Distribution2D (row CDF/PDF, per-row CDF/PDF) */
   int32_t sky_w = 0, sky_h = 0, sky_light = -1;
   std::vector<float> sky_rgb, sky_ycdf, sky_ypdf, sky_xcdf, sky_xpdf;
+  /* instancing: meshes (object space, each with its own median BVH) and the
+   * instances placing them; instance i's triangle k has the virtual id
+   * tris.size() + inst[i].vbase + k */
+  std::vector<or_scene *> meshes;
+  struct Instance {
+    float fwd[12], inv[12]; /* object-to-world rows, world-to-object rows */
+    int32_t mesh, material;
+    uint32_t vbase;
+  };
+  std::vector<Instance> inst;
+  ~or_scene() {
+    for (or_scene *m : meshes) delete m;
+  }
 };
 
 namespace {
@@ -360,20 +373,106 @@ bool traverse(const or_scene &s, V3<R> o, V3<R> d, R tmax, R &bt, R &bu, R &bv, 
   return bid != GI_INVALID_REF;
 }
 
+/* Affine maps by 3x4 row-major matrices, summed left to right:
+ * ((m0 x + m1 y) + m2 z) + m3 per row.  Normals go by the transpose of the
+ * inverse's 3x3 part (its columns). */
+template <class R> inline V3<R> xf_point(const float *m, V3<R> p) {
+  return mk<R>(((R(m[0]) * p.x + R(m[1]) * p.y) + R(m[2]) * p.z) + R(m[3]),
+               ((R(m[4]) * p.x + R(m[5]) * p.y) + R(m[6]) * p.z) + R(m[7]),
+               ((R(m[8]) * p.x + R(m[9]) * p.y) + R(m[10]) * p.z) + R(m[11]));
+}
+template <class R> inline V3<R> xf_dir(const float *m, V3<R> v) {
+  return mk<R>((R(m[0]) * v.x + R(m[1]) * v.y) + R(m[2]) * v.z, (R(m[4]) * v.x + R(m[5]) * v.y) + R(m[6]) * v.z,
+               (R(m[8]) * v.x + R(m[9]) * v.y) + R(m[10]) * v.z);
+}
+template <class R> inline V3<R> xf_normal(const float *inv, V3<R> n) {
+  return mk<R>((R(inv[0]) * n.x + R(inv[4]) * n.y) + R(inv[8]) * n.z,
+               (R(inv[1]) * n.x + R(inv[5]) * n.y) + R(inv[9]) * n.z,
+               (R(inv[2]) * n.x + R(inv[6]) * n.y) + R(inv[10]) * n.z);
+}
+
+/* World geometry and every instance (no instance tree: each instance's mesh
+ * is traversed with the ray mapped into its object space by the inverse
+ * rows).  The object-space ray keeps the world ray's parameter, so t values
+ * compare directly; the closest hit is the lexicographic minimum of (t,
+ * virtual id) over everything. */
+template <class R, bool ANY>
+bool trace_scene(const or_scene &s, V3<R> o, V3<R> d, R tmax, R &bt, R &bu, R &bv, uint32_t &bid,
+                 uint64_t &nbox, uint64_t &ntri) {
+  bool hit = traverse<R, ANY>(s, o, d, tmax, bt, bu, bv, bid, nbox, ntri);
+  if (ANY && hit) return true;
+  const uint32_t nw = (uint32_t)s.tris.size();
+  for (const or_scene::Instance &I : s.inst) {
+    R t, u, v;
+    uint32_t id;
+    if (!traverse<R, ANY>(*s.meshes[(size_t)I.mesh], xf_point<R>(I.inv, o), xf_dir<R>(I.inv, d), tmax, t, u, v, id,
+                          nbox, ntri))
+      continue;
+    const uint32_t vid = nw + I.vbase + id;
+    if (ANY || t < bt || (t == bt && vid < bid)) { bt = t; bu = u; bv = v; bid = vid; }
+    if (ANY) return true;
+  }
+  return bid != GI_INVALID_REF;
+}
+
 template <class R>
-bool brute(const or_scene &s, V3<R> o, V3<R> d, R tmax, R &bt, R &bu, R &bv, uint32_t &bid) {
-  bt = R(FLT_MAX);
-  bid = GI_INVALID_REF;
-  for (uint32_t id = 0; id < s.tris.size(); ++id) {
-    const gi_tri &T = s.tris[id];
+bool brute_tris(const or_scene &s, V3<R> o, V3<R> d, R tmax, uint32_t base, R &bt, R &bu, R &bv, uint32_t &bid) {
+  for (uint32_t k = 0; k < s.tris.size(); ++k) {
+    const gi_tri &T = s.tris[k];
     V3<R> a = ld<R>(T.a), b = ld<R>(T.b), c = ld<R>(T.c);
     R t, u, v;
+    const uint32_t id = base + k;
     if (intersect_tri<R>(a, b - a, c - a, o, d, t, u, v) && t <= tmax &&
         (t < bt || (t == bt && id < bid))) {
       bt = t; bu = u; bv = v; bid = id;
     }
   }
   return bid != GI_INVALID_REF;
+}
+
+template <class R>
+bool brute(const or_scene &s, V3<R> o, V3<R> d, R tmax, R &bt, R &bu, R &bv, uint32_t &bid) {
+  bt = R(FLT_MAX);
+  bid = GI_INVALID_REF;
+  brute_tris<R>(s, o, d, tmax, 0u, bt, bu, bv, bid);
+  const uint32_t nw = (uint32_t)s.tris.size();
+  for (const or_scene::Instance &I : s.inst)
+    brute_tris<R>(*s.meshes[(size_t)I.mesh], xf_point<R>(I.inv, o), xf_dir<R>(I.inv, d), tmax, nw + I.vbase, bt, bu,
+                  bv, bid);
+  return bid != GI_INVALID_REF;
+}
+
+/* The surface at hit (id, beta, gamma): position and unnormalised
+ * interpolated normal, in world space, and the material.  An instance
+ * triangle is interpolated in object space, then mapped (positions by the
+ * forward rows, normals by the inverse transpose); the instance's material,
+ * when set, replaces the triangle's. */
+template <class R>
+void surface(const or_scene &s, uint32_t id, R bu, R bv, V3<R> &P, V3<R> &Ni, int32_t &mat) {
+  const or_scene::Instance *I = nullptr;
+  const gi_tri *T;
+  const uint32_t nw = (uint32_t)s.tris.size();
+  if (id < nw) {
+    T = &s.tris[id];
+  } else {
+    size_t lo = 0, hi = s.inst.size() - 1;
+    while (lo < hi) {
+      const size_t mid = (lo + hi + 1) / 2;
+      if (nw + s.inst[mid].vbase <= id) lo = mid;
+      else hi = mid - 1;
+    }
+    I = &s.inst[lo];
+    T = &s.meshes[(size_t)I->mesh]->tris[id - nw - I->vbase];
+  }
+  V3<R> bc = mk<R>((R(1) - bu) - bv, bu, bv);
+  P = (ld<R>(T->a) * bc.x + ld<R>(T->b) * bc.y) + ld<R>(T->c) * bc.z;
+  Ni = (ld<R>(T->na) * bc.x + ld<R>(T->nb) * bc.y) + ld<R>(T->nc) * bc.z;
+  mat = T->material_index;
+  if (I) {
+    P = xf_point<R>(I->fwd, P);
+    Ni = xf_normal<R>(I->inv, Ni);
+    if (I->material >= 0) mat = I->material;
+  }
 }
 
 /* ------------------------------------------------------------------ BRDFs -- */
@@ -679,12 +778,12 @@ void cam_ray(const Cam &c, int32_t x, int32_t y, float jx, float jy, float lx, f
 template <class R> struct Tracer {
   const or_scene &s;
   bool closest(V3<R> o, V3<R> d, R tmax, R &t, R &u, R &v, uint32_t &id, uint64_t &nb, uint64_t &nt) const {
-    return traverse<R, false>(s, o, d, tmax, t, u, v, id, nb, nt);
+    return trace_scene<R, false>(s, o, d, tmax, t, u, v, id, nb, nt);
   }
   bool any(V3<R> o, V3<R> d, R tmax, uint64_t &nb, uint64_t &nt) const {
     R t, u, v;
     uint32_t id;
-    return traverse<R, true>(s, o, d, tmax, t, u, v, id, nb, nt);
+    return trace_scene<R, true>(s, o, d, tmax, t, u, v, id, nb, nt);
   }
 };
 
@@ -780,11 +879,10 @@ V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H,
       break;
     }
     ++st.vertices;
-    const gi_tri &T = s.tris[id];
-    V3<R> bc = mk<R>((R(1) - bu) - bv, bu, bv);
-    V3<R> P = (ld<R>(T.a) * bc.x + ld<R>(T.b) * bc.y) + ld<R>(T.c) * bc.z;
-    V3<R> Ni = (ld<R>(T.na) * bc.x + ld<R>(T.nb) * bc.y) + ld<R>(T.nc) * bc.z;
-    MatP<R> mp(s.mats[T.material_index]);
+    V3<R> P, Ni;
+    int32_t mat;
+    surface<R>(s, id, bu, bv, P, Ni, mat);
+    MatP<R> mp(s.mats[mat]);
 
     /* NEE: pixel_generate_arealight_sample_cpu, src/arealight-sampler-cpu.cpp:25-119.
      * The normal is the unnormalised interpolation, as there. */
@@ -1081,13 +1179,12 @@ void adjoint_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H
       break;
     }
     v.hit = true;
-    const gi_tri &T = s.tris[id];
-    V3<R> bc = mk<R>((R(1) - bu) - bv, bu, bv);
-    V3<R> P = (ld<R>(T.a) * bc.x + ld<R>(T.b) * bc.y) + ld<R>(T.c) * bc.z;
-    V3<R> Ni = (ld<R>(T.na) * bc.x + ld<R>(T.nb) * bc.y) + ld<R>(T.nc) * bc.z;
-    MatP<R> mp(s.mats[T.material_index]);
+    V3<R> P, Ni;
+    int32_t mat;
+    surface<R>(s, id, bu, bv, P, Ni, mat);
+    MatP<R> mp(s.mats[mat]);
     MatD<R> md(mp);
-    v.mat = T.material_index;
+    v.mat = mat;
     V3<R> xl = mk<R>(R(H.sample(n, dim + 0)), R(H.sample(n, dim + 1)), R(H.sample(n, dim + 2)));
     R choice = xl.z * R(s.power_total), acc = R(0);
     int k = 0;
@@ -1277,6 +1374,47 @@ or_scene *or_scene_create(const gi_tri *tris, int64_t n, const gi_material *mats
 }
 
 void or_scene_free(or_scene *s) { delete s; }
+
+int32_t or_scene_add_mesh(or_scene *s, const gi_tri *tris, int64_t n, int32_t max_leaf) {
+  if (!s || !tris || n <= 0) return -1;
+  s->meshes.push_back(or_scene_create(tris, n, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, max_leaf));
+  return (int32_t)s->meshes.size() - 1;
+}
+
+/* Instance i: the rows of xform (object to world) and of the inverse map,
+ * (A^-1 | -A^-1 t), computed in double by cofactors over the determinant and
+ * rounded to float; vbase = sum of the mesh sizes of the instances before
+ * i.  -1 for a bad mesh id or material, or a singular 3x3 part. */
+int or_scene_set_instances(or_scene *s, const gi_instance *inst, int32_t n) {
+  if (!s || n < 0 || (n > 0 && !inst)) return -1;
+  std::vector<or_scene::Instance> out((size_t)n);
+  uint32_t vbase = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    const gi_instance &g = inst[i];
+    or_scene::Instance &I = out[(size_t)i];
+    if (g.mesh < 0 || g.mesh >= (int32_t)s->meshes.size() || g.material < -1) return -1;
+    const float *m = g.xform;
+    const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], gg = m[8], h = m[9], k = m[10];
+    const double c00 = e * k - f * h, c01 = f * gg - d * k, c02 = d * h - e * gg;
+    const double det = a * c00 + b * c01 + c * c02;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return -1;
+    /* the adjugate (transposed cofactors) over det */
+    const double r[3][3] = {{c00 / det, (c * h - b * k) / det, (b * f - c * e) / det},
+                            {c01 / det, (a * k - c * gg) / det, (c * d - a * f) / det},
+                            {c02 / det, (b * gg - a * h) / det, (a * e - b * d) / det}};
+    for (int row = 0; row < 3; ++row) {
+      for (int col = 0; col < 3; ++col) I.inv[4 * row + col] = (float)r[row][col];
+      I.inv[4 * row + 3] = (float)-(r[row][0] * (double)m[3] + r[row][1] * (double)m[7] + r[row][2] * (double)m[11]);
+    }
+    std::memcpy(I.fwd, m, sizeof(I.fwd));
+    I.mesh = g.mesh;
+    I.material = g.material;
+    I.vbase = vbase;
+    vbase += (uint32_t)s->meshes[(size_t)g.mesh]->tris.size();
+  }
+  s->inst = std::move(out);
+  return 0;
+}
 int64_t or_scene_bvh_nodes(const or_scene *s) { return (int64_t)s->nodes.size() / 8; }
 void or_scene_export_bvh(const or_scene *s, float *nodes, uint32_t *indices) {
   std::memcpy(nodes, s->nodes.data(), s->nodes.size() * 4);
@@ -1295,7 +1433,7 @@ void or_trace_closest(const or_scene *s, const float *o, const float *d, const f
     float t, u = 0.f, v = 0.f;
     uint32_t id;
     uint64_t b = 0, c = 0;
-    traverse<float, false>(*s, ld<float>(o + 3 * i), ld<float>(d + 3 * i), tmax[i], t, u, v, id, b, c);
+    trace_scene<float, false>(*s, ld<float>(o + 3 * i), ld<float>(d + 3 * i), tmax[i], t, u, v, id, b, c);
     nb += b; nt += c;
     out[i].t = t; out[i].beta = id == GI_INVALID_REF ? 0.f : u; out[i].gamma = id == GI_INVALID_REF ? 0.f : v;
     out[i].ref = id;

@@ -32,6 +32,10 @@ or_scene *or_scene_create(const gi_tri *tris, int64_t n, const gi_material *mats
                           int64_t n_nodes, const uint32_t *indices, int64_t n_indices,
                           int32_t max_leaf);
 void or_scene_free(or_scene *s);
+/* Instancing: a mesh (object-space triangles, own BVH) and the instances
+ * placing meshes; the scene's triangles stay the world geometry. */
+int32_t or_scene_add_mesh(or_scene *s, const gi_tri *tris, int64_t n, int32_t max_leaf);
+int or_scene_set_instances(or_scene *s, const gi_instance *inst, int32_t n);
 int64_t or_scene_bvh_nodes(const or_scene *s);
 void or_scene_export_bvh(const or_scene *s, float *nodes, uint32_t *indices);
 float or_scene_diagonal(const or_scene *s);

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 TRI = np.dtype([("a", "<f4", 3), ("b", "<f4", 3), ("c", "<f4", 3), ("na", "<f4", 3), ("nb", "<f4", 3),
                 ("nc", "<f4", 3), ("ta", "<f4", 2), ("tb", "<f4", 2), ("tc", "<f4", 2),
                 ("material_index", "<i4")])
+INSTANCE = np.dtype([("xform", "<f4", 12), ("mesh", "<i4"), ("material", "<i4")])
 HIT = np.dtype([("t", "<f4"), ("beta", "<f4"), ("gamma", "<f4"), ("ref", "<u4")])
 STATS = np.dtype([(k, "<u8") for k in ("rays_closest", "rays_shadow", "vertices", "samples", "box_closest",
                                        "tri_closest", "box_shadow", "tri_shadow")])
@@ -42,6 +43,8 @@ def load():
         sig = {
             "or_scene_create": (P, [P, I64, P, I32, P, I32, P, I64, P, I64, I32]),
             "or_scene_free": (None, [P]),
+            "or_scene_add_mesh": (I32, [P, P, I64, I32]),
+            "or_scene_set_instances": (C.c_int, [P, P, I32]),
             "or_scene_bvh_nodes": (I64, [P]),
             "or_scene_export_bvh": (None, [P, P, P]),
             "or_scene_diagonal": (C.c_float, [P]),
@@ -89,7 +92,7 @@ def _p(a):
 class OracleScene:
     """The oracle's scene: its own object-median BVH unless bvh2 is given."""
 
-    def __init__(self, tris, mats=None, lights=None, bvh2=None, max_leaf=4, sky=None):
+    def __init__(self, tris, mats=None, lights=None, bvh2=None, max_leaf=4, sky=None, meshes=None, instances=None):
         self.lib = load()
         self._keep = [np.ascontiguousarray(tris)]
         mats = None if mats is None else np.ascontiguousarray(mats)
@@ -110,6 +113,15 @@ class OracleScene:
             self._keep.append(m)
             if self.lib.or_scene_set_sky(self.h, _p(m), m.shape[1], m.shape[0]) != 0:
                 raise ValueError("or_scene_set_sky: the scene needs exactly one sky light")
+        for m in meshes or []:
+            m = np.ascontiguousarray(m)
+            self._keep.append(m)
+            if self.lib.or_scene_add_mesh(self.h, _p(m), len(m), max_leaf) < 0:
+                raise ValueError("or_scene_add_mesh")
+        if instances is not None:
+            inst = np.ascontiguousarray(instances, INSTANCE)
+            if self.lib.or_scene_set_instances(self.h, _p(inst), len(inst)) != 0:
+                raise ValueError("or_scene_set_instances: bad mesh id, material or singular transform")
 
     def __del__(self):
         if getattr(self, "h", None):
