@@ -49,6 +49,7 @@ def parse():
     p.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batching leg")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     p.add_argument("--no-bvh", action="store_true", help="skip the BVH construction leg")
+    p.add_argument("--no-instancing", action="store_true", help="skip the two-level instancing leg")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (untimed): the known-size "
                         "kernel that scales FETCH_SIZE / WRITE_SIZE counter runs")
@@ -193,6 +194,49 @@ def bvh_leg(steps: int):
     return out
 
 
+def instancing_leg(steps: int):
+    """The large combined configuration's forward pass (2048x1024-cell
+    heightfield world + 2000 instanced 160-triangle spheres, 256 principled
+    materials, 64 rect lights, 4 views of 3840x2160, 16 spp, depth 6; no
+    albedo texture) rendered with the two-level BVH and with the same
+    spheres flattened into the world BVH (4.51M triangles)."""
+    import numpy as np
+
+    from globillum_amd import scenes
+    from globillum_amd._lib import FLAG_BATCHED
+    from globillum_amd.context import Context
+
+    sc = scenes.config_scene(5)
+    out = {"workload": "large_combined: 2048x1024-cell heightfield (4194304 triangles) + 2000 instances of a "
+                       "160-triangle sphere, 256 principled materials, 64 rect lights, 4 views 3840x2160, 16 spp, "
+                       "depth 6, no texture"}
+    means = {}
+    for name, s in (("instanced", sc), ("flattened", scenes.flatten(sc))):
+        with Context(0) as ctx:
+            t0 = time.perf_counter()
+            ctx.upload(s)
+            up_ms = (time.perf_counter() - t0) * 1e3
+            ctx.render(flags=FLAG_BATCHED)
+            rays, kms = 0, {"ms_trace_closest": 0.0, "ms_trace_any": 0.0, "ms_shade": 0.0}
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.render(flags=FLAG_BATCHED)
+                c = ctx.counters()
+                rays += c["rays_closest"] + c["rays_shadow"]
+                for k in kms:
+                    kms[k] += c[k]
+            dt = time.perf_counter() - t0
+            means[name] = np.array([ctx.image(v).mean(0).mean(0) for v in range(len(s.views))])
+            out[name] = {"mrays_per_s": round(rays / dt / 1e6, 1), "ms_per_frame": round(dt / steps * 1e3, 1),
+                         "upload_and_bvh_ms": round(up_ms, 1),
+                         "kernel_ms_per_frame": {k[3:]: round(v / steps, 2) for k, v in kms.items()}}
+        print(f"[bench] instancing leg: {name} {out[name]}", file=sys.stderr, flush=True)
+    out["instanced_over_flattened"] = round(out["instanced"]["mrays_per_s"] / out["flattened"]["mrays_per_s"], 3)
+    out["mean_radiance_rel_diff"] = float(np.abs(means["instanced"] - means["flattened"]).max()
+                                          / np.abs(means["flattened"]).max())
+    return out
+
+
 def cpu_baseline(scene, bvh2):
     """Oracle on 1/4 of the samples: same scene, camera, depth and BVH at
     640x360, 32 spp.  Returns the baseline record and the per-ray traversal
@@ -307,6 +351,8 @@ def main():
         out["adjoint"] = adjoint_leg(max(1, min(a.steps, 3)))
     if not a.no_bvh:
         out["bvh_build"] = bvh_leg(max(1, min(a.steps, 3)))
+    if not a.no_instancing:
+        out["instancing"] = instancing_leg(max(1, min(a.steps, 2)))
     print(json.dumps(out), flush=True)
 
 

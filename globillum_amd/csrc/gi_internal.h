@@ -80,12 +80,10 @@ void pad_box(float *lo, float *hi);
 void build_bvh2_boxes(const float *boxes, int64_t n, int32_t max_leaf, gi_bvh &out);
 
 // Special references (count 0): an empty slot, the end of a traversal, the
-// return from an instance to world space, the world BVH as a leaf of the
-// instance tree, and instance i (kLeafBit | i + 1).
+// return from an instance to world space, and instance i (kLeafBit | i + 1).
 constexpr uint32_t kDoneRef = kLeafBit | kMaxLeafFirst;
 constexpr uint32_t kReturnRef = kLeafBit | (kMaxLeafFirst - 1);
-constexpr uint32_t kWorldRef = kLeafBit | (kMaxLeafFirst - 2);
-constexpr uint32_t kMaxInstances = kMaxLeafFirst - 3;
+constexpr uint32_t kMaxInstances = kMaxLeafFirst - 2;
 
 // ------------------------------------------------------------- Halton ----
 // Per-dimension constants of the scrambled radical inverse, laid out for

@@ -387,6 +387,44 @@ def instanced_terrain(seed: int = 5, cells=(64, 64), n_instances: int = 200, n_m
                  meshes=meshes, instances=np.concatenate(inst))
 
 
+def large_combined(seed: int = 6, cells=(2048, 1024), n_spheres: int = 2000, n_mats: int = 256,
+                   n_lights: int = 64, w: int = 3840, h: int = 2160, spp: int = 16, depth: int = 6,
+                   n_views: int = 4, size=(200.0, 100.0), radius=(0.2, 1.5)) -> Scene:
+    """The large combined configuration's geometry and lighting: a
+    2048x1024-cell fBm heightfield (4,194,304 triangles) as the world, 2000
+    instances of one 160-triangle unit UV sphere (uniform scale U[radius],
+    resting on the terrain, each with its own principled material), 256
+    principled materials, 64 rect lights on an 8x8 grid, n_views views of
+    3840x2160 at 16 spp, depth 6."""
+    rng = np.random.default_rng(seed)
+    nx, nz = cells
+    sx, sz = size
+    ter, H = heightfield(rng, nx, nz, sx, sz, 10.0, n_mats)
+    sphere = uv_sphere((0.0, 0.0, 0.0), 1.0, material=0)
+    inst = []
+    for _ in range(n_spheres):
+        cx, cz = rng.uniform(0.0, sx), rng.uniform(0.0, sz)
+        r = rng.uniform(*radius)
+        hx = min(int(cx / sx * nx), nx)
+        hz = min(int(cz / sz * nz), nz)
+        inst.append(make_instance(np.eye(3) * r, (cx, H[hx, hz] + 0.5 * r, cz), 0, int(rng.integers(0, n_mats))))
+    mats = np.concatenate([
+        principled(rng.uniform(0.0, 1.0, 3).astype(np.float32), rng.uniform(0.05, 1.0),
+                   float(rng.random() < 0.5)) for _ in range(n_mats)])
+    g = int(np.ceil(np.sqrt(n_lights)))
+    lights = []
+    for li in range(n_lights):
+        lx = (li % g + 0.5) / g * sx
+        lz = (li // g + 0.5) / g * sz
+        lights.append(make_light((lx, 40.0, lz), (0.0, -1.0, 0.0), (20.0, 20.0, 20.0), 6.0, 6.0))
+    views = []
+    for v in range(n_views):
+        x = (v + 0.5) / n_views * sx
+        views.append(make_view((x, 35.0, -5.0), (x, 0.0, 0.5 * sz), (0, 1, 0), 55.0, w, h, spp))
+    return Scene(ter, mats, np.concatenate(lights), np.concatenate(views), make_opts(depth), name="large_combined",
+                 meshes=[sphere], instances=np.concatenate(inst))
+
+
 def flatten(scene: Scene) -> Scene:
     """The same scene without instancing: every instance's mesh mapped to
     world space in float64 (positions by the transform, normals by its
@@ -484,4 +522,6 @@ def config_scene(idx: int, **kw) -> Scene:
         return cornell_batch(**kw)
     if idx == 4:
         return diff_scene(**kw)
+    if idx == 5:
+        return large_combined(**kw)
     raise ValueError(f"no scene generator for configuration {idx}")
