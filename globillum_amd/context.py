@@ -54,6 +54,8 @@ class Context:
         float4 layout (bvh2=(nodes, indices))."""
         tris = np.ascontiguousarray(scene.tris, TRI)
         self._ck(self.lib.gi_upload_triangles(self.h, ptr(tris), len(tris)))
+        for t in getattr(scene, "textures", None) or []:
+            self.upload_texture(t)
         if scene.mats is not None and len(scene.mats):
             mats = np.ascontiguousarray(scene.mats, MATERIAL)
             self._ck(self.lib.gi_upload_materials(self.h, ptr(mats), len(mats)))
@@ -89,6 +91,24 @@ class Context:
         sc = self.scene
         if sc is not None and getattr(sc, "instances", None) is not None and len(sc.instances):
             self._ck(self.lib.gi_upload_instances(self.h, ptr(self._inst), len(self._inst)))
+
+    def upload_texture(self, rgba) -> int:
+        """An RGBA8 texture ((h, w, 4) uint8, row 0 at t = 1); its mip chain
+        is built on the device.  Returns its id (gi_material.texture)."""
+        t = np.ascontiguousarray(rgba, np.uint8)
+        if t.ndim != 3 or t.shape[2] != 4:
+            raise ValueError("texture must be (h, w, 4) uint8")
+        tid = C.c_int32()
+        self._ck(self.lib.gi_upload_texture(self.h, ptr(t), t.shape[1], t.shape[0], C.byref(tid)))
+        return tid.value
+
+    def texture_chain(self, tex: int = 0):
+        """(levels, flat uint8 RGBA chain: level 0, then each halved level)."""
+        lv, n = C.c_int32(), C.c_int64()
+        self._ck(self.lib.gi_texture_info(self.h, tex, C.byref(lv), C.byref(n)))
+        out = np.zeros(4 * n.value, np.uint8)
+        self._ck(self.lib.gi_download_texture(self.h, tex, ptr(out)))
+        return lv.value, out
 
     def upload_instances(self, meshes, instances):
         """Meshes (object-space TRI arrays) and INSTANCE records placing

@@ -202,6 +202,19 @@ __device__ __forceinline__ bool intersect_tri(f3 a, f3 e1, f3 e2, f3 o, f3 d, fl
   return (det != 0.0f) & (u >= 0.0f) & (u <= 1.0f) & (v >= 0.0f) & (u + v <= 1.0f) & (t > 0.0f);
 }
 
+// Barycentrics of the ray's crossing with the triangle's plane, without
+// the inside tests (intersect_tri_opt_nocheck, as computeRayDiffs uses it,
+// src/hybrid-pt.cpp:300-328): the arithmetic of intersect_tri.
+__device__ __forceinline__ void plane_bary(f3 a, f3 e1, f3 e2, f3 o, f3 d, float &u, float &v) {
+  const f3 pvec = cross(d, e2);
+  const float det = dot(e1, pvec);
+  const f3 tvec = sub(o, a);
+  const f3 qvec = cross(tvec, e1);
+  const float inv_det = 1.0f / det;
+  u = dot(tvec, pvec) * inv_det;
+  v = dot(d, qvec) * inv_det;
+}
+
 __device__ __forceinline__ float rcp_clamped(float d) {
   const float ooeps = 8.2718061255302767e-25f;  // 2^-80, as src/tracers.cu:48-51
   return __builtin_amdgcn_rcpf(fabsf(d) > ooeps ? d : copysignf(ooeps, d));
@@ -224,7 +237,151 @@ struct alignas(16) DevMat {
   float opacity, ax, ay, roughg2;
   float color[3];
   float specular;
+  int32_t texture;  // diffuse texture, -1: none
+  float spec_tint, sheen_tint, pad;
 };
+
+// --------------------------------------------------------------- pow 2.2 --
+// x^2.2 (mon2lin) for a base colour that varies per hit.  x = m 2^e with m
+// in [sqrt(1/2), sqrt(2)); x^2.2 = m^2.2 * 2^k * 2^(j/5) with 11 e = 5 k + j.
+// m^2.2 is a degree-6 polynomial in m - 1 (relative least squares, float
+// coefficients); 2^(j/5) a float table.  Relative error < 3e-7.  Basic
+// operations only, so host and device agree bit for bit.  x <= 0 or NaN ->
+// 0; x < 2^-50 -> 0; x >= 2^50 -> infinity.
+__host__ __device__ __forceinline__ float pow22(float x) {
+  if (!(x >= 0x1p-50f)) return 0.0f;
+  if (x >= 0x1p50f) return __builtin_inff();
+  uint32_t bits;
+  __builtin_memcpy(&bits, &x, 4);
+  int32_t e = (int32_t)(bits >> 23) - 127;
+  uint32_t mb = (bits & 0x7fffffu) | 0x3f800000u;
+  if (mb >= 0x3fb504f3u) {  // m >= sqrt(2): halve it
+    mb -= 0x00800000u;
+    e += 1;
+  }
+  float m;
+  __builtin_memcpy(&m, &mb, 4);
+  const float t = m - 1.0f;
+  float p = -0.002734926762059331f;
+  p = p * t + 0.006638647988438606f;
+  p = p * t + -0.017642781138420105f;
+  p = p * t + 0.0879826545715332f;
+  p = p * t + 1.3200013637542725f;
+  p = p * t + 2.200000286102295f;
+  p = p * t + 1.0f;
+  const int32_t q = 11 * e;
+  const int32_t j = ((q % 5) + 5) % 5;
+  const int32_t k = (q - j) / 5;
+  const float c = j == 0 ? 1.0f
+                         : j == 1 ? 1.1486983549970351f
+                                  : j == 2 ? 1.3195079107728942f : j == 3 ? 1.515716566510398f : 1.7411011265922482f;
+  const uint32_t sb = (uint32_t)(k + 127) << 23;
+  float scale;
+  __builtin_memcpy(&scale, &sb, 4);
+  return (c * p) * scale;
+}
+
+// --------------------------------------------------------------- texture --
+// RGBA8 texture with its box-filtered mip chain: level 0 (w x h), then
+// level l of ((W+1)/2) x ((H+1)/2) texels, `levels` levels after level 0
+// (floor(log2(min(w, h))), texture_data, src/material.h:27-35).
+struct DevTex {
+  const uchar4 *texels;
+  int32_t w, h, levels, pad;
+};
+
+// Mip level of a footprint: ceil(log2(diff * max(w, h))) clamped to
+// [0, levels], taken as the number of i < levels with 2^i < diff*max(w,h)
+// (the loop `for (i = 0; i < lod; ++i)` of sample_bilin_lod,
+// src/material.h:88-99; compared exactly, without log2f's last-ulp rounding
+// near powers of two); a NaN footprint takes the coarsest level, as
+// fminf(NaN, max_mm) does there.
+__host__ __device__ __forceinline__ int tex_level(float diff, int32_t w, int32_t h, int32_t levels) {
+  const float x = diff * (float)(w > h ? w : h);
+  if (x != x) return levels;
+  int l = 0;
+  float p = 1.0f;
+  while (l < levels && p < x) {
+    ++l;
+    p = p * 2.0f;
+  }
+  return l;
+}
+
+// sample_bilin_lod (src/material.h:88-137): wrap (s, t) into [0, 1), pick
+// the level, x = s W, y = (1 - t) H; blend the texel holding (x, y) with its
+// neighbours on the side of the texel centre the point lies, weights
+// |neighbour centre - point|.  Declared difference: addressing repeats
+// (the reference leaves a neighbour index of exactly W or H, and y = H for
+// t = 0, unwrapped, which reads the next row or the next level); texture
+// coordinates that are NaN, infinite or at least 2^23 in magnitude wrap to 0.
+__device__ __forceinline__ f3 tex_sample_lod(const DevTex &T, float s, float t, float diff) {
+  int32_t W = T.w, H = T.h;
+  const int lv = tex_level(diff, T.w, T.h, T.levels);
+  uint32_t off = 0;
+  for (int i = 0; i < lv; ++i) {
+    off += (uint32_t)W * (uint32_t)H;
+    W = (W + 1) / 2;
+    H = (H + 1) / 2;
+  }
+  if (!(fabsf(s) < 0x1p23f)) s = 0.0f;  // NaN, infinite or integral: wraps to 0
+  if (!(fabsf(t) < 0x1p23f)) t = 0.0f;
+  if (s < 0.0f) s = s + (-truncf(s) + 1.0f);
+  if (s >= 1.0f) s = s - truncf(s);
+  if (t < 0.0f) t = t + (-truncf(t) + 1.0f);
+  if (t >= 1.0f) t = t - truncf(t);
+  const float x = s * (float)W;
+  const float y = (1.0f - t) * (float)H;
+  int32_t nx = (int32_t)x, ny = (int32_t)y;
+  int32_t ox = nx + 1, oy = ny + 1;
+  if (x - floorf(x) < 0.5f) ox = nx - 1;
+  if (y - floorf(y) < 0.5f) oy = ny - 1;
+  const float wx = fabsf(((float)ox + 0.5f) - x);
+  const float wy = fabsf(((float)oy + 0.5f) - y);
+  nx = nx % W;
+  ny = ny % H;
+  ox = (ox % W + W) % W;
+  oy = (oy % H + H) % H;
+  const uchar4 *L = T.texels + off;
+  const uchar4 a00 = L[ny * W + nx], a01 = L[ny * W + ox], a10 = L[oy * W + nx], a11 = L[oy * W + ox];
+  const float k = 255.0f;
+  const f3 c00 = f3{(float)a00.x / k, (float)a00.y / k, (float)a00.z / k};
+  const f3 c01 = f3{(float)a01.x / k, (float)a01.y / k, (float)a01.z / k};
+  const f3 c10 = f3{(float)a10.x / k, (float)a10.y / k, (float)a10.z / k};
+  const f3 c11 = f3{(float)a11.x / k, (float)a11.y / k, (float)a11.z / k};
+  const float vx = 1.0f - wx, vy = 1.0f - wy;
+  const f3 r0 = f3{wx * c00.x + vx * c01.x, wx * c00.y + vx * c01.y, wx * c00.z + vx * c01.z};
+  const f3 r1 = f3{wx * c10.x + vx * c11.x, wx * c10.y + vx * c11.y, wx * c10.z + vx * c11.z};
+  return f3{wy * r0.x + vy * r1.x, wy * r0.y + vy * r1.y, wy * r0.z + vy * r1.z};
+}
+
+// The material with its diffuse texture's value tx applied
+// (material_t::diffuseColor, src/material.h:155-168): Lambert scales the
+// diffuse colour; principled scales the base colour, and the quantities
+// derived from it are recomputed as on the host (mon2lin by pow22).
+__device__ __forceinline__ void apply_texel(DevMat &m, f3 tx) {
+  if (!m.principled) {
+    m.diffuse[0] = m.diffuse[0] * tx.x;
+    m.diffuse[1] = m.diffuse[1] * tx.y;
+    m.diffuse[2] = m.diffuse[2] * tx.z;
+    return;
+  }
+  m.color[0] = m.color[0] * tx.x;
+  m.color[1] = m.color[1] * tx.y;
+  m.color[2] = m.color[2] * tx.z;
+  float cd[3];
+  for (int k = 0; k < 3; ++k) cd[k] = pow22(m.color[k]);
+  const float lum = 0.3f * cd[0] + 0.6f * cd[1] + 0.1f * cd[2];
+  const float sp = m.specular * 0.08f;
+  for (int k = 0; k < 3; ++k) {
+    const float tint = lum > 0.f ? cd[k] / lum : 1.f;
+    const float mt = (1.0f - m.spec_tint) * 1.0f + m.spec_tint * tint;
+    const float x = sp * mt;
+    m.cspec0[k] = (1.0f - m.metallic) * x + m.metallic * cd[k];
+    m.csheen[k] = (1.0f - m.sheen_tint) * 1.0f + m.sheen_tint * tint;
+    m.cdlin[k] = cd[k];
+  }
+}
 
 enum { kTypeSpecular = 0, kTypeDiffuse = 1, kTypeSpecRefl = 3, kTypeSpecTrans = 4, kTypeDisneyRefl = 5 };
 

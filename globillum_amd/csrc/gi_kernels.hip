@@ -116,6 +116,9 @@ struct ShadeParams {
   int32_t depth, rr_start;
   SkyDev sky;
   InstScene is;
+  const DevTex *tex;       // textures (diffuse, by DevMat::texture)
+  float4 *rdiff;           // ray differentials per path slot, or null (no texture in use):
+  uint32_t rdiff_stride;   // planes [0..3] = upper origin, upper dir, right origin, right dir
 };
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -519,12 +522,44 @@ __device__ __forceinline__ int find_chunk(const PassDesc *pass, uint32_t slot) {
   return c;
 }
 
+// One level of the box-filtered mip chain (k::mipmap, src/material.cu:235-256):
+// texel (x, y) of the dw x dh level is the truncating integer mean, per
+// channel, of the (up to) 2x2 texels of the sw x sh level below it that
+// exist.  (The reference zeroes the wrong weight at an odd right edge and
+// reads an unset value; the missing texel's weight is the one dropped here.)
+__global__ __launch_bounds__(256) void k_mipmap(const uchar4 *__restrict__ src, int32_t sw, int32_t sh,
+                                                uchar4 *__restrict__ dst, int32_t dw, int32_t dh) {
+  const uint32_t total = (uint32_t)dw * (uint32_t)dh;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int32_t x = (int32_t)(i % (uint32_t)dw), y = (int32_t)(i / (uint32_t)dw);
+    const int32_t X = 2 * x, Y = 2 * y;
+    const bool rx = X + 1 < sw, ry = Y + 1 < sh;
+    const uchar4 a = src[Y * sw + X];
+    uint32_t r = a.x, g = a.y, b = a.z, w = a.w, cnt = 1;
+    if (rx) {
+      const uchar4 c = src[Y * sw + X + 1];
+      r += c.x; g += c.y; b += c.z; w += c.w; ++cnt;
+    }
+    if (ry) {
+      const uchar4 c = src[(Y + 1) * sw + X];
+      r += c.x; g += c.y; b += c.z; w += c.w; ++cnt;
+    }
+    if (rx && ry) {
+      const uchar4 c = src[(Y + 1) * sw + X + 1];
+      r += c.x; g += c.y; b += c.z; w += c.w; ++cnt;
+    }
+    dst[i] = make_uchar4((unsigned char)(r / cnt), (unsigned char)(g / cnt), (unsigned char)(b / cnt),
+                         (unsigned char)(w / cnt));
+  }
+}
+
 // Primary rays of every path slot of a pass (new_path: generate_rays and
 // the throughput / colour reset, src/gpu-pt.h:110-118).
 __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pass, const DevView *__restrict__ views,
                                                 HaltonDev H, float4 *__restrict__ ray_o, float4 *__restrict__ ray_d,
                                                 float4 *__restrict__ tp, float4 *__restrict__ path_l,
-                                                float4 *__restrict__ rec, uint32_t rec_stride, int32_t depth) {
+                                                float4 *__restrict__ rec, uint32_t rec_stride, int32_t depth,
+                                                float4 *__restrict__ rdiff, uint32_t rdiff_stride) {
   const uint32_t total = pass->total_slots;
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
     const PassChunk &ck = pass->chunk[find_chunk(pass, slot)];
@@ -542,6 +577,22 @@ __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pas
     path_l[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (rec)
       for (int32_t b = 0; b < depth; ++b) rec[(uint32_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (rdiff) {
+      // ray differentials: pinhole rays from the camera position through the
+      // centres of pixels (x, y+1) ("upper") and (x+1, y) ("right"); rta's
+      // camera_ray_generator_shirley, which makes them in the reference, is
+      // external (src/raygen.h:122-130)
+      DevView pv = v;
+      pv.aperture = 0.0f;
+      const int32_t x = (int32_t)(p % (uint32_t)v.w), y = (int32_t)(p / (uint32_t)v.w);
+      f3 uo, ud, ro, rd;
+      camera_ray(pv, x, y + 1, 0.5f, 0.5f, 0.5f, 0.5f, uo, ud);
+      camera_ray(pv, x + 1, y, 0.5f, 0.5f, 0.5f, 0.5f, ro, rd);
+      rdiff[slot] = make_float4(uo.x, uo.y, uo.z, 0.f);
+      rdiff[rdiff_stride + slot] = make_float4(ud.x, ud.y, ud.z, 0.f);
+      rdiff[2 * rdiff_stride + slot] = make_float4(ro.x, ro.y, ro.z, 0.f);
+      rdiff[3 * rdiff_stride + slot] = make_float4(rd.x, rd.y, rd.z, 0.f);
+    }
   }
 }
 
@@ -600,10 +651,26 @@ struct VertexEval {
   int type, light;
   uint32_t mat;
   bool glass;
+  DevMat m;  // the material at this vertex (its texture applied)
 };
 
-__device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, int dim, f3 org_dir, float4 h,
-                                            VertexEval &ve) {
+// reflect(d, N) = d - 2 (d.N) N
+__device__ __forceinline__ f3 reflect_dir(f3 d, f3 N) {
+  const float k = 2.0f * dot(d, N);
+  return f3{d.x - N.x * k, d.y - N.y * k, d.z - N.z * k};
+}
+
+// kTex: some material has a texture (P.rdiff is set): the ray differentials
+// are carried along and ve.m holds the material with its texel applied;
+// otherwise the material is read from P.mats where it is used.
+template <bool kTex>
+__device__ __forceinline__ const DevMat &vertex_mat(const ShadeParams &P, const VertexEval &ve) {
+  return kTex ? ve.m : P.mats[ve.mat];
+}
+
+template <bool kTex>
+__device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, int dim, uint32_t slot, f3 org_dir,
+                                            float4 h, VertexEval &ve) {
   uint32_t tid = fbits(h.w);
   const DevInstance *I = nullptr;
   if (tid >= P.is.n_world) {
@@ -634,7 +701,50 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
             (r0.z * Ni.x + r1.z * Ni.y) + r2.z * Ni.z};
     if (I->material >= 0) ve.mat = (uint32_t)I->material;
   }
-  const DevMat &m = P.mats[ve.mat];
+  if (kTex) ve.m = P.mats[ve.mat];
+  const DevMat &m = vertex_mat<kTex>(P, ve);
+  const f3 N0 = normalize(Ni);
+  if (kTex) {
+    // computeRayDiffs (src/hybrid-pt.cpp:300-328): the differentials cross
+    // the hit triangle's plane; their texture coordinates give the lookup's
+    // footprint (material_t::diffuseColor, src/material.h:160-166); the
+    // origins move to the crossings and the upper direction is reflected
+    // about the shading normal (the right direction is kept, as there)
+    const uint32_t st = P.rdiff_stride;
+    f3 A = xyz(T.a), B = xyz(T.b), C = xyz(T.c);
+    if (I) {
+      const float4 f0 = I->fwd[0], f1 = I->fwd[1], f2 = I->fwd[2];
+      A = f3{((f0.x * A.x + f0.y * A.y) + f0.z * A.z) + f0.w, ((f1.x * A.x + f1.y * A.y) + f1.z * A.z) + f1.w,
+             ((f2.x * A.x + f2.y * A.y) + f2.z * A.z) + f2.w};
+      B = f3{((f0.x * B.x + f0.y * B.y) + f0.z * B.z) + f0.w, ((f1.x * B.x + f1.y * B.y) + f1.z * B.z) + f1.w,
+             ((f2.x * B.x + f2.y * B.y) + f2.z * B.z) + f2.w};
+      C = f3{((f0.x * C.x + f0.y * C.y) + f0.z * C.z) + f0.w, ((f1.x * C.x + f1.y * C.y) + f1.z * C.z) + f1.w,
+             ((f2.x * C.x + f2.y * C.y) + f2.z * C.z) + f2.w};
+    }
+    const f3 e1 = sub(B, A), e2 = sub(C, A);
+    const float4 ud4 = P.rdiff[st + slot];
+    float ub, ug, rb, rg;
+    plane_bary(A, e1, e2, xyz(P.rdiff[slot]), xyz(ud4), ub, ug);
+    plane_bary(A, e1, e2, xyz(P.rdiff[2 * st + slot]), xyz(P.rdiff[3 * st + slot]), rb, rg);
+    const float u0 = (1.0f - ub) - ug, r0 = (1.0f - rb) - rg;
+    if (m.texture >= 0) {
+      const float4 tab = T.tab, tcm = T.tcm;
+      const float tcx = (tab.x * bc.x + tab.z * bc.y) + tcm.x * bc.z;
+      const float tcy = (tab.y * bc.x + tab.w * bc.y) + tcm.y * bc.z;
+      const float ux = (tab.x * u0 + tab.z * ub) + tcm.x * ug, uy = (tab.y * u0 + tab.w * ub) + tcm.y * ug;
+      const float rx = (tab.x * r0 + tab.z * rb) + tcm.x * rg, ry = (tab.y * r0 + tab.w * rb) + tcm.y * rg;
+      float dx = fabsf(tcx - ux), dy = fabsf(tcy - uy);
+      dx = fmaxf(fabsf(tcx - rx), dx);
+      dy = fmaxf(fabsf(tcy - ry), dy);
+      apply_texel(ve.m, tex_sample_lod(P.tex[m.texture], tcx, tcy, fmaxf(dx, dy)));
+    }
+    const f3 uo = add(add(scl(A, u0), scl(B, ub)), scl(C, ug));
+    const f3 ro = add(add(scl(A, r0), scl(B, rb)), scl(C, rg));
+    const f3 ud = reflect_dir(xyz(ud4), N0);
+    P.rdiff[slot] = make_float4(uo.x, uo.y, uo.z, 0.0f);
+    P.rdiff[st + slot] = make_float4(ud.x, ud.y, ud.z, 0.0f);
+    P.rdiff[2 * st + slot] = make_float4(ro.x, ro.y, ro.z, 0.0f);
+  }
   ve.org_dir = org_dir;
   ve.P = Pp;
 
@@ -677,7 +787,7 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
   ve.gk = factor * sel;
 
   // bounce: tangent frame, normal flip, BRDF sample
-  f3 N = normalize(Ni);
+  f3 N = N0;
   f3 Tt, Bt;
   tangent_frame(N, Tt, Bt);
   if (dot(org_dir, N) > 0.0f) N = neg(N);
@@ -711,6 +821,7 @@ constexpr uint32_t kRecBackground = 8u;
 // Shading of path vertex b.  The NEE contribution travels with the shadow
 // ray; the shadow kernel adds it when unoccluded.  rec != nullptr: also
 // record the vertex for the adjoint (see kRecBackground).
+template <bool kTex>
 __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__restrict__ count_ptr, ShadeParams P,
                                                const float4 *__restrict__ ray_o, const float4 *__restrict__ ray_d,
                                                const float4 *__restrict__ tp_in, const float4 *__restrict__ hits,
@@ -752,8 +863,8 @@ __global__ __launch_bounds__(256) void k_shade(int32_t b, const uint32_t *__rest
         }
       } else {
         VertexEval ve;
-        eval_vertex(P, n, dim, org_dir, h, ve);
-        const DevMat &m = P.mats[ve.mat];
+        eval_vertex<kTex>(P, n, dim, slot, org_dir, h, ve);
+        const DevMat &m = vertex_mat<kTex>(P, ve);
         if (!ve.glass) {
           const f3 C = mul(ve.fnee, mul(TP, ve.psc));
           if (C.x > 0.0f || C.y > 0.0f || C.z > 0.0f) {
@@ -953,8 +1064,8 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
       const f3 org_dir = normalize(xyz(rd));
       if (id != GI_INVALID_REF) {
         VertexEval ve;
-        eval_vertex(P, n, dim, org_dir, h, ve);
-        const DevMat &m = P.mats[ve.mat];
+        eval_vertex<false>(P, n, dim, slot, org_dir, h, ve);
+        const DevMat &m = vertex_mat<false>(P, ve);
         const DevMatGrad &mg = A.mg[ve.mat];
         const f3 inv = neg(org_dir);
         // adjoint weight of the path's pixel: dL/dI / spp
@@ -1154,6 +1265,15 @@ struct gi_ctx {
   // instancing: meshes (object space), instances, the instance tree
   std::vector<std::vector<gi_tri>> meshes;
   DBuf d_inst, d_inst_vbase;
+  // textures: one buffer per texture (level 0 + mip chain), their records
+  struct TexHost {
+    int32_t w, h, levels;
+    size_t texels;
+  };
+  std::vector<DBuf> tex_bufs;
+  std::vector<TexHost> texs;
+  DBuf d_tex;
+  bool textured = false;  // some material names a texture
   int64_t world_nnodes = 0;
   int32_t world_depth = 0, n_inst = 0;
   // range of the material indices shading can reach: world triangles, and
@@ -1170,6 +1290,7 @@ struct gi_ctx {
   DBuf d_views, d_passes, d_counters;
   DBuf d_matgrad, d_rec, d_dl, d_gmats, d_gem;
   DBuf q_o[2], q_d[2], q_tp[2], q_hit, s_o, s_d, s_c, path_l;
+  DBuf rdiff;  // 4 planes of float4 per slot: ray differentials (textures in use)
   size_t slot_cap = 0;
   // primitives
   DBuf p_orig, p_dir, p_maxt, p_ro, p_rd, p_hit, p_occ;
@@ -1261,6 +1382,9 @@ DevMat make_devmat(const gi_material &m) {
   d.cc_log_a2 = std::log(d.cc_a * d.cc_a);
   d.opacity = m.opacity;
   d.specular = m.specular;
+  d.texture = m.texture;
+  d.spec_tint = m.specular_tint;
+  d.sheen_tint = m.sheen_tint;
   float aspect = std::sqrt(1.0f - m.anisotropic * 0.9f);
   float r2 = m.roughness * m.roughness;
   d.ax = std::max(0.001f, r2 / aspect);
@@ -1394,6 +1518,9 @@ int check_scene(gi_ctx *ctx, bool need_shading) {
     }
     if (lo < 0 || hi >= (int32_t)ctx->mats.size())
       return fail(ctx, GI_ERR_STATE, "a triangle or instance material index is outside the uploaded materials");
+    for (const gi_material &m : ctx->mats)
+      if (m.texture < -1 || m.texture >= (int32_t)ctx->texs.size())
+        return fail(ctx, GI_ERR_STATE, "a material names a texture that was not uploaded (gi_upload_texture)");
   }
   return GI_OK;
 }
@@ -1547,8 +1674,13 @@ void plan_passes(const std::vector<gi_view> &views, const std::vector<int32_t> &
 }
 
 int ensure_workspace(gi_ctx *ctx, size_t slots) {
-  if (slots <= ctx->slot_cap && ctx->path_l.p) return GI_OK;
+  if (slots <= ctx->slot_cap && ctx->path_l.p) {
+    // ray differentials only while a material has a texture
+    if (ctx->textured) GI_HIP(ctx, ctx->rdiff.ensure(4 * ctx->slot_cap * sizeof(float4)));
+    return GI_OK;
+  }
   const size_t b4 = slots * sizeof(float4);
+  if (ctx->textured) GI_HIP(ctx, ctx->rdiff.ensure(4 * b4));
   for (int i = 0; i < 2; ++i) {
     GI_HIP(ctx, ctx->q_o[i].ensure(b4));
     GI_HIP(ctx, ctx->q_d[i].ensure(b4));
@@ -1581,6 +1713,9 @@ ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
                  ctx->sky_w, ctx->sky_h, ctx->sky_light,
                  ctx->sky_light >= 0 ? ctx->lights[(size_t)ctx->sky_light].col[0] : 0.0f};
   P.is = inst_scene(ctx);
+  P.tex = ctx->d_tex.as<DevTex>();
+  P.rdiff = ctx->textured ? ctx->rdiff.as<float4>() : nullptr;
+  P.rdiff_stride = (uint32_t)ctx->slot_cap;
   return P;
 }
 
@@ -1638,7 +1773,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
     const int rg = (int)std::min<uint64_t>((pd.total_slots + 255) / 256, (uint64_t)persist);
     hipLaunchKernelGGL(k_raygen, dim3(std::max(1, rg)), dim3(256), 0, ctx->stream, dp, dviews, SP.H,
                        ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->q_tp[0].as<float4>(),
-                       ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, depth);
+                       ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, depth, SP.rdiff, SP.rdiff_stride);
     uint32_t host_count = pd.total_slots;
     for (int32_t b = 0; b < depth; ++b) {
       const int cur = b & 1, nxt = cur ^ 1;
@@ -1655,11 +1790,12 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
       ++launches;
       if (timing && (rc = mark(0))) return rc;
       const int sg = (int)std::max<uint64_t>(1, std::min<uint64_t>((bound + sblock - 1) / sblock, (uint64_t)persist));
-      hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b, cnt + b, SP, ctx->q_o[cur].as<float4>(),
-                         ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(), ctx->q_hit.as<float4>(),
-                         ctx->path_l.as<float4>(), ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(),
-                         ctx->q_tp[nxt].as<float4>(), cnt + b + 1, ctx->s_o.as<float4>(), ctx->s_d.as<float4>(),
-                         ctx->s_c.as<float4>(), cnt + depth + 1 + b, (float4 *)nullptr, 0u);
+      hipLaunchKernelGGL(SP.rdiff ? k_shade<true> : k_shade<false>, dim3(sg), dim3(sblock), 0, ctx->stream, b,
+                         cnt + b, SP, ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(),
+                         ctx->q_tp[cur].as<float4>(), ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(),
+                         ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(), ctx->q_tp[nxt].as<float4>(),
+                         cnt + b + 1, ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>(),
+                         cnt + depth + 1 + b, (float4 *)nullptr, 0u);
       GI_HIP(ctx, hipGetLastError());
       if (timing && (rc = mark(2))) return rc;
       uint32_t shadow_host = 0;
@@ -1756,14 +1892,15 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
       hipLaunchKernelGGL(k_pass_setup, dim3(1), dim3(256), 0, ctx->stream, dp, cnt, ncnt);
       hipLaunchKernelGGL(k_raygen, dim3(std::max(1, rg)), dim3(256), 0, ctx->stream, dp, dviews, SP.H,
                          ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->q_tp[0].as<float4>(),
-                         ctx->path_l.as<float4>(), phase == 0 ? rec : (float4 *)nullptr, stride, depth);
+                         ctx->path_l.as<float4>(), phase == 0 ? rec : (float4 *)nullptr, stride, depth,
+                         (float4 *)nullptr, 0u);
       for (int32_t b = 0; b < depth; ++b) {
         const int cur = b & 1, nxt = cur ^ 1;
         rc = launch_closest(ctx, false, cnt + b, 0, pd.total_slots, ctx->q_o[cur].as<float4>(),
                             ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
         if (rc) return rc;
         if (phase == 0) {
-          hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b, cnt + b, SP,
+          hipLaunchKernelGGL(k_shade<false>, dim3(sg), dim3(sblock), 0, ctx->stream, b, cnt + b, SP,
                              ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
                              ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(), ctx->q_o[nxt].as<float4>(),
                              ctx->q_d[nxt].as<float4>(), ctx->q_tp[nxt].as<float4>(), cnt + b + 1,
@@ -2005,6 +2142,8 @@ int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n) {
   if (!ctx || !mats || n <= 0) return fail(ctx, GI_ERR_INVALID_ARG, "materials");
   GI_HIP(ctx, hipSetDevice(ctx->device));
   ctx->mats.assign(mats, mats + n);
+  ctx->textured = false;
+  for (int32_t i = 0; i < n; ++i) ctx->textured = ctx->textured || mats[i].texture >= 0;
   std::vector<DevMat> dm(n);
   std::vector<DevMatGrad> dg(n);
   for (int32_t i = 0; i < n; ++i) {
@@ -2128,6 +2267,65 @@ int gi_build_bvh_device(gi_ctx *ctx, int32_t max_leaf) {
                   ctx->chi, ml, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), r, err))
     return fail(ctx, GI_ERR_HIP, err);
   return finish_bvh(ctx, r.nnodes, r.depth);
+}
+
+int gi_upload_texture(gi_ctx *ctx, const uint8_t *rgba, int32_t w, int32_t h, int32_t *tex_id) {
+  if (!ctx || !rgba || w <= 0 || h <= 0 || !tex_id) return fail(ctx, GI_ERR_INVALID_ARG, "texture");
+  if ((int64_t)w * h > ((int64_t)1 << 28)) return fail(ctx, GI_ERR_LIMIT, "texture of more than 2^28 texels");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  int32_t levels = 0;
+  while ((2 << levels) <= std::min(w, h)) ++levels;  // floor(log2(min(w, h)))
+  std::vector<size_t> off(levels + 1);
+  size_t total = 0;
+  int32_t W = w, H = h;
+  for (int32_t l = 0; l <= levels; ++l) {
+    off[l] = total;
+    total += (size_t)W * H;
+    W = (W + 1) / 2;
+    H = (H + 1) / 2;
+  }
+  DBuf buf;
+  GI_HIP(ctx, buf.ensure(total * sizeof(uchar4)));
+  GI_HIP(ctx, copy_sync(ctx, buf.p, rgba, (size_t)w * h * sizeof(uchar4), hipMemcpyHostToDevice));
+  W = w;
+  H = h;
+  for (int32_t l = 0; l < levels; ++l) {
+    const int32_t dw = (W + 1) / 2, dh = (H + 1) / 2;
+    const uint32_t n = (uint32_t)dw * (uint32_t)dh;
+    hipLaunchKernelGGL(k_mipmap, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0, ctx->stream,
+                       buf.as<uchar4>() + off[l], W, H, buf.as<uchar4>() + off[l + 1], dw, dh);
+    GI_HIP(ctx, hipGetLastError());
+    W = dw;
+    H = dh;
+  }
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->tex_bufs.emplace_back();
+  std::swap(ctx->tex_bufs.back().p, buf.p);
+  std::swap(ctx->tex_bufs.back().bytes, buf.bytes);
+  ctx->texs.push_back({w, h, levels, total});
+  std::vector<DevTex> dt(ctx->texs.size());
+  for (size_t i = 0; i < dt.size(); ++i)
+    dt[i] = DevTex{ctx->tex_bufs[i].as<uchar4>(), ctx->texs[i].w, ctx->texs[i].h, ctx->texs[i].levels, 0};
+  GI_HIP(ctx, ctx->d_tex.ensure(dt.size() * sizeof(DevTex)));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_tex.p, dt.data(), dt.size() * sizeof(DevTex), hipMemcpyHostToDevice));
+  *tex_id = (int32_t)ctx->texs.size() - 1;
+  return GI_OK;
+}
+
+int gi_texture_info(gi_ctx *ctx, int32_t tex_id, int32_t *levels, int64_t *texels) {
+  if (!ctx || tex_id < 0 || tex_id >= (int32_t)ctx->texs.size()) return fail(ctx, GI_ERR_INVALID_ARG, "texture id");
+  if (levels) *levels = ctx->texs[(size_t)tex_id].levels;
+  if (texels) *texels = (int64_t)ctx->texs[(size_t)tex_id].texels;
+  return GI_OK;
+}
+
+int gi_download_texture(gi_ctx *ctx, int32_t tex_id, uint8_t *rgba) {
+  if (!ctx || !rgba || tex_id < 0 || tex_id >= (int32_t)ctx->texs.size())
+    return fail(ctx, GI_ERR_INVALID_ARG, "texture id");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  GI_HIP(ctx, copy_sync(ctx, rgba, ctx->tex_bufs[(size_t)tex_id].p, ctx->texs[(size_t)tex_id].texels * sizeof(uchar4),
+                        hipMemcpyDeviceToHost));
+  return GI_OK;
 }
 
 int gi_upload_mesh(gi_ctx *ctx, const gi_tri *tris, int64_t n, int32_t *mesh_id) {
@@ -2469,7 +2667,8 @@ int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint3
   HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>()};
   hipLaunchKernelGGL(k_raygen, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_passes.as<PassDesc>(),
                      ctx->d_views.as<DevView>(), H, ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(),
-                     ctx->q_tp[0].as<float4>(), ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, 1);
+                     ctx->q_tp[0].as<float4>(), ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, 1,
+                     (float4 *)nullptr, 0u);
   GI_HIP(ctx, hipGetLastError());
   std::vector<float4> ro(npix), rd(npix);
   GI_HIP(ctx, hipMemcpyAsync(ro.data(), ctx->q_o[0].p, npix * 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -2605,6 +2804,7 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
   if (ctx->views.empty()) return fail(ctx, GI_ERR_STATE, "gi_render first");
   if (ctx->opts.rr_start > 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support Russian roulette");
   if (ctx->sky_light >= 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support sky lights");
+  if (ctx->textured) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support textured materials");
   const int32_t nm = (int32_t)ctx->mats.size(), nl = (int32_t)ctx->lights.size();
   if ((size_t)(5 * nm + 3 * nl) * sizeof(float) > 48 * 1024)
     return fail(ctx, GI_ERR_LIMIT, "adjoint: 5*materials + 3*lights must fit 48 KiB of LDS");

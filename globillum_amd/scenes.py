@@ -32,6 +32,9 @@ class Scene:
     # records placing them; tris stays the world geometry
     meshes: list | None = None
     instances: np.ndarray | None = None
+    # RGBA8 textures, (h, w, 4) uint8, row 0 at t = 1; gi_material.texture
+    # names one by its index
+    textures: list | None = None
 
 
 def rect_light_power(w: float, h: float, col) -> np.float32:
@@ -208,9 +211,10 @@ def _fbm(x, z, rng, octaves=6, gain=0.5, lacunarity=2.0, base_res=4):
 
 
 def heightfield(rng, nx: int, nz: int, size_x: float, size_z: float, amplitude: float, n_patch_mats: int,
-                mat_offset: int = 0):
-    """nx x nz cells -> 2*nx*nz triangles with smooth vertex normals; the
-    material index cycles over 8x8 patches."""
+                mat_offset: int = 0, uv_repeat: float = 1.0):
+    """nx x nz cells -> 2*nx*nz triangles with smooth vertex normals and
+    texture coordinates (x, z) / size * uv_repeat; the material index cycles
+    over 8x8 patches."""
     xs = np.linspace(0.0, 1.0, nx + 1)
     zs = np.linspace(0.0, 1.0, nz + 1)
     X, Z = np.meshgrid(xs, zs, indexing="ij")
@@ -228,6 +232,9 @@ def heightfield(rng, nx: int, nz: int, size_x: float, size_z: float, amplitude: 
     t["na"][0::2], t["nb"][0::2], t["nc"][0::2] = N[v00], N[v01], N[v11]
     t["a"][1::2], t["b"][1::2], t["c"][1::2] = P[v00], P[v11], P[v10]
     t["na"][1::2], t["nb"][1::2], t["nc"][1::2] = N[v00], N[v11], N[v10]
+    UV = np.stack([X, Z], -1) * uv_repeat  # texture coordinates: (x, z) over the extent
+    t["ta"][0::2], t["tb"][0::2], t["tc"][0::2] = UV[v00], UV[v01], UV[v11]
+    t["ta"][1::2], t["tb"][1::2], t["tc"][1::2] = UV[v00], UV[v11], UV[v10]
     patch = ((i * 8) // nx) * 8 + (k * 8) // nz
     mi = (mat_offset + patch % n_patch_mats).astype(np.int32)
     t["material_index"][0::2] = mi
@@ -253,10 +260,12 @@ def uv_sphere(center, radius, segments=10, rings=9, material=0) -> np.ndarray:
                 tris.append((b, c, d))
     tris = tris
     out = np.zeros(len(tris), TRI)
+    UV = np.stack([Pp / (2 * np.pi), 1.0 - T / np.pi], -1)  # texture coordinates (phi, theta)
     for j, key in enumerate(("a", "b", "c")):
         idx = np.array([tt[j] for tt in tris])
         out[key] = V[idx[:, 0], idx[:, 1]]
         out["n" + key] = N[idx[:, 0], idx[:, 1]]
+        out["t" + key] = UV[idx[:, 0], idx[:, 1]]
     out["material_index"] = material
     return out
 
@@ -425,6 +434,57 @@ def large_combined(seed: int = 6, cells=(2048, 1024), n_spheres: int = 2000, n_m
                  meshes=[sphere], instances=np.concatenate(inst))
 
 
+def checker_texture(w: int = 64, h: int = 64, cell: int = 1, c0=(230, 230, 230), c1=(20, 40, 160)) -> np.ndarray:
+    """RGBA8 checkerboard of cell x cell texel squares."""
+    y, x = np.mgrid[0:h, 0:w]
+    on = ((x // cell + y // cell) % 2).astype(bool)
+    t = np.empty((h, w, 4), np.uint8)
+    t[...] = (*c0, 255)
+    t[on] = (*c1, 255)
+    return t
+
+
+def noise_texture(w: int = 1024, h: int = 1024, seed: int = 7) -> np.ndarray:
+    """RGBA8 albedo: fBm value noise tinted per channel plus texel noise."""
+    rng = np.random.default_rng(seed)
+    ys, xs = np.meshgrid(np.linspace(0.0, 1.0, h), np.linspace(0.0, 1.0, w), indexing="ij")
+    base = 0.5 + 0.25 * _fbm(xs, ys, rng, octaves=7, base_res=8)
+    rgb = np.stack([base * 0.9, base * 0.8 + 0.1, base * 0.6], -1) + 0.08 * rng.random((h, w, 3))
+    out = np.empty((h, w, 4), np.uint8)
+    out[..., :3] = np.clip(rgb * 255.0, 0, 255).astype(np.uint8)
+    out[..., 3] = 255
+    return out
+
+
+def textured_scene(seed: int = 8, cells=(32, 32), w: int = 96, h: int = 64, spp: int = 4, depth: int = 4,
+                   n_instances: int = 40, tex_size=(64, 64)) -> Scene:
+    """A heightfield under a grazing camera with a 1-texel checkerboard
+    (Lambert and principled patches) and instanced textured spheres: every
+    mip level of the texture is reached."""
+    rng = np.random.default_rng(seed)
+    size = 30.0
+    ter, H = heightfield(rng, *cells, size, size, 1.5, 4, uv_repeat=4.0)
+    mats = np.concatenate([
+        lambert((0.9, 0.9, 0.9)), principled((0.8, 0.7, 0.6), 0.5, 0.0), lambert((0.7, 0.8, 0.9)),
+        principled((0.9, 0.9, 0.9), 0.3, 1.0), lambert((0.6, 0.6, 0.6)), principled((0.5, 0.8, 0.5), 0.8, 0.0)])
+    mats["texture"] = [0, 0, 1, 1, -1, 0]
+    sphere = uv_sphere((0.0, 0.0, 0.0), 1.0, 12, 10, material=5)
+    inst = []
+    for _ in range(n_instances):
+        cx, cz = rng.uniform(2.0, size - 2.0, 2)
+        hx = min(int(cx / size * cells[0]), cells[0])
+        hz = min(int(cz / size * cells[1]), cells[1])
+        r = rng.uniform(0.3, 1.2)
+        A = rotation(rng.normal(size=3), rng.uniform(0.0, 2 * np.pi)) * r
+        inst.append(make_instance(A, (cx, H[hx, hz] + r, cz), 0, int(rng.integers(-1, 6))))
+    lights = np.concatenate([make_light((10.0, 12.0, 10.0), (0, -1, 0), (60, 60, 60), 3.0, 3.0),
+                             make_light((22.0, 10.0, 20.0), (0, -1, 0), (40, 40, 40), 2.0, 2.0)])
+    view = make_view((0.3 * size, 3.0, -2.0), (0.5 * size, 0.0, size), (0, 1, 0), 60.0, w, h, spp)
+    return Scene(ter, mats, lights, view, make_opts(depth), name="textured_scene", meshes=[sphere],
+                 instances=np.concatenate(inst),
+                 textures=[checker_texture(*tex_size, cell=1), noise_texture(37, 23, seed=seed)])
+
+
 def flatten(scene: Scene) -> Scene:
     """The same scene without instancing: every instance's mesh mapped to
     world space in float64 (positions by the transform, normals by its
@@ -445,7 +505,7 @@ def flatten(scene: Scene) -> Scene:
             m["material_index"] = r["material"]
         parts.append(m)
     return Scene(np.concatenate(parts), scene.mats, scene.lights, scene.views, scene.opts, name=scene.name + "_flat",
-                 sky=scene.sky)
+                 sky=scene.sky, textures=scene.textures)
 
 
 def save_scene(scene: Scene, path: str) -> None:

@@ -88,7 +88,7 @@ typedef struct {
   float specular_tint, anisotropic, sheen, sheen_tint;
   float clearcoat, clearcoat_gloss, opacity, transmission;
   float color[3];
-  int32_t texture;          /* reserved: -1 (textures are not on this path) */
+  int32_t texture;          /* diffuse texture (gi_upload_texture id), -1: none */
 } gi_material;
 
 /* Light: gi::light with the rect_light or sky_light member
@@ -275,6 +275,24 @@ int gi_progress_step(gi_ctx *ctx, int32_t *done);
  * score-function term.  Not defined with Russian roulette (rr_start > 0). */
 int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats,
                float *d_emission);
+
+/* ---- Textures (texture_data, src/material.h:21-137; compute_mipmaps,
+ * src/material.cu:235-274) ----------------------------------------------------
+ * An RGBA8 image, w x h texels row-major (row 0 is the top, t = 1), and its
+ * box-filtered mip chain built on the device: floor(log2(min(w, h))) levels
+ * after level 0, each ((W+1)/2) x ((H+1)/2), texels the truncating integer
+ * mean of the 2x2 below.  A material names its diffuse texture in
+ * gi_material.texture.  Shading then scales the diffuse colour (Lambert) or
+ * the base colour (principled) by sample_bilin_lod at the hit's texture
+ * coordinates, with the level from the footprint of the path's two ray
+ * differentials (computeRayDiffs, src/hybrid-pt.cpp:300-328).  Replaces
+ * convert_texture / compute_mipmaps (src/material-conversion.cpp:18-41);
+ * the image file is decoded by the caller. */
+int gi_upload_texture(gi_ctx *ctx, const uint8_t *rgba, int32_t w, int32_t h, int32_t *tex_id);
+/* Levels after level 0, and texels of the whole chain. */
+int gi_texture_info(gi_ctx *ctx, int32_t tex_id, int32_t *levels, int64_t *texels);
+/* The chain, level 0 first: 4 * texels bytes. */
+int gi_download_texture(gi_ctx *ctx, int32_t tex_id, uint8_t *rgba);
 
 /* ---- Instancing (two-level BVH) --------------------------------------------
  * No counterpart in the reference; needed by the large configuration

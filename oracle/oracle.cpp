@@ -242,6 +242,13 @@ struct or_scene {
     uint32_t vbase;
   };
   std::vector<Instance> inst;
+  /* RGBA8 textures with their mip chains (level 0, then each halved level) */
+  struct Texture {
+    int32_t w, h, levels;
+    std::vector<uint8_t> chain;
+  };
+  std::vector<Texture> textures;
+  bool textured = false; /* some material has a diffuse texture */
   ~or_scene() {
     for (or_scene *m : meshes) delete m;
   }
@@ -475,6 +482,165 @@ void surface(const or_scene &s, uint32_t id, R bu, R bv, V3<R> &P, V3<R> &Ni, in
   }
 }
 
+/* The hit triangle in world space (vertices mapped by the instance's
+ * forward rows) and its texture coordinates. */
+inline void world_tri(const or_scene &s, uint32_t id, V3<float> &A, V3<float> &B, V3<float> &C, const float *&ta,
+                      const float *&tb, const float *&tc) {
+  const uint32_t nw = (uint32_t)s.tris.size();
+  const gi_tri *T;
+  const or_scene::Instance *I = nullptr;
+  if (id < nw) {
+    T = &s.tris[id];
+  } else {
+    size_t lo = 0, hi = s.inst.size() - 1;
+    while (lo < hi) {
+      const size_t mid = (lo + hi + 1) / 2;
+      if (nw + s.inst[mid].vbase <= id) lo = mid;
+      else hi = mid - 1;
+    }
+    I = &s.inst[lo];
+    T = &s.meshes[(size_t)I->mesh]->tris[id - nw - I->vbase];
+  }
+  A = ld<float>(T->a); B = ld<float>(T->b); C = ld<float>(T->c);
+  if (I) { A = xf_point<float>(I->fwd, A); B = xf_point<float>(I->fwd, B); C = xf_point<float>(I->fwd, C); }
+  ta = T->ta; tb = T->tb; tc = T->tc;
+}
+
+/* ------------------------------------------------------------- textures -- */
+/* Box-filtered mip chain (k::mipmap + compute_mipmaps,
+ * src/material.cu:235-274): floor(log2(min(w, h))) levels after level 0,
+ * each ((W+1)/2) x ((H+1)/2); a texel is the truncating integer mean of the
+ * (up to) 2x2 texels below it that exist.  (The reference zeroes the wrong
+ * weight at an odd right edge and reads an unset value; here the missing
+ * texel's weight is the one dropped.) */
+void build_chain(or_scene::Texture &t, const uint8_t *rgba) {
+  int32_t mn = std::min(t.w, t.h), levels = 0;
+  while ((2 << levels) <= mn) ++levels;
+  t.levels = levels;
+  size_t total = 0;
+  int32_t W = t.w, H = t.h;
+  for (int l = 0; l <= levels; ++l) { total += (size_t)W * H; W = (W + 1) / 2; H = (H + 1) / 2; }
+  t.chain.assign(4 * total, 0);
+  std::memcpy(t.chain.data(), rgba, 4 * (size_t)t.w * t.h);
+  size_t so = 0;
+  W = t.w; H = t.h;
+  for (int l = 0; l < levels; ++l) {
+    const int32_t dw = (W + 1) / 2, dh = (H + 1) / 2;
+    const size_t d0 = so + (size_t)W * H;
+    for (int32_t y = 0; y < dh; ++y)
+      for (int32_t x = 0; x < dw; ++x) {
+        const int32_t X = 2 * x, Y = 2 * y;
+        const bool rx = X + 1 < W, ry = Y + 1 < H;
+        for (int c = 0; c < 4; ++c) {
+          unsigned sum = t.chain[4 * (so + (size_t)Y * W + X) + c], cnt = 1;
+          if (rx) { sum += t.chain[4 * (so + (size_t)Y * W + X + 1) + c]; ++cnt; }
+          if (ry) { sum += t.chain[4 * (so + (size_t)(Y + 1) * W + X) + c]; ++cnt; }
+          if (rx && ry) { sum += t.chain[4 * (so + (size_t)(Y + 1) * W + X + 1) + c]; ++cnt; }
+          t.chain[4 * (d0 + (size_t)y * dw + x) + c] = (uint8_t)(sum / cnt);
+        }
+      }
+    so = d0;
+    W = dw; H = dh;
+  }
+}
+
+/* Number of i < levels with 2^i < diff * max(w, h): the iterations of
+ * sample_bilin_lod's `for (i = 0; i < lod; ++i)` (src/material.h:90-99), the
+ * log2f compared exactly; NaN -> levels as fminf(NaN, max_mm) makes it. */
+int tex_level(float diff, int32_t w, int32_t h, int32_t levels) {
+  const float x = diff * (float)std::max(w, h);
+  if (std::isnan(x)) return levels;
+  int l = 0;
+  for (double p = 1.0; l < levels && p < (double)x; p *= 2.0) ++l;
+  return l;
+}
+
+/* sample_bilin_lod, src/material.h:88-137, with repeating addressing where
+ * the reference leaves an index of W or H unwrapped (reading the next row
+ * or level).  Coordinates that are NaN, infinite or >= 2^23 in magnitude
+ * wrap to 0. */
+V3<float> tex_sample(const or_scene::Texture &T, float s, float t, float diff) {
+  int32_t W = T.w, H = T.h;
+  const int lv = tex_level(diff, T.w, T.h, T.levels);
+  size_t off = 0;
+  for (int i = 0; i < lv; ++i) { off += (size_t)W * H; W = (W + 1) / 2; H = (H + 1) / 2; }
+  if (!(std::fabs(s) < 0x1p23f)) s = 0.0f;
+  if (!(std::fabs(t) < 0x1p23f)) t = 0.0f;
+  if (s < 0) s += -std::trunc(s) + 1;
+  if (s >= 1) s -= std::trunc(s);
+  if (t < 0) t += -std::trunc(t) + 1;
+  if (t >= 1) t -= std::trunc(t);
+  const float x = s * (float)W, y = (1.0f - t) * (float)H;
+  int32_t nx = (int32_t)x, ny = (int32_t)y, ox = nx + 1, oy = ny + 1;
+  if (x - std::floor(x) < .5) ox = nx - 1;
+  if (y - std::floor(y) < .5) oy = ny - 1;
+  const float wx = (float)std::fabs((double)ox + .5 - (double)x);
+  const float wy = (float)std::fabs((double)oy + .5 - (double)y);
+  nx %= W; ny %= H;
+  ox = (ox % W + W) % W; oy = (oy % H + H) % H;
+  auto texel = [&](int32_t xx, int32_t yy) {
+    const uint8_t *q = &T.chain[4 * (off + (size_t)yy * W + xx)];
+    return mk<float>(float(q[0]) / 255.0f, float(q[1]) / 255.0f, float(q[2]) / 255.0f);
+  };
+  const V3<float> a00 = texel(nx, ny), a01 = texel(ox, ny), a10 = texel(nx, oy), a11 = texel(ox, oy);
+  const V3<float> a0x = wx * a00 + (1.0f - wx) * a01, a1x = wx * a10 + (1.0f - wx) * a11;
+  return wy * a0x + (1.0f - wy) * a1x;
+}
+
+/* The two ray differentials of a path (the "upper" and "right" rays of the
+ * camera's neighbouring pixels, src/hybrid-pt.cpp:300-328). */
+struct Diffs {
+  V3<float> uo, ud, ro, rd;
+};
+
+/* reflect(d, N) = d - 2 (d.N) N */
+inline V3<float> reflect_dir(V3<float> d, V3<float> N) {
+  const float k = 2.0f * dot(d, N);
+  return mk<float>(d.x - N.x * k, d.y - N.y * k, d.z - N.z * k);
+}
+
+/* Barycentrics of a ray's crossing with the triangle's plane, no inside
+ * tests (intersect_tri_opt_nocheck): the arithmetic of intersect_tri. */
+inline void plane_bary(V3<float> a, V3<float> e1, V3<float> e2, V3<float> o, V3<float> d, float &u, float &v) {
+  V3<float> pvec = cross(d, e2);
+  float det = dot(e1, pvec);
+  float inv_det = 1.0f / det;
+  V3<float> tvec = o - a;
+  u = dot(tvec, pvec) * inv_det;
+  V3<float> qvec = cross(tvec, e1);
+  v = dot(d, qvec) * inv_det;
+}
+
+/* computeRayDiffs (src/hybrid-pt.cpp:300-328) at hit (id, bu, bv) with the
+ * normalised shading normal N: the differentials' crossings with the hit
+ * triangle's plane give their texture coordinates; the origins move there,
+ * the upper direction is reflected about N (the right direction is kept, as
+ * there).  Returns the texture footprint of the material's diffuse texture
+ * lookup (material_t::diffuseColor, src/material.h:160-166) and the hit's
+ * texture coordinates. */
+float update_diffs(const or_scene &s, uint32_t id, float bu, float bv, V3<float> N, Diffs &df, float &tcx,
+                   float &tcy) {
+  V3<float> A, B, C;
+  const float *ta, *tb, *tc;
+  world_tri(s, id, A, B, C, ta, tb, tc);
+  const V3<float> e1 = B - A, e2 = C - A;
+  float ub, ug, rb, rg;
+  plane_bary(A, e1, e2, df.uo, df.ud, ub, ug);
+  plane_bary(A, e1, e2, df.ro, df.rd, rb, rg);
+  const float h0 = (1.0f - bu) - bv, u0 = (1.0f - ub) - ug, r0 = (1.0f - rb) - rg;
+  tcx = (ta[0] * h0 + tb[0] * bu) + tc[0] * bv;
+  tcy = (ta[1] * h0 + tb[1] * bu) + tc[1] * bv;
+  const float ux = (ta[0] * u0 + tb[0] * ub) + tc[0] * ug, uy = (ta[1] * u0 + tb[1] * ub) + tc[1] * ug;
+  const float rx = (ta[0] * r0 + tb[0] * rb) + tc[0] * rg, ry = (ta[1] * r0 + tb[1] * rb) + tc[1] * rg;
+  float dx = std::fabs(tcx - ux), dy = std::fabs(tcy - uy);
+  dx = std::fmax(std::fabs(tcx - rx), dx);
+  dy = std::fmax(std::fabs(tcy - ry), dy);
+  df.uo = (A * u0 + B * ub) + C * ug;
+  df.ro = (A * r0 + B * rb) + C * rg;
+  df.ud = reflect_dir(df.ud, N);
+  return std::fmax(dx, dy);
+}
+
 /* ------------------------------------------------------------------ BRDFs -- */
 enum { T_DIFFUSE = 1, T_SPECULAR = 0, T_SPEC_REFL = 3, T_SPEC_TRANS = 4, T_DISNEY_REFL = 5 };
 
@@ -578,6 +744,36 @@ template <class R> V3<R> mon2lin(V3<R> p) {
   return mk<R>(std::pow(p.x, R(2.2)), std::pow(p.y, R(2.2)), std::pow(p.z, R(2.2)));
 }
 
+/* x^2.2 for a base colour that varies per hit (a texture): x = m 2^e,
+ * m in [sqrt(1/2), sqrt(2)), 11 e = 5 k + j, x^2.2 = m^2.2 2^k 2^(j/5);
+ * m^2.2 a degree-6 polynomial in m - 1 (relative least-squares fit, float
+ * coefficients), 2^(j/5) a float table.  Basic operations only.  x < 2^-50
+ * (or NaN) -> 0, x >= 2^50 -> infinity. */
+inline float pow22(float x) {
+  if (!(x >= 0x1p-50f)) return 0.0f;
+  if (x >= 0x1p50f) return INFINITY;
+  uint32_t bits;
+  std::memcpy(&bits, &x, 4);
+  int32_t e = (int32_t)(bits >> 23) - 127;
+  uint32_t mb = (bits & 0x7fffffu) | 0x3f800000u;
+  if (mb >= 0x3fb504f3u) { mb -= 0x00800000u; e += 1; }
+  float m;
+  std::memcpy(&m, &mb, 4);
+  const float t = m - 1.0f;
+  const float c6[7] = {1.0f, 2.200000286102295f, 1.3200013637542725f, 0.0879826545715332f,
+                       -0.017642781138420105f, 0.006638647988438606f, -0.002734926762059331f};
+  float p = c6[6];
+  for (int i = 5; i >= 0; --i) p = p * t + c6[i];
+  const int32_t q = 11 * e, j = ((q % 5) + 5) % 5, k = (q - j) / 5;
+  const float tab[5] = {1.0f, 1.1486983549970351f, 1.3195079107728942f, 1.515716566510398f, 1.7411011265922482f};
+  const uint32_t sb = (uint32_t)(k + 127) << 23;
+  float scale;
+  std::memcpy(&scale, &sb, 4);
+  return (tab[j] * p) * scale;
+}
+inline double pow22(double x) { return std::pow(x, 2.2); }
+template <class R> V3<R> mon2lin_poly(V3<R> p) { return mk<R>(pow22(p.x), pow22(p.y), pow22(p.z)); }
+
 template <class R> R smith_ggx(R ndv, R ag) {
   R a = ag * ag, b = ndv * ndv;
   return R(1) / (ndv + std::sqrt(a + b - a * b));
@@ -594,12 +790,20 @@ template <class R> struct MatP {
       cc_gloss, opacity;
   V3<R> color, diffuse;
   bool principled;
+  bool textured = false; /* colour scaled by a texel: mon2lin by pow22 */
   explicit MatP(const gi_material &m)
       : metallic(m.metallic), subsurface(m.subsurface), specular(m.specular), roughness(m.roughness),
         spec_tint(m.specular_tint), aniso(m.anisotropic), sheen(m.sheen), sheen_tint(m.sheen_tint),
         clearcoat(m.clearcoat), cc_gloss(m.clearcoat_gloss), opacity(m.opacity),
         color(ld<R>(m.color)), diffuse(ld<R>(m.diffuse_color)), principled(m.principled != 0) {}
   bool glass() const { return principled && opacity < R(1); }
+  V3<R> cdlin() const { return textured ? mon2lin_poly(color) : mon2lin(color); }
+  /* material_t::diffuseColor (src/material.h:155-168): the texel scales the
+   * diffuse colour (Lambert) or the principled base colour */
+  void apply_texel(V3<R> tx) {
+    if (principled) { color = mul(color, tx); textured = true; }
+    else diffuse = mul(diffuse, tx);
+  }
 };
 
 /* evaluatePrincipledBRDF_specular, src/principledMaterial.h:210-252 */
@@ -608,7 +812,7 @@ template <class R> V3<R> eval_spec(const MatP<R> &p, V3<R> V, V3<R> N, V3<R> L) 
   if (ndl < R(0) || ndv < R(0)) return mk<R>(0, 0, 0);
   V3<R> H = normalize(L + V);
   R ndh = dot(N, H), ldh = dot(L, H);
-  V3<R> cd = mon2lin(p.color);
+  V3<R> cd = p.cdlin();
   R lum = R(0.3) * cd.x + R(0.6) * cd.y + R(0.1) * cd.z;
   V3<R> one = mk<R>(1, 1, 1);
   V3<R> tint = lum > R(0) ? divs(cd, lum) : one;
@@ -632,7 +836,7 @@ template <class R> V3<R> eval_diff(const MatP<R> &p, V3<R> V, V3<R> N, V3<R> L) 
   if (ndl < R(0) || ndv < R(0)) return mk<R>(0, 0, 0);
   V3<R> H = normalize(L + V);
   R ldh = dot(L, H);
-  V3<R> cd = mon2lin(p.color);
+  V3<R> cd = p.cdlin();
   R lum = R(0.3) * cd.x + R(0.6) * cd.y + R(0.1) * cd.z;
   V3<R> one = mk<R>(1, 1, 1);
   V3<R> tint = lum > R(0) ? divs(cd, lum) : one;
@@ -774,6 +978,18 @@ void cam_ray(const Cam &c, int32_t x, int32_t y, float jx, float jy, float lx, f
   }
 }
 
+/* Ray differentials of pixel (x, y): pinhole rays from the camera position
+ * through the centres of pixels (x, y+1) ("upper") and (x+1, y) ("right").
+ * The reference takes them from rta's camera_ray_generator_shirley, an
+ * external library (src/raygen.h:122-130), whose exact rays are not in the
+ * reference; the lens generator keeps them pinhole. */
+void cam_diffs(const Cam &c, int32_t x, int32_t y, Diffs &df) {
+  Cam p = c;
+  p.aperture = 0.0f;
+  cam_ray(p, x, y + 1, 0.5f, 0.5f, 0.5f, 0.5f, df.uo, df.ud);
+  cam_ray(p, x + 1, y, 0.5f, 0.5f, 0.5f, 0.5f, df.ro, df.rd);
+}
+
 /* ----------------------------------------------------------------- render -- */
 template <class R> struct Tracer {
   const or_scene &s;
@@ -858,7 +1074,7 @@ template <class R> V3<R> sky_eval(const or_scene &s, V3<R> d) {
  * path.  Returns the path's accumulated colour (path_accum_color). */
 template <class R>
 V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H, const Options &op,
-                   uint64_t n, V3<R> o, V3<R> d, or_stats &st) {
+                   uint64_t n, V3<R> o, V3<R> d, or_stats &st, Diffs *df = nullptr) {
   V3<R> L = mk<R>(0, 0, 0), TP = mk<R>(1, 1, 1);
   R tmax = R(FLT_MAX);
   const int nl = (int)s.lights.size();
@@ -883,6 +1099,17 @@ V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H,
     int32_t mat;
     surface<R>(s, id, bu, bv, P, Ni, mat);
     MatP<R> mp(s.mats[mat]);
+    if (df) {
+      const V3<R> Nn = normalize(Ni);
+      float tcx, tcy;
+      const float diff = update_diffs(s, id, (float)bu, (float)bv, mk<float>((float)Nn.x, (float)Nn.y, (float)Nn.z),
+                                      *df, tcx, tcy);
+      const int32_t tex = s.mats[mat].texture;
+      if (tex >= 0) {
+        const V3<float> tx = tex_sample(s.textures[(size_t)tex], tcx, tcy, diff);
+        mp.apply_texel(mk<R>(tx.x, tx.y, tx.z));
+      }
+    }
 
     /* NEE: pixel_generate_arealight_sample_cpu, src/arealight-sampler-cpu.cpp:25-119.
      * The normal is the unnormalised interpolation, as there. */
@@ -994,7 +1221,10 @@ void render_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_r
         uint64_t n = base + (uint64_t)p * (uint64_t)v.spp + (uint64_t)sidx;
         V3<float> of, df;
         cam_ray(cam, x, y, H.sample(n, DIM_JX), H.sample(n, DIM_JY), H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
-        V3<R> Ls = trace_sample<R>(s, tr, H, op, n, mk<R>(of.x, of.y, of.z), mk<R>(df.x, df.y, df.z), st);
+        Diffs dfs;
+        if (s.textured) cam_diffs(cam, x, y, dfs);
+        V3<R> Ls = trace_sample<R>(s, tr, H, op, n, mk<R>(of.x, of.y, of.z), mk<R>(df.x, df.y, df.z), st,
+                                   s.textured ? &dfs : nullptr);
         ++st.samples;
         /* combine_color_samples, src/util-kernels.cu:17-25 */
         acc = divs(R(sidx) * acc + Ls, R(sidx + 1));
@@ -1036,7 +1266,7 @@ template <class R> struct MatD {
   V3<R> cd, dcd, tint;
   R lum, w[3];
   explicit MatD(const MatP<R> &p) {
-    cd = mon2lin(p.color);
+    cd = p.cdlin();
     dcd = mk<R>(p.color.x > R(0) ? R(2.2) * std::pow(p.color.x, R(1.2)) : R(0),
                 p.color.y > R(0) ? R(2.2) * std::pow(p.color.y, R(1.2)) : R(0),
                 p.color.z > R(0) ? R(2.2) * std::pow(p.color.z, R(1.2)) : R(0));
@@ -1353,6 +1583,7 @@ or_scene *or_scene_create(const gi_tri *tris, int64_t n, const gi_material *mats
   s->tris.assign(tris, tris + n);
   if (mats) s->mats.assign(mats, mats + nm);
   if (lights) s->lights.assign(lights, lights + nl);
+  for (const gi_material &m : s->mats) s->textured = s->textured || m.texture >= 0;
   for (auto &l : s->lights) s->power_total += l.power;
   for (int k = 0; k < 3; ++k) { s->lo[k] = FLT_MAX; s->hi[k] = -FLT_MAX; }
   for (auto &t : s->tris)
@@ -1374,6 +1605,28 @@ or_scene *or_scene_create(const gi_tri *tris, int64_t n, const gi_material *mats
 }
 
 void or_scene_free(or_scene *s) { delete s; }
+
+int32_t or_scene_add_texture(or_scene *s, const uint8_t *rgba, int32_t w, int32_t h) {
+  if (!s || !rgba || w <= 0 || h <= 0) return -1;
+  or_scene::Texture t;
+  t.w = w;
+  t.h = h;
+  build_chain(t, rgba);
+  s->textures.push_back(std::move(t));
+  return (int32_t)s->textures.size() - 1;
+}
+int32_t or_texture_levels(const or_scene *s, int32_t id) { return s->textures[(size_t)id].levels; }
+int64_t or_texture_chain(const or_scene *s, int32_t id, uint8_t *out) {
+  const std::vector<uint8_t> &c = s->textures[(size_t)id].chain;
+  if (out) std::memcpy(out, c.data(), c.size());
+  return (int64_t)c.size();
+}
+void or_tex_sample(const or_scene *s, int32_t id, float u, float v, float diff, float *rgb) {
+  const V3<float> c = tex_sample(s->textures[(size_t)id], u, v, diff);
+  rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+int32_t or_tex_level(float diff, int32_t w, int32_t h, int32_t levels) { return tex_level(diff, w, h, levels); }
+float or_pow22(float x) { return pow22(x); }
 
 int32_t or_scene_add_mesh(or_scene *s, const gi_tri *tris, int64_t n, int32_t max_leaf) {
   if (!s || !tris || n <= 0) return -1;
@@ -1654,7 +1907,7 @@ float or_cdf_sample(const float *cdf, const float *pdf, int32_t size, float u, f
 
 int or_adjoint(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
                const float *dL_dimage, double *d_mats, double *d_emission, int32_t threads, int32_t f64) {
-  if (opts->rr_start > 0 || opts->max_depth > 32 || s->sky_light >= 0) return 1;
+  if (opts->rr_start > 0 || opts->max_depth > 32 || s->sky_light >= 0 || s->textured) return 1;
   if (f64) adjoint_impl<double>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, threads);
   else adjoint_impl<float>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, threads);
   return 0;

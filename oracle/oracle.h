@@ -35,6 +35,14 @@ void or_scene_free(or_scene *s);
 /* Instancing: a mesh (object-space triangles, own BVH) and the instances
  * placing meshes; the scene's triangles stay the world geometry. */
 int32_t or_scene_add_mesh(or_scene *s, const gi_tri *tris, int64_t n, int32_t max_leaf);
+/* Textures: RGBA8 w x h (row 0 at t = 1), its mip chain built here;
+ * materials name them by gi_material.texture (diffuse texture). */
+int32_t or_scene_add_texture(or_scene *s, const uint8_t *rgba, int32_t w, int32_t h);
+int32_t or_texture_levels(const or_scene *s, int32_t id);
+int64_t or_texture_chain(const or_scene *s, int32_t id, uint8_t *out);
+void or_tex_sample(const or_scene *s, int32_t id, float u, float v, float diff, float *rgb);
+int32_t or_tex_level(float diff, int32_t w, int32_t h, int32_t levels);
+float or_pow22(float x);
 int or_scene_set_instances(or_scene *s, const gi_instance *inst, int32_t n);
 int64_t or_scene_bvh_nodes(const or_scene *s);
 void or_scene_export_bvh(const or_scene *s, float *nodes, uint32_t *indices);

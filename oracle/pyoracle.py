@@ -44,6 +44,12 @@ def load():
             "or_scene_create": (P, [P, I64, P, I32, P, I32, P, I64, P, I64, I32]),
             "or_scene_free": (None, [P]),
             "or_scene_add_mesh": (I32, [P, P, I64, I32]),
+            "or_scene_add_texture": (I32, [P, P, I32, I32]),
+            "or_texture_levels": (I32, [P, I32]),
+            "or_texture_chain": (I64, [P, I32, P]),
+            "or_tex_sample": (None, [P, I32, C.c_float, C.c_float, C.c_float, P]),
+            "or_tex_level": (I32, [C.c_float, I32, I32, I32]),
+            "or_pow22": (C.c_float, [C.c_float]),
             "or_scene_set_instances": (C.c_int, [P, P, I32]),
             "or_scene_bvh_nodes": (I64, [P]),
             "or_scene_export_bvh": (None, [P, P, P]),
@@ -92,7 +98,8 @@ def _p(a):
 class OracleScene:
     """The oracle's scene: its own object-median BVH unless bvh2 is given."""
 
-    def __init__(self, tris, mats=None, lights=None, bvh2=None, max_leaf=4, sky=None, meshes=None, instances=None):
+    def __init__(self, tris, mats=None, lights=None, bvh2=None, max_leaf=4, sky=None, meshes=None, instances=None,
+                 textures=None):
         self.lib = load()
         self._keep = [np.ascontiguousarray(tris)]
         mats = None if mats is None else np.ascontiguousarray(mats)
@@ -113,6 +120,10 @@ class OracleScene:
             self._keep.append(m)
             if self.lib.or_scene_set_sky(self.h, _p(m), m.shape[1], m.shape[0]) != 0:
                 raise ValueError("or_scene_set_sky: the scene needs exactly one sky light")
+        for t in textures or []:
+            t = np.ascontiguousarray(t, np.uint8)
+            if self.lib.or_scene_add_texture(self.h, _p(t), t.shape[1], t.shape[0]) < 0:
+                raise ValueError("or_scene_add_texture")
         for m in meshes or []:
             m = np.ascontiguousarray(m)
             self._keep.append(m)
@@ -127,6 +138,18 @@ class OracleScene:
         if getattr(self, "h", None):
             self.lib.or_scene_free(self.h)
             self.h = None
+
+    def texture_chain(self, tex: int = 0):
+        """(levels, flat uint8 RGBA chain: level 0 then each halved level)."""
+        n = self.lib.or_texture_chain(self.h, tex, None)
+        out = np.zeros(n, np.uint8)
+        self.lib.or_texture_chain(self.h, tex, _p(out))
+        return self.lib.or_texture_levels(self.h, tex), out
+
+    def tex_sample(self, tex: int, u: float, v: float, diff: float):
+        out = np.zeros(3, np.float32)
+        self.lib.or_tex_sample(self.h, tex, u, v, diff, _p(out))
+        return out
 
     def diagonal(self) -> float:
         return float(self.lib.or_scene_diagonal(self.h))
@@ -302,3 +325,11 @@ def brdf_eval(mat, typ, wo, wi, n, f64=False):
     fn = load().or_brdf_eval_f64 if f64 else load().or_brdf_eval
     fn(_p(m), typ, _p(wo), _p(wi), _p(n), _p(f))
     return f
+
+
+def pow22(x: float) -> float:
+    return float(load().or_pow22(x))
+
+
+def tex_level(diff: float, w: int, h: int, levels: int) -> int:
+    return int(load().or_tex_level(diff, w, h, levels))

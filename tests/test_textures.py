@@ -1,0 +1,161 @@
+"""Textures in the oracle, pinned without the product: pow 2.2 against
+numpy, the box-filtered mip chain against a numpy restatement and closed
+forms (constant texture, odd sizes), the level choice at powers of two,
+sample_bilin_lod at level 0 against plain bilinear interpolation with
+repeat addressing, and renders: a white texture leaves a Lambert image
+bit-identical, a grey texture scales it, the FP32 render against FP64,
+and a one-texel checkerboard that turns into its mean grey with distance
+(the ray-differential level choice)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from globillum_amd import scenes
+
+
+def test_pow22_against_numpy(oracle):
+    xs = np.concatenate([np.float32([0.0, 1.0, 0.5, 2.0, 1e-12, 7.0]),
+                         np.random.default_rng(0).uniform(0, 1.5, 20000).astype(np.float32),
+                         np.geomspace(1e-14, 1e6, 5000).astype(np.float32)])
+    got = np.array([oracle.pow22(float(x)) for x in xs], np.float64)
+    want = xs.astype(np.float64) ** 2.2
+    ok = xs >= 2.0 ** -50
+    assert np.all(np.abs(got[ok] / want[ok] - 1) < 3e-7)
+    assert np.all(got[~ok] == 0.0)
+    assert oracle.pow22(1.0) == 1.0 and oracle.pow22(-1.0) == 0.0 and oracle.pow22(float("nan")) == 0.0
+
+
+def _np_chain(img):
+    """numpy restatement of the mip chain: truncating mean of the existing
+    texels of each 2x2 block."""
+    h, w = img.shape[:2]
+    levels = int(np.floor(np.log2(min(w, h))))
+    out = [img.astype(np.int64)]
+    cur = out[0]
+    for _ in range(levels):
+        H, W = cur.shape[:2]
+        dh, dw = (H + 1) // 2, (W + 1) // 2
+        pad = np.zeros((2 * dh, 2 * dw, 4), np.int64)
+        cnt = np.zeros((2 * dh, 2 * dw, 1), np.int64)
+        pad[:H, :W] = cur
+        cnt[:H, :W] = 1
+        s = pad.reshape(dh, 2, dw, 2, 4).sum((1, 3))
+        c = cnt.reshape(dh, 2, dw, 2, 1).sum((1, 3))
+        cur = s // c
+        out.append(cur)
+    return levels, np.concatenate([o.reshape(-1) for o in out]).astype(np.uint8)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 2), (4, 4), (5, 3), (3, 7), (16, 9), (64, 64), (33, 65)])
+def test_mip_chain(oracle, shape):
+    h, w = shape
+    img = np.random.default_rng(h * 100 + w).integers(0, 256, (h, w, 4), dtype=np.uint8)
+    osc = oracle.OracleScene(scenes.cornell_boxes().tris, textures=[img])
+    lv, chain = osc.texture_chain(0)
+    want_lv, want = _np_chain(img)
+    assert lv == want_lv and np.array_equal(chain, want)
+    # closed form: a constant texture stays constant on every level
+    const = np.full((h, w, 4), 77, np.uint8)
+    osc = oracle.OracleScene(scenes.cornell_boxes().tris, textures=[const])
+    assert (osc.texture_chain(0)[1] == 77).all()
+
+
+def test_level_choice(oracle):
+    # levels counted as i < levels with 2^i < diff * max(w, h)
+    for x, want in ((0.0, 0), (0.5, 0), (1.0, 0), (1.0001, 1), (2.0, 1), (2.001, 2), (4.0, 2), (1e9, 6)):
+        assert oracle.tex_level(x / 64.0, 64, 32, 6) == want, x
+    assert oracle.tex_level(float("nan"), 64, 64, 6) == 6
+    assert oracle.tex_level(float("inf"), 64, 64, 6) == 6
+
+
+def test_bilinear_at_level_0(oracle):
+    rng = np.random.default_rng(1)
+    h, w = 13, 21
+    img = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    osc = oracle.OracleScene(scenes.cornell_boxes().tris, textures=[img])
+    f = img[..., :3].astype(np.float64) / 255.0
+    for s, t in rng.uniform(-2.5, 2.5, (400, 2)):
+        x = (s % 1.0) * w
+        y = (1.0 - t % 1.0) * h
+        # texel centres at i + 0.5, repeat addressing
+        fx, fy = x - 0.5, y - 0.5
+        i0, j0 = int(np.floor(fx)), int(np.floor(fy))
+        ax, ay = fx - i0, fy - j0
+        c = lambda i, j: f[j % h, i % w]
+        want = ((1 - ay) * ((1 - ax) * c(i0, j0) + ax * c(i0 + 1, j0))
+                + ay * ((1 - ax) * c(i0, j0 + 1) + ax * c(i0 + 1, j0 + 1)))
+        got = osc.tex_sample(0, float(s), float(t), 0.0)
+        assert np.allclose(got, want, atol=2e-6), (s, t)
+    # texel centres give the texel
+    got = osc.tex_sample(0, (3 + 0.5) / w, 1.0 - (5 + 0.5) / h, 0.0)
+    assert np.array_equal(got, np.float32(img[5, 3, :3]) / np.float32(255.0))
+
+
+def _lambert_plane(tex=None, albedo=0.8, w=48, h=32, spp=2):
+    ground = np.array([[(-10, 0, -10), (-10, 0, 30), (10, 0, 30), (10, 0, -10)]], np.float32)
+    tris = scenes._quads_to_tris(ground, np.array([(0, 1, 0)], np.float32), np.zeros(1, np.int32))
+    tris["ta"], tris["tb"], tris["tc"] = tris["a"][:, [0, 2]] / 4, tris["b"][:, [0, 2]] / 4, tris["c"][:, [0, 2]] / 4
+    mats = scenes.lambert((albedo, albedo, albedo))
+    if tex is not None:
+        mats["texture"] = 0
+    light = scenes.make_light((0, 5, 5), (0, -1, 0), (20, 20, 20), 2.0, 2.0)
+    view = scenes.make_view((0, 2, -8), (0, 0, 10), (0, 1, 0), 60.0, w, h, spp)
+    return scenes.Scene(tris, mats, light, view, scenes.make_opts(3), textures=None if tex is None else [tex])
+
+
+def _render(oracle, sc, **kw):
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=sc.textures, meshes=sc.meshes,
+                             instances=sc.instances)
+    return osc.render(sc.views, sc.opts, threads=4, **kw)
+
+
+def test_white_texture_is_identity_and_grey_scales(oracle):
+    plain, _ = _render(oracle, _lambert_plane())
+    white, _ = _render(oracle, _lambert_plane(np.full((8, 8, 4), 255, np.uint8)))
+    assert plain.sum() > 0 and np.array_equal(plain, white)
+    # grey 51/255 = 0.2 at every level: the Lambert albedo 0.8 becomes 0.16
+    grey, _ = _render(oracle, _lambert_plane(np.full((8, 8, 4), 51, np.uint8)))
+    ref, _ = _render(oracle, _lambert_plane(albedo=0.8 * 0.2))
+    assert np.allclose(grey, ref, rtol=1e-5, atol=1e-7)
+
+
+def test_checkerboard_greys_with_distance(oracle):
+    """One-texel checkerboard of 0 and 255: level 0 alternates, every coarser
+    level is (255 + 255 + 0 + 0) // 4 = 127.  Far rows of the plane (a
+    footprint of several texels) render exactly as a constant 127 texture
+    does; near rows keep the pattern (the ray-differential level choice)."""
+    def render(tex):
+        sc = _lambert_plane(tex, albedo=1.0, w=160, h=120, spp=1)
+        sc.opts["max_depth"] = 1
+        sc.lights = scenes.make_light((0, 30, 10), (0, -1, 0), (5e5, 5e5, 5e5), 0.02, 0.02)
+        return _render(oracle, sc)[0].reshape(120, 160, 3)
+
+    checker = render(scenes.checker_texture(64, 64, cell=1, c0=(255, 255, 255), c1=(0, 0, 0)))
+    grey = render(np.full((64, 64, 4), 127, np.uint8))
+    contrast = lambda img, r: float(np.abs(np.diff(img[r, 40:120, 0])).mean() / img[r, 40:120, 0].mean())
+    # image row 0 is the bottom of the view: the nearest ground
+    assert min(contrast(checker, r) for r in range(0, 12)) > 0.1 > 50 * contrast(grey, 6)
+    assert np.array_equal(checker[30:60], grey[30:60]) and grey[30:60].min() > 0
+
+
+def test_textured_render_fp32_vs_fp64(oracle):
+    sc = scenes.textured_scene(w=64, h=48, spp=2, depth=3)
+    img, st = _render(oracle, sc)
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=sc.textures, meshes=sc.meshes,
+                             instances=sc.instances)
+    img64 = osc.render_f64(sc.views, sc.opts, threads=4)
+    assert st["rays_shadow"] > 0 and np.isfinite(img).all()
+    assert np.sqrt(((img - img64) ** 2).mean()) / np.sqrt((img64 ** 2).mean()) < 2e-3
+    # the textures change the image
+    plain = dataclasses.replace(sc, mats=sc.mats.copy())
+    plain.mats["texture"] = -1
+    img0, _ = _render(oracle, plain)
+    assert np.abs(img - img0).mean() > 0.01 * np.abs(img0).mean()
+
+
+def test_adjoint_refuses_textures(oracle):
+    sc = _lambert_plane(np.full((8, 8, 4), 200, np.uint8))
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=sc.textures)
+    with pytest.raises(Exception):
+        oracle.adjoint(osc, sc.views, sc.opts, np.ones((32 * 48, 3), np.float32))
