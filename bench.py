@@ -17,6 +17,7 @@ of the same workload.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -197,9 +198,10 @@ def bvh_leg(steps: int):
 def instancing_leg(steps: int):
     """The large combined configuration's forward pass (2048x1024-cell
     heightfield world + 2000 instanced 160-triangle spheres, 256 principled
-    materials, 64 rect lights, 4 views of 3840x2160, 16 spp, depth 6; no
-    albedo texture) rendered with the two-level BVH and with the same
-    spheres flattened into the world BVH (4.51M triangles)."""
+    materials with a 1024x1024 albedo texture, 64 rect lights, 4 views of
+    3840x2160, 16 spp, depth 6) rendered with the two-level BVH and with the
+    same spheres flattened into the world BVH (4.51M triangles), and the
+    instanced scene without its texture."""
     import numpy as np
 
     from globillum_amd import scenes
@@ -208,10 +210,12 @@ def instancing_leg(steps: int):
 
     sc = scenes.config_scene(5)
     out = {"workload": "large_combined: 2048x1024-cell heightfield (4194304 triangles) + 2000 instances of a "
-                       "160-triangle sphere, 256 principled materials, 64 rect lights, 4 views 3840x2160, 16 spp, "
-                       "depth 6, no texture"}
+                       "160-triangle sphere, 256 principled materials with a 1024x1024 albedo texture, 64 rect "
+                       "lights, 4 views 3840x2160, 16 spp, depth 6"}
     means = {}
-    for name, s in (("instanced", sc), ("flattened", scenes.flatten(sc))):
+    untextured = dataclasses.replace(sc, mats=sc.mats.copy(), textures=None)
+    untextured.mats["texture"] = -1
+    for name, s in (("instanced", sc), ("flattened", scenes.flatten(sc)), ("instanced_untextured", untextured)):
         with Context(0) as ctx:
             t0 = time.perf_counter()
             ctx.upload(s)

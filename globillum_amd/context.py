@@ -110,6 +110,14 @@ class Context:
         self._ck(self.lib.gi_download_texture(self.h, tex, ptr(out)))
         return lv.value, out
 
+    def texture_grad(self, tex: int = 0) -> np.ndarray:
+        """(h, w, 3) gradient of the last adjoint w.r.t. the texture's
+        level-0 texel values (byte / 255)."""
+        t = self.scene.textures[tex]
+        out = np.zeros((t.shape[0], t.shape[1], 3), np.float32)
+        self._ck(self.lib.gi_texture_grad(self.h, tex, ptr(out)))
+        return out
+
     def upload_instances(self, meshes, instances):
         """Meshes (object-space TRI arrays) and INSTANCE records placing
         them over the world geometry (two-level BVH).  Call after the world

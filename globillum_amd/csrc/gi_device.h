@@ -315,7 +315,14 @@ __host__ __device__ __forceinline__ int tex_level(float diff, int32_t w, int32_t
 // (the reference leaves a neighbour index of exactly W or H, and y = H for
 // t = 0, unwrapped, which reads the next row or the next level); texture
 // coordinates that are NaN, infinite or at least 2^23 in magnitude wrap to 0.
-__device__ __forceinline__ f3 tex_sample_lod(const DevTex &T, float s, float t, float diff) {
+// The four texels a lookup blends (indices into the chain) and their
+// weights: d value / d texel.
+struct TexFetch {
+  uint32_t idx[4];
+  float w[4];
+};
+
+__device__ __forceinline__ f3 tex_sample_lod(const DevTex &T, float s, float t, float diff, TexFetch &tf) {
   int32_t W = T.w, H = T.h;
   const int lv = tex_level(diff, T.w, T.h, T.levels);
   uint32_t off = 0;
@@ -344,6 +351,14 @@ __device__ __forceinline__ f3 tex_sample_lod(const DevTex &T, float s, float t, 
   oy = (oy % H + H) % H;
   const uchar4 *L = T.texels + off;
   const uchar4 a00 = L[ny * W + nx], a01 = L[ny * W + ox], a10 = L[oy * W + nx], a11 = L[oy * W + ox];
+  tf.idx[0] = off + (uint32_t)(ny * W + nx);
+  tf.idx[1] = off + (uint32_t)(ny * W + ox);
+  tf.idx[2] = off + (uint32_t)(oy * W + nx);
+  tf.idx[3] = off + (uint32_t)(oy * W + ox);
+  tf.w[0] = wy * wx;
+  tf.w[1] = wy * (1.0f - wx);
+  tf.w[2] = (1.0f - wy) * wx;
+  tf.w[3] = (1.0f - wy) * (1.0f - wx);
   const float k = 255.0f;
   const f3 c00 = f3{(float)a00.x / k, (float)a00.y / k, (float)a00.z / k};
   const f3 c01 = f3{(float)a01.x / k, (float)a01.y / k, (float)a01.z / k};

@@ -553,6 +553,25 @@ __global__ __launch_bounds__(256) void k_mipmap(const uchar4 *__restrict__ src, 
   }
 }
 
+// Texel gradients of mip level l+1 passed down to level l: the chain is the
+// box filter's mean of the (up to) 2x2 texels below (the truncation of the
+// integer mean is not differentiated), so each child gets its parent's
+// gradient over the parent's child count.  Run from the coarsest level down.
+__global__ __launch_bounds__(256) void k_tex_grad_down(float *__restrict__ g, uint32_t child_off, int32_t cw,
+                                                       int32_t ch, uint32_t parent_off, int32_t pw) {
+  const uint32_t total = (uint32_t)cw * (uint32_t)ch;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int32_t x = (int32_t)(i % (uint32_t)cw), y = (int32_t)(i / (uint32_t)cw);
+    const int32_t px = x >> 1, py = y >> 1;
+    const float cnt = (float)((2 * px + 1 < cw ? 2 : 1) * (2 * py + 1 < ch ? 2 : 1));
+    const float *gp = g + 3 * ((size_t)parent_off + (size_t)py * pw + px);
+    float *gc = g + 3 * ((size_t)child_off + i);
+    gc[0] = gc[0] + gp[0] / cnt;
+    gc[1] = gc[1] + gp[1] / cnt;
+    gc[2] = gc[2] + gp[2] / cnt;
+  }
+}
+
 // Primary rays of every path slot of a pass (new_path: generate_rays and
 // the throughput / colour reset, src/gpu-pt.h:110-118).
 __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pass, const DevView *__restrict__ views,
@@ -652,6 +671,9 @@ struct VertexEval {
   uint32_t mat;
   bool glass;
   DevMat m;  // the material at this vertex (its texture applied)
+  bool textured;  // the material has a texture: tx was applied, tf says where from
+  f3 tx;
+  TexFetch tf;
 };
 
 // reflect(d, N) = d - 2 (d.N) N
@@ -736,8 +758,10 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
       float dx = fabsf(tcx - ux), dy = fabsf(tcy - uy);
       dx = fmaxf(fabsf(tcx - rx), dx);
       dy = fmaxf(fabsf(tcy - ry), dy);
-      apply_texel(ve.m, tex_sample_lod(P.tex[m.texture], tcx, tcy, fmaxf(dx, dy)));
+      ve.tx = tex_sample_lod(P.tex[m.texture], tcx, tcy, fmaxf(dx, dy), ve.tf);
+      apply_texel(ve.m, ve.tx);
     }
+    ve.textured = m.texture >= 0;
     const f3 uo = add(add(scl(A, u0), scl(B, ub)), scl(C, ug));
     const f3 ro = add(add(scl(A, r0), scl(B, rb)), scl(C, rg));
     const f3 ud = reflect_dir(xyz(ud4), N0);
@@ -1031,6 +1055,8 @@ struct AdjParams {
   int32_t nmats, nlights;
   const PassDesc *pass;
   const DevView *views;
+  float *g_tex;               // texel gradients, RGB per texel of every chain
+  const uint32_t *tex_goff;   // first texel of texture i in g_tex (in texels)
 };
 
 // Adjoint replay of path vertex b: the same hit, NEE point and BRDF sample
@@ -1039,6 +1065,11 @@ struct AdjParams {
 // estimator (NEE, throughput chain up to the next clamp, score of the lobe
 // choice and of the GGX half-vector, emission).  Gradients gather in LDS per
 // block and go to global memory with one atomic per parameter per block.
+// kTex: the ray differentials are carried (as in the forward pass) and a
+// textured vertex's base colour b = colour * texel: the colour gradient is
+// dL/db * texel, and dL/db * colour goes to the four blended texels with
+// their bilinear weights (global atomics into the chain's gradient).
+template <bool kTex>
 __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t *__restrict__ count_ptr,
                                                        ShadeParams P, AdjParams A, const float4 *__restrict__ ray_o,
                                                        const float4 *__restrict__ ray_d,
@@ -1064,9 +1095,19 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
       const f3 org_dir = normalize(xyz(rd));
       if (id != GI_INVALID_REF) {
         VertexEval ve;
-        eval_vertex<false>(P, n, dim, slot, org_dir, h, ve);
-        const DevMat &m = vertex_mat<false>(P, ve);
-        const DevMatGrad &mg = A.mg[ve.mat];
+        eval_vertex<kTex>(P, n, dim, slot, org_dir, h, ve);
+        const DevMat &m = vertex_mat<kTex>(P, ve);
+        DevMatGrad mg = A.mg[ve.mat];
+        if (kTex && ve.textured && m.principled) {
+          // the derivative quantities of the textured base colour b:
+          // d Cdlin / d b = 2.2 b^1.2 = 2.2 Cdlin / b
+          const float lum = 0.3f * m.cdlin[0] + 0.6f * m.cdlin[1] + 0.1f * m.cdlin[2];
+          mg.lum = lum;
+          for (int c = 0; c < 3; ++c) {
+            mg.tint[c] = lum > 0.f ? m.cdlin[c] / lum : 1.f;
+            mg.dcd[c] = m.color[c] > 0.f ? (2.2f * m.cdlin[c]) / m.color[c] : 0.f;
+          }
+        }
         const f3 inv = neg(org_dir);
         // adjoint weight of the path's pixel: dL/dI / spp
         const PassChunk &ck = A.pass->chunk[find_chunk(A.pass, slot)];
@@ -1130,6 +1171,18 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
             if (vis && !ve.glass) gq += dl[c] * dfn[q][c] * Tc[c] * Pc[c];
             if (!last && Fc[c] != 0.0f && Rc[c] != 0.0f)
               gq += dl[c] * Rc[c] * (dfb[q][c] / Fc[c] - (q == 3 ? dlogw : 0.0f));
+          }
+          if (kTex && ve.textured && q < 3) {
+            // b_q = colour_q * tx_q
+            const DevMat &m0 = P.mats[ve.mat];
+            const float col = m.principled ? m0.color[q] : m0.diffuse[q];
+            const float gt = gq * col;
+            if (gt != 0.0f) {
+              float *gtex = A.g_tex + 3 * (size_t)A.tex_goff[m.texture];
+#pragma unroll
+              for (int k = 0; k < 4; ++k) atomicAdd(&gtex[3 * (size_t)ve.tf.idx[k] + q], gt * ve.tf.w[k]);
+            }
+            gq = gq * f3c(ve.tx, q);
           }
           if (gq != 0.0f) atomicAdd(&sg[5 * ve.mat + q], gq);
         }
@@ -1274,6 +1327,8 @@ struct gi_ctx {
   std::vector<TexHost> texs;
   DBuf d_tex;
   bool textured = false;  // some material names a texture
+  DBuf d_gtex, d_gtex_off;                    // adjoint: texel gradients of the chains
+  std::vector<std::vector<float>> tex_grad;   // level-0 texel gradients of the last adjoint
   int64_t world_nnodes = 0;
   int32_t world_depth = 0, n_inst = 0;
   // range of the material indices shading can reach: world triangles, and
@@ -1893,14 +1948,15 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
       hipLaunchKernelGGL(k_raygen, dim3(std::max(1, rg)), dim3(256), 0, ctx->stream, dp, dviews, SP.H,
                          ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->q_tp[0].as<float4>(),
                          ctx->path_l.as<float4>(), phase == 0 ? rec : (float4 *)nullptr, stride, depth,
-                         (float4 *)nullptr, 0u);
+                         SP.rdiff, SP.rdiff_stride);
       for (int32_t b = 0; b < depth; ++b) {
         const int cur = b & 1, nxt = cur ^ 1;
         rc = launch_closest(ctx, false, cnt + b, 0, pd.total_slots, ctx->q_o[cur].as<float4>(),
                             ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
         if (rc) return rc;
         if (phase == 0) {
-          hipLaunchKernelGGL(k_shade<false>, dim3(sg), dim3(sblock), 0, ctx->stream, b, cnt + b, SP,
+          hipLaunchKernelGGL(SP.rdiff ? k_shade<true> : k_shade<false>, dim3(sg), dim3(sblock), 0, ctx->stream, b,
+                             cnt + b, SP,
                              ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
                              ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(), ctx->q_o[nxt].as<float4>(),
                              ctx->q_d[nxt].as<float4>(), ctx->q_tp[nxt].as<float4>(), cnt + b + 1,
@@ -1912,7 +1968,8 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
                           stride, b);
           if (rc) return rc;
         } else {
-          hipLaunchKernelGGL(k_shade_adjoint, dim3(sg), dim3(sblock), lds_adj, ctx->stream, b, cnt + b, SP, A,
+          hipLaunchKernelGGL(SP.rdiff ? k_shade_adjoint<true> : k_shade_adjoint<false>, dim3(sg), dim3(sblock),
+                             lds_adj, ctx->stream, b, cnt + b, SP, A,
                              ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
                              ctx->q_hit.as<float4>(), ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(),
                              ctx->q_tp[nxt].as<float4>(), cnt + b + 1);
@@ -2316,6 +2373,15 @@ int gi_texture_info(gi_ctx *ctx, int32_t tex_id, int32_t *levels, int64_t *texel
   if (!ctx || tex_id < 0 || tex_id >= (int32_t)ctx->texs.size()) return fail(ctx, GI_ERR_INVALID_ARG, "texture id");
   if (levels) *levels = ctx->texs[(size_t)tex_id].levels;
   if (texels) *texels = (int64_t)ctx->texs[(size_t)tex_id].texels;
+  return GI_OK;
+}
+
+int gi_texture_grad(gi_ctx *ctx, int32_t tex_id, float *d_rgb) {
+  if (!ctx || !d_rgb || tex_id < 0 || tex_id >= (int32_t)ctx->texs.size())
+    return fail(ctx, GI_ERR_INVALID_ARG, "texture id");
+  if ((size_t)tex_id >= ctx->tex_grad.size() || ctx->tex_grad[(size_t)tex_id].empty())
+    return fail(ctx, GI_ERR_STATE, "no texture gradient: gi_adjoint of a textured scene first");
+  std::memcpy(d_rgb, ctx->tex_grad[(size_t)tex_id].data(), ctx->tex_grad[(size_t)tex_id].size() * sizeof(float));
   return GI_OK;
 }
 
@@ -2804,7 +2870,6 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
   if (ctx->views.empty()) return fail(ctx, GI_ERR_STATE, "gi_render first");
   if (ctx->opts.rr_start > 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support Russian roulette");
   if (ctx->sky_light >= 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support sky lights");
-  if (ctx->textured) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support textured materials");
   const int32_t nm = (int32_t)ctx->mats.size(), nl = (int32_t)ctx->lights.size();
   if ((size_t)(5 * nm + 3 * nl) * sizeof(float) > 48 * 1024)
     return fail(ctx, GI_ERR_LIMIT, "adjoint: 5*materials + 3*lights must fit 48 KiB of LDS");
@@ -2836,8 +2901,47 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
   A.g_em = ctx->d_gem.as<float>();
   A.nmats = nm;
   A.nlights = nl;
+  // texel gradients: RGB per texel of every texture's chain
+  std::vector<uint32_t> goff(ctx->texs.size() + 1, 0);
+  for (size_t t = 0; t < ctx->texs.size(); ++t) goff[t + 1] = goff[t] + (uint32_t)ctx->texs[t].texels;
+  const size_t gtex = ctx->textured ? (size_t)goff.back() : 0;
+  if (gtex) {
+    GI_HIP(ctx, ctx->d_gtex.ensure(gtex * 3 * sizeof(float)));
+    GI_HIP(ctx, hipMemsetAsync(ctx->d_gtex.p, 0, gtex * 3 * sizeof(float), ctx->stream));
+    GI_HIP(ctx, ctx->d_gtex_off.ensure(goff.size() * sizeof(uint32_t)));
+    GI_HIP(ctx, copy_sync(ctx, ctx->d_gtex_off.p, goff.data(), goff.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+    A.g_tex = ctx->d_gtex.as<float>();
+    A.tex_goff = ctx->d_gtex_off.as<uint32_t>();
+  }
   rc = run_adjoint_passes(ctx, plan, ctx->opts.max_depth, A);
   if (rc) return rc;
+  ctx->tex_grad.assign(ctx->texs.size(), std::vector<float>());
+  for (size_t t = 0; gtex && t < ctx->texs.size(); ++t) {
+    const gi_ctx::TexHost &T = ctx->texs[t];
+    std::vector<uint32_t> loff(T.levels + 1);
+    std::vector<int32_t> lw(T.levels + 1), lh(T.levels + 1);
+    uint32_t o = 0;
+    int32_t W = T.w, H = T.h;
+    for (int32_t l = 0; l <= T.levels; ++l) {
+      loff[l] = o;
+      lw[l] = W;
+      lh[l] = H;
+      o += (uint32_t)W * (uint32_t)H;
+      W = (W + 1) / 2;
+      H = (H + 1) / 2;
+    }
+    float *g = ctx->d_gtex.as<float>() + 3 * (size_t)goff[t];
+    for (int32_t l = T.levels - 1; l >= 0; --l) {
+      const uint32_t n = (uint32_t)lw[l] * (uint32_t)lh[l];
+      hipLaunchKernelGGL(k_tex_grad_down, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0,
+                         ctx->stream, g, loff[l], lw[l], lh[l], loff[l + 1], lw[l + 1]);
+      GI_HIP(ctx, hipGetLastError());
+    }
+    ctx->tex_grad[t].resize((size_t)T.w * T.h * 3);
+    GI_HIP(ctx, copy_sync(ctx, ctx->tex_grad[t].data(), g, ctx->tex_grad[t].size() * sizeof(float),
+                          hipMemcpyDeviceToHost));
+  }
   std::vector<float> gm((size_t)5 * nm), ge((size_t)3 * nl);
   GI_HIP(ctx, copy_sync(ctx, gm.data(), ctx->d_gmats.p, gm.size() * sizeof(float), hipMemcpyDeviceToHost));
   GI_HIP(ctx, copy_sync(ctx, ge.data(), ctx->d_gem.p, ge.size() * sizeof(float), hipMemcpyDeviceToHost));

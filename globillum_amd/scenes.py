@@ -398,17 +398,17 @@ def instanced_terrain(seed: int = 5, cells=(64, 64), n_instances: int = 200, n_m
 
 def large_combined(seed: int = 6, cells=(2048, 1024), n_spheres: int = 2000, n_mats: int = 256,
                    n_lights: int = 64, w: int = 3840, h: int = 2160, spp: int = 16, depth: int = 6,
-                   n_views: int = 4, size=(200.0, 100.0), radius=(0.2, 1.5)) -> Scene:
-    """The large combined configuration's geometry and lighting: a
-    2048x1024-cell fBm heightfield (4,194,304 triangles) as the world, 2000
-    instances of one 160-triangle unit UV sphere (uniform scale U[radius],
-    resting on the terrain, each with its own principled material), 256
-    principled materials, 64 rect lights on an 8x8 grid, n_views views of
-    3840x2160 at 16 spp, depth 6."""
+                   n_views: int = 4, size=(200.0, 100.0), radius=(0.2, 1.5), texture=(1024, 1024)) -> Scene:
+    """The large combined configuration: a 2048x1024-cell fBm heightfield
+    (4,194,304 triangles) as the world, 2000 instances of one 160-triangle
+    unit UV sphere (uniform scale U[radius], resting on the terrain, each
+    with its own principled material), 256 principled materials sharing a
+    1024x1024 RGBA8 albedo texture (texture=None: untextured), 64 rect
+    lights on an 8x8 grid, n_views views of 3840x2160 at 16 spp, depth 6."""
     rng = np.random.default_rng(seed)
     nx, nz = cells
     sx, sz = size
-    ter, H = heightfield(rng, nx, nz, sx, sz, 10.0, n_mats)
+    ter, H = heightfield(rng, nx, nz, sx, sz, 10.0, n_mats, uv_repeat=4.0)
     sphere = uv_sphere((0.0, 0.0, 0.0), 1.0, material=0)
     inst = []
     for _ in range(n_spheres):
@@ -430,8 +430,11 @@ def large_combined(seed: int = 6, cells=(2048, 1024), n_spheres: int = 2000, n_m
     for v in range(n_views):
         x = (v + 0.5) / n_views * sx
         views.append(make_view((x, 35.0, -5.0), (x, 0.0, 0.5 * sz), (0, 1, 0), 55.0, w, h, spp))
+    if texture:
+        mats["texture"] = 0
     return Scene(ter, mats, np.concatenate(lights), np.concatenate(views), make_opts(depth), name="large_combined",
-                 meshes=[sphere], instances=np.concatenate(inst))
+                 meshes=[sphere], instances=np.concatenate(inst),
+                 textures=[noise_texture(*texture, seed=seed)] if texture else None)
 
 
 def checker_texture(w: int = 64, h: int = 64, cell: int = 1, c0=(230, 230, 230), c1=(20, 40, 160)) -> np.ndarray:

@@ -293,6 +293,13 @@ int gi_upload_texture(gi_ctx *ctx, const uint8_t *rgba, int32_t w, int32_t h, in
 int gi_texture_info(gi_ctx *ctx, int32_t tex_id, int32_t *levels, int64_t *texels);
 /* The chain, level 0 first: 4 * texels bytes. */
 int gi_download_texture(gi_ctx *ctx, int32_t tex_id, uint8_t *rgba);
+/* Gradient of the last gi_adjoint's objective with respect to the level-0
+ * texel values of a texture (value = byte / 255): w*h*3 floats, RGB,
+ * row-major.  A lookup blends four texels of one level with the bilinear
+ * weights; a coarser level's texel is the mean of the (up to) 2x2 below
+ * (the integer truncation is not differentiated).  The material colour
+ * gradient of a textured material is d/d colour of colour * texel. */
+int gi_texture_grad(gi_ctx *ctx, int32_t tex_id, float *d_rgb);
 
 /* ---- Instancing (two-level BVH) --------------------------------------------
  * No counterpart in the reference; needed by the large configuration

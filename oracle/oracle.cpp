@@ -559,7 +559,12 @@ int tex_level(float diff, int32_t w, int32_t h, int32_t levels) {
  * the reference leaves an index of W or H unwrapped (reading the next row
  * or level).  Coordinates that are NaN, infinite or >= 2^23 in magnitude
  * wrap to 0. */
-V3<float> tex_sample(const or_scene::Texture &T, float s, float t, float diff) {
+struct TexFetch {
+  size_t idx[4]; /* texel indices into the chain */
+  float w[4];    /* bilinear weights: d value / d texel */
+};
+
+V3<float> tex_sample(const or_scene::Texture &T, float s, float t, float diff, TexFetch *tf = nullptr) {
   int32_t W = T.w, H = T.h;
   const int lv = tex_level(diff, T.w, T.h, T.levels);
   size_t off = 0;
@@ -583,6 +588,16 @@ V3<float> tex_sample(const or_scene::Texture &T, float s, float t, float diff) {
     return mk<float>(float(q[0]) / 255.0f, float(q[1]) / 255.0f, float(q[2]) / 255.0f);
   };
   const V3<float> a00 = texel(nx, ny), a01 = texel(ox, ny), a10 = texel(nx, oy), a11 = texel(ox, oy);
+  if (tf) {
+    tf->idx[0] = off + (size_t)ny * W + nx;
+    tf->idx[1] = off + (size_t)ny * W + ox;
+    tf->idx[2] = off + (size_t)oy * W + nx;
+    tf->idx[3] = off + (size_t)oy * W + ox;
+    tf->w[0] = wy * wx;
+    tf->w[1] = wy * (1.0f - wx);
+    tf->w[2] = (1.0f - wy) * wx;
+    tf->w[3] = (1.0f - wy) * (1.0f - wx);
+  }
   const V3<float> a0x = wx * a00 + (1.0f - wx) * a01, a1x = wx * a10 + (1.0f - wx) * a11;
   return wy * a0x + (1.0f - wy) * a1x;
 }
@@ -1267,9 +1282,14 @@ template <class R> struct MatD {
   R lum, w[3];
   explicit MatD(const MatP<R> &p) {
     cd = p.cdlin();
-    dcd = mk<R>(p.color.x > R(0) ? R(2.2) * std::pow(p.color.x, R(1.2)) : R(0),
-                p.color.y > R(0) ? R(2.2) * std::pow(p.color.y, R(1.2)) : R(0),
-                p.color.z > R(0) ? R(2.2) * std::pow(p.color.z, R(1.2)) : R(0));
+    if (p.textured) /* d b^2.2 / d b = 2.2 b^2.2 / b for the textured base colour b */
+      dcd = mk<R>(p.color.x > R(0) ? (R(2.2) * cd.x) / p.color.x : R(0),
+                  p.color.y > R(0) ? (R(2.2) * cd.y) / p.color.y : R(0),
+                  p.color.z > R(0) ? (R(2.2) * cd.z) / p.color.z : R(0));
+    else
+      dcd = mk<R>(p.color.x > R(0) ? R(2.2) * std::pow(p.color.x, R(1.2)) : R(0),
+                  p.color.y > R(0) ? R(2.2) * std::pow(p.color.y, R(1.2)) : R(0),
+                  p.color.z > R(0) ? R(2.2) * std::pow(p.color.z, R(1.2)) : R(0));
     lum = R(0.3) * cd.x + R(0.6) * cd.y + R(0.1) * cd.z;
     tint = lum > R(0) ? divs(cd, lum) : mk<R>(1, 1, 1);
     w[0] = R(0.3); w[1] = R(0.6); w[2] = R(0.1);
@@ -1382,11 +1402,15 @@ template <class R> struct VRec {
   V3<R> dfn[5] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   V3<R> dlr[5] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   R gk = 0, score[5] = {0, 0, 0, 0, 0};
+  int tex = -1;              /* textured vertex: texture, lookup, texel value, untextured colour */
+  TexFetch tf{};
+  V3<R> tx{0, 0, 0}, col{0, 0, 0};
 };
 
 template <class R>
 void adjoint_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H, const Options &op, uint64_t n,
-                    V3<R> o, V3<R> d, V3<R> delta, std::vector<double> &gm, std::vector<double> &ge) {
+                    V3<R> o, V3<R> d, V3<R> delta, std::vector<double> &gm, std::vector<double> &ge, Diffs *df,
+                    std::vector<double> &gt, const std::vector<size_t> &toff) {
   VRec<R> vr[32];
   int nv = 0;
   V3<R> TP = mk<R>(1, 1, 1);
@@ -1413,6 +1437,20 @@ void adjoint_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H
     int32_t mat;
     surface<R>(s, id, bu, bv, P, Ni, mat);
     MatP<R> mp(s.mats[mat]);
+    if (df) {
+      const V3<R> Nn = normalize(Ni);
+      float tcx, tcy;
+      const float diff = update_diffs(s, id, (float)bu, (float)bv, mk<float>((float)Nn.x, (float)Nn.y, (float)Nn.z),
+                                      *df, tcx, tcy);
+      const int32_t tex = s.mats[mat].texture;
+      if (tex >= 0) {
+        const V3<float> tx = tex_sample(s.textures[(size_t)tex], tcx, tcy, diff, &v.tf);
+        v.tex = tex;
+        v.tx = mk<R>(tx.x, tx.y, tx.z);
+        v.col = mp.principled ? mp.color : mp.diffuse;
+        mp.apply_texel(v.tx);
+      }
+    }
     MatD<R> md(mp);
     v.mat = mat;
     V3<R> xl = mk<R>(R(H.sample(n, dim + 0)), R(H.sample(n, dim + 1)), R(H.sample(n, dim + 2)));
@@ -1512,6 +1550,12 @@ void adjoint_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H
         if (v.vis) g += cat(delta, c) * cat(v.dfn[q], c) * cat(v.T, c) * cat(v.psc, c);
         if (v.chain) g += cat(delta, c) * cat(Rj, c) * cat(v.dlr[q], c);
       }
+      if (v.tex >= 0 && q < 3) {
+        /* base colour b_q = colour_q * texel_q */
+        const double gq = (double)(g * cat(v.col, q));
+        for (int k = 0; k < 4; ++k) gt[3 * (toff[(size_t)v.tex] + v.tf.idx[k]) + q] += gq * (double)v.tf.w[k];
+        g = g * cat(v.tx, q);
+      }
       gm[5 * v.mat + q] += (double)g;
     }
     if (v.vis)
@@ -1522,7 +1566,7 @@ void adjoint_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H
 
 template <class R>
 void adjoint_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_render_opts &ro, const float *dL,
-                  double *d_mats, double *d_em, int32_t threads) {
+                  double *d_mats, double *d_em, double *d_tex, int32_t threads) {
   Options op = resolve(s, ro);
   HaltonTable H(ro.seed, dims_for_depth(op.depth));
   Tracer<R> tr{s};
@@ -1534,6 +1578,9 @@ void adjoint_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_
 #endif
   const size_t nm = s.mats.size(), nl = s.lights.size();
   std::vector<std::vector<double>> gm(nth, std::vector<double>(5 * nm, 0.0)), ge(nth, std::vector<double>(3 * nl, 0.0));
+  std::vector<size_t> toff(s.textures.size() + 1, 0);
+  for (size_t t = 0; t < s.textures.size(); ++t) toff[t + 1] = toff[t] + s.textures[t].chain.size() / 4;
+  std::vector<std::vector<double>> gt(nth, std::vector<double>(s.textured ? 3 * toff.back() : 0, 0.0));
   uint64_t base = 0;
   int64_t img_off = 0;
   for (int32_t vi = 0; vi < nv; ++vi) {
@@ -1555,7 +1602,10 @@ void adjoint_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_
         uint64_t n = base + (uint64_t)p * (uint64_t)v.spp + (uint64_t)si;
         V3<float> of, df;
         cam_ray(cam, x, y, H.sample(n, DIM_JX), H.sample(n, DIM_JY), H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
-        adjoint_sample<R>(s, tr, H, op, n, mk<R>(of.x, of.y, of.z), mk<R>(df.x, df.y, df.z), delta, gm[th], ge[th]);
+        Diffs dfs;
+        if (s.textured) cam_diffs(cam, x, y, dfs);
+        adjoint_sample<R>(s, tr, H, op, n, mk<R>(of.x, of.y, of.z), mk<R>(df.x, df.y, df.z), delta, gm[th], ge[th],
+                          s.textured ? &dfs : nullptr, gt[th], toff);
       }
     }
     base += (uint64_t)npix * (uint64_t)v.spp;
@@ -1568,6 +1618,32 @@ void adjoint_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_
   for (size_t i = 0; i < 3 * nl; ++i) {
     d_em[i] = 0;
     for (int t = 0; t < nth; ++t) d_em[i] += ge[t][i];
+  }
+  if (!d_tex || !s.textured) return;
+  /* texel gradients: sum the threads, then pass each level's gradient down
+   * to the (up to) 2x2 texels it averages, coarsest first; output level 0 */
+  std::vector<double> g(3 * toff.back(), 0.0);
+  for (int t = 0; t < nth; ++t)
+    for (size_t i = 0; i < g.size(); ++i) g[i] += gt[t][i];
+  size_t out = 0;
+  for (size_t ti = 0; ti < s.textures.size(); ++ti) {
+    const or_scene::Texture &T = s.textures[ti];
+    std::vector<size_t> lo(T.levels + 1);
+    std::vector<int32_t> lw(T.levels + 1), lh(T.levels + 1);
+    size_t o = 0;
+    int32_t W = T.w, Hh = T.h;
+    for (int l = 0; l <= T.levels; ++l) { lo[l] = o; lw[l] = W; lh[l] = Hh; o += (size_t)W * Hh; W = (W + 1) / 2; Hh = (Hh + 1) / 2; }
+    double *gt0 = g.data() + 3 * toff[ti];
+    for (int l = T.levels - 1; l >= 0; --l)
+      for (int32_t y = 0; y < lh[l]; ++y)
+        for (int32_t x = 0; x < lw[l]; ++x) {
+          const int32_t px = x / 2, py = y / 2;
+          const double cnt = (double)((2 * px + 1 < lw[l] ? 2 : 1) * (2 * py + 1 < lh[l] ? 2 : 1));
+          for (int c = 0; c < 3; ++c)
+            gt0[3 * (lo[l] + (size_t)y * lw[l] + x) + c] += gt0[3 * (lo[l + 1] + (size_t)py * lw[l + 1] + px) + c] / cnt;
+        }
+    for (size_t i = 0; i < 3 * (size_t)T.w * T.h; ++i) d_tex[out + i] = gt0[i];
+    out += 3 * (size_t)T.w * T.h;
   }
 }
 
@@ -1905,12 +1981,18 @@ float or_cdf_sample(const float *cdf, const float *pdf, int32_t size, float u, f
   return cdf_sample(cdf, pdf, size, u, *p);
 }
 
+int or_adjoint_tex(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
+                   const float *dL_dimage, double *d_mats, double *d_emission, double *d_tex, int32_t threads,
+                   int32_t f64) {
+  if (opts->rr_start > 0 || opts->max_depth > 32 || s->sky_light >= 0) return 1;
+  if (f64) adjoint_impl<double>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, d_tex, threads);
+  else adjoint_impl<float>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, d_tex, threads);
+  return 0;
+}
+
 int or_adjoint(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
                const float *dL_dimage, double *d_mats, double *d_emission, int32_t threads, int32_t f64) {
-  if (opts->rr_start > 0 || opts->max_depth > 32 || s->sky_light >= 0 || s->textured) return 1;
-  if (f64) adjoint_impl<double>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, threads);
-  else adjoint_impl<float>(*s, views, n_views, *opts, dL_dimage, d_mats, d_emission, threads);
-  return 0;
+  return or_adjoint_tex(s, views, n_views, opts, dL_dimage, d_mats, d_emission, nullptr, threads, f64);
 }
 
 }  // extern "C"

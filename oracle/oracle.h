@@ -110,6 +110,11 @@ void or_render_f64(const or_scene *s, const gi_view *views, int32_t n_views,
  * options (Russian roulette). */
 int or_adjoint(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
                const float *dL_dimage, double *d_mats, double *d_emission, int32_t threads, int32_t f64);
+/* or_adjoint plus the gradient w.r.t. every texture's level-0 texel values
+ * (byte / 255), d_tex = sum over textures of w*h*3 doubles, RGB. */
+int or_adjoint_tex(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts,
+                   const float *dL_dimage, double *d_mats, double *d_emission, double *d_tex, int32_t threads,
+                   int32_t f64);
 
 /* Sky light (gi::sky_light): the map of the scene's single GI_LIGHT_SKY
  * light (scale in col[0]); returns nonzero unless exactly one sky light. */

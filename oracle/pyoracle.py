@@ -74,6 +74,7 @@ def load():
             "or_render": (None, [P, P, I32, P, P, P, I32, I64, I64]),
             "or_render_f64": (None, [P, P, I32, P, P, I32]),
             "or_adjoint": (C.c_int, [P, P, I32, P, P, P, P, I32, I32]),
+            "or_adjoint_tex": (C.c_int, [P, P, I32, P, P, P, P, P, I32, I32]),
             "or_scene_set_sky": (C.c_int, [P, P, I32, I32]),
             "or_sky_eval": (None, [P, P, P]),
             "or_sky_sample": (None, [P, P, P, P, P]),
@@ -120,8 +121,10 @@ class OracleScene:
             self._keep.append(m)
             if self.lib.or_scene_set_sky(self.h, _p(m), m.shape[1], m.shape[0]) != 0:
                 raise ValueError("or_scene_set_sky: the scene needs exactly one sky light")
+        self.tex_shapes = []
         for t in textures or []:
             t = np.ascontiguousarray(t, np.uint8)
+            self.tex_shapes.append(t.shape[:2])
             if self.lib.or_scene_add_texture(self.h, _p(t), t.shape[1], t.shape[0]) < 0:
                 raise ValueError("or_scene_add_texture")
         for m in meshes or []:
@@ -211,8 +214,9 @@ class OracleScene:
         return img
 
 
-def adjoint(osc: OracleScene, views, opts, dL, threads=0, f64=True):
-    """(d_mats (n_mats, 5): color rgb, roughness, metallic; d_emission (n_lights, 3))."""
+def adjoint(osc: OracleScene, views, opts, dL, threads=0, f64=True, textures=False):
+    """(d_mats (n_mats, 5): color rgb, roughness, metallic; d_emission (n_lights, 3))
+    and with textures=True also the list of (h, w, 3) level-0 texel gradients."""
     views = np.ascontiguousarray(views).reshape(-1)
     opts = np.ascontiguousarray(opts).reshape(1)
     dL = np.ascontiguousarray(dL, np.float32).reshape(-1, 3)
@@ -220,10 +224,18 @@ def adjoint(osc: OracleScene, views, opts, dL, threads=0, f64=True):
     nl = len(osc._keep[2]) if osc._keep[2] is not None else 0
     dm = np.zeros((nm, 5))
     de = np.zeros((nl, 3))
-    rc = osc.lib.or_adjoint(osc.h, _p(views), len(views), _p(opts), _p(dL), _p(dm), _p(de), threads, int(f64))
+    dt = np.zeros(sum(3 * h * w for h, w in osc.tex_shapes))
+    rc = osc.lib.or_adjoint_tex(osc.h, _p(views), len(views), _p(opts), _p(dL), _p(dm), _p(de),
+                                _p(dt) if len(dt) else None, threads, int(f64))
     if rc != 0:
         raise ValueError("or_adjoint: unsupported options")
-    return dm, de
+    if not textures:
+        return dm, de
+    out, k = [], 0
+    for h, w in osc.tex_shapes:
+        out.append(dt[k:k + 3 * h * w].reshape(h, w, 3))
+        k += 3 * h * w
+    return dm, de, out
 
 
 def sky_eval(osc: OracleScene, d):

@@ -2,7 +2,8 @@
 mip chain byte for byte against the oracle's, textured renders (Lambert
 and principled, world and instanced geometry, every mip level reached)
 bit-exact against the oracle with the ray differentials carried per path,
-and the error paths."""
+the adjoint's material, emission and per-texel gradients against the
+oracle's, and the error paths."""
 import dataclasses
 
 import numpy as np
@@ -81,9 +82,27 @@ def test_texture_errors(gpu_ctx_factory):
             ctx.render()
     with gpu_ctx_factory(sc) as ctx:
         ctx.render()
-        with pytest.raises(GiError, match="textured"):
-            ctx.adjoint(np.ones((16, 16, 3), np.float32))
+        with pytest.raises(GiError, match="no texture gradient"):
+            ctx.texture_grad(0)
         with pytest.raises((GiError, ValueError)):
             ctx.upload_texture(np.zeros((4, 4, 3), np.uint8))
         with pytest.raises(GiError):
             ctx.texture_chain(7)
+
+
+def test_adjoint_texel_gradients_match_oracle(oracle, gpu_ctx_factory):
+    sc = scenes.textured_scene(w=64, h=48, spp=4, depth=3)
+    rng = np.random.default_rng(9)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render().image(0).reshape(-1, 3)
+        dL = (rng.normal(size=img.shape) * 1e-3).astype(np.float32)
+        gm, ge = ctx.adjoint(dL)
+        gt = [ctx.texture_grad(t) for t in range(len(sc.textures))]
+    osc = _osc(oracle, sc)
+    om, oe, ot = oracle.adjoint(osc, sc.views, sc.opts, dL, threads=8, f64=False, textures=True)
+    # the same paths as the FP32 oracle: agreement up to summation order
+    assert np.allclose(gm, om, rtol=2e-3, atol=2e-4 * np.abs(om).max())
+    assert np.allclose(ge, oe, rtol=2e-3, atol=2e-4 * np.abs(oe).max())
+    for a, b in zip(gt, ot):
+        assert np.abs(b).max() > 0
+        assert np.allclose(a, b, rtol=2e-3, atol=2e-4 * np.abs(b).max())

@@ -154,8 +154,86 @@ def test_textured_render_fp32_vs_fp64(oracle):
     assert np.abs(img - img0).mean() > 0.01 * np.abs(img0).mean()
 
 
-def test_adjoint_refuses_textures(oracle):
-    sc = _lambert_plane(np.full((8, 8, 4), 200, np.uint8))
+def _topdown(tex, principled=False, depth=1, w=24, h=24, spp=2):
+    """A textured plane seen from straight above, close enough that every
+    lookup is on level 0 (a footprint well under one texel)."""
+    sc = _lambert_plane(tex, albedo=0.9, w=w, h=h, spp=spp)
+    if principled:
+        sc.mats = scenes.principled((0.9, 0.8, 0.7), 0.6, 0.0)
+        sc.mats["texture"] = 0
+    sc.views = scenes.make_view((0.3, 3.0, 5.2), (0.3, 0.0, 5.0), (0, 0, 1), 30.0, w, h, spp)
+    sc.opts["max_depth"] = depth
+    return sc
+
+
+def _loss_grads(oracle, sc, dL, f64=True):
     osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=sc.textures)
-    with pytest.raises(Exception):
-        oracle.adjoint(osc, sc.views, sc.opts, np.ones((32 * 48, 3), np.float32))
+    return oracle.adjoint(osc, sc.views, sc.opts, dL, threads=4, f64=f64, textures=True)
+
+
+def test_texel_gradient_exact_for_a_linear_image(oracle):
+    """Depth 1, Lambert, level-0 lookups: the image is linear in every texel
+    value, so 40 bytes more on texel k change sum(dL * I) by 40 g_k / 255."""
+    rng = np.random.default_rng(3)
+    tex = rng.integers(60, 200, (16, 16, 4), dtype=np.uint8)
+    sc = _topdown(tex)
+    dL = rng.normal(size=(24 * 24, 3)).astype(np.float32)
+    _, _, (g,) = _loss_grads(oracle, sc, dL)
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=[tex])
+    base = (osc.render_f64(sc.views, sc.opts, threads=4).reshape(-1, 3) * dL).sum()
+    hot = np.argwhere(np.abs(g).sum(-1) > 0)
+    assert len(hot) > 10
+    for (y, x) in hot[rng.choice(len(hot), 6, replace=False)]:
+        for c in range(3):
+            t2 = tex.copy()
+            t2[y, x, c] += 40
+            o2 = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=[t2])
+            l2 = (o2.render_f64(sc.views, sc.opts, threads=4).reshape(-1, 3) * dL).sum()
+            # texel values are float32(byte / 255): 40 bytes are 40/255 up to float rounding
+            assert (l2 - base) * 255.0 / 40.0 == pytest.approx(g[y, x, c], rel=2e-5, abs=1e-9), (y, x, c)
+
+
+@pytest.mark.parametrize("principled", [False, True])
+def test_texel_gradient_sum_through_the_mip_chain(oracle, principled):
+    """A constant texture is constant on every level, so scaling all its
+    texels moves every lookup, whatever its level: the sum of the level-0
+    gradients (coarser levels passed down) is the derivative along that
+    direction, checked by central differences of the FP64 render."""
+    c0 = 140
+    sc = _lambert_plane(np.full((32, 32, 4), c0, np.uint8), albedo=0.9, w=40, h=30, spp=2)
+    sc.opts["max_depth"] = 2
+    if principled:
+        sc.mats = scenes.principled((0.9, 0.8, 0.7), 0.6, 0.0)
+        sc.mats["texture"] = 0
+    dL = np.random.default_rng(4).normal(size=(40 * 30, 3)).astype(np.float32)
+    _, _, (g,) = _loss_grads(oracle, sc, dL)
+
+    def loss(c):
+        t = np.full((32, 32, 4), c, np.uint8)
+        t[..., 3] = 255
+        o = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=[t])
+        return (o.render_f64(sc.views, sc.opts, threads=4).reshape(-1, 3) * dL).sum()
+
+    fd = (loss(c0 + 4) - loss(c0 - 4)) / (8 / 255.0)
+    assert g.sum() == pytest.approx(fd, rel=2e-3)
+
+
+def test_texel_gradients_fp32_vs_fp64_and_white_texture(oracle):
+    sc = scenes.textured_scene(w=32, h=24, spp=2, depth=3)
+    dL = np.random.default_rng(5).normal(size=(32 * 24, 3)).astype(np.float32)
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=sc.textures, meshes=sc.meshes,
+                             instances=sc.instances)
+    m64, e64, t64 = oracle.adjoint(osc, sc.views, sc.opts, dL, threads=4, f64=True, textures=True)
+    m32, e32, t32 = oracle.adjoint(osc, sc.views, sc.opts, dL, threads=4, f64=False, textures=True)
+    for a, b in zip(t32, t64):
+        assert np.abs(b).max() > 0
+        assert np.allclose(a, b, rtol=5e-2, atol=5e-3 * np.abs(b).max())
+    assert np.allclose(m32, m64, rtol=5e-2, atol=5e-3 * np.abs(m64).max())
+    # a white texture: material gradients equal the untextured ones (Lambert)
+    tex = _lambert_plane(np.full((8, 8, 4), 255, np.uint8))
+    plain = _lambert_plane()
+    dL = np.random.default_rng(6).normal(size=(32 * 48, 3)).astype(np.float32)
+    gt, _, _ = _loss_grads(oracle, tex, dL)
+    gp = oracle.adjoint(oracle.OracleScene(plain.tris, plain.mats, plain.lights), plain.views, plain.opts, dL,
+                        threads=4)[0]
+    assert np.allclose(gt, gp, rtol=1e-12, atol=0)
