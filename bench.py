@@ -235,6 +235,33 @@ def instancing_leg(steps: int):
                          "upload_and_bvh_ms": round(up_ms, 1),
                          "kernel_ms_per_frame": {k[3:]: round(v / steps, 2) for k, v in kms.items()}}
         print(f"[bench] instancing leg: {name} {out[name]}", file=sys.stderr, flush=True)
+    # the configuration's adjoint: material, light and per-texel gradients
+    # of an L2 loss against a perturbed target (same geometry and texture)
+    tgt = dataclasses.replace(scenes.perturbed(sc, seed=1), meshes=sc.meshes, instances=sc.instances,
+                              textures=sc.textures)
+    with Context(0) as ctx:
+        ctx.upload(tgt)
+        ctx.render(flags=FLAG_BATCHED)
+        timgs = [ctx.image(v) for v in range(len(sc.views))]
+    with Context(0) as ctx:
+        ctx.upload(sc)
+        ctx.render(flags=FLAG_BATCHED)
+        imgs = [ctx.image(v) for v in range(len(sc.views))]
+        dL = np.concatenate([(2.0 * (a - b) / a.size).reshape(-1, 3) for a, b in zip(imgs, timgs)]).astype(np.float32)
+        t0 = time.perf_counter()
+        ctx.render(flags=FLAG_BATCHED)
+        fwd_ms = (time.perf_counter() - t0) * 1e3
+        ctx.adjoint(dL)  # warm (gradient buffers)
+        t0 = time.perf_counter()
+        gm, ge = ctx.adjoint(dL)
+        adj_ms = (time.perf_counter() - t0) * 1e3
+        gt = ctx.texture_grad(0)
+    n_par = gm.size + ge.size + gt.size
+    out["adjoint"] = {"ms_forward": round(fwd_ms, 1), "ms_adjoint": round(adj_ms, 1),
+                      "adjoint_over_forward": round(adj_ms / fwd_ms, 3), "params_per_gradient": int(n_par),
+                      "texels_with_gradient": int((np.abs(gt).sum(-1) > 0).sum()),
+                      "grad_finite": bool(np.isfinite(gm).all() and np.isfinite(ge).all() and np.isfinite(gt).all())}
+    print(f"[bench] large configuration adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
     out["instanced_over_flattened"] = round(out["instanced"]["mrays_per_s"] / out["flattened"]["mrays_per_s"], 3)
     out["mean_radiance_rel_diff"] = float(np.abs(means["instanced"] - means["flattened"]).max()
                                           / np.abs(means["flattened"]).max())

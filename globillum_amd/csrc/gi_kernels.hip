@@ -553,6 +553,25 @@ __global__ __launch_bounds__(256) void k_mipmap(const uchar4 *__restrict__ src, 
   }
 }
 
+// The coarse levels' replicas summed into the chain's gradient: coarse
+// texel c0 + j of the replicas is chain texel dst + j.
+__global__ __launch_bounds__(256) void k_tex_rep_sum(float *__restrict__ g, uint32_t dst, const float *__restrict__ rep,
+                                                     uint32_t c0, uint32_t n, uint32_t n_coarse, uint32_t n_rep) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    float r = 0.f, gg = 0.f, bb = 0.f;
+    for (uint32_t k = 0; k < n_rep; ++k) {
+      const float *q = rep + 3 * ((size_t)k * n_coarse + c0 + j);
+      r += q[0];
+      gg += q[1];
+      bb += q[2];
+    }
+    float *o = g + 3 * ((size_t)dst + j);
+    o[0] = o[0] + r;
+    o[1] = o[1] + gg;
+    o[2] = o[2] + bb;
+  }
+}
+
 // Texel gradients of mip level l+1 passed down to level l: the chain is the
 // box filter's mean of the (up to) 2x2 texels below (the truncation of the
 // integer mean is not differentiated), so each child gets its parent's
@@ -1057,7 +1076,19 @@ struct AdjParams {
   const DevView *views;
   float *g_tex;               // texel gradients, RGB per texel of every chain
   const uint32_t *tex_goff;   // first texel of texture i in g_tex (in texels)
+  // replicas of the coarse levels' gradients (levels of <= kCoarseTexels
+  // texels, where many lanes across the chip add to few texels): replica
+  // r of coarse texel c at g_rep[3 (r n_coarse + c)], r from the block
+  float *g_rep;
+  const uint32_t *tex_coarse;  // chain index of texture i's first replicated texel
+  const uint32_t *tex_gc;      // texture i's first coarse texel in the replicas' indexing
+  uint32_t n_coarse;
 };
+
+// Coarse-level replication of the texel gradients: levels of at most
+// kCoarseTexels texels get kGradReplicas copies.
+constexpr uint32_t kCoarseTexels = 16384;
+constexpr uint32_t kGradReplicas = 64;
 
 // Adjoint replay of path vertex b: the same hit, NEE point and BRDF sample
 // as the forward pass (same Halton indices), the visibility read off the
@@ -1065,6 +1096,51 @@ struct AdjParams {
 // estimator (NEE, throughput chain up to the next clamp, score of the lobe
 // choice and of the GGX half-vector, emission).  Gradients gather in LDS per
 // block and go to global memory with one atomic per parameter per block.
+// Sum over the 64 lanes of a wave (butterfly); every lane gets the total.
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Texel gradients of one lookup per lane (has: the lane has one): g (RGB)
+// times each of the four bilinear weights, added at texel gi[k] (index into
+// g_tex in texels).  Called by the whole wave.  Lanes whose texel equals the
+// first pending lane's are summed across the wave first and added by one
+// lane: coarse mip levels, which far and secondary footprints reach, are
+// shared by many lanes, and atomics to one address serialise.  ta[k] is
+// where texel gi[k]'s RGB gradient is added (a coarse texel's replica).
+__device__ __forceinline__ void scatter_texel_grads(bool has, float *const ta[4], const uint32_t gi[4],
+                                                    const float w[4], const float g[3]) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    bool pend = has;
+    for (int round = 0; round < 2; ++round) {
+      const uint64_t act = __ballot(pend);
+      if (act == 0) break;
+      const int leader = __ffsll((unsigned long long)act) - 1;
+      const uint32_t key = (uint32_t)__shfl((int)gi[k], leader, 64);
+      const bool mine = pend && gi[k] == key;
+      if (__popcll(__ballot(mine)) < 4) break;
+      const float r = wave_sum(mine ? g[0] * w[k] : 0.0f);
+      const float gg = wave_sum(mine ? g[1] * w[k] : 0.0f);
+      const float bb = wave_sum(mine ? g[2] * w[k] : 0.0f);
+      if (lane == (uint32_t)leader) {
+        atomicAdd(ta[k] + 0, r);
+        atomicAdd(ta[k] + 1, gg);
+        atomicAdd(ta[k] + 2, bb);
+      }
+      pend = pend && !mine;
+    }
+    if (pend) {
+      atomicAdd(ta[k] + 0, g[0] * w[k]);
+      atomicAdd(ta[k] + 1, g[1] * w[k]);
+      atomicAdd(ta[k] + 2, g[2] * w[k]);
+    }
+  }
+}
+
 // kTex: the ray differentials are carried (as in the forward pass) and a
 // textured vertex's base colour b = colour * texel: the colour gradient is
 // dL/db * texel, and dL/db * colour goes to the four blended texels with
@@ -1088,6 +1164,10 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
     const uint32_t i = base + threadIdx.x;
     bool emit_next = false;
     float4 o_next, d_next, tp_next;
+    bool tex_has = false;  // texel gradient of this lane's lookup, scattered below by the whole wave
+    uint32_t tex_gi[4] = {0u, 0u, 0u, 0u};
+    float *tex_a[4] = {nullptr, nullptr, nullptr, nullptr};
+    float tex_w[4] = {0.f, 0.f, 0.f, 0.f}, tex_g[3] = {0.f, 0.f, 0.f};
     if (i < count) {
       const float4 ro = ray_o[i], rd = ray_d[i], tpv = tp_in[i], h = hits[i];
       const uint32_t n = fbits(ro.w), slot = fbits(rd.w), id = fbits(h.w);
@@ -1175,16 +1255,22 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
           if (kTex && ve.textured && q < 3) {
             // b_q = colour_q * tx_q
             const DevMat &m0 = P.mats[ve.mat];
-            const float col = m.principled ? m0.color[q] : m0.diffuse[q];
-            const float gt = gq * col;
-            if (gt != 0.0f) {
-              float *gtex = A.g_tex + 3 * (size_t)A.tex_goff[m.texture];
-#pragma unroll
-              for (int k = 0; k < 4; ++k) atomicAdd(&gtex[3 * (size_t)ve.tf.idx[k] + q], gt * ve.tf.w[k]);
-            }
+            tex_g[q] = gq * (m.principled ? m0.color[q] : m0.diffuse[q]);
             gq = gq * f3c(ve.tx, q);
           }
           if (gq != 0.0f) atomicAdd(&sg[5 * ve.mat + q], gq);
+        }
+        if (kTex && ve.textured) {
+          tex_has = tex_g[0] != 0.0f || tex_g[1] != 0.0f || tex_g[2] != 0.0f;
+          const uint32_t base = A.tex_goff[m.texture], coarse = A.tex_coarse[m.texture];
+          const uint32_t rep = (blockIdx.x % kGradReplicas) * A.n_coarse + A.tex_gc[m.texture];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t ix = ve.tf.idx[k];
+            tex_gi[k] = base + ix;
+            tex_w[k] = ve.tf.w[k];
+            tex_a[k] = ix >= coarse ? A.g_rep + 3 * ((size_t)rep + (ix - coarse)) : A.g_tex + 3 * (size_t)(base + ix);
+          }
         }
         if (vis && !ve.glass) {
           atomicAdd(&sg[5 * A.nmats + 3 * ve.light + 0], dl[0] * ve.fnee.x * Tc[0] * ve.gk);
@@ -1193,6 +1279,7 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
         }
       }
     }
+    if (kTex) scatter_texel_grads(tex_has, tex_a, tex_gi, tex_w, tex_g);
     uint32_t qn, unused;
     block_append2(emit_next, next_count, false, next_count, qn, unused);
     if (emit_next) {
@@ -1327,7 +1414,7 @@ struct gi_ctx {
   std::vector<TexHost> texs;
   DBuf d_tex;
   bool textured = false;  // some material names a texture
-  DBuf d_gtex, d_gtex_off;                    // adjoint: texel gradients of the chains
+  DBuf d_gtex, d_gtex_off, d_grep;            // adjoint: texel gradients of the chains, coarse replicas
   std::vector<std::vector<float>> tex_grad;   // level-0 texel gradients of the last adjoint
   int64_t world_nnodes = 0;
   int32_t world_depth = 0, n_inst = 0;
@@ -2905,14 +2992,37 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
   std::vector<uint32_t> goff(ctx->texs.size() + 1, 0);
   for (size_t t = 0; t < ctx->texs.size(); ++t) goff[t + 1] = goff[t] + (uint32_t)ctx->texs[t].texels;
   const size_t gtex = ctx->textured ? (size_t)goff.back() : 0;
+  // coarse levels (<= kCoarseTexels texels) and their replicas
+  std::vector<uint32_t> coarse(ctx->texs.size() + 1, 0), gc(ctx->texs.size() + 1, 0);
+  for (size_t t = 0; t < ctx->texs.size(); ++t) {
+    const gi_ctx::TexHost &T = ctx->texs[t];
+    uint32_t o = 0;
+    int32_t W = T.w, H = T.h;
+    for (int32_t l = 0; l <= T.levels && (uint64_t)W * H > kCoarseTexels; ++l) {
+      o += (uint32_t)W * (uint32_t)H;  // o <= texels: at most every level is skipped
+      W = (W + 1) / 2;
+      H = (H + 1) / 2;
+    }
+    coarse[t] = o;
+    gc[t + 1] = gc[t] + ((uint32_t)T.texels - o);
+  }
+  const size_t nrep = (size_t)kGradReplicas * gc[ctx->texs.size()] * 3;
   if (gtex) {
     GI_HIP(ctx, ctx->d_gtex.ensure(gtex * 3 * sizeof(float)));
     GI_HIP(ctx, hipMemsetAsync(ctx->d_gtex.p, 0, gtex * 3 * sizeof(float), ctx->stream));
-    GI_HIP(ctx, ctx->d_gtex_off.ensure(goff.size() * sizeof(uint32_t)));
-    GI_HIP(ctx, copy_sync(ctx, ctx->d_gtex_off.p, goff.data(), goff.size() * sizeof(uint32_t),
-                          hipMemcpyHostToDevice));
+    GI_HIP(ctx, ctx->d_grep.ensure(nrep * sizeof(float)));
+    GI_HIP(ctx, hipMemsetAsync(ctx->d_grep.p, 0, nrep * sizeof(float), ctx->stream));
+    std::vector<uint32_t> tab(goff);
+    tab.insert(tab.end(), coarse.begin(), coarse.end());
+    tab.insert(tab.end(), gc.begin(), gc.end());
+    GI_HIP(ctx, ctx->d_gtex_off.ensure(tab.size() * sizeof(uint32_t)));
+    GI_HIP(ctx, copy_sync(ctx, ctx->d_gtex_off.p, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     A.g_tex = ctx->d_gtex.as<float>();
     A.tex_goff = ctx->d_gtex_off.as<uint32_t>();
+    A.tex_coarse = A.tex_goff + goff.size();
+    A.tex_gc = A.tex_coarse + coarse.size();
+    A.g_rep = ctx->d_grep.as<float>();
+    A.n_coarse = gc[ctx->texs.size()];
   }
   rc = run_adjoint_passes(ctx, plan, ctx->opts.max_depth, A);
   if (rc) return rc;
@@ -2932,6 +3042,12 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
       H = (H + 1) / 2;
     }
     float *g = ctx->d_gtex.as<float>() + 3 * (size_t)goff[t];
+    {
+      const uint32_t n = gc[t + 1] - gc[t];
+      hipLaunchKernelGGL(k_tex_rep_sum, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0, ctx->stream, g,
+                         coarse[t], ctx->d_grep.as<float>(), gc[t], n, gc[ctx->texs.size()], kGradReplicas);
+      GI_HIP(ctx, hipGetLastError());
+    }
     for (int32_t l = T.levels - 1; l >= 0; --l) {
       const uint32_t n = (uint32_t)lw[l] * (uint32_t)lh[l];
       hipLaunchKernelGGL(k_tex_grad_down, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0,
