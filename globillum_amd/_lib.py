@@ -100,6 +100,10 @@ SIGNATURES = {
     "gi_texture_info": (_I, [_P, _I32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "gi_download_texture": (_I, [_P, _I32, _P]),
     "gi_texture_grad": (_I, [_P, _I32, _P]),
+    "gi_image_size": (_I, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "gi_image_read": (_I, [C.c_char_p, _P]),
+    "gi_obj_texture_count": (_I32, [_P]),
+    "gi_obj_texture_path": (C.c_char_p, [_P, _I32]),
     "gi_upload_instances": (_I, [_P, _P, _I32]),
     "gi_obj_open": (_I, [C.POINTER(_P)]),
     "gi_obj_append": (_I, [_P, C.c_char_p, _P]),

@@ -29,7 +29,11 @@
 //     M's upper 3x3, renormalised;
 //   - "usemtl" of a name no MTL file defines uses the fallback material;
 //   - MTL keys used: newmtl, Kd, Ks, d, Tr (= 1 - d), map_Kd; the rest are
-//     ignored.
+//     ignored.  map_Kd's file (its last token, relative to the MTL file's
+//     directory) joins the list of the scene's textures in order of first
+//     use, and the material's gi_material.texture is its index there (the
+//     image is decoded by gi_image_read, the texture uploaded by the
+//     caller, as convert_texture does, src/material-conversion.cpp:18-41).
 // Unlike the reference (which loops forever on a malformed number in a
 // .pbrdf file and leaves unset principled fields uninitialised) a parse
 // error is reported, and unset fields take PrincipledBRDFParameters'
@@ -53,6 +57,7 @@ struct gi_scene_file {
   std::vector<gi_material> mats;
   std::vector<std::string> names;
   std::vector<bool> has_diffuse_texture;
+  std::vector<std::string> textures;     // distinct map_Kd files (resolved), gi_material.texture indexes them
   std::map<std::string, int32_t> index;  // registered material name -> index
   std::string error;
 };
@@ -63,6 +68,7 @@ struct MtlEntry {
   float kd[3] = {0.f, 0.f, 0.f}, ks[3] = {0.f, 0.f, 0.f};
   float alpha = 1.f;
   bool tex_d = false;
+  std::string tex_path;  // map_Kd file, resolved against the MTL's directory
 };
 
 gi_material lambert_material(const float kd[3], const float ks[3], float alpha) {
@@ -130,7 +136,12 @@ bool read_mtl(const std::string &path, std::map<std::string, MtlEntry> &out, std
       ok = parse_floats(in, &tr, 1);
       if (ok) m.alpha = 1.f - tr;
     } else if (key == "map_Kd") {
+      // the file is the last token (options such as -bm come before it);
+      // relative paths are relative to the MTL file
       m.tex_d = true;
+      std::string tok, last;
+      while (in >> tok) last = tok;
+      if (!last.empty()) m.tex_path = (last[0] == '/') ? last : dir_of(path) + "/" + last;
     }
     if (!ok) {
       err = path + ":" + std::to_string(ln) + ": bad number after " + key;
@@ -188,6 +199,12 @@ int32_t register_material(gi_scene_file *f, const std::string &name, const MtlEn
   const float red[3] = {1.f, 0.f, 0.f};
   const int32_t id = (int32_t)f->mats.size();
   f->mats.push_back(e ? lambert_material(e->kd, e->ks, e->alpha) : lambert_material(red, red, 1.f));
+  if (e && e->tex_d && !e->tex_path.empty()) {
+    int32_t t = 0;
+    while (t < (int32_t)f->textures.size() && f->textures[(size_t)t] != e->tex_path) ++t;
+    if (t == (int32_t)f->textures.size()) f->textures.push_back(e->tex_path);
+    f->mats.back().texture = t;
+  }
   f->names.push_back(name);
   f->has_diffuse_texture.push_back(e ? e->tex_d : false);
   f->index[name] = id;
@@ -412,6 +429,13 @@ int gi_obj_apply_pbrdf(gi_scene_file *f, const char *dir) {
     if (gi_pbrdf_read(p.c_str(), &f->mats[i]) != GI_OK) return fail(f, GI_ERR_INVALID_ARG, "bad number in " + p);
   }
   return GI_OK;
+}
+
+int32_t gi_obj_texture_count(const gi_scene_file *f) { return f ? (int32_t)f->textures.size() : 0; }
+
+const char *gi_obj_texture_path(const gi_scene_file *f, int32_t i) {
+  if (!f || i < 0 || i >= (int32_t)f->textures.size()) return nullptr;
+  return f->textures[(size_t)i].c_str();
 }
 
 int gi_obj_sizes(const gi_scene_file *f, int64_t *ntri, int32_t *nmat) {

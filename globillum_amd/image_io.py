@@ -43,3 +43,20 @@ def save(path, rgb):
             f.write(img.tobytes())
     else:
         raise ValueError(f"unknown image extension {ext!r}")
+
+
+def read_image(path) -> np.ndarray:
+    """A texture image (PNG or binary PPM) as (h, w, 4) uint8 RGBA, row 0 the
+    file's top row -- the layout gi_upload_texture takes."""
+    lib = _lib.load()
+    import ctypes as C
+    w, h = C.c_int32(), C.c_int32()
+    p = os.fsencode(str(path))
+    rc = lib.gi_image_size(p, C.byref(w), C.byref(h))
+    if rc != _lib.GI_OK:
+        raise GiError(rc, f"cannot read image {path}")
+    out = np.zeros((h.value, w.value, 4), np.uint8)
+    rc = lib.gi_image_read(p, ptr(out))
+    if rc != _lib.GI_OK:
+        raise GiError(rc, f"cannot decode image {path}")
+    return out

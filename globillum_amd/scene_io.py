@@ -15,11 +15,12 @@ from ._lib import MATERIAL, TRI, GiError, ptr
 from .scenes import Scene
 
 
-def load_obj(paths, trafo=None, pbrdf_dir: str | None = None):
+def load_obj(paths, trafo=None, pbrdf_dir: str | None = None, with_textures: bool = False):
     """Read one or more OBJ files into one table.  Returns (tris, mats,
     names): gi_tri records, gi_material records (index 0 = the fallback
-    material) and the material names.  trafo: optional 4x4 (row-major)
-    applied to every file.  pbrdf_dir: directory searched for
+    material) and the material names; with_textures also the list of map_Kd
+    files that gi_material.texture indexes.  trafo: optional 4x4
+    (row-major) applied to every file.  pbrdf_dir: directory searched for
     <material name>.pbrdf files."""
     if isinstance(paths, (str, os.PathLike)):
         paths = [paths]
@@ -45,6 +46,9 @@ def load_obj(paths, trafo=None, pbrdf_dir: str | None = None):
         mats = np.zeros(nm.value, MATERIAL)
         lib.gi_obj_export(h, ptr(tris) if len(tris) else None, ptr(mats) if len(mats) else None)
         names = [lib.gi_obj_material_name(h, i).decode() for i in range(nm.value)]
+        if with_textures:
+            tex = [lib.gi_obj_texture_path(h, i).decode() for i in range(lib.gi_obj_texture_count(h))]
+            return tris, mats, names, tex
         return tris, mats, names
     finally:
         lib.gi_obj_free(h)
@@ -65,11 +69,25 @@ def read_pbrdf(path: str, base=None) -> np.ndarray:
     return m
 
 
-def obj_scene(paths, views, lights, opts, trafo=None, pbrdf_dir: str | None = None, name: str = "") -> Scene:
+def obj_scene(paths, views, lights, opts, trafo=None, pbrdf_dir: str | None = None, name: str = "",
+              textures: str = "load") -> Scene:
     """A renderable Scene from OBJ files plus the camera(s), lights and
-    options a scene script would give."""
-    tris, mats, names = load_obj(paths, trafo, pbrdf_dir)
+    options a scene script would give.  textures="load" decodes the map_Kd
+    images (PNG / PPM; a missing or unreadable file is an error),
+    "ignore" renders those materials untextured."""
+    from .image_io import read_image
+
+    tris, mats, names, tex = load_obj(paths, trafo, pbrdf_dir, with_textures=True)
+    if textures == "ignore":
+        mats["texture"] = -1
+        images = None
+    elif textures == "load":
+        images = [read_image(p) for p in tex] or None
+    else:
+        raise ValueError(f"textures must be 'load' or 'ignore', not {textures!r}")
     sc = Scene(tris, mats, np.array(lights).reshape(-1), np.array(views).reshape(-1), np.array(opts).reshape(1),
-               name=name or os.path.basename(str(paths if isinstance(paths, (str, os.PathLike)) else paths[0])))
+               name=name or os.path.basename(str(paths if isinstance(paths, (str, os.PathLike)) else paths[0])),
+               textures=images)
     sc.meta["material_names"] = names
+    sc.meta["texture_files"] = tex
     return sc

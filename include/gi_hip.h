@@ -359,6 +359,11 @@ int gi_obj_apply_pbrdf(gi_scene_file *f, const char *dir);
 int gi_obj_sizes(const gi_scene_file *f, int64_t *n_tris, int32_t *n_mats);
 int gi_obj_export(const gi_scene_file *f, gi_tri *tris, gi_material *mats);
 const char *gi_obj_material_name(const gi_scene_file *f, int32_t i);
+/* The distinct map_Kd files of the loaded materials, in order of first use
+ * (resolved against their MTL file's directory); gi_material.texture of an
+ * exported material is its index in this list (-1: none). */
+int32_t gi_obj_texture_count(const gi_scene_file *f);
+const char *gi_obj_texture_path(const gi_scene_file *f, int32_t i);
 const char *gi_obj_error(const gi_scene_file *f);
 void gi_obj_free(gi_scene_file *f);
 /* One .pbrdf file (PrincipledBRDFParameters::readFrom,
@@ -376,6 +381,13 @@ int gi_pbrdf_read(const char *path, gi_material *m);
  * rounded to nearest even. */
 int gi_write_png(const char *path, const float *rgb, int32_t w, int32_t h);
 int gi_write_exr(const char *path, const float *rgb, int32_t w, int32_t h);
+/* Texture images for map_Kd (the reference decodes them with rta's image
+ * library, external): PNG (grey, grey+alpha, RGB, RGBA at 8/16 bits,
+ * palette at 1/2/4/8 bits with tRNS; not interlaced) and binary PPM (P6).
+ * gi_image_size gives w and h; gi_image_read writes w*h RGBA8 texels, row 0
+ * the file's top row (as gi_upload_texture takes them). */
+int gi_image_size(const char *path, int32_t *w, int32_t *h);
+int gi_image_read(const char *path, uint8_t *rgba);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,9 @@ def read_mtl(path):
             out[cur]["alpha"] = 1.0 - float(tok[1])
         elif tok[0] == "map_Kd":
             out[cur]["tex_d"] = True
+            if len(tok) > 1:  # the file is the last token, relative to the MTL's directory
+                f = tok[-1]
+                out[cur]["tex_path"] = f if f.startswith("/") else (os.path.dirname(path) or ".") + "/" + f
     return out
 
 
@@ -56,16 +59,23 @@ class ObjTable:
         self.mats = []       # dicts: kd, ks, alpha, principled params
         self.names = []
         self.tex = []
+        self.textures = []   # distinct map_Kd files in order of first use
 
     def material(self, name, entry):
         if name in self.names:
             return self.names.index(name)
         self.names.append(name)
         if entry is None:
-            self.mats.append({"kd": [1.0, 0.0, 0.0], "ks": [1.0, 0.0, 0.0], "alpha": 1.0})
+            self.mats.append({"kd": [1.0, 0.0, 0.0], "ks": [1.0, 0.0, 0.0], "alpha": 1.0, "texture": -1})
             self.tex.append(False)
         else:
-            self.mats.append({"kd": list(entry["kd"]), "ks": list(entry["ks"]), "alpha": entry["alpha"]})
+            t = -1
+            if entry["tex_d"] and entry.get("tex_path"):
+                if entry["tex_path"] not in self.textures:
+                    self.textures.append(entry["tex_path"])
+                t = self.textures.index(entry["tex_path"])
+            self.mats.append({"kd": list(entry["kd"]), "ks": list(entry["ks"]), "alpha": entry["alpha"],
+                              "texture": t})
             self.tex.append(entry["tex_d"])
         return len(self.names) - 1
 

@@ -190,6 +190,48 @@ def test_obj_scene_bit_exact(oracle, gpu_ctx_factory, tmp_path):
     assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
 
 
+def test_obj_scene_with_map_kd_texture_bit_exact(oracle, gpu_ctx_factory, tmp_path):
+    """The room with map_Kd PNG textures on two materials (one Lambert, one
+    principled through its .pbrdf): obj_scene decodes the images, and the
+    render matches the oracle given the same decoded textures."""
+    from globillum_amd import image_io, scene_io
+
+    rng = np.random.default_rng(0)
+    image_io.save_png(tmp_path / "bricks.png", rng.uniform(0.2, 1.0, (32, 48, 3)).astype(np.float32))
+    image_io.save_png(tmp_path / "gold.png", rng.uniform(0.5, 1.0, (16, 16, 3)).astype(np.float32))
+    (tmp_path / "room.mtl").write_text(
+        "newmtl wall\nKd 0.7 0.7 0.7\nnewmtl redwall\nKd 0.9 0.3 0.3\nmap_Kd bricks.png\n"
+        "newmtl gold\nKd 0.8 0.6 0.2\nmap_Kd -bm 1.0 gold.png\n")
+    v = ["0 0 0", "4 0 0", "4 3 0", "0 3 0", "0 0 4", "4 0 4", "4 3 4", "0 3 4",
+         "1 0 2", "2 0 2", "2 1 2", "1 1 2", "1 0 3", "2 0 3", "2 1 3", "1 1 3"]
+    vt = ["0 0", "3 0", "3 2", "0 2"]
+    f = ["usemtl wall", "f 1 5 6 2", "f 4 3 7 8", "f 5 8 7 6",
+         "usemtl redwall", "f 1/1 4/4 8/3 5/2",
+         "usemtl wall", "f 2 6 7 3",
+         "usemtl gold", "f 9/1 12/4 11/3 10/2", "f 13/1 14/2 15/3 16/4", "f 9/1 13/2 16/3 12/4",
+         "f 10/1 11/4 15/3 14/2", "f 12/1 16/2 15/3 11/4"]
+    (tmp_path / "room.obj").write_text("mtllib room.mtl\n" + "".join(f"v {x}\n" for x in v)
+                                       + "".join(f"vt {x}\n" for x in vt) + "\n".join(f) + "\n")
+    mats_dir = tmp_path / "materials"
+    mats_dir.mkdir()
+    (mats_dir / "gold.pbrdf").write_text("color 0.9 0.7 0.3 roughness 0.3 metallic 0.4 specular 0.5\n")
+    view = scenes.make_view((2.0, 1.5, -3.0), (1.5, 1.2, 2.0), (0, 1, 0), 50.0, 64, 48, 4)
+    light = scenes.make_light((2.0, 2.95, 2.0), (0, -1, 0), (8, 8, 8), 1.0, 1.0)
+    sc = scene_io.obj_scene(str(tmp_path / "room.obj"), view, light, scenes.make_opts(4), pbrdf_dir=str(mats_dir))
+    assert sc.meta["texture_files"] == [str(tmp_path) + "/bricks.png", str(tmp_path) + "/gold.png"]
+    assert sc.mats["texture"].tolist() == [-1, -1, 0, 1] and [t.shape for t in sc.textures] == [(32, 48, 4), (16, 16, 4)]
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=sc.textures)
+    ref, st = osc.render(sc.views, sc.opts)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render().image(0).reshape(-1, 3)
+    assert img.max() > 0
+    _same(img, ref)
+    plain = scene_io.obj_scene(str(tmp_path / "room.obj"), view, light, scenes.make_opts(4),
+                               pbrdf_dir=str(mats_dir), textures="ignore")
+    with gpu_ctx_factory(plain) as ctx:
+        assert np.abs(ctx.render().image(0).reshape(-1, 3) - img).max() > 0.01
+
+
 def test_device_bvh_render_bit_exact(oracle, gpu_ctx_factory):
     sc = scenes.terrain_spheres(cells=(64, 64), n_spheres=100, w=80, h=45, spp=4)
     ref, st = _oracle(oracle, sc)

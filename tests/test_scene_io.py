@@ -14,13 +14,14 @@ REF = "/root/reference"
 
 
 def _compare(path_list, trafo=None, pbrdf_dir=None):
-    tris, mats, names = scene_io.load_obj(path_list, trafo, pbrdf_dir)
+    tris, mats, names, tex = scene_io.load_obj(path_list, trafo, pbrdf_dir, with_textures=True)
     o = obj_oracle.ObjTable()
     for p in path_list:
         o.append(p, trafo)
     if pbrdf_dir:
         o.apply_pbrdf(pbrdf_dir)
     assert names == o.names
+    assert tex == o.textures
     assert len(tris) == len(o.tris)
     for k in ("a", "b", "c", "na", "nb", "nc", "ta", "tb", "tc"):
         want = np.array([t[k] for t in o.tris], np.float32).reshape(len(o.tris), -1)
@@ -30,7 +31,7 @@ def _compare(path_list, trafo=None, pbrdf_dir=None):
         assert np.array_equal(mats[i]["diffuse_color"], np.float32(m["kd"]))
         assert np.array_equal(mats[i]["specular_color"], np.float32(m["ks"]))
         assert mats[i]["alpha"] == np.float32(m["alpha"])
-        assert mats[i]["texture"] == -1
+        assert mats[i]["texture"] == m["texture"]
         if "pbrdf" in m:
             assert mats[i]["principled"] == 1
             for f, v in m["pbrdf"].items():
@@ -81,6 +82,9 @@ def test_obj_matches_oracle(objdir):
     # fallback material is red/red, alpha from d and from Tr
     assert np.array_equal(mats[0]["diffuse_color"], [1, 0, 0]) and np.array_equal(mats[0]["specular_color"], [1, 0, 0])
     assert mats[1]["alpha"] == np.float32(0.5) and mats[2]["alpha"] == np.float32(0.75)
+    # map_Kd: the texture list holds the file (relative to the MTL's directory)
+    _, mats, _, tex = scene_io.load_obj(str(objdir / "a.obj"), with_textures=True)
+    assert tex == [str(objdir) + "/some/file.png"] and list(mats["texture"]) == [-1, -1, 0, -1]
 
 
 def test_obj_closed_forms(objdir):
