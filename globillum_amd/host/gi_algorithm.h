@@ -32,6 +32,25 @@ struct scene_desc {
   int64_t bvh2_n_nodes = 0;
   const uint32_t *bvh2_indices = nullptr;
   int64_t bvh2_n_indices = 0;
+  // optional: the sky light's map (rgb, w*h; load_skylight_data,
+  // src/lights.cpp:58-79), RGBA8 textures (gi_material.texture indexes
+  // them), and placed meshes (two-level instancing)
+  const float *sky_rgb = nullptr;
+  int32_t sky_w = 0, sky_h = 0;
+  struct image {
+    const uint8_t *rgba;
+    int32_t w, h;
+  };
+  const image *textures = nullptr;
+  int32_t n_textures = 0;
+  struct mesh {
+    const gi_tri *tris;
+    int64_t n;
+  };
+  const mesh *meshes = nullptr;
+  int32_t n_meshes = 0;
+  const gi_instance *instances = nullptr;
+  int32_t n_instances = 0;
 };
 
 // Camera as the reference extracts it from its lookat matrix

@@ -3,9 +3,14 @@
 // integrator, compute(), then update() while in_progress(), and write the
 // image.  Scenes come as a binary dump of C-ABI records (written by
 // globillum_amd.scenes.save_scene):
-//   "GISC" u32 version=1, u64 n_tris, u32 n_mats, u32 n_lights, u32 n_views,
-//   gi_tri[n_tris], gi_material[n_mats], gi_light[n_lights],
+//   "GISC" u32 version (1 or 2), u64 n_tris, u32 n_mats, u32 n_lights,
+//   u32 n_views, gi_tri[n_tris], gi_material[n_mats], gi_light[n_lights],
 //   gi_view[n_views], gi_render_opts
+// and in version 2 tagged sections until "END ":
+//   "SKY " u32 w, u32 h, float rgb[3*w*h]      the sky light's map
+//   "TEX " u32 w, u32 h, u8 rgba[4*w*h]        a texture (in id order)
+//   "MESH" u64 n, gi_tri[n]                    a mesh (in id order)
+//   "INST" u32 n, gi_instance[n]               the instances
 // Output by extension: .png / .exr as the reference's save_image writes them
 // (gi_write_png / gi_write_exr), anything else PFM (3 floats per pixel,
 // bottom row first, as the renderer's row order y*w+x already is).
@@ -29,6 +34,12 @@ struct SceneFile {
   std::vector<gi_light> lights;
   std::vector<gi_view> views;
   gi_render_opts opts{};
+  std::vector<float> sky;
+  uint32_t sky_w = 0, sky_h = 0;
+  std::vector<std::vector<uint8_t>> tex;
+  std::vector<std::pair<uint32_t, uint32_t>> tex_wh;
+  std::vector<std::vector<gi_tri>> meshes;
+  std::vector<gi_instance> inst;
 };
 
 template <class T> void read_n(std::ifstream &f, std::vector<T> &v, uint64_t n) {
@@ -44,7 +55,8 @@ SceneFile load(const std::string &path) {
   uint64_t nt;
   f.read(magic, 4);
   f.read(reinterpret_cast<char *>(&version), 4);
-  if (std::memcmp(magic, "GISC", 4) != 0 || version != 1) throw std::runtime_error(path + ": not a GISC v1 file");
+  if (std::memcmp(magic, "GISC", 4) != 0 || (version != 1 && version != 2))
+    throw std::runtime_error(path + ": not a GISC v1/v2 file");
   f.read(reinterpret_cast<char *>(&nt), 8);
   f.read(reinterpret_cast<char *>(&nm), 4);
   f.read(reinterpret_cast<char *>(&nl), 4);
@@ -55,6 +67,37 @@ SceneFile load(const std::string &path) {
   read_n(f, s.lights, nl);
   read_n(f, s.views, nv);
   f.read(reinterpret_cast<char *>(&s.opts), sizeof(s.opts));
+  while (version == 2 && f) {
+    char tag[4];
+    f.read(tag, 4);
+    if (!f || !std::memcmp(tag, "END ", 4)) break;
+    uint32_t a = 0, b = 0;
+    uint64_t n = 0;
+    if (!std::memcmp(tag, "SKY ", 4) || !std::memcmp(tag, "TEX ", 4)) {
+      f.read(reinterpret_cast<char *>(&a), 4);
+      f.read(reinterpret_cast<char *>(&b), 4);
+      if ((uint64_t)a * b > ((uint64_t)1 << 28)) throw std::runtime_error(path + ": image too large");
+      if (tag[0] == 'S') {
+        s.sky_w = a;
+        s.sky_h = b;
+        read_n(f, s.sky, 3 * (uint64_t)a * b);
+      } else {
+        s.tex.emplace_back();
+        s.tex_wh.emplace_back(a, b);
+        read_n(f, s.tex.back(), 4 * (uint64_t)a * b);
+      }
+    } else if (!std::memcmp(tag, "MESH", 4)) {
+      f.read(reinterpret_cast<char *>(&n), 8);
+      if (n > ((uint64_t)1 << 27)) throw std::runtime_error(path + ": mesh too large");
+      s.meshes.emplace_back();
+      read_n(f, s.meshes.back(), n);
+    } else if (!std::memcmp(tag, "INST", 4)) {
+      f.read(reinterpret_cast<char *>(&a), 4);
+      read_n(f, s.inst, a);
+    } else {
+      throw std::runtime_error(path + ": unknown section");
+    }
+  }
   if (!f) throw std::runtime_error(path + ": truncated");
   return s;
 }
@@ -96,6 +139,22 @@ int main(int argc, char **argv) {
     sd.n_mats = (int32_t)s.mats.size();
     sd.lights = s.lights.data();
     sd.n_lights = (int32_t)s.lights.size();
+    if (!s.sky.empty()) {
+      sd.sky_rgb = s.sky.data();
+      sd.sky_w = (int32_t)s.sky_w;
+      sd.sky_h = (int32_t)s.sky_h;
+    }
+    std::vector<gi::scene_desc::image> images;
+    for (size_t i = 0; i < s.tex.size(); ++i)
+      images.push_back({s.tex[i].data(), (int32_t)s.tex_wh[i].first, (int32_t)s.tex_wh[i].second});
+    sd.textures = images.data();
+    sd.n_textures = (int32_t)images.size();
+    std::vector<gi::scene_desc::mesh> meshes;
+    for (const auto &m : s.meshes) meshes.push_back({m.data(), (int64_t)m.size()});
+    sd.meshes = meshes.data();
+    sd.n_meshes = (int32_t)meshes.size();
+    sd.instances = s.inst.data();
+    sd.n_instances = (int32_t)s.inst.size();
     gi::gi_algorithm::original_scene = &sd;
 
     gi::mi355x_pt pt(v.w, v.h);

@@ -25,13 +25,26 @@ void mi355x_pt::activate(const scene_desc *scene) {
   if (!scene) throw std::invalid_argument("mi355x_pt: activate without a scene");
   check(gi_create(&ctx, 0), "gi_create");
   check(gi_upload_triangles(ctx, scene->tris, scene->n_tris), "upload triangles");
+  for (int32_t i = 0; i < scene->n_textures; ++i) {
+    int32_t id = -1;
+    check(gi_upload_texture(ctx, scene->textures[i].rgba, scene->textures[i].w, scene->textures[i].h, &id),
+          "upload texture");
+  }
   check(gi_upload_materials(ctx, scene->mats, scene->n_mats), "upload materials");
   check(gi_upload_lights(ctx, scene->lights, scene->n_lights), "upload lights");
+  if (scene->sky_rgb) check(gi_upload_sky(ctx, scene->sky_rgb, scene->sky_w, scene->sky_h), "upload sky map");
   if (scene->bvh2_nodes)
     check(gi_upload_bvh2(ctx, scene->bvh2_nodes, scene->bvh2_n_nodes, scene->bvh2_indices, scene->bvh2_n_indices),
           "upload bvh");
   else
     check(gi_build_bvh(ctx, 4), "build bvh");
+  if (scene->n_instances > 0) {
+    for (int32_t i = 0; i < scene->n_meshes; ++i) {
+      int32_t id = -1;
+      check(gi_upload_mesh(ctx, scene->meshes[i].tris, scene->meshes[i].n, &id), "upload mesh");
+    }
+    check(gi_upload_instances(ctx, scene->instances, scene->n_instances), "upload instances");
+  }
   gi_algorithm::activate(scene);
 }
 

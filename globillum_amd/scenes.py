@@ -513,15 +513,39 @@ def flatten(scene: Scene) -> Scene:
 
 def save_scene(scene: Scene, path: str) -> None:
     """Binary dump read by the C++ driver (host/gi_driver.cpp): "GISC", u32
-    version 1, u64 n_tris, u32 n_mats, n_lights, n_views, then the records."""
+    version, u64 n_tris, u32 n_mats, n_lights, n_views, then the records;
+    version 2 (a sky map, textures or instances) appends tagged sections
+    ("SKY ", "TEX ", "MESH", "INST") and "END "."""
+    extra = scene.sky is not None or scene.textures or (scene.instances is not None and len(scene.instances))
     with open(path, "wb") as f:
         f.write(b"GISC")
-        np.array([1], "<u4").tofile(f)
+        np.array([2 if extra else 1], "<u4").tofile(f)
         np.array([len(scene.tris)], "<u8").tofile(f)
         np.array([len(scene.mats), len(scene.lights), len(scene.views)], "<u4").tofile(f)
         for a, dt in ((scene.tris, TRI), (scene.mats, MATERIAL), (scene.lights, LIGHT), (scene.views, VIEW),
                       (scene.opts, OPTS)):
             np.ascontiguousarray(a, dt).tofile(f)
+        if not extra:
+            return
+        if scene.sky is not None:
+            m = np.ascontiguousarray(scene.sky, "<f4")
+            f.write(b"SKY ")
+            np.array([m.shape[1], m.shape[0]], "<u4").tofile(f)
+            m.tofile(f)
+        for t in scene.textures or []:
+            t = np.ascontiguousarray(t, np.uint8)
+            f.write(b"TEX ")
+            np.array([t.shape[1], t.shape[0]], "<u4").tofile(f)
+            t.tofile(f)
+        if scene.instances is not None and len(scene.instances):
+            for m in scene.meshes:
+                f.write(b"MESH")
+                np.array([len(m)], "<u8").tofile(f)
+                np.ascontiguousarray(m, TRI).tofile(f)
+            f.write(b"INST")
+            np.array([len(scene.instances)], "<u4").tofile(f)
+            np.ascontiguousarray(scene.instances, INSTANCE).tofile(f)
+        f.write(b"END ")
 
 
 def cornell_batch(seed: int = 3, n_views: int = 16, depth: int = 8, rr_start: int = 3,

@@ -74,3 +74,42 @@ def test_driver_writes_png_and_exr(tmp_path, oracle):
         else:
             got = _read_exr(out)
             assert np.array_equal(got[..., 3], ref.astype(np.float16)[::-1, ::-1, 0])
+
+
+def _combined(w=48, h=32, spp=3):
+    """Instanced geometry, textures and a sky light in one scene."""
+    import dataclasses
+
+    sc = scenes.textured_scene(w=w, h=h, spp=spp, depth=3)
+    return dataclasses.replace(sc, lights=np.concatenate([sc.lights, scenes.make_sky_light(0.5, 40.0)]),
+                               sky=scenes.sky_map(32, 16))
+
+
+def test_scene_dump_v2_sections(tmp_path):
+    sc = _combined()
+    p = tmp_path / "x.gisc"
+    scenes.save_scene(sc, str(p))
+    b = p.read_bytes()
+    assert b[:4] == b"GISC" and int.from_bytes(b[4:8], "little") == 2 and b.endswith(b"END ")
+    base = 4 + 4 + 8 + 12 + len(sc.tris) * 100 + len(sc.mats) * 96 + len(sc.lights) * 64 + 60 + 40
+    sky = 4 + 8 + 32 * 16 * 12
+    tex = sum(4 + 8 + t.size for t in sc.textures)
+    mesh = sum(4 + 8 + len(m) * 100 for m in sc.meshes)
+    inst = 4 + 4 + len(sc.instances) * 56
+    assert len(b) == base + sky + tex + mesh + inst + 4
+
+
+@pytest.mark.gpu
+def test_driver_renders_instances_textures_and_sky(tmp_path, oracle):
+    sc = _combined()
+    path = tmp_path / "x.gisc"
+    scenes.save_scene(sc, str(path))
+    out = tmp_path / "x.pfm"
+    r = subprocess.run([os.path.join(PKG, "gi_driver"), str(path), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    img = _read_pfm(out).reshape(-1, 3)
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, sky=sc.sky, textures=sc.textures, meshes=sc.meshes,
+                             instances=sc.instances)
+    ref, st = osc.render(sc.views, sc.opts)
+    assert img.max() > 0 and np.array_equal(img, ref)
