@@ -352,16 +352,15 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
     std::memset(&nd, 0, sizeof(nd));
     for (int k = 0; k < 4; ++k) {
       if (k >= (int)kids.size()) {
-        nd.lox[k] = nd.loy[k] = nd.loz[k] = 1.f;
-        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = -1.f;
-        nd.child[k] = kEmptyRef;
+        set_empty_slot(nd, k);
         continue;
       }
       int32_t c = kids[k];
       nd.lox[k] = N[c].lo[0]; nd.loy[k] = N[c].lo[1]; nd.loz[k] = N[c].lo[2];
       nd.hix[k] = N[c].hi[0]; nd.hiy[k] = N[c].hi[1]; nd.hiz[k] = N[c].hi[2];
       if (N[c].left < 0) {
-        nd.child[k] = N[c].count == 0 ? kEmptyRef : leaf_ref(N[c]);
+        if (N[c].count == 0) set_empty_slot(nd, k);
+        else nd.child[k] = leaf_ref(N[c]);
       } else {
         int32_t slot = (int32_t)wnodes.size();
         wnodes.emplace_back();

@@ -234,9 +234,7 @@ __global__ void k_collapse(CollapseArgs A, const int32_t *__restrict__ frontier,
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (k >= nk) {
-        nd.lox[k] = nd.loy[k] = nd.loz[k] = 1.0f;
-        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = -1.0f;
-        nd.child[k] = kEmptyRef;
+        set_empty_slot(nd, k);
         nd.pad[k] = 0;
         continue;
       }
@@ -351,13 +349,17 @@ bool lbvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, c
     WideNode nd;
     std::memset(&nd, 0, sizeof(nd));
     for (int k = 0; k < 4; ++k) {
-      nd.lox[k] = k == 0 ? lb[0] : 1.0f;
-      nd.loy[k] = k == 0 ? lb[1] : 1.0f;
-      nd.loz[k] = k == 0 ? lb[2] : 1.0f;
-      nd.hix[k] = k == 0 ? lb[4] : -1.0f;
-      nd.hiy[k] = k == 0 ? lb[5] : -1.0f;
-      nd.hiz[k] = k == 0 ? lb[6] : -1.0f;
-      nd.child[k] = k == 0 ? (kLeafBit | (1u << 27)) : kEmptyRef;
+      if (k > 0) {
+        set_empty_slot(nd, k);
+        continue;
+      }
+      nd.lox[k] = lb[0];
+      nd.loy[k] = lb[1];
+      nd.loz[k] = lb[2];
+      nd.hix[k] = lb[4];
+      nd.hiy[k] = lb[5];
+      nd.hiz[k] = lb[6];
+      nd.child[k] = kLeafBit | (1u << 27);
     }
     GI_CK(hipMemcpyAsync(nodes, &nd, sizeof(nd), hipMemcpyHostToDevice, s));
     GI_CK(hipStreamSynchronize(s));

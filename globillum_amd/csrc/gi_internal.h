@@ -37,6 +37,18 @@ constexpr uint32_t kEmptyRef = 0x80000000u;  // a leaf with zero triangles
 constexpr int kMaxLeafTris = 15;
 constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1;
 
+// An empty child slot: its box is [+inf, +inf] on every axis, so the slab
+// test never reports it (each plane distance is +inf or -inf for any finite
+// nonzero inverse direction) and the traversal needs no reference check.
+#if defined(__HIP__)
+__host__ __device__
+#endif
+inline void set_empty_slot(WideNode &nd, int k) {
+  nd.lox[k] = nd.loy[k] = nd.loz[k] = __builtin_inff();
+  nd.hix[k] = nd.hiy[k] = nd.hiz[k] = __builtin_inff();
+  nd.child[k] = kEmptyRef;
+}
+
 // Triangle in leaf order for intersection: vertex a (w = original id as
 // bits), edges b-a and c-a.  48 bytes.
 struct alignas(16) IsectTri {

@@ -239,7 +239,7 @@ __device__ __forceinline__ Stack<kLdsDepth> make_stack(uint32_t *smem, uint32_t 
 // space changes when it is taken up): that reloads the world ray and pops.
 // Triangle ids found in a mesh are offset by the instance's virtual base.
 __device__ __forceinline__ bool special_ref(uint32_t ref) {
-  return (ref & kLeafBit) && ((ref >> 27) & 15u) == 0u && ref != kDoneRef;
+  return (ref & kLeafBit) && ((ref >> 27) & 15u) == 0u && ref != kDoneRef && ref != kEmptyRef;
 }
 
 template <bool kAny, bool kInst, int kLdsDepth, class Start, class Finish>
@@ -307,7 +307,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
           const float z0 = fmaf(lz[k], idir.z, -oid.z), z1 = fmaf(hz[k], idir.z, -oid.z);
           const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
           const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), lim));
-          const bool hit = tn <= tf * kSlabSlack && cref[k] != kEmptyRef;
+          const bool hit = tn <= tf * kSlabSlack;  // empty slots have +inf boxes: never hit
           dist[k] = hit ? tn : INFINITY;
         }
         // sort (dist, ref) ascending: network (0,1)(2,3)(0,2)(1,3)(1,2)
