@@ -139,6 +139,26 @@ def test_full_size_c2_properties_and_crop_parity(oracle, gpu_ctx_factory):
     _same(img[p0:p1], ref[p0:p1])
 
 
+def test_full_size_large_configuration_properties_and_crop_parity(oracle, gpu_ctx_factory):
+    """One view of the large configuration at full size (3840x2160, 16 spp,
+    depth 6: 4.2M-triangle world, 2000 instanced spheres, 1024x1024 albedo
+    texture with ray-differential mip levels, 64 lights), and the oracle on
+    pixel ranges of the same view."""
+    sc = scenes.config_scene(5, n_views=1)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render(flags=FLAG_BATCHED).image(0).reshape(-1, 3)
+        c = ctx.counters()
+    n = 3840 * 2160 * 16
+    assert np.isfinite(img).all() and (img >= 0).all() and img.mean() > 0
+    assert c["samples"] == n and n <= c["rays_closest"] <= 6 * n and c["rays_shadow"] <= c["rays_closest"]
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=sc.textures, meshes=sc.meshes,
+                             instances=sc.instances)
+    for row, x0 in ((1400, 1500), (800, 2600)):
+        p0 = row * 3840 + x0
+        ref, _ = osc.render(sc.views, sc.opts, p_begin=p0, p_end=p0 + 300)
+        _same(img[p0:p0 + 300], ref[p0:p0 + 300])
+
+
 def test_render_argument_errors(gpu_ctx_factory):
     sc = scenes.cornell_boxes(w=8, h=8, spp=1)
     with gpu_ctx_factory(sc) as ctx:
