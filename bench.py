@@ -151,9 +151,10 @@ def adjoint_leg(steps: int):
 
 def bvh_leg(steps: int):
     """BVH construction: the host binned-SAH builder against the device
-    linear-BVH builder on the headline scene (291k triangles) and on a
-    2048x1024-cell heightfield (4.19M triangles, the large configuration's
-    terrain), plus the headline frame time traversing each tree."""
+    linear-BVH and PLOC (radius 16) builders on the headline scene (291k
+    triangles) and on a 2048x1024-cell heightfield (4.19M triangles, the
+    large configuration's terrain), plus the headline frame time
+    traversing each tree."""
     import numpy as np
 
     from globillum_amd import scenes
@@ -166,20 +167,22 @@ def bvh_leg(steps: int):
         t0 = time.perf_counter()
         ctx.upload(sc)  # host SAH (includes the upload of the tables)
         host_ms = (time.perf_counter() - t0) * 1e3
-        frames = {}
-        for name in ("sah", "lbvh"):
-            if name == "lbvh":
-                ctx.build_bvh_device()  # warm (allocations, code objects)
+        frames, dev_ms = {}, {}
+        for name, method in (("sah", None), ("lbvh", "device"), ("ploc", "ploc")):
+            if method is not None:
+                ctx.build_bvh_device(4, method)  # warm (allocations, code objects)
                 t0 = time.perf_counter()
-                ctx.build_bvh_device()
-                dev_ms = (time.perf_counter() - t0) * 1e3
+                ctx.build_bvh_device(4, method)
+                dev_ms[name] = (time.perf_counter() - t0) * 1e3
             ctx.render(flags=FLAG_BATCHED)
             t0 = time.perf_counter()
             for _ in range(steps):
                 ctx.render(flags=FLAG_BATCHED)
             frames[name] = (time.perf_counter() - t0) / steps * 1e3
-    out["terrain_spheres_291k"] = {"host_sah_ms_incl_upload": round(host_ms, 1), "device_lbvh_ms": round(dev_ms, 2),
-                                   "frame_ms_sah": round(frames["sah"], 2), "frame_ms_lbvh": round(frames["lbvh"], 2)}
+    out["terrain_spheres_291k"] = {"host_sah_ms_incl_upload": round(host_ms, 1),
+                                   "device_lbvh_ms": round(dev_ms["lbvh"], 2), "device_ploc_ms": round(dev_ms["ploc"], 2),
+                                   "frame_ms_sah": round(frames["sah"], 2), "frame_ms_lbvh": round(frames["lbvh"], 2),
+                                   "frame_ms_ploc": round(frames["ploc"], 2)}
     rng = np.random.default_rng(5)
     hf, _ = scenes.heightfield(rng, 2048, 1024, 200.0, 100.0, 10.0, 8)
     big = scenes.Scene(hf, sc.mats, sc.lights, sc.views, sc.opts)
@@ -187,11 +190,14 @@ def bvh_leg(steps: int):
         t0 = time.perf_counter()
         ctx.upload(big)
         host_ms = (time.perf_counter() - t0) * 1e3
-        ctx.build_bvh_device()
-        t0 = time.perf_counter()
-        ctx.build_bvh_device()
-        dev_ms = (time.perf_counter() - t0) * 1e3
-    out["heightfield_4194k"] = {"host_sah_ms_incl_upload": round(host_ms, 1), "device_lbvh_ms": round(dev_ms, 2)}
+        dev_ms = {}
+        for name, method in (("lbvh", "device"), ("ploc", "ploc")):
+            ctx.build_bvh_device(4, method)
+            t0 = time.perf_counter()
+            ctx.build_bvh_device(4, method)
+            dev_ms[name] = (time.perf_counter() - t0) * 1e3
+    out["heightfield_4194k"] = {"host_sah_ms_incl_upload": round(host_ms, 1), "device_lbvh_ms": round(dev_ms["lbvh"], 2),
+                                "device_ploc_ms": round(dev_ms["ploc"], 2)}
     return out
 
 

@@ -95,6 +95,7 @@ SIGNATURES = {
     "gi_sky_light_power": (_F, [_F, _F]),
     "gi_upload_sky": (_I, [_P, _P, _I32, _I32]),
     "gi_build_bvh_device": (_I, [_P, _I32]),
+    "gi_build_bvh_device_ex": (_I, [_P, _I32, _I32, _I32]),
     "gi_upload_mesh": (_I, [_P, _P, _I64, C.POINTER(C.c_int32)]),
     "gi_upload_texture": (_I, [_P, _P, _I32, _I32, C.POINTER(C.c_int32)]),
     "gi_texture_info": (_I, [_P, _I32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),

@@ -10,6 +10,9 @@ import numpy as np
 from . import _lib
 from ._lib import COUNTERS, HIT, INSTANCE, LIGHT, MATERIAL, OPTS, TRI, TUNING, VIEW, GiError, ptr
 
+# device builders of gi_build_bvh_device_ex (GI_BVH_LBVH, GI_BVH_PLOC)
+_DEVICE_METHODS = {"device": 0, "ploc": 1}
+
 
 def _check(lib, ctx, rc):
     if rc != _lib.GI_OK:
@@ -47,11 +50,13 @@ class Context:
         _check(self.lib, self.h, rc)
 
     # ---- scene ---------------------------------------------------------
-    def upload(self, scene, max_leaf: int = 4, bvh2: tuple | None = None, builder: str = "host"):
+    def upload(self, scene, max_leaf: int = 4, bvh2: tuple | None = None, builder: str = "host",
+               radius: int = 16):
         """Triangles, materials and lights of `scene`, and its BVH: the
         host binned-SAH builder (builder="host"), the device LBVH builder
-        (builder="device"), or an external tree in the reference's 2 x
-        float4 layout (bvh2=(nodes, indices))."""
+        (builder="device"), the device PLOC builder with the given search
+        radius (builder="ploc"), or an external tree in the reference's
+        2 x float4 layout (bvh2=(nodes, indices))."""
         tris = np.ascontiguousarray(scene.tris, TRI)
         self._ck(self.lib.gi_upload_triangles(self.h, ptr(tris), len(tris)))
         for t in getattr(scene, "textures", None) or []:
@@ -66,11 +71,11 @@ class Context:
         if sky is not None:
             m = np.ascontiguousarray(sky, np.float32)
             self._ck(self.lib.gi_upload_sky(self.h, ptr(m), m.shape[1], m.shape[0]))
-        if bvh2 is None and builder == "device":
-            self._ck(self.lib.gi_build_bvh_device(self.h, max_leaf))
+        if bvh2 is None and builder in _DEVICE_METHODS:
+            self._ck(self.lib.gi_build_bvh_device_ex(self.h, max_leaf, _DEVICE_METHODS[builder], radius))
         elif bvh2 is None:
             if builder != "host":
-                raise ValueError(f"builder must be 'host' or 'device', not {builder!r}")
+                raise ValueError(f"builder must be 'host', 'device' or 'ploc', not {builder!r}")
             self._ck(self.lib.gi_build_bvh(self.h, max_leaf))
         else:
             nodes, idx = bvh2
@@ -84,10 +89,13 @@ class Context:
                                          lights=None if scene.lights is None else scene.lights.copy())
         return self
 
-    def build_bvh_device(self, max_leaf: int = 4):
-        """Rebuild the BVH of the uploaded triangles on the device (a new
-        world tree drops the instances; the scene's are placed again)."""
-        self._ck(self.lib.gi_build_bvh_device(self.h, max_leaf))
+    def build_bvh_device(self, max_leaf: int = 4, method: str = "device", radius: int = 16):
+        """Rebuild the BVH of the uploaded triangles on the device, as a
+        linear BVH (method="device") or by PLOC clustering (method="ploc").
+        A new world tree drops the instances; the scene's are placed again."""
+        if method not in _DEVICE_METHODS:
+            raise ValueError(f"method must be 'device' or 'ploc', not {method!r}")
+        self._ck(self.lib.gi_build_bvh_device_ex(self.h, max_leaf, _DEVICE_METHODS[method], radius))
         sc = self.scene
         if sc is not None and getattr(sc, "instances", None) is not None and len(sc.instances):
             self._ck(self.lib.gi_upload_instances(self.h, ptr(self._inst), len(self._inst)))

@@ -1,7 +1,9 @@
-// Device BVH build: linear BVH (Karras 2012, "Maximizing parallelism in the
-// construction of BVHs, octrees and k-d trees") over 30-bit Morton codes of
-// the triangle centroids, bottom-up box refit, and a level-by-level collapse
-// into the traversal kernels' 4-wide nodes (children picked by opening the
+// Device BVH build.  The binary tree is either a linear BVH (Karras 2012,
+// "Maximizing parallelism in the construction of BVHs, octrees and k-d
+// trees") over 30-bit Morton codes of the triangle centroids with a
+// bottom-up box refit, or a PLOC clustering over the 63-bit Morton order
+// (below).  Either is then collapsed level by level into the traversal
+// kernels' 4-wide nodes (children picked by opening the
 // child of largest surface area, subtrees of at most max_leaf triangles
 // becoming leaves) -- the same node format and leaf order conventions as
 // the host builder's collapse_wide.  The device counterpart of the
@@ -55,6 +57,35 @@ __global__ void k_morton(const float4 *__restrict__ tri, uint32_t stride, uint32
   const uint32_t x = quant10((cx - lo.x) * scale.x), y = quant10((cy - lo.y) * scale.y),
                  z = quant10((cz - lo.z) * scale.z);
   keys[i] = (spread10(x) << 2) | (spread10(y) << 1) | spread10(z);
+  vals[i] = i;
+}
+
+__device__ __forceinline__ uint64_t spread21(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+__device__ __forceinline__ uint32_t quant21(float x) {
+  const float q = x * 2097152.0f;
+  return q <= 0.0f ? 0u : (q >= 2097151.0f ? 2097151u : (uint32_t)q);
+}
+
+// 63-bit Morton key (21 bits per axis), for the clustering builder.
+__global__ void k_morton64(const float4 *__restrict__ tri, uint32_t stride, uint32_t n, float3 lo, float3 scale,
+                           uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = tri[(size_t)i * stride], b = tri[(size_t)i * stride + 1], c = tri[(size_t)i * stride + 2];
+  const float cx = (a.x + b.x + c.x) * (1.0f / 3.0f), cy = (a.y + b.y + c.y) * (1.0f / 3.0f),
+              cz = (a.z + b.z + c.z) * (1.0f / 3.0f);
+  const uint32_t x = quant21((cx - lo.x) * scale.x), y = quant21((cy - lo.y) * scale.y),
+                 z = quant21((cz - lo.z) * scale.z);
+  keys[i] = (spread21(x) << 2) | (spread21(y) << 1) | spread21(z);
   vals[i] = i;
 }
 
@@ -123,13 +154,8 @@ __device__ __forceinline__ void store_box(float *p, const Box &b) {
 
 // Leaf boxes, then every interior box from the second of its two children
 // to finish (release / acquire on the node's arrival counter).
-__global__ void k_refit(const float4 *__restrict__ tri, uint32_t stride, int n, const uint32_t *__restrict__ order,
-                        const int32_t *__restrict__ left, const int32_t *__restrict__ right,
-                        const int32_t *__restrict__ parent, const int32_t *__restrict__ leaf_parent,
-                        float *leafbox, float *nodebox, uint32_t *visit) {
-  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (k >= n) return;
-  const size_t t = order[k];
+// The padded box of triangle t (the host builder's padding).
+__device__ __forceinline__ Box leaf_box(const float4 *__restrict__ tri, uint32_t stride, size_t t) {
   const float4 a = tri[t * stride], b = tri[t * stride + 1], c = tri[t * stride + 2];
   Box bx;
   bx.lo[0] = fminf(a.x, fminf(b.x, c.x));
@@ -145,6 +171,16 @@ __global__ void k_refit(const float4 *__restrict__ tri, uint32_t stride, int n, 
     bx.lo[q] -= pad;
     bx.hi[q] += pad;
   }
+  return bx;
+}
+
+__global__ void k_refit(const float4 *__restrict__ tri, uint32_t stride, int n, const uint32_t *__restrict__ order,
+                        const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                        const int32_t *__restrict__ parent, const int32_t *__restrict__ leaf_parent,
+                        float *leafbox, float *nodebox, uint32_t *visit) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= n) return;
+  Box bx = leaf_box(tri, stride, order[k]);
   store_box(leafbox + 8 * (size_t)k, bx);
   int p = n > 1 ? leaf_parent[k] : -1;
   while (p >= 0) {
@@ -255,6 +291,197 @@ __global__ void k_collapse(CollapseArgs A, const int32_t *__restrict__ frontier,
   }
 }
 
+// ---------------------------------------------------------------------------
+// PLOC (Meister & Bittner 2018, "Parallel locally-ordered clustering for
+// bounding volume hierarchy construction").  Clusters start as the leaves in
+// Morton order.  Each round, every cluster finds the neighbour within
+// `radius` positions whose union box has the smallest surface area; mutual
+// nearest neighbours merge into a new interior node at the lower position,
+// and a scan compacts the surviving clusters in order.  Pairs are ranked by
+// the 64-bit key (area bits, hash of the two positions), then by position:
+// a strict total order that both ends of a pair compute alike, so the
+// globally smallest pair is always mutual and every round merges.  The hash
+// spreads ties (equal boxes) so that about a third of them merge per round
+// instead of one pair.
+//
+// Interior nodes are numbered in merge order (the root is n - 2).  A leaf's
+// slot in the final leaf order is the number of leaves left of it in a
+// depth-first walk; the collapse then sees contiguous leaf ranges, as with
+// the radix tree.
+
+constexpr int kPlocMaxRadius = 64;
+
+__device__ __forceinline__ uint32_t pair_hash(uint32_t a, uint32_t b) {
+  uint32_t x = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
+  x ^= x >> 15;
+  x *= 0x2C1B3C6Du;
+  x ^= x >> 12;
+  x *= 0x297A2D39u;
+  x ^= x >> 15;
+  return x;
+}
+
+// Leaf boxes in sorted order, and the first clusters (one leaf each).
+__global__ void k_ploc_init(const float4 *__restrict__ tri, uint32_t stride, uint32_t n,
+                            const uint32_t *__restrict__ order, float4 *__restrict__ leafbox,
+                            float4 *__restrict__ cbox, int32_t *__restrict__ cl) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const Box b = leaf_box(tri, stride, order[k]);
+  const float4 lo = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f), hi = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
+  leafbox[2 * (size_t)k] = lo;
+  leafbox[2 * (size_t)k + 1] = hi;
+  cbox[2 * (size_t)k] = lo;
+  cbox[2 * (size_t)k + 1] = hi;
+  cl[k] = ~(int32_t)k;
+}
+
+__device__ __forceinline__ float union_area(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+  const float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
+  const float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
+  const float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
+  return fabsf(dx * dy + dy * dz + dz * dx);  // |.|: +0 and -0 rank alike
+}
+
+// Nearest neighbour of every cluster within `radius` positions.  The block's
+// clusters and `radius` on either side are staged in LDS.
+__global__ void __launch_bounds__(kBlock) k_ploc_nn(const float4 *__restrict__ cbox, uint32_t cnt, int radius,
+                                                    int32_t *__restrict__ nn) {
+  __shared__ float4 slo[kBlock + 2 * kPlocMaxRadius], shi[kBlock + 2 * kPlocMaxRadius];
+  const int start = (int)(blockIdx.x * kBlock);
+  const int lo = max(0, start - radius), hi = min((int)cnt, start + kBlock + radius);
+  for (int k = lo + (int)threadIdx.x; k < hi; k += kBlock) {
+    slo[k - lo] = cbox[2 * (size_t)k];
+    shi[k - lo] = cbox[2 * (size_t)k + 1];
+  }
+  __syncthreads();
+  const int i = start + (int)threadIdx.x;
+  if (i >= (int)cnt) return;
+  const float4 alo = slo[i - lo], ahi = shi[i - lo];
+  uint64_t best = ~0ull;
+  int bj = -1;
+  const int jlo = max(0, i - radius), jhi = min((int)cnt - 1, i + radius);
+  for (int j = jlo; j <= jhi; ++j) {
+    if (j == i) continue;
+    const float a = union_area(alo, ahi, slo[j - lo], shi[j - lo]);
+    const uint64_t key = ((uint64_t)__float_as_uint(a) << 32) | pair_hash((uint32_t)min(i, j), (uint32_t)max(i, j));
+    if (key < best) {  // ascending j: a full tie keeps the lower position
+      best = key;
+      bj = j;
+    }
+  }
+  nn[i] = bj;
+}
+
+// flags[i]: low word 1 if cluster i survives the round (it does not merge
+// into a lower position), high word 1 if it starts a merge.
+__global__ void k_ploc_flags(const int32_t *__restrict__ nn, uint32_t cnt, uint64_t *__restrict__ flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const int32_t m = nn[i];
+  const bool mutual = m >= 0 && nn[m] == (int32_t)i;
+  const uint64_t lead = mutual && (int32_t)i < m, keep = !(mutual && (int32_t)i > m);
+  flags[i] = keep | (lead << 32);
+}
+
+__device__ __forceinline__ uint32_t leaves_of(const uint32_t *__restrict__ ncount, int32_t c) {
+  return c < 0 ? 1u : ncount[c];
+}
+
+// Merges the mutual pairs into interior nodes node_base + (merge rank) and
+// writes the surviving clusters in order.  totals = {clusters, merges}.
+__global__ void k_ploc_merge(const int32_t *__restrict__ cl, const float4 *__restrict__ cbox,
+                             const int32_t *__restrict__ nn, const uint64_t *__restrict__ flags,
+                             const uint64_t *__restrict__ pre, uint32_t cnt, uint32_t node_base,
+                             int32_t *__restrict__ left, int32_t *__restrict__ right, int32_t *__restrict__ parent,
+                             int32_t *__restrict__ leaf_parent, uint32_t *__restrict__ ncount,
+                             float4 *__restrict__ nodebox, int32_t *__restrict__ cl_out,
+                             float4 *__restrict__ cbox_out, uint32_t *__restrict__ totals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint64_t f = flags[i], p = pre[i];
+  const uint32_t keep = (uint32_t)f, lead = (uint32_t)(f >> 32);
+  const uint32_t slot = (uint32_t)p, rank = (uint32_t)(p >> 32);
+  if (i == cnt - 1) {
+    totals[0] = slot + keep;
+    totals[1] = rank + lead;
+  }
+  if (!keep) return;
+  float4 lo = cbox[2 * (size_t)i], hi = cbox[2 * (size_t)i + 1];
+  int32_t c = cl[i];
+  if (lead) {
+    const int32_t m = nn[i];
+    const int32_t d = cl[m];
+    const float4 blo = cbox[2 * (size_t)m], bhi = cbox[2 * (size_t)m + 1];
+    lo = make_float4(fminf(lo.x, blo.x), fminf(lo.y, blo.y), fminf(lo.z, blo.z), 0.0f);
+    hi = make_float4(fmaxf(hi.x, bhi.x), fmaxf(hi.y, bhi.y), fmaxf(hi.z, bhi.z), 0.0f);
+    const int32_t node = (int32_t)(node_base + rank);
+    left[node] = c;
+    right[node] = d;
+    ncount[node] = leaves_of(ncount, c) + leaves_of(ncount, d);
+    nodebox[2 * (size_t)node] = lo;
+    nodebox[2 * (size_t)node + 1] = hi;
+    if (c < 0) leaf_parent[~c] = node;
+    else parent[c] = node;
+    if (d < 0) leaf_parent[~d] = node;
+    else parent[d] = node;
+    c = node;
+  }
+  cl_out[slot] = c;
+  cbox_out[2 * (size_t)slot] = lo;
+  cbox_out[2 * (size_t)slot + 1] = hi;
+}
+
+// Depth-first slot of every leaf: the leaves of the left siblings on its
+// way to the root.
+__global__ void k_ploc_slot(const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                            const int32_t *__restrict__ parent, const int32_t *__restrict__ leaf_parent,
+                            const uint32_t *__restrict__ ncount, uint32_t n, uint32_t *__restrict__ slot) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int32_t c = ~(int32_t)k, p = leaf_parent[k];
+  uint32_t s = 0;
+  while (p >= 0) {
+    if (right[p] == c) s += leaves_of(ncount, left[p]);
+    c = p;
+    p = parent[p];
+  }
+  slot[k] = s;
+}
+
+// Leaf ranges of the interior nodes (each node's range starts at its
+// leftmost leaf), the leaves in slot order, and leaf children renamed to
+// their slots.
+__global__ void k_ploc_ranges(const int32_t *__restrict__ left, const int32_t *__restrict__ parent,
+                              const int32_t *__restrict__ leaf_parent, const uint32_t *__restrict__ ncount,
+                              const uint32_t *__restrict__ slot, const uint32_t *__restrict__ order,
+                              const float4 *__restrict__ leafbox, uint32_t n, uint32_t *__restrict__ first,
+                              uint32_t *__restrict__ last, uint32_t *__restrict__ order_out,
+                              float4 *__restrict__ leafbox_out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t s = slot[k];
+  order_out[s] = order[k];
+  leafbox_out[2 * (size_t)s] = leafbox[2 * (size_t)k];
+  leafbox_out[2 * (size_t)s + 1] = leafbox[2 * (size_t)k + 1];
+  int32_t c = ~(int32_t)k, p = leaf_parent[k];
+  while (p >= 0 && left[p] == c) {
+    first[p] = s;
+    last[p] = s + ncount[p] - 1u;
+    c = p;
+    p = parent[p];
+  }
+}
+
+__global__ void k_ploc_rename(int32_t *__restrict__ left, int32_t *__restrict__ right, const uint32_t *__restrict__ slot,
+                              uint32_t ni) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ni) return;
+  const int32_t l = left[p], r = right[p];
+  if (l < 0) left[p] = ~(int32_t)slot[~l];
+  if (r < 0) right[p] = ~(int32_t)slot[~r];
+}
+
 // Intersection triangles in leaf (sorted) order: a with the original id in
 // w, e1 = b - a, e2 = c - a (the host builder's rounding).
 __global__ void k_itris(const float4 *__restrict__ tri, uint32_t stride, uint32_t n, const uint32_t *__restrict__ order,
@@ -286,8 +513,9 @@ inline uint32_t blocks(size_t n) { return (uint32_t)std::max<size_t>(1, (n + kBl
     }                                                                 \
   } while (0)
 
-bool lbvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, const float clo[3], const float chi[3],
-                int max_leaf, WideNode *nodes, float4 *itris, LbvhResult &res, std::string &err) {
+bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, const float clo[3],
+                      const float chi[3], int max_leaf, DeviceBvhMethod method, int radius, WideNode *nodes,
+                      float4 *itris, LbvhResult &res, std::string &err) {
   std::vector<void *> mem;
   auto alloc = [&](size_t bytes) -> void * {
     void *p = nullptr;
@@ -296,11 +524,17 @@ bool lbvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, c
     return p;
   };
   if (n == 0 || max_leaf < 1 || max_leaf > kMaxLeafTris) {
-    err = "lbvh: n > 0 and 1 <= max_leaf <= 15 required";
+    err = "device bvh: n > 0 and 1 <= max_leaf <= 15 required";
     return false;
   }
+  if (method == DeviceBvhMethod::kPloc && (radius < 1 || radius > kPlocMaxRadius)) {
+    err = "device bvh: PLOC radius must be in 1.." + std::to_string(kPlocMaxRadius);
+    return false;
+  }
+  const bool ploc = method == DeviceBvhMethod::kPloc;
   const size_t ni = n > 1 ? n - 1 : 1;
-  uint32_t *keys = (uint32_t *)alloc(4 * (size_t)n), *keys2 = (uint32_t *)alloc(4 * (size_t)n);
+  // 30-bit keys for the radix tree, 63-bit keys for the clustering
+  uint32_t *keys = (uint32_t *)alloc(8 * (size_t)n), *keys2 = (uint32_t *)alloc(8 * (size_t)n);
   uint32_t *vals = (uint32_t *)alloc(4 * (size_t)n), *vals2 = (uint32_t *)alloc(4 * (size_t)n);
   int32_t *left = (int32_t *)alloc(4 * ni), *right = (int32_t *)alloc(4 * ni), *parent = (int32_t *)alloc(4 * ni);
   int32_t *leaf_parent = (int32_t *)alloc(4 * (size_t)n);
@@ -308,9 +542,28 @@ bool lbvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, c
   float *leafbox = (float *)alloc(32 * (size_t)n), *nodebox = (float *)alloc(32 * ni);
   int32_t *front[2] = {(int32_t *)alloc(4 * ni), (int32_t *)alloc(4 * ni)};
   uint32_t *cnt = (uint32_t *)alloc(16);
+  // PLOC: clusters (double-buffered), their boxes, neighbours, round flags
+  // and their scan, node leaf counts, leaf slots, the reordered leaves.
+  int32_t *cl[2] = {nullptr, nullptr}, *nn = nullptr;
+  float4 *cbox[2] = {nullptr, nullptr}, *leafbox2 = nullptr;
+  uint64_t *flags = nullptr, *pre = nullptr;
+  uint32_t *ncount = nullptr, *slot = nullptr, *order2 = nullptr;
+  if (ploc) {
+    cl[0] = (int32_t *)alloc(4 * (size_t)n);
+    cl[1] = (int32_t *)alloc(4 * (size_t)n);
+    cbox[0] = (float4 *)alloc(32 * (size_t)n);
+    cbox[1] = (float4 *)alloc(32 * (size_t)n);
+    nn = (int32_t *)alloc(4 * (size_t)n);
+    flags = (uint64_t *)alloc(8 * (size_t)n);
+    pre = (uint64_t *)alloc(8 * (size_t)n);
+    ncount = (uint32_t *)alloc(4 * ni);
+    slot = (uint32_t *)alloc(4 * (size_t)n);
+    order2 = (uint32_t *)alloc(4 * (size_t)n);
+    leafbox2 = (float4 *)alloc(32 * (size_t)n);
+  }
   for (void *p : mem)
     if (!p) {
-      err = "lbvh: out of device memory";
+      err = "device bvh: out of device memory";
       for (void *q : mem) (void)hipFree(q);
       return false;
     }
@@ -319,27 +572,77 @@ bool lbvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, c
   sc.x = chi[0] > clo[0] ? 1.0f / (chi[0] - clo[0]) : 0.0f;
   sc.y = chi[1] > clo[1] ? 1.0f / (chi[1] - clo[1]) : 0.0f;
   sc.z = chi[2] > clo[2] ? 1.0f / (chi[2] - clo[2]) : 0.0f;
-  hipLaunchKernelGGL(k_morton, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, lo, sc, keys, vals);
+  uint64_t *keys64 = (uint64_t *)keys, *keys64_2 = (uint64_t *)keys2;
+  if (ploc) hipLaunchKernelGGL(k_morton64, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, lo, sc, keys64, vals);
+  else hipLaunchKernelGGL(k_morton, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, lo, sc, keys, vals);
   GI_CK(hipGetLastError());
-  size_t temp_bytes = 0;
-  GI_CK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, s));
-  void *temp = alloc(temp_bytes);
+  size_t temp_bytes = 0, scan_bytes = 0;
+  if (ploc)
+    GI_CK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys64, keys64_2, vals, vals2, (int)n, 0, 63, s));
+  else GI_CK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, s));
+  if (ploc) GI_CK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags, pre, (int)n, s));
+  void *temp = alloc(std::max(temp_bytes, scan_bytes));
   if (!temp) {
-    err = "lbvh: out of device memory (sort)";
+    err = "device bvh: out of device memory (sort)";
     for (void *q : mem) (void)hipFree(q);
     return false;
   }
-  GI_CK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, s));
-  hipLaunchKernelGGL(k_zero, dim3(blocks(ni)), dim3(kBlock), 0, s, visit, ni);
-  if (n > 1) {
-    hipLaunchKernelGGL(k_karras, dim3(blocks(ni)), dim3(kBlock), 0, s, keys2, (int)n, left, right, parent,
-                       leaf_parent, first, last);
+  if (ploc) GI_CK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys64, keys64_2, vals, vals2, (int)n, 0, 63, s));
+  else GI_CK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, s));
+  const uint32_t *order = vals2;
+  if (!ploc) {
+    hipLaunchKernelGGL(k_zero, dim3(blocks(ni)), dim3(kBlock), 0, s, visit, ni);
+    if (n > 1) {
+      hipLaunchKernelGGL(k_karras, dim3(blocks(ni)), dim3(kBlock), 0, s, keys2, (int)n, left, right, parent,
+                         leaf_parent, first, last);
+      GI_CK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_refit, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, (int)n, vals2, left, right, parent,
+                       leaf_parent, leafbox, nodebox, visit);
     GI_CK(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(k_ploc_init, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, vals2, (float4 *)leafbox,
+                       cbox[0], cl[0]);
+    GI_CK(hipGetLastError());
+    GI_CK(hipMemsetAsync(parent, 0xFF, 4 * ni, s));  // the root keeps parent -1
+    uint32_t count = n, merged = 0;
+    int rounds = 0, cur = 0;
+    while (count > 1) {
+      hipLaunchKernelGGL(k_ploc_nn, dim3(blocks(count)), dim3(kBlock), 0, s, cbox[cur], count, radius, nn);
+      hipLaunchKernelGGL(k_ploc_flags, dim3(blocks(count)), dim3(kBlock), 0, s, nn, count, flags);
+      GI_CK(hipGetLastError());
+      GI_CK(hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, flags, pre, (int)count, s));
+      hipLaunchKernelGGL(k_ploc_merge, dim3(blocks(count)), dim3(kBlock), 0, s, cl[cur], cbox[cur], nn, flags, pre,
+                         count, merged, left, right, parent, leaf_parent, ncount, (float4 *)nodebox, cl[cur ^ 1],
+                         cbox[cur ^ 1], cnt);
+      GI_CK(hipGetLastError());
+      uint32_t tot[2] = {0, 0};
+      GI_CK(hipMemcpyAsync(tot, cnt, 8, hipMemcpyDeviceToHost, s));
+      GI_CK(hipStreamSynchronize(s));
+      // every round merges at least the globally closest pair
+      if (tot[1] == 0 || tot[0] + tot[1] != count || merged + tot[1] > n - 1) {
+        err = "device bvh: PLOC clustering stalled (non-finite vertex coordinates?)";
+        for (void *q : mem) (void)hipFree(q);
+        return false;
+      }
+      merged += tot[1];
+      count = tot[0];
+      cur ^= 1;
+      ++rounds;
+    }
+    res.rounds = rounds;
+    if (n > 1) {
+      hipLaunchKernelGGL(k_ploc_slot, dim3(blocks(n)), dim3(kBlock), 0, s, left, right, parent, leaf_parent, ncount, n,
+                         slot);
+      hipLaunchKernelGGL(k_ploc_ranges, dim3(blocks(n)), dim3(kBlock), 0, s, left, parent, leaf_parent, ncount, slot,
+                         vals2, (const float4 *)leafbox, n, first, last, order2, leafbox2);
+      hipLaunchKernelGGL(k_ploc_rename, dim3(blocks(n - 1)), dim3(kBlock), 0, s, left, right, slot, n - 1);
+      GI_CK(hipGetLastError());
+      order = order2;
+      leafbox = (float *)leafbox2;
+    }
   }
-  hipLaunchKernelGGL(k_refit, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, (int)n, vals2, left, right, parent,
-                     leaf_parent, leafbox, nodebox, visit);
-  GI_CK(hipGetLastError());
-  hipLaunchKernelGGL(k_itris, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, vals2, itris);
+  hipLaunchKernelGGL(k_itris, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, order, itris);
   GI_CK(hipGetLastError());
   if (n == 1) {
     // one triangle: a root with a single leaf child
@@ -369,7 +672,8 @@ bool lbvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, c
     return true;
   }
   CollapseArgs A{left, right, first, last, leafbox, nodebox, max_leaf};
-  const int32_t root = 0;
+  // the radix tree's root is node 0; PLOC's is the last merge, n - 2
+  const int32_t root = ploc ? (int32_t)(n - 2) : 0;
   GI_CK(hipMemcpyAsync(front[0], &root, 4, hipMemcpyHostToDevice, s));
   uint32_t count = 1, base = 0, next_base = 1;
   int levels = 0;

@@ -2396,9 +2396,16 @@ int gi_build_bvh(gi_ctx *ctx, int32_t max_leaf) {
 }
 
 int gi_build_bvh_device(gi_ctx *ctx, int32_t max_leaf) {
+  return gi_build_bvh_device_ex(ctx, max_leaf, GI_BVH_LBVH, 0);
+}
+
+int gi_build_bvh_device_ex(gi_ctx *ctx, int32_t max_leaf, int32_t method, int32_t radius) {
   if (!ctx || ctx->tris.empty()) return fail(ctx, GI_ERR_STATE, "upload triangles first");
   const int ml = max_leaf > 0 ? max_leaf : 4;
   if (ml > kMaxLeafTris) return fail(ctx, GI_ERR_INVALID_ARG, "max_leaf must be at most 15");
+  if (method != GI_BVH_LBVH && method != GI_BVH_PLOC) return fail(ctx, GI_ERR_INVALID_ARG, "unknown BVH method");
+  const int rad = radius == 0 ? 16 : radius;
+  if (method == GI_BVH_PLOC && (rad < 1 || rad > 64)) return fail(ctx, GI_ERR_INVALID_ARG, "PLOC radius must be in 1..64");
   GI_HIP(ctx, hipSetDevice(ctx->device));
   const uint32_t n = (uint32_t)ctx->ntri;
   GI_HIP(ctx, ctx->d_nodes.ensure(std::max<size_t>(1, n) * sizeof(WideNode)));
@@ -2407,8 +2414,9 @@ int gi_build_bvh_device(gi_ctx *ctx, int32_t max_leaf) {
   ctx->have_bvh = false;
   LbvhResult r;
   std::string err;
-  if (!lbvh_build(ctx->stream, ctx->d_stris.as<float4>(), (uint32_t)(sizeof(ShadeTri) / sizeof(float4)), n, ctx->clo,
-                  ctx->chi, ml, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), r, err))
+  if (!device_bvh_build(ctx->stream, ctx->d_stris.as<float4>(), (uint32_t)(sizeof(ShadeTri) / sizeof(float4)), n,
+                        ctx->clo, ctx->chi, ml, method == GI_BVH_PLOC ? DeviceBvhMethod::kPloc : DeviceBvhMethod::kLbvh,
+                        rad, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), r, err))
     return fail(ctx, GI_ERR_HIP, err);
   return finish_bvh(ctx, r.nnodes, r.depth);
 }

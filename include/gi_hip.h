@@ -219,6 +219,17 @@ int gi_build_bvh(gi_ctx *ctx, int32_t max_leaf);
  * leaves of at most max_leaf (<= 15) triangles.  Boxes are padded as the host
  * builder pads them, so closest-hit results are the same with either tree. */
 int gi_build_bvh_device(gi_ctx *ctx, int32_t max_leaf);
+#define GI_BVH_LBVH 0 /* radix tree over the Morton keys (gi_build_bvh_device) */
+#define GI_BVH_PLOC 1 /* locally-ordered clustering over the Morton order      */
+/* gi_build_bvh_device with a choice of binary tree before the collapse.
+ * GI_BVH_PLOC merges mutual nearest neighbours (smallest union surface
+ * area) within `radius` (1..64, 0 = 16) positions of the Morton order,
+ * round by round (Meister and Bittner 2018); its trees trace close to the
+ * host SAH tree at device build speed.  Same node format, padding and
+ * closest-hit results as the other builders.  The reference builds its
+ * GPU trees with rta's LBVH (src/main.cpp:218-225); this method has no
+ * counterpart there. */
+int gi_build_bvh_device_ex(gi_ctx *ctx, int32_t max_leaf, int32_t method, int32_t radius);
 /* Upload an externally built binary BVH in the reference's 2 x float4 layout
  * (src/tracers.cu:26-36): nodes[8*n_nodes], indices[n_indices]. */
 int gi_upload_bvh2(gi_ctx *ctx, const float *nodes, int64_t n_nodes,

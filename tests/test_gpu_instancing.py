@@ -45,7 +45,7 @@ def _rays(rng, n, size):
     return o, d
 
 
-@pytest.mark.parametrize("builder", ["host", "device"])
+@pytest.mark.parametrize("builder", ["host", "device", "ploc"])
 def test_closest_and_any_parity(builder, inst, oracle, gpu_ctx_factory):
     rng = np.random.default_rng(21)
     o, d = _rays(rng, 200_000, 40.0)

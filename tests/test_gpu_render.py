@@ -252,10 +252,11 @@ def test_obj_scene_with_map_kd_texture_bit_exact(oracle, gpu_ctx_factory, tmp_pa
         assert np.abs(ctx.render().image(0).reshape(-1, 3) - img).max() > 0.01
 
 
-def test_device_bvh_render_bit_exact(oracle, gpu_ctx_factory):
+@pytest.mark.parametrize("builder", ["device", "ploc"])
+def test_device_bvh_render_bit_exact(builder, oracle, gpu_ctx_factory):
     sc = scenes.terrain_spheres(cells=(64, 64), n_spheres=100, w=80, h=45, spp=4)
     ref, st = _oracle(oracle, sc)
-    with gpu_ctx_factory(sc, builder="device") as ctx:
+    with gpu_ctx_factory(sc, builder=builder) as ctx:
         img = ctx.render().image(0).reshape(-1, 3)
         c = ctx.counters()
     _same(img, ref)

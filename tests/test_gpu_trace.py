@@ -133,12 +133,14 @@ def test_call_order_and_argument_errors(c1, gpu_ctx_factory):
         assert e.value.status == 1
 
 
-# ---- device-built BVH (linear BVH, refit, 4-wide collapse) -----------------
+# ---- device-built BVH (linear BVH or PLOC, 4-wide collapse) ----------------
 # Closest hit is the minimum (t, id) over all triangles, so the hits do not
 # depend on the tree: the device-built tree must give the oracle's answers.
 
-@pytest.mark.parametrize("max_leaf", [1, 4, 15])
-def test_device_bvh_closest_and_any(max_leaf, terrain, oracle, gpu_ctx_factory):
+@pytest.mark.parametrize("builder,max_leaf,radius", [("device", 1, 16), ("device", 4, 16), ("device", 15, 16),
+                                                     ("ploc", 1, 16), ("ploc", 4, 16), ("ploc", 15, 16),
+                                                     ("ploc", 4, 1), ("ploc", 4, 64)])
+def test_device_bvh_closest_and_any(builder, max_leaf, radius, terrain, oracle, gpu_ctx_factory):
     sc = terrain
     rng = np.random.default_rng(5 + max_leaf)
     o, d = _rays(rng, 200_000, -5, 105, 0, 20)
@@ -146,13 +148,27 @@ def test_device_bvh_closest_and_any(max_leaf, terrain, oracle, gpu_ctx_factory):
     osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights)
     ref = osc.trace_closest(o, d, tmax)
     occ = osc.trace_any(o, d, tmax)
-    with gpu_ctx_factory(sc, builder="device", max_leaf=max_leaf) as ctx:
+    with gpu_ctx_factory(sc, builder=builder, max_leaf=max_leaf, radius=radius) as ctx:
         _assert_hits_equal(ctx.trace_closest(o, d, tmax), ref, osc, o, d, tmax)
         assert np.array_equal(ctx.trace_any(o, d, tmax), occ)
 
 
+def test_ploc_argument_errors(c1, gpu_ctx_factory):
+    with gpu_ctx_factory(c1) as ctx:
+        for method, radius in (("ploc", 65), ("ploc", -1)):
+            with pytest.raises(GiError) as e:
+                ctx.build_bvh_device(4, method, radius)
+            assert e.value.status == 1
+        with pytest.raises(GiError) as e:
+            ctx._ck(ctx.lib.gi_build_bvh_device_ex(ctx.h, 4, 2, 16))
+        assert e.value.status == 1
+        with pytest.raises(ValueError):
+            ctx.build_bvh_device(4, "sah")
+
+
+@pytest.mark.parametrize("builder", ["device", "ploc"])
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 17])
-def test_device_bvh_tiny_and_degenerate(n, oracle, gpu_ctx_factory):
+def test_device_bvh_tiny_and_degenerate(n, builder, oracle, gpu_ctx_factory):
     rng = np.random.default_rng(n)
     sc = scenes.cornell_boxes(w=8, h=8, spp=1)
     tris = sc.tris[:n].copy() if n <= len(sc.tris) else np.concatenate([sc.tris] * 2)[:n].copy()
@@ -169,5 +185,5 @@ def test_device_bvh_tiny_and_degenerate(n, oracle, gpu_ctx_factory):
     osc = oracle.OracleScene(tris, sc.mats, sc.lights)
     ref = osc.trace_closest(o, d, t)
     one = scenes.Scene(tris, sc.mats, sc.lights, sc.views, sc.opts)
-    with gpu_ctx_factory(one, builder="device") as ctx:
+    with gpu_ctx_factory(one, builder=builder) as ctx:
         _assert_hits_equal(ctx.trace_closest(o, d, t), ref, osc, o, d, t)
