@@ -10,9 +10,13 @@ renders, in Mrays/s.  roofline: the closest-hit traversal kernel's
 algorithmic bytes (per ray 32 B in + 16 B out + 32 B per child box + 40 B
 per triangle test, with the box / triangle test counts of the reference's
 binary-BVH traversal rules measured by the oracle on the same BVH) over its
-HIP-event time, against 8 TB/s HBM.  cpu_baseline: the oracle (a C++
+HIP-event time, against 8 TB/s HBM, and beside it against the read
+bandwidth of L2, Infinity Cache and HBM measured on the same GPU by a
+streaming-read kernel (libgi_membench.so).  cpu_baseline: the oracle (a C++
 restatement of the reference's CPU path tracer, OpenMP) on a bounded sample
-of the same workload.
+of the same workload.  Further legs: the small 34-triangle configuration
+(with RMSE against its 1024-spp image), 16-view batching, the adjoint, BVH
+construction and the large instanced configuration.
 """
 from __future__ import annotations
 
@@ -36,7 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # reads for the 512 MiB device copy (FETCH 262,168 KiB -> x2.000, WRITE
 # 524,288 KiB -> x1.000).  FETCH counts L2 misses to the fabric, so
 # Infinity Cache hits are included: an upper bound of the HBM bytes.
-TRACE_CLOSEST_TRAFFIC_BYTES = (3083962.2 * 1.99982 + 507187.8 * 1.0) * 1024
+TRACE_CLOSEST_FETCH_BYTES = 3083962.2 * 1.99982 * 1024
+TRACE_CLOSEST_TRAFFIC_BYTES = TRACE_CLOSEST_FETCH_BYTES + 507187.8 * 1.0 * 1024
 
 
 def parse():
@@ -51,10 +56,101 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     p.add_argument("--no-bvh", action="store_true", help="skip the BVH construction leg")
     p.add_argument("--no-instancing", action="store_true", help="skip the two-level instancing leg")
+    p.add_argument("--no-small", action="store_true", help="skip the small (34-triangle) configuration leg")
+    p.add_argument("--no-membench", action="store_true", help="skip the memory read-bandwidth measurement")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (untimed): the known-size "
                         "kernel that scales FETCH_SIZE / WRITE_SIZE counter runs")
     return p.parse_args()
+
+
+def read_bandwidth():
+    """Measured read bandwidth (GB/s) of each level of the memory hierarchy:
+    the streaming-read kernel of libgi_membench.so over 16 MiB (2 MiB per
+    XCD: L2), 128 MiB (16 MiB per XCD: Infinity Cache) and 4 GiB (HBM)."""
+    import ctypes
+
+    path = os.path.join(ROOT, "globillum_amd", "libgi_membench.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (python -m globillum_amd.build)")
+    lib = ctypes.CDLL(path)
+    lib.gi_membench_read.restype = ctypes.c_int
+    lib.gi_membench_read.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for name, mib, passes in (("l2", 16, 5), ("infinity_cache", 128, 5), ("hbm", 4096, 3)):
+        g = ctypes.c_double(0.0)
+        rc = lib.gi_membench_read(0, mib << 20, passes, ctypes.byref(g))
+        if rc != 0:
+            raise RuntimeError(f"gi_membench_read({mib} MiB) failed with HIP error {rc}")
+        out[name] = round(g.value, 1)
+    print(f"[bench] measured read bandwidth GB/s: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def small_leg(steps: int, with_cpu: bool, bw: dict | None):
+    """The small plain configuration (cornell_boxes: 5 walls and 2 boxes, 34
+    triangles, 1 rect light of emission 10, Lambert albedo U[0.2, 0.8],
+    256x256, 16 spp, depth 4), rendered as one hipGraph: Mrays/s,
+    samples/s, the relative RMSE of its image against the same scene at
+    1024 spp, and the closest-hit kernel's algorithmic bytes (box and
+    triangle test counts from the oracle on the same BVH, which runs the
+    whole configuration as the CPU baseline) against the measured L2 read
+    bandwidth -- the scene's BVH and triangles take a few KB -- and
+    against the HBM peak."""
+    import numpy as np
+
+    from globillum_amd import scenes
+    from globillum_amd._lib import FLAG_BATCHED, FLAG_GRAPH
+    from globillum_amd.context import Context, build_bvh2
+
+    sc = scenes.config_scene(1)
+    bvh2 = build_bvh2(sc.tris)
+    flags = FLAG_BATCHED | FLAG_GRAPH
+    tot = {"rays_closest": 0, "rays_shadow": 0, "samples": 0}
+    kern = {"rays_closest": 0, "ms_trace_closest": 0.0, "launches_closest": 0}
+    with Context(0) as ctx:
+        ctx.upload(sc, bvh2=bvh2)
+        for _ in range(3):
+            ctx.render(flags=flags)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.render(flags=flags)
+            c = ctx.counters()
+            for k in tot:
+                tot[k] += c[k]
+        dt = time.perf_counter() - t0
+        # kernel times come from renders outside a graph (events per launch)
+        for _ in range(steps):
+            ctx.render(flags=FLAG_BATCHED)
+            c = ctx.counters()
+            for k in kern:
+                kern[k] += c[k]
+        img = ctx.image(0).astype(np.float64)
+        v = sc.views.copy()
+        v["spp"] = 1024
+        ctx.render(v, sc.opts, flags=FLAG_BATCHED)
+        ref = ctx.image(0).astype(np.float64)
+    rays = tot["rays_closest"] + tot["rays_shadow"]
+    out = {"workload": "cornell_boxes: 5 walls + 2 boxes (34 triangles), 1 rect light, Lambert, 256x256, 16 spp, "
+                       "depth 4 (the small plain configuration), batched + hipGraph",
+           "mrays_per_s": round(rays / dt / 1e6, 2), "samples_per_s": round(tot["samples"] / dt, 1),
+           "ms_per_frame": round(dt / steps * 1e3, 3),
+           "rmse_rel_vs_1024spp": round(float(np.sqrt(((img - ref) ** 2).mean()) / np.sqrt((ref ** 2).mean())), 5)}
+    if with_cpu:
+        base, per_ray = cpu_baseline(sc, bvh2, full=True)
+        out["cpu_baseline"] = base
+        if kern["launches_closest"] > 0 and kern["ms_trace_closest"] > 0:
+            b_ray = 32 + 16 + 32 * per_ray["box_closest"] + 40 * per_ray["tri_closest"]
+            avg_s = kern["ms_trace_closest"] / kern["launches_closest"] / 1e3
+            achieved = kern["rays_closest"] / kern["launches_closest"] * b_ray / avg_s / 1e9
+            roof = {"kernel": "k_trace_closest", "achieved": round(achieved, 1), "unit": "GB/s",
+                    "algorithmic_bytes_per_ray": round(b_ray, 1), "avg_launch_ms": round(avg_s * 1e3, 4),
+                    "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4)}
+            if bw:
+                roof["measured_l2_read"] = bw["l2"]
+                roof["frac_of_measured_l2"] = round(achieved / bw["l2"], 4)
+            out["roofline"] = roof
+    return out
 
 
 def batching_leg(steps: int, tune: bool):
@@ -274,9 +370,10 @@ def instancing_leg(steps: int):
     return out
 
 
-def cpu_baseline(scene, bvh2):
-    """Oracle on 1/4 of the samples: same scene, camera, depth and BVH at
-    640x360, 32 spp.  Returns the baseline record and the per-ray traversal
+def cpu_baseline(scene, bvh2, full: bool = False):
+    """Oracle on 1/4 of the samples (same scene, camera, depth and BVH at
+    640x360, 32 spp), or on the whole workload (full=True, for the small
+    configuration).  Returns the baseline record and the per-ray traversal
     statistics (box / triangle tests per closest and shadow ray)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -284,14 +381,16 @@ def cpu_baseline(scene, bvh2):
     threads = min(16, os.cpu_count() or 1)
     osc = pyoracle.OracleScene(scene.tris, scene.mats, scene.lights, bvh2=bvh2)
     v = scene.views.copy()
-    v["w"], v["h"], v["spp"] = 640, 360, 32
+    if not full:
+        v["w"], v["h"], v["spp"] = 640, 360, 32
     t0 = time.perf_counter()
     _, st = osc.render(v, scene.opts, threads=threads)
     dt = time.perf_counter() - t0
     rays = st["rays_closest"] + st["rays_shadow"]
+    what = (f"the whole workload ({int(v['w'][0])}x{int(v['h'][0])}, {int(v['spp'][0])} spp)" if full else
+            "same scene/camera/depth/BVH at 640x360, 32 spp (1/4 of the samples)")
     rec = {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-           "sample": f"same scene/camera/depth/BVH at 640x360, 32 spp (1/4 of the samples): {rays} rays "
-                     f"in {dt:.2f} s on {threads} threads",
+           "sample": f"{what}: {rays} rays in {dt:.2f} s on {threads} threads",
            "samples_per_s": round(st["samples"] / dt, 1)}
     per_ray = {"box_closest": st["box_closest"] / max(1, st["rays_closest"]),
                "tri_closest": st["tri_closest"] / max(1, st["rays_closest"]),
@@ -344,6 +443,7 @@ def main():
     value = rays / dt / 1e6
 
     base, per_ray = (None, None) if a.no_cpu else cpu_baseline(scene, bvh2)
+    bw = None if a.no_membench else read_bandwidth()
     roof = None
     if per_ray is not None and tot["launches_closest"] > 0 and tot["ms_trace_closest"] > 0:
         b_ray = 32 + 16 + 32 * per_ray["box_closest"] + 40 * per_ray["tri_closest"]
@@ -355,6 +455,17 @@ def main():
                 "traffic": round(TRACE_CLOSEST_TRAFFIC_BYTES) if tot["launches_closest"] == 8 * a.steps else None,
                 "kernel": "k_trace_closest", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                 "algorithmic_bytes_per_ray": round(b_ray, 1)}
+        if bw:
+            # the BVH and triangles (~27 MB) live in the Infinity Cache
+            roof["measured_read_gbs"] = bw
+            roof["frac_of_measured_infinity_cache"] = round(achieved / bw["infinity_cache"], 4)
+            roof["frac_of_measured_l2"] = round(achieved / bw["l2"], 4)
+            if roof["traffic"] is not None:
+                # time bound of the measured hierarchy: the bytes fetched past
+                # L2 at the Infinity Cache read rate, the rest at the L2 rate
+                t_bound = ((bytes_per_launch - TRACE_CLOSEST_FETCH_BYTES) / (bw["l2"] * 1e9) +
+                           TRACE_CLOSEST_FETCH_BYTES / (bw["infinity_cache"] * 1e9))
+                roof["frac_of_measured_hierarchy"] = round(t_bound / avg_launch_s, 4)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -382,6 +493,8 @@ def main():
         "cpu_baseline": base,
     }
     ctx.close()
+    if not a.no_small:
+        out["small"] = small_leg(20, not a.no_cpu, bw)
     if not a.no_batching:
         out["batching"] = batching_leg(max(1, min(a.steps, 3)), not a.no_tune)
     if not a.no_adjoint:

@@ -1,5 +1,6 @@
 """Build step of the package: compiles the gfx950 HIP library libgi_hip.so
-in-tree with hipcc (no JIT cache), so the built file travels with the tree.
+(and the read-bandwidth benchmark libgi_membench.so) in-tree with hipcc (no
+JIT cache), so the built files travel with the tree.
 
     python -m globillum_amd.build
 """
@@ -14,6 +15,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libgi_hip.so")
 SOURCES = ["gi_kernels.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene_io.cpp", "image_io.cpp"]
+# the memory-hierarchy read benchmark used by bench.py (not part of the path)
+MEMBENCH = os.path.join(PKG, "libgi_membench.so")
 
 # -ffp-contract=off: hit ids, sample directions and image values are
 # reproduced bit-for-bit on the host, so no a*b+c may be fused.
@@ -73,6 +76,13 @@ def build(verbose: bool = False) -> str:
     if verbose and r.stderr.strip():
         sys.stderr.write(r.stderr)
     os.replace(tmp, LIB)
+    tmp = MEMBENCH + ".tmp"
+    r = subprocess.run([hipcc(), *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "membench.hip")],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libgi_membench.so")
+    os.replace(tmp, MEMBENCH)
     build_host(verbose)
     return LIB
 

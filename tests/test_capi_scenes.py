@@ -28,6 +28,20 @@ def test_every_declared_symbol_is_exported():
     assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
 
 
+def test_read_benchmark_library_loads():
+    """bench.py's memory read-bandwidth kernel: built beside the library,
+    exports its one entry point, and refuses bad sizes before touching a
+    device."""
+    lib = C.CDLL(os.path.join(ROOT, "globillum_amd", "libgi_membench.so"))
+    f = lib.gi_membench_read
+    f.restype = C.c_int
+    f.argtypes = [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+    g = C.c_double(0.0)
+    for nbytes, passes in ((1024, 1), (16 * 1024 + 8, 1), (1 << 20, 0)):
+        assert f(0, nbytes, passes, C.byref(g)) != 0
+    assert f(0, 1 << 20, 1, None) != 0
+
+
 def test_version_and_status_strings():
     lib = _lib.load()
     assert lib.gi_abi_version() == 1
