@@ -1,7 +1,8 @@
 // Device BVH build.  The binary tree is either a linear BVH (Karras 2012,
 // "Maximizing parallelism in the construction of BVHs, octrees and k-d
-// trees") over 30-bit Morton codes of the triangle centroids with a
-// bottom-up box refit, or a PLOC clustering over the 63-bit Morton order
+// trees") over 30-bit Morton codes of the triangle centroids with a box
+// refit level by level from the deepest, or a PLOC clustering over the
+// 63-bit Morton order
 // (below).  Either is then collapsed level by level into the traversal
 // kernels' 4-wide nodes (children picked by opening the
 // child of largest surface area, subtrees of at most max_leaf triangles
@@ -136,24 +137,6 @@ struct Box {
   float lo[3], hi[3];
 };
 
-__device__ __forceinline__ float relaxed_load(const float *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void load_box(const float *p, Box &b) {
-  for (int k = 0; k < 3; ++k) {
-    b.lo[k] = relaxed_load(p + k);
-    b.hi[k] = relaxed_load(p + 4 + k);
-  }
-}
-__device__ __forceinline__ void store_box(float *p, const Box &b) {
-  for (int k = 0; k < 3; ++k) {
-    __hip_atomic_store(p + k, b.lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(p + 4 + k, b.hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// Leaf boxes, then every interior box from the second of its two children
-// to finish (release / acquire on the node's arrival counter).
 // The padded box of triangle t (the host builder's padding).
 __device__ __forceinline__ Box leaf_box(const float4 *__restrict__ tri, uint32_t stride, size_t t) {
   const float4 a = tri[t * stride], b = tri[t * stride + 1], c = tri[t * stride + 2];
@@ -174,31 +157,6 @@ __device__ __forceinline__ Box leaf_box(const float4 *__restrict__ tri, uint32_t
   return bx;
 }
 
-__global__ void k_refit(const float4 *__restrict__ tri, uint32_t stride, int n, const uint32_t *__restrict__ order,
-                        const int32_t *__restrict__ left, const int32_t *__restrict__ right,
-                        const int32_t *__restrict__ parent, const int32_t *__restrict__ leaf_parent,
-                        float *leafbox, float *nodebox, uint32_t *visit) {
-  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (k >= n) return;
-  Box bx = leaf_box(tri, stride, order[k]);
-  store_box(leafbox + 8 * (size_t)k, bx);
-  int p = n > 1 ? leaf_parent[k] : -1;
-  while (p >= 0) {
-    const uint32_t old = __hip_atomic_fetch_add(visit + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;  // the sibling subtree is not done yet: it continues
-    Box l, r;
-    const int32_t lc = left[p], rc = right[p];
-    load_box(lc < 0 ? leafbox + 8 * (size_t)~lc : nodebox + 8 * (size_t)lc, l);
-    load_box(rc < 0 ? leafbox + 8 * (size_t)~rc : nodebox + 8 * (size_t)rc, r);
-    for (int q = 0; q < 3; ++q) {
-      bx.lo[q] = fminf(l.lo[q], r.lo[q]);
-      bx.hi[q] = fmaxf(l.hi[q], r.hi[q]);
-    }
-    store_box(nodebox + 8 * (size_t)p, bx);
-    p = parent[p];
-  }
-}
-
 __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
   const uint64_t mask = __ballot(pred);
   if (mask == 0) return 0;
@@ -208,6 +166,47 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
   if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
   base = (uint32_t)__shfl((int)base, leader);
   return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Leaf boxes in sorted order.
+__global__ void k_leaf_boxes(const float4 *__restrict__ tri, uint32_t stride, uint32_t n,
+                             const uint32_t *__restrict__ order, float4 *__restrict__ leafbox) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const Box b = leaf_box(tri, stride, order[k]);
+  leafbox[2 * (size_t)k] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f);
+  leafbox[2 * (size_t)k + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
+}
+
+// One breadth-first level of the radix tree: the interior children of the
+// level's nodes, appended to the next level.
+__global__ void k_tree_level(const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                             const int32_t *__restrict__ level, uint32_t count, int32_t *__restrict__ next,
+                             uint32_t *next_count) {
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < count; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;  // uniform per block: wave_append needs every lane
+    const int32_t p = i < count ? level[i] : 0;
+    const int32_t l = i < count ? left[p] : -1, r = i < count ? right[p] : -1;
+    const uint32_t ql = wave_append(l >= 0, next_count);
+    if (l >= 0) next[ql] = l;
+    const uint32_t qr = wave_append(r >= 0, next_count);
+    if (r >= 0) next[qr] = r;
+  }
+}
+
+// Boxes of one level's nodes from their children's (the deeper levels are
+// done by earlier launches).
+__global__ void k_refit_level(const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                              const int32_t *__restrict__ level, uint32_t count, const float4 *__restrict__ leafbox,
+                              float4 *__restrict__ nodebox) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int32_t p = level[i], l = left[p], r = right[p];
+  const float4 *bl = l < 0 ? leafbox + 2 * (size_t)~l : nodebox + 2 * (size_t)l;
+  const float4 *br = r < 0 ? leafbox + 2 * (size_t)~r : nodebox + 2 * (size_t)r;
+  const float4 al = bl[0], ah = bl[1], cl = br[0], ch = br[1];
+  nodebox[2 * (size_t)p] = make_float4(fminf(al.x, cl.x), fminf(al.y, cl.y), fminf(al.z, cl.z), 0.0f);
+  nodebox[2 * (size_t)p + 1] = make_float4(fmaxf(ah.x, ch.x), fmaxf(ah.y, ch.y), fmaxf(ah.z, ch.z), 0.0f);
 }
 
 struct CollapseArgs {
@@ -538,7 +537,8 @@ bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_
   uint32_t *vals = (uint32_t *)alloc(4 * (size_t)n), *vals2 = (uint32_t *)alloc(4 * (size_t)n);
   int32_t *left = (int32_t *)alloc(4 * ni), *right = (int32_t *)alloc(4 * ni), *parent = (int32_t *)alloc(4 * ni);
   int32_t *leaf_parent = (int32_t *)alloc(4 * (size_t)n);
-  uint32_t *first = (uint32_t *)alloc(4 * ni), *last = (uint32_t *)alloc(4 * ni), *visit = (uint32_t *)alloc(4 * ni);
+  uint32_t *first = (uint32_t *)alloc(4 * ni), *last = (uint32_t *)alloc(4 * ni);
+  int32_t *levels_buf = (int32_t *)alloc(4 * ni);  // radix tree nodes by breadth-first level
   float *leafbox = (float *)alloc(32 * (size_t)n), *nodebox = (float *)alloc(32 * ni);
   int32_t *front[2] = {(int32_t *)alloc(4 * ni), (int32_t *)alloc(4 * ni)};
   uint32_t *cnt = (uint32_t *)alloc(16);
@@ -591,15 +591,41 @@ bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_
   else GI_CK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, s));
   const uint32_t *order = vals2;
   if (!ploc) {
-    hipLaunchKernelGGL(k_zero, dim3(blocks(ni)), dim3(kBlock), 0, s, visit, ni);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, vals2, (float4 *)leafbox);
+    GI_CK(hipGetLastError());
     if (n > 1) {
       hipLaunchKernelGGL(k_karras, dim3(blocks(ni)), dim3(kBlock), 0, s, keys2, (int)n, left, right, parent,
                          leaf_parent, first, last);
       GI_CK(hipGetLastError());
+      // breadth-first levels (the host reads each level's size), then the
+      // boxes level by level from the deepest: kernel boundaries order them
+      std::vector<uint32_t> off{0, 1};
+      const int32_t root0 = 0;
+      GI_CK(hipMemcpyAsync(levels_buf, &root0, 4, hipMemcpyHostToDevice, s));
+      while (off.back() > off[off.size() - 2] && off.back() < ni) {
+        const uint32_t lo_i = off[off.size() - 2], cnt_l = off.back() - lo_i;
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, cnt, (size_t)1);
+        hipLaunchKernelGGL(k_tree_level, dim3(std::min<uint32_t>(blocks(cnt_l), 4096)), dim3(kBlock), 0, s, left,
+                           right, levels_buf + lo_i, cnt_l, levels_buf + off.back(), cnt);
+        GI_CK(hipGetLastError());
+        uint32_t nc = 0;
+        GI_CK(hipMemcpyAsync(&nc, cnt, 4, hipMemcpyDeviceToHost, s));
+        GI_CK(hipStreamSynchronize(s));
+        if (nc > ni - off.back()) {
+          err = "device bvh: radix tree levels exceed its node count";
+          for (void *q : mem) (void)hipFree(q);
+          return false;
+        }
+        off.push_back(off.back() + nc);
+      }
+      for (size_t L = off.size() - 1; L-- > 0;) {
+        const uint32_t cnt_l = off[L + 1] - off[L];
+        if (cnt_l == 0) continue;
+        hipLaunchKernelGGL(k_refit_level, dim3(blocks(cnt_l)), dim3(kBlock), 0, s, left, right, levels_buf + off[L],
+                           cnt_l, (const float4 *)leafbox, (float4 *)nodebox);
+      }
+      GI_CK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_refit, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, (int)n, vals2, left, right, parent,
-                       leaf_parent, leafbox, nodebox, visit);
-    GI_CK(hipGetLastError());
   } else {
     hipLaunchKernelGGL(k_ploc_init, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, vals2, (float4 *)leafbox,
                        cbox[0], cl[0]);
