@@ -441,6 +441,12 @@ def main():
     assert np.isfinite(img).all()
     rays = tot["rays_closest"] + tot["rays_shadow"]
     value = rays / dt / 1e6
+    # one more (untimed) frame through the counting build of the traversal
+    # kernels: node visits and triangle tests of the wide BVH per ray
+    ctx.traversal_stats(True)
+    ctx.render(flags=flags)
+    ts = ctx.traversal_stats()
+    ctx.traversal_stats(False)
 
     base, per_ray = (None, None) if a.no_cpu else cpu_baseline(scene, bvh2)
     bw = None if a.no_membench else read_bandwidth()
@@ -455,6 +461,16 @@ def main():
                 "traffic": round(TRACE_CLOSEST_TRAFFIC_BYTES) if tot["launches_closest"] == 8 * a.steps else None,
                 "kernel": "k_trace_closest", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                 "algorithmic_bytes_per_ray": round(b_ray, 1)}
+        # bytes the wide traversal requests per closest ray: ray in, hit
+        # out, one 128-byte node per interior visit, 48 bytes per triangle
+        req = 32 + 16 + (128 * ts["closest_nodes"] + 48 * ts["closest_triangles"]) / max(1, ts["closest_rays"])
+        roof["wide_traversal"] = {"nodes_per_ray": round(ts["closest_nodes"] / max(1, ts["closest_rays"]), 3),
+                                  "triangles_per_ray": round(ts["closest_triangles"] / max(1, ts["closest_rays"]), 3),
+                                  "requested_bytes_per_ray": round(req, 1),
+                                  "requested_gbs": round(bytes_per_launch / b_ray * req / avg_launch_s / 1e9, 1)}
+        if bw:
+            roof["wide_traversal"]["requested_frac_of_measured_l2"] = round(
+                roof["wide_traversal"]["requested_gbs"] / bw["l2"], 4)
         if bw:
             # the BVH and triangles (~27 MB) live in the Infinity Cache
             roof["measured_read_gbs"] = bw

@@ -38,6 +38,8 @@ COUNTERS = np.dtype([("rays_closest", "<u8"), ("rays_shadow", "<u8"), ("samples"
 TUNING = np.dtype([("trace_block", "<i4"), ("trace_blocks_per_cu", "<i4"), ("shade_block", "<i4"),
                    ("samples_per_pass", "<i4"), ("stack_lds", "<i4"), ("target_slots", "<i4")])
 INSTANCE = np.dtype([("xform", "<f4", 12), ("mesh", "<i4"), ("material", "<i4")])
+TRAVERSAL_STATS = np.dtype([(f"{kind}_{what}", "<u8") for kind in ("closest", "any")
+                            for what in ("rays", "nodes", "triangles", "instances")])
 MATGRAD = np.dtype([("d_color", "<f4", 3), ("d_roughness", "<f4"), ("d_metallic", "<f4")])
 
 assert TRI.itemsize == 100 and MATERIAL.itemsize == 96 and LIGHT.itemsize == 64
@@ -96,6 +98,8 @@ SIGNATURES = {
     "gi_upload_sky": (_I, [_P, _P, _I32, _I32]),
     "gi_build_bvh_device": (_I, [_P, _I32]),
     "gi_build_bvh_device_ex": (_I, [_P, _I32, _I32, _I32]),
+    "gi_set_traversal_stats": (_I, [_P, _I32]),
+    "gi_get_traversal_stats": (_I, [_P, _P]),
     "gi_upload_mesh": (_I, [_P, _P, _I64, C.POINTER(C.c_int32)]),
     "gi_upload_texture": (_I, [_P, _P, _I32, _I32, C.POINTER(C.c_int32)]),
     "gi_texture_info": (_I, [_P, _I32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),

@@ -8,7 +8,8 @@ import dataclasses
 import numpy as np
 
 from . import _lib
-from ._lib import COUNTERS, HIT, INSTANCE, LIGHT, MATERIAL, OPTS, TRI, TUNING, VIEW, GiError, ptr
+from ._lib import (COUNTERS, HIT, INSTANCE, LIGHT, MATERIAL, OPTS, TRAVERSAL_STATS, TRI, TUNING, VIEW, GiError,
+                   ptr)
 
 # device builders of gi_build_bvh_device_ex (GI_BVH_LBVH, GI_BVH_PLOC)
 _DEVICE_METHODS = {"device": 0, "ploc": 1}
@@ -227,6 +228,18 @@ class Context:
         c = np.zeros(1, COUNTERS)
         self._ck(self.lib.gi_get_counters(self.h, ptr(c)))
         return {k: c[k][0].item() for k in COUNTERS.names}
+
+    def traversal_stats(self, enable: bool | None = None) -> dict | None:
+        """enable=True zeroes the traversal statistics and counts every
+        later trace; enable=False stops counting; with no argument returns
+        the totals (rays, interior nodes visited, triangles tested,
+        instance entries per kind of trace)."""
+        if enable is not None:
+            self._ck(self.lib.gi_set_traversal_stats(self.h, 1 if enable else 0))
+            return None
+        t = np.zeros(1, TRAVERSAL_STATS)
+        self._ck(self.lib.gi_get_traversal_stats(self.h, ptr(t)))
+        return {k: t[k][0].item() for k in TRAVERSAL_STATS.names}
 
     def set_tuning(self, **kw):
         t = np.zeros(1, TUNING)

@@ -242,10 +242,15 @@ __device__ __forceinline__ bool special_ref(uint32_t ref) {
   return (ref & kLeafBit) && ((ref >> 27) & 15u) == 0u && ref != kDoneRef && ref != kEmptyRef;
 }
 
-template <bool kAny, bool kInst, int kLdsDepth, class Start, class Finish>
+// kCount: the lane also counts its rays, interior-node visits, triangle
+// tests and instance entries, and the wave adds them to stats[0..3] at the
+// end (traversal statistics; a separate instantiation).
+template <bool kAny, bool kInst, int kLdsDepth, bool kCount, class Start, class Finish>
 __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode *__restrict__ nodes,
                                                  const float4 *__restrict__ itris, Stack<kLdsDepth> &st,
-                                                 uint32_t *overflow, InstScene IS, Start start, Finish finish) {
+                                                 uint32_t *overflow, InstScene IS, unsigned long long *stats,
+                                                 Start start, Finish finish) {
+  uint32_t c_rays = 0, c_nodes = 0, c_tris = 0, c_inst = 0;
   const uint32_t lane = __lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
@@ -267,6 +272,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
         const uint32_t i = c * kChunk + r % kChunk;
         if (r < (n_idle / kChunk) * kChunk && c < nchunks && i < count) {
           ray = i;
+          if constexpr (kCount) ++c_rays;
           start(ray, o, d, tmax);
           idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
           oid = f3{o.x * idir.x, o.y * idir.y, o.z * idir.z};
@@ -291,6 +297,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
     // interior work left has a parked leaf (or none has interior work).
     for (;;) {
       if (!(ref & kLeafBit)) {
+        if constexpr (kCount) ++c_nodes;
         const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
         const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
         const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
@@ -359,6 +366,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
           atomicAdd(overflow, 1u);
           if (!keep) ref = st.sp > 0 ? st.take() : kDoneRef;
         } else {
+          if constexpr (kCount) ++c_inst;
           if (keep) st.put(sp, ref);
           st.put(sp + keep, kReturnRef);
           st.sp = sp + keep + 1;
@@ -395,6 +403,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
           n2 = tp[3 * j + 5];
         }
         float t, u, v;
+        if constexpr (kCount) ++c_tris;
         if (intersect_tri(xyz(ta), xyz(t1), xyz(t2), o, d, t, u, v) && t <= tmax) {
           const uint32_t id = fbits(ta.w) + vbase;
           if (kAny) {
@@ -414,6 +423,15 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
       ray = kNoRay;
     }
   }
+  if constexpr (kCount) {
+    // every lane of the wave leaves the loop together
+    uint32_t c[4] = {c_rays, c_nodes, c_tris, c_inst};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      for (int off = 32; off > 0; off >>= 1) c[k] += (uint32_t)__shfl_xor((int)c[k], off);
+      if (lane == 0 && c[k]) atomicAdd(stats + k, (unsigned long long)c[k]);
+    }
+  }
 }
 
 constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the global part
@@ -421,17 +439,17 @@ constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the g
 // Closest-hit kernel over a ray queue.  kHasTmax: max_t in ray_o.w (the
 // primitive entry point); otherwise ray_o.w carries the Halton index and
 // max_t is FLT_MAX (camera and bounce rays, src/hybrid-pt.cpp:558).
-template <bool kHasTmax, bool kInst, int kLdsDepth>
+template <bool kHasTmax, bool kInst, int kLdsDepth, bool kCount>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_closest(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, float4 *__restrict__ hits, const WideNode *__restrict__ nodes,
     const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow,
-    InstScene IS) {
+    InstScene IS, unsigned long long *stats) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
-  trace_persistent<false, kInst, kLdsDepth>(
-      count, nodes, itris, st, overflow, IS,
+  trace_persistent<false, kInst, kLdsDepth, kCount>(
+      count, nodes, itris, st, overflow, IS, stats,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
         const float4 ro = ray_o[i], rd = ray_d[i];
         o = xyz(ro);
@@ -445,18 +463,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WA
 // ray adds its precomputed NEE contribution to its path's colour (the
 // !is.valid() branch of src/hybrid-pt.cpp:355-365).  Otherwise writes the
 // occlusion byte (primitive entry point).
-template <bool kDeposit, bool kInst, int kLdsDepth>
+template <bool kDeposit, bool kInst, int kLdsDepth, bool kCount>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_any(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
     uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes, const float4 *__restrict__ itris,
     uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
-    uint32_t rec_stride, int32_t b, InstScene IS) {
+    uint32_t rec_stride, int32_t b, InstScene IS, unsigned long long *stats) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
-  trace_persistent<true, kInst, kLdsDepth>(
-      count, nodes, itris, st, overflow, IS,
+  trace_persistent<true, kInst, kLdsDepth, kCount>(
+      count, nodes, itris, st, overflow, IS, stats,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
         const float4 ro = ray_o[i], rd = ray_d[i];
         o = xyz(ro);
@@ -1401,6 +1419,9 @@ struct gi_ctx {
   DBuf d_hdims, d_hchunks;
   // traversal stack + overflow flag
   DBuf d_gstack, d_overflow;
+  // traversal statistics (gi_set_traversal_stats): 8 counters
+  bool count_traversal = false;
+  DBuf d_tstats;
   DBuf d_sky_rgb, d_sky_x, d_sky_y;  // sky map, per-row pdf|cdf, row pdf|cdf
   // instancing: meshes (object space), instances, the instance tree
   std::vector<std::vector<gi_tri>> meshes;
@@ -1675,23 +1696,25 @@ size_t trace_lds_bytes(int lds_depth, int block) {
 
 InstScene inst_scene(const gi_ctx *ctx);
 
-template <bool kHasTmax, bool kInst, int kLds>
+template <bool kHasTmax, bool kInst, int kLds, bool kCount = false>
 void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
                       const float4 *ro, const float4 *rd, float4 *hits) {
-  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kInst, kLds>), dim3(grid), dim3(block), trace_lds_bytes(kLds, block),
-                     ctx->stream, count_ptr, count_host, ro, rd, hits, ctx->d_nodes.as<WideNode>(),
-                     ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
-                     ctx->d_overflow.as<uint32_t>(), inst_scene(ctx));
+  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kInst, kLds, kCount>), dim3(grid), dim3(block),
+                     trace_lds_bytes(kLds, block), ctx->stream, count_ptr, count_host, ro, rd, hits,
+                     ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
+                     ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), inst_scene(ctx),
+                     kCount ? ctx->d_tstats.as<unsigned long long>() : nullptr);
 }
 
-template <bool kDeposit, bool kInst, int kLds>
+template <bool kDeposit, bool kInst, int kLds, bool kCount = false>
 void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
                   const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ, float4 *rec,
                   uint32_t rec_stride, int32_t b) {
-  hipLaunchKernelGGL((k_trace_any<kDeposit, kInst, kLds>), dim3(grid), dim3(block), trace_lds_bytes(kLds, block),
-                     ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ, ctx->d_nodes.as<WideNode>(),
-                     ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
-                     ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b, inst_scene(ctx));
+  hipLaunchKernelGGL((k_trace_any<kDeposit, kInst, kLds, kCount>), dim3(grid), dim3(block),
+                     trace_lds_bytes(kLds, block), ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ,
+                     ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
+                     ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b, inst_scene(ctx),
+                     kCount ? ctx->d_tstats.as<unsigned long long>() + 4 : nullptr);
 }
 
 int stack_lds_of(const gi_ctx *ctx) {
@@ -1710,7 +1733,8 @@ int launch_closest(gi_ctx *ctx, bool has_tmax, const uint32_t *count_ptr, uint32
   const bool I = instanced(ctx);
 #define GI_CL(T, II, LL) launch_closest_t<T, II, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, hits)
 #define GI_CL_L(T, II) \
-  if (L == 8) GI_CL(T, II, 8); \
+  if (ctx->count_traversal) launch_closest_t<T, II, 16, true>(ctx, grid, block, count_ptr, count_host, ro, rd, hits); \
+  else if (L == 8) GI_CL(T, II, 8); \
   else if (L == 32) GI_CL(T, II, 32); \
   else GI_CL(T, II, 16)
   if (has_tmax) {
@@ -1735,7 +1759,10 @@ int launch_any(gi_ctx *ctx, bool deposit, const uint32_t *count_ptr, uint32_t co
 #define GI_ANY(D, II, LL) \
   launch_any_t<D, II, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, b)
 #define GI_ANY_L(D, II) \
-  if (L == 8) GI_ANY(D, II, 8); \
+  if (ctx->count_traversal) \
+    launch_any_t<D, II, 16, true>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, \
+                                  b); \
+  else if (L == 8) GI_ANY(D, II, 8); \
   else if (L == 32) GI_ANY(D, II, 32); \
   else GI_ANY(D, II, 16)
   if (deposit) {
@@ -2931,6 +2958,26 @@ int gi_get_image_device(gi_ctx *ctx, int32_t view, float **d_rgb) {
 int gi_get_counters(const gi_ctx *ctx, gi_counters *out) {
   if (!ctx || !out) return GI_ERR_INVALID_ARG;
   *out = ctx->counters;
+  return GI_OK;
+}
+
+int gi_set_traversal_stats(gi_ctx *ctx, int32_t enable) {
+  if (!ctx) return GI_ERR_INVALID_ARG;
+  if (enable) {
+    GI_HIP(ctx, hipSetDevice(ctx->device));
+    GI_HIP(ctx, ctx->d_tstats.ensure(sizeof(gi_traversal_stats)));
+    const gi_traversal_stats zero{};
+    GI_HIP(ctx, copy_sync(ctx, ctx->d_tstats.p, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  }
+  ctx->count_traversal = enable != 0;
+  return GI_OK;
+}
+
+int gi_get_traversal_stats(gi_ctx *ctx, gi_traversal_stats *out) {
+  if (!ctx || !out) return fail(ctx, GI_ERR_INVALID_ARG, "traversal stats");
+  if (!ctx->d_tstats.p) return fail(ctx, GI_ERR_STATE, "traversal statistics were never enabled");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  GI_HIP(ctx, copy_sync(ctx, out, ctx->d_tstats.p, sizeof(*out), hipMemcpyDeviceToHost));
   return GI_OK;
 }
 

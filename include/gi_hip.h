@@ -264,6 +264,23 @@ int gi_get_image(gi_ctx *ctx, int32_t view, float *rgb);
 /* Device pointer of the image of view v (valid until the next gi_render). */
 int gi_get_image_device(gi_ctx *ctx, int32_t view, float **d_rgb);
 int gi_get_counters(const gi_ctx *ctx, gi_counters *out);
+/* Traversal statistics: per kind of trace (closest hit; any hit), the rays
+ * traced, the wide-BVH interior nodes visited (one 128-byte node fetch
+ * each), the triangles tested (48 bytes each) and the instance entries.
+ * Counted by a separate build of the traversal kernels (LDS stack depth 16)
+ * that runs only while statistics are on, so the default kernels pay
+ * nothing.  The reference prints per-bounce trace times only
+ * (src/gpu-pt.h:185); these counts have no counterpart there. */
+typedef struct {
+  uint64_t closest_rays, closest_nodes, closest_triangles, closest_instances;
+  uint64_t any_rays, any_nodes, any_triangles, any_instances;
+} gi_traversal_stats;
+/* enable != 0: zero the totals and count every later traversal launch
+ * (renders, adjoint replays, gi_trace_closest / gi_trace_any); 0: stop
+ * counting (the totals are kept). */
+int gi_set_traversal_stats(gi_ctx *ctx, int32_t enable);
+/* Totals since the last gi_set_traversal_stats(ctx, 1). */
+int gi_get_traversal_stats(gi_ctx *ctx, gi_traversal_stats *out);
 int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);
 int gi_get_tuning(const gi_ctx *ctx, gi_tuning *t);
 

@@ -66,6 +66,7 @@ def test_null_context_calls_fail_cleanly():
     assert lib.gi_upload_triangles(None, None, 0) == 1
     assert lib.gi_render(None, None, 0, None) != 0
     assert lib.gi_get_counters(None, None) == 1
+    assert lib.gi_set_traversal_stats(None, 1) == 1 and lib.gi_get_traversal_stats(None, None) == 1
     assert lib.gi_last_error(None) == b"null context"
     lib.gi_destroy(None)
 
