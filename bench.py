@@ -35,13 +35,14 @@ HBM_PEAK_GBS = 8000.0
 # Memory traffic of one k_trace_closest launch of this workload at the
 # default tuning (one pass, 8 launches per frame, 18.7 M rays per launch),
 # from two rocprofv3 counter runs of `bench.py --steps 1 --warmup 0 --no-cpu
-# --no-batching --no-adjoint --copy-mib 512`: FETCH_SIZE 3,083,962 KiB and
-# WRITE_SIZE 507,188 KiB per dispatch, each scaled by what the counter
-# reads for the 512 MiB device copy (FETCH 262,168 KiB -> x2.000, WRITE
-# 524,288 KiB -> x1.000).  FETCH counts L2 misses to the fabric, so
-# Infinity Cache hits are included: an upper bound of the HBM bytes.
-TRACE_CLOSEST_FETCH_BYTES = 3083962.2 * 1.99982 * 1024
-TRACE_CLOSEST_TRAFFIC_BYTES = TRACE_CLOSEST_FETCH_BYTES + 507187.8 * 1.0 * 1024
+# --no-batching --no-adjoint --no-bvh --no-instancing --no-small
+# --no-membench --copy-mib 512`: FETCH_SIZE 2,909,255 KiB and WRITE_SIZE
+# 511,091 KiB per dispatch, each scaled by what the counter reads for the
+# 512 MiB device copy (FETCH 262,168 KiB -> x2.000, WRITE 524,288 KiB ->
+# x1.000).  FETCH counts L2 misses to the fabric, so Infinity Cache hits
+# are included: an upper bound of the HBM bytes.
+TRACE_CLOSEST_FETCH_BYTES = 2909255.1 * 1.99982 * 1024
+TRACE_CLOSEST_TRAFFIC_BYTES = TRACE_CLOSEST_FETCH_BYTES + 511090.5 * 1.0 * 1024
 
 
 def parse():

@@ -6,6 +6,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <functional>
 #include <numeric>
 
@@ -312,41 +313,113 @@ void leaf_order_tris(const gi_bvh &b, const gi_tri *tris, std::vector<IsectTri> 
   }
 }
 
+// Binary -> 4-wide collapse chosen by surface-area cost (the dynamic
+// program of Ylitie, Karras and Laine 2017, "Efficient incoherent ray
+// traversal on GPUs through compressed wide BVHs", for width 4).  C[n][i] is
+// the least cost of binary subtree n when it may take up to i slots of its
+// parent: one slot holds n as a wide node (area(n) * kNodeCost plus the best
+// split of 4 slots between its children) or, when its triangles are
+// contiguous and at most merge_limit, as one leaf (area(n) * count); more
+// slots open n and split them between its children.  Costs are in triangle
+// tests; a wide-node visit (four box tests, one 128-byte fetch) counts as
+// kNodeCost of them.
+#ifndef GI_COLLAPSE_NODE_COST
+#define GI_COLLAPSE_NODE_COST 1.0
+#endif
+namespace {
+constexpr double kNodeCost = GI_COLLAPSE_NODE_COST;
+}  // namespace
+
 void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh2Node &)> &leaf_ref,
-                         uint32_t node_base, std::vector<WideNode> &wnodes, int32_t &wide_depth) {
+                         uint32_t node_base, std::vector<WideNode> &wnodes, int32_t &wide_depth,
+                         int32_t merge_limit) {
   wnodes.clear();
   wide_depth = 0;
   const auto &N = b.nodes;
-  auto area = [&](int32_t i) {
-    float dx = N[i].hi[0] - N[i].lo[0], dy = N[i].hi[1] - N[i].lo[1], dz = N[i].hi[2] - N[i].lo[2];
-    return dx * dy + dy * dz + dz * dx;
+  const size_t nn = N.size();
+  // post-order of the nodes reachable from the root
+  std::vector<int32_t> post;
+  post.reserve(nn);
+  {
+    std::vector<std::pair<int32_t, bool>> st{{0, false}};
+    while (!st.empty()) {
+      const auto [i, open] = st.back();
+      st.pop_back();
+      if (open || N[i].left < 0) {
+        post.push_back(i);
+        continue;
+      }
+      st.push_back({i, true});
+      st.push_back({N[i].right, false});
+      st.push_back({N[i].left, false});
+    }
+  }
+  constexpr double kInf = std::numeric_limits<double>::infinity();
+  std::vector<double> C(5 * nn, 0.0);        // C[5n + i], i = 1..4
+  std::vector<int8_t> split(5 * nn, 0);      // left share of the best split of i slots
+  std::vector<int8_t> opened(5 * nn, 0);     // C[n][i] opens n (else it equals C[n][i-1])
+  std::vector<uint8_t> as_leaf(nn, 0), contiguous(nn, 1);
+  std::vector<int32_t> cnt(nn, 0), first(nn, 0);
+  for (const int32_t n : post) {
+    const Bvh2Node &nd = N[n];
+    const double dx = (double)nd.hi[0] - nd.lo[0], dy = (double)nd.hi[1] - nd.lo[1], dz = (double)nd.hi[2] - nd.lo[2];
+    double area = dx * dy + dy * dz + dz * dx;
+    if (!(area > 0.0) || !std::isfinite(area)) area = 0.0;
+    double *c = &C[5 * (size_t)n];
+    if (nd.left < 0) {
+      cnt[n] = nd.count;
+      first[n] = nd.first;
+      as_leaf[n] = 1;
+      for (int i = 1; i <= 4; ++i) c[i] = area * nd.count;
+      continue;
+    }
+    const int32_t l = nd.left, r = nd.right;
+    cnt[n] = cnt[l] + cnt[r];
+    first[n] = std::min(first[l], first[r]);
+    contiguous[n] = contiguous[l] && contiguous[r] &&
+                    (first[r] == first[l] + cnt[l] || first[l] == first[r] + cnt[r]);
+    const double *cl = &C[5 * (size_t)l], *cr = &C[5 * (size_t)r];
+    double D[5] = {kInf, kInf, kInf, kInf, kInf};
+    for (int j = 2; j <= 4; ++j)
+      for (int k = 1; k < j; ++k)
+        if (cl[k] + cr[j - k] < D[j]) {
+          D[j] = cl[k] + cr[j - k];
+          split[5 * (size_t)n + j] = (int8_t)k;
+        }
+    const double node_cost = area * kNodeCost + D[4];
+    const double leaf_cost =
+        contiguous[n] && cnt[n] <= merge_limit && cnt[n] <= kMaxLeafTris ? area * cnt[n] : kInf;
+    as_leaf[n] = leaf_cost <= node_cost;
+    c[1] = std::min(leaf_cost, node_cost);
+    for (int i = 2; i <= 4; ++i) {
+      opened[5 * (size_t)n + i] = D[i] < c[i - 1];
+      c[i] = std::min(D[i], c[i - 1]);
+    }
+  }
+  // the binary nodes that take the slots of a wide node opened at n
+  std::function<void(int32_t, int, std::vector<int32_t> &)> take = [&](int32_t n, int i, std::vector<int32_t> &out) {
+    while (i > 1 && N[n].left >= 0 && !opened[5 * (size_t)n + i]) --i;
+    if (i == 1 || N[n].left < 0) {
+      out.push_back(n);
+      return;
+    }
+    const int k = split[5 * (size_t)n + i];
+    take(N[n].left, k, out);
+    take(N[n].right, i - k, out);
   };
-  // BFS over binary nodes that become wide nodes.
-  std::vector<std::pair<int32_t, int32_t>> work;  // (binary node, wide slot)
+  std::vector<std::pair<int32_t, int32_t>> work;  // (binary node, wide slot), breadth first
   wnodes.emplace_back();
   std::vector<int32_t> wdepth{0};
   work.push_back({0, 0});
   for (size_t w = 0; w < work.size(); ++w) {
-    int32_t bi = work[w].first, wi = work[w].second;
+    const int32_t bi = work[w].first, wi = work[w].second;
     std::vector<int32_t> kids;
     if (N[bi].left < 0) {
       kids.push_back(bi);  // root is a leaf: one child slot
     } else {
-      kids = {N[bi].left, N[bi].right};
-      while (kids.size() < 4) {
-        int best = -1;
-        float ba = -1.f;
-        for (int k = 0; k < (int)kids.size(); ++k)
-          if (N[kids[k]].left >= 0 && area(kids[k]) > ba) {
-            ba = area(kids[k]);
-            best = k;
-          }
-        if (best < 0) break;
-        int32_t open = kids[best];
-        kids.erase(kids.begin() + best);
-        kids.push_back(N[open].left);
-        kids.push_back(N[open].right);
-      }
+      const int k = split[5 * (size_t)bi + 4];
+      take(N[bi].left, k, kids);
+      take(N[bi].right, 4 - k, kids);
     }
     WideNode nd;
     std::memset(&nd, 0, sizeof(nd));
@@ -355,14 +428,21 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
         set_empty_slot(nd, k);
         continue;
       }
-      int32_t c = kids[k];
+      const int32_t c = kids[k];
       nd.lox[k] = N[c].lo[0]; nd.loy[k] = N[c].lo[1]; nd.loz[k] = N[c].lo[2];
       nd.hix[k] = N[c].hi[0]; nd.hiy[k] = N[c].hi[1]; nd.hiz[k] = N[c].hi[2];
-      if (N[c].left < 0) {
-        if (N[c].count == 0) set_empty_slot(nd, k);
-        else nd.child[k] = leaf_ref(N[c]);
+      if (N[c].left < 0 || as_leaf[c]) {
+        if (cnt[c] == 0) {
+          set_empty_slot(nd, k);
+        } else {
+          Bvh2Node leaf = N[c];
+          leaf.left = leaf.right = -1;
+          leaf.first = first[c];
+          leaf.count = cnt[c];
+          nd.child[k] = leaf_ref(leaf);
+        }
       } else {
-        int32_t slot = (int32_t)wnodes.size();
+        const int32_t slot = (int32_t)wnodes.size();
         wnodes.emplace_back();
         wdepth.push_back(wdepth[wi] + 1);
         wide_depth = std::max(wide_depth, wdepth.back());
@@ -379,7 +459,7 @@ void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &w
   leaf_order_tris(b, tris, leaf_tris);
   collapse_wide_nodes(
       b, [](const Bvh2Node &n) { return kLeafBit | ((uint32_t)n.count << 27) | (uint32_t)n.first; }, 0u, wnodes,
-      wide_depth);
+      wide_depth, kCollapseMergeLimit);
 }
 
 // ------------------------------------------------------------- Halton ----

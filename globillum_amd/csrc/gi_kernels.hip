@@ -2585,7 +2585,7 @@ int gi_upload_instances(gi_ctx *ctx, const gi_instance *inst, int32_t n) {
     const uint32_t tb = tri_base;
     collapse_wide_nodes(
         b, [tb](const Bvh2Node &nd) { return kLeafBit | ((uint32_t)nd.count << 27) | ((uint32_t)nd.first + tb); },
-        node_base, wn, depth);
+        node_base, wn, depth, kCollapseMergeLimit);
     root[m] = node_base;
     stri[m] = tri_base;
     std::memcpy(&mbox[6 * m], b.nodes[0].lo, 12);
@@ -2689,7 +2689,7 @@ int gi_upload_instances(gi_ctx *ctx, const gi_instance *inst, int32_t n) {
         const uint32_t prim = tb.perm[(size_t)nd.first];
         return prim >= (uint32_t)n ? pref[prim - (uint32_t)n] : (kLeafBit | (prim + 1u));
       },
-      tlas_base, tn, tlas_depth);
+      tlas_base, tn, tlas_depth, 1);  // instance-tree leaves hold one instance or piece each
   // combined device arrays: world part copied on the device, the rest uploaded
   const size_t nw = (size_t)ctx->world_nnodes, nx = xnodes.size(), nt = tn.size();
   DBuf nodes, itris, stris;
