@@ -36,7 +36,8 @@ COUNTERS = np.dtype([("rays_closest", "<u8"), ("rays_shadow", "<u8"), ("samples"
                      ("ms_trace_any", "<f8"), ("ms_shade", "<f8"), ("ms_other", "<f8"),
                      ("launches_closest", "<i4"), ("passes", "<i4")])
 TUNING = np.dtype([("trace_block", "<i4"), ("trace_blocks_per_cu", "<i4"), ("shade_block", "<i4"),
-                   ("samples_per_pass", "<i4"), ("stack_lds", "<i4"), ("target_slots", "<i4")])
+                   ("samples_per_pass", "<i4"), ("stack_lds", "<i4"), ("target_slots", "<i4"),
+                   ("node_width", "<i4"), ("refill_lanes", "<i4")])
 INSTANCE = np.dtype([("xform", "<f4", 12), ("mesh", "<i4"), ("material", "<i4")])
 TRAVERSAL_STATS = np.dtype([(f"{kind}_{what}", "<u8") for kind in ("closest", "any")
                             for what in ("rays", "nodes", "triangles", "instances")])

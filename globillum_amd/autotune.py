@@ -1,10 +1,13 @@
 """Autotuner of the renderer's launch parameters.
 
-Searches traversal block size, blocks per CU, LDS stack depth, shading
+Searches traversal block size, blocks per CU, LDS stack depth, BVH node
+width (child slots per node: the world tree is collapsed again for each
+candidate), the traversal kernels' refill (compaction) threshold, shading
 block size and the number of paths kept in flight per pass by timing real
 renders of the caller's workload (coordinate descent from the defaults,
 two repeats per candidate, best median wins).  The images do not depend on
-these parameters -- only the schedule does -- so any choice is safe.
+these parameters -- only the schedule and the tree's shape do, and closest
+hits are the same in any tree -- so any choice is safe.
 """
 from __future__ import annotations
 
@@ -15,10 +18,13 @@ SPACE = {
     "trace_block": [64, 128, 256],
     "trace_blocks_per_cu": [4, 5, 6, 8],
     "stack_lds": [8, 16, 32],
+    "node_width": [2, 3, 4],
+    "refill_lanes": [8, 16, 32, 64],
     "shade_block": [64, 128, 256],
     "target_slots": [16, 32, 64],
 }
-DEFAULT = {"trace_block": 256, "trace_blocks_per_cu": 5, "stack_lds": 16, "shade_block": 256, "target_slots": 64}
+DEFAULT = {"trace_block": 256, "trace_blocks_per_cu": 5, "stack_lds": 16, "node_width": 4, "refill_lanes": 32,
+           "shade_block": 256, "target_slots": 64}
 
 
 def _time(ctx, views, opts, repeats: int) -> float:

@@ -318,7 +318,7 @@ void leaf_order_tris(const gi_bvh &b, const gi_tri *tris, std::vector<IsectTri> 
 // traversal on GPUs through compressed wide BVHs", for width 4).  C[n][i] is
 // the least cost of binary subtree n when it may take up to i slots of its
 // parent: one slot holds n as a wide node (area(n) * kNodeCost plus the best
-// split of 4 slots between its children) or, when its triangles are
+// split of `width` (2..4) slots between its children) or, when its triangles are
 // contiguous and at most merge_limit, as one leaf (area(n) * count); more
 // slots open n and split them between its children.  Costs are in triangle
 // tests; a wide-node visit (four box tests, one 128-byte fetch) counts as
@@ -332,9 +332,10 @@ constexpr double kNodeCost = GI_COLLAPSE_NODE_COST;
 
 void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh2Node &)> &leaf_ref,
                          uint32_t node_base, std::vector<WideNode> &wnodes, int32_t &wide_depth,
-                         int32_t merge_limit) {
+                         int32_t merge_limit, int32_t width) {
   wnodes.clear();
   wide_depth = 0;
+  const int W = std::max(2, std::min(4, (int)width));  // slots filled per wide node
   const auto &N = b.nodes;
   const size_t nn = N.size();
   // post-order of the nodes reachable from the root
@@ -380,20 +381,20 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
                     (first[r] == first[l] + cnt[l] || first[l] == first[r] + cnt[r]);
     const double *cl = &C[5 * (size_t)l], *cr = &C[5 * (size_t)r];
     double D[5] = {kInf, kInf, kInf, kInf, kInf};
-    for (int j = 2; j <= 4; ++j)
+    for (int j = 2; j <= W; ++j)
       for (int k = 1; k < j; ++k)
         if (cl[k] + cr[j - k] < D[j]) {
           D[j] = cl[k] + cr[j - k];
           split[5 * (size_t)n + j] = (int8_t)k;
         }
-    const double node_cost = area * kNodeCost + D[4];
+    const double node_cost = area * kNodeCost + D[W];
     const double leaf_cost =
         contiguous[n] && cnt[n] <= merge_limit && cnt[n] <= kMaxLeafTris ? area * cnt[n] : kInf;
     as_leaf[n] = leaf_cost <= node_cost;
     c[1] = std::min(leaf_cost, node_cost);
     for (int i = 2; i <= 4; ++i) {
-      opened[5 * (size_t)n + i] = D[i] < c[i - 1];
-      c[i] = std::min(D[i], c[i - 1]);
+      opened[5 * (size_t)n + i] = i <= W && D[i] < c[i - 1];
+      c[i] = i <= W ? std::min(D[i], c[i - 1]) : c[i - 1];
     }
   }
   // the binary nodes that take the slots of a wide node opened at n
@@ -417,9 +418,9 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
     if (N[bi].left < 0) {
       kids.push_back(bi);  // root is a leaf: one child slot
     } else {
-      const int k = split[5 * (size_t)bi + 4];
+      const int k = split[5 * (size_t)bi + W];
       take(N[bi].left, k, kids);
-      take(N[bi].right, 4 - k, kids);
+      take(N[bi].right, W - k, kids);
     }
     WideNode nd;
     std::memset(&nd, 0, sizeof(nd));
@@ -455,11 +456,11 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
 }
 
 void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &wnodes,
-                   std::vector<IsectTri> &leaf_tris, int32_t &wide_depth) {
+                   std::vector<IsectTri> &leaf_tris, int32_t &wide_depth, int32_t width) {
   leaf_order_tris(b, tris, leaf_tris);
   collapse_wide_nodes(
       b, [](const Bvh2Node &n) { return kLeafBit | ((uint32_t)n.count << 27) | (uint32_t)n.first; }, 0u, wnodes,
-      wide_depth, kCollapseMergeLimit);
+      wide_depth, kCollapseMergeLimit, width);
 }
 
 // ------------------------------------------------------------- Halton ----

@@ -23,10 +23,10 @@ enum class DeviceBvhMethod { kLbvh = 0, kPloc = 1 };
 // centroid bounds clo..chi.  Writes nodes[0..nnodes) (capacity >= n) and
 // the 3n float4 of the leaf-ordered intersection triangles.  kLbvh: the
 // radix tree over the Morton keys; kPloc: clustering with the given search
-// radius (1..64).  Synchronises the stream; scratch memory is allocated and
+// radius (1..64).  width: child slots filled per wide node (2..4).  Synchronises the stream; scratch memory is allocated and
 // freed inside.
 bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, const float clo[3],
-                      const float chi[3], int max_leaf, DeviceBvhMethod method, int radius, WideNode *nodes,
-                      float4 *itris, LbvhResult &res, std::string &err);
+                      const float chi[3], int max_leaf, DeviceBvhMethod method, int radius, int width,
+                      WideNode *nodes, float4 *itris, LbvhResult &res, std::string &err);
 
 }  // namespace gi

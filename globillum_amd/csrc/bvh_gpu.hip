@@ -214,6 +214,7 @@ struct CollapseArgs {
   const uint32_t *first, *last;
   const float *leafbox, *nodebox;
   int max_leaf;
+  int width;  // child slots filled per wide node (2..4)
 };
 
 __device__ __forceinline__ uint32_t count_of(const CollapseArgs &A, int32_t c) {
@@ -236,7 +237,7 @@ __global__ void k_collapse(CollapseArgs A, const int32_t *__restrict__ frontier,
       kids[0] = A.left[b];
       kids[1] = A.right[b];
       nk = 2;
-      while (nk < 4) {
+      while (nk < A.width) {
         int best = -1;
         float ba = -1.0f;
         for (int k = 0; k < nk; ++k) {
@@ -513,7 +514,7 @@ inline uint32_t blocks(size_t n) { return (uint32_t)std::max<size_t>(1, (n + kBl
   } while (0)
 
 bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, const float clo[3],
-                      const float chi[3], int max_leaf, DeviceBvhMethod method, int radius, WideNode *nodes,
+                      const float chi[3], int max_leaf, DeviceBvhMethod method, int radius, int width, WideNode *nodes,
                       float4 *itris, LbvhResult &res, std::string &err) {
   std::vector<void *> mem;
   auto alloc = [&](size_t bytes) -> void * {
@@ -697,7 +698,7 @@ bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_
     for (void *q : mem) (void)hipFree(q);
     return true;
   }
-  CollapseArgs A{left, right, first, last, leafbox, nodebox, max_leaf};
+  CollapseArgs A{left, right, first, last, leafbox, nodebox, max_leaf, std::max(2, std::min(4, width))};
   // the radix tree's root is node 0; PLOC's is the last merge, n - 2
   const int32_t root = ploc ? (int32_t)(n - 2) : 0;
   GI_CK(hipMemcpyAsync(front[0], &root, 4, hipMemcpyHostToDevice, s));

@@ -77,12 +77,13 @@ bool import_bvh2(const float *nodes, int64_t n_nodes, const uint32_t *indices, i
 // Collapse to the 4-wide layout (surface-area cost, dynamic program over
 // slot counts) and emit the leaf-ordered intersection triangles.
 void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &nodes,
-                   std::vector<IsectTri> &leaf_tris, int32_t &wide_depth);
+                   std::vector<IsectTri> &leaf_tris, int32_t &wide_depth, int32_t width = 4);
 // The pieces of collapse_wide: leaf-ordered triangles (a.w = index in tris),
 // and the 4-wide nodes with interior references offset by node_base and
 // leaf references from leaf_ref (called for leaves with count > 0).  The
 // collapse may turn a binary subtree of at most merge_limit primitives (a
 // contiguous range) into one leaf; 1 keeps the binary leaves as they are.
+// width (2..4) is the number of child slots it fills per wide node.
 #ifndef GI_COLLAPSE_MERGE
 #define GI_COLLAPSE_MERGE 4
 #endif
@@ -90,7 +91,7 @@ constexpr int32_t kCollapseMergeLimit = GI_COLLAPSE_MERGE;
 void leaf_order_tris(const gi_bvh &b, const gi_tri *tris, std::vector<IsectTri> &leaf_tris);
 void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh2Node &)> &leaf_ref,
                          uint32_t node_base, std::vector<WideNode> &nodes, int32_t &wide_depth,
-                         int32_t merge_limit);
+                         int32_t merge_limit, int32_t width = 4);
 // Relative box padding of the builders (2^-20 of the largest coordinate
 // magnitude / extent): keeps the float slab tests conservative.
 void pad_box(float *lo, float *hi);

@@ -159,9 +159,10 @@ __device__ __forceinline__ void block_append2(bool p0, uint32_t *c0, bool p1, ui
 
 // -------------------------------------------------------------- traversal --
 // Ordered 4-wide BVH traversal, one ray per lane, persistent.  The queue is
-// cut into chunks of kChunk consecutive rays; wave w of W owns chunks w,
+// cut into chunks of C consecutive rays (C = the refill threshold, a tuning
+// parameter: 8..64); wave w of W owns chunks w,
 // w + W, w + 2W, ... (no atomics).  A wave keeps its lanes busy: whenever
-// at least kChunk lanes have finished their rays, they take the wave's next
+// at least C lanes have finished their rays, they take the wave's next
 // whole chunks, one ray each.  Interior nodes are processed in an inner
 // loop in which a lane that reaches a leaf parks it and goes on with its
 // stack (speculative traversal, Aila & Laine 2009), until every lane with
@@ -189,10 +190,6 @@ __device__ __forceinline__ void block_append2(bool p0, uint32_t *c0, bool p1, ui
 #endif
 
 constexpr uint32_t kNoRay = 0xffffffffu;
-#ifndef GI_TRACE_CHUNK
-#define GI_TRACE_CHUNK 32
-#endif
-constexpr uint32_t kChunk = GI_TRACE_CHUNK;
 constexpr float kSlabSlack = 1.0f + 0x1p-21f;
 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
@@ -245,8 +242,9 @@ __device__ __forceinline__ bool special_ref(uint32_t ref) {
 // kCount: the lane also counts its rays, interior-node visits, triangle
 // tests and instance entries, and the wave adds them to stats[0..3] at the
 // end (traversal statistics; a separate instantiation).
+// cs: log2 of the chunk size (the refill threshold, 8..64 rays).
 template <bool kAny, bool kInst, int kLdsDepth, bool kCount, class Start, class Finish>
-__device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode *__restrict__ nodes,
+__device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, const WideNode *__restrict__ nodes,
                                                  const float4 *__restrict__ itris, Stack<kLdsDepth> &st,
                                                  uint32_t *overflow, InstScene IS, unsigned long long *stats,
                                                  Start start, Finish finish) {
@@ -254,7 +252,8 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
   const uint32_t lane = __lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  const uint32_t nchunks = (count + kChunk - 1) / kChunk;
+  const uint32_t chunk_n = 1u << cs;  // rays per chunk = idle lanes a refill needs
+  const uint32_t nchunks = (count + chunk_n - 1) >> cs;
   uint32_t chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // wave-uniform: next chunk
   uint32_t ray = kNoRay, ref = kDoneRef, parked = kDoneRef;
   f3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f}, idir{0.f, 0.f, 0.f}, oid{0.f, 0.f, 0.f};
@@ -265,12 +264,12 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
     // ---- refill: the idle lanes, in lane order, take whole chunks
     const uint64_t idle = __ballot(ray == kNoRay);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
-    if (n_idle >= kChunk && chunk < nchunks) {
+    if (n_idle >= chunk_n && chunk < nchunks) {
       if (ray == kNoRay) {
         const uint32_t r = (uint32_t)__popcll(idle & below);
-        const uint32_t c = chunk + (r / kChunk) * waves;
-        const uint32_t i = c * kChunk + r % kChunk;
-        if (r < (n_idle / kChunk) * kChunk && c < nchunks && i < count) {
+        const uint32_t c = chunk + (r >> cs) * waves;
+        const uint32_t i = (c << cs) + (r & (chunk_n - 1));
+        if (r < (n_idle & ~(chunk_n - 1)) && c < nchunks && i < count) {
           ray = i;
           if constexpr (kCount) ++c_rays;
           start(ray, o, d, tmax);
@@ -284,7 +283,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, const WideNode 
           st.sp = 0;
         }
       }
-      chunk += (n_idle / kChunk) * waves;
+      chunk += (n_idle >> cs) * waves;
     }
     if (__ballot(ray != kNoRay) == 0) {
       if (chunk >= nchunks) break;
@@ -444,12 +443,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WA
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, float4 *__restrict__ hits, const WideNode *__restrict__ nodes,
     const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow,
-    InstScene IS, unsigned long long *stats) {
+    InstScene IS, unsigned long long *stats, uint32_t chunk_shift) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
   trace_persistent<false, kInst, kLdsDepth, kCount>(
-      count, nodes, itris, st, overflow, IS, stats,
+      count, chunk_shift, nodes, itris, st, overflow, IS, stats,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
         const float4 ro = ray_o[i], rd = ray_d[i];
         o = xyz(ro);
@@ -469,12 +468,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WA
     const float4 *__restrict__ ray_d, const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
     uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes, const float4 *__restrict__ itris,
     uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
-    uint32_t rec_stride, int32_t b, InstScene IS, unsigned long long *stats) {
+    uint32_t rec_stride, int32_t b, InstScene IS, unsigned long long *stats, uint32_t chunk_shift) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
   trace_persistent<true, kInst, kLdsDepth, kCount>(
-      count, nodes, itris, st, overflow, IS, stats,
+      count, chunk_shift, nodes, itris, st, overflow, IS, stats,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
         const float4 ro = ray_o[i], rd = ray_d[i];
         o = xyz(ro);
@@ -1461,7 +1460,7 @@ struct gi_ctx {
   int32_t prog_sample = 0, prog_max = 0;
   bool prog_active = false;
   gi_counters counters{};
-  gi_tuning tuning{256, 5, 256, 0, 16, 0};
+  gi_tuning tuning{256, 5, 256, 0, 16, 0, 4, 32};
   std::vector<hipEvent_t> events;
 };
 
@@ -1633,7 +1632,7 @@ int upload_wide(gi_ctx *ctx) {
   std::vector<WideNode> wn;
   std::vector<IsectTri> it;
   int32_t wd = 0;
-  collapse_wide(*ctx->bvh, ctx->tris.data(), wn, it, wd);
+  collapse_wide(*ctx->bvh, ctx->tris.data(), wn, it, wd, ctx->tuning.node_width);
   GI_HIP(ctx, ctx->d_nodes.ensure(wn.size() * sizeof(WideNode)));
   GI_HIP(ctx, ctx->d_itris.ensure(std::max<size_t>(1, it.size()) * sizeof(IsectTri)));
   GI_HIP(ctx, copy_sync(ctx, ctx->d_nodes.p, wn.data(), wn.size() * sizeof(WideNode), hipMemcpyHostToDevice));
@@ -1696,6 +1695,12 @@ size_t trace_lds_bytes(int lds_depth, int block) {
 
 InstScene inst_scene(const gi_ctx *ctx);
 
+// log2 of the traversal kernels' refill threshold (gi_tuning.refill_lanes)
+uint32_t chunk_shift_of(const gi_ctx *ctx) {
+  const int r = ctx->tuning.refill_lanes;
+  return r == 8 ? 3u : r == 16 ? 4u : r == 64 ? 6u : 5u;
+}
+
 template <bool kHasTmax, bool kInst, int kLds, bool kCount = false>
 void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
                       const float4 *ro, const float4 *rd, float4 *hits) {
@@ -1703,7 +1708,7 @@ void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_pt
                      trace_lds_bytes(kLds, block), ctx->stream, count_ptr, count_host, ro, rd, hits,
                      ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
                      ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), inst_scene(ctx),
-                     kCount ? ctx->d_tstats.as<unsigned long long>() : nullptr);
+                     kCount ? ctx->d_tstats.as<unsigned long long>() : nullptr, chunk_shift_of(ctx));
 }
 
 template <bool kDeposit, bool kInst, int kLds, bool kCount = false>
@@ -1714,7 +1719,7 @@ void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, u
                      trace_lds_bytes(kLds, block), ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ,
                      ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
                      ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b, inst_scene(ctx),
-                     kCount ? ctx->d_tstats.as<unsigned long long>() + 4 : nullptr);
+                     kCount ? ctx->d_tstats.as<unsigned long long>() + 4 : nullptr, chunk_shift_of(ctx));
 }
 
 int stack_lds_of(const gi_ctx *ctx) {
@@ -2443,7 +2448,7 @@ int gi_build_bvh_device_ex(gi_ctx *ctx, int32_t max_leaf, int32_t method, int32_
   std::string err;
   if (!device_bvh_build(ctx->stream, ctx->d_stris.as<float4>(), (uint32_t)(sizeof(ShadeTri) / sizeof(float4)), n,
                         ctx->clo, ctx->chi, ml, method == GI_BVH_PLOC ? DeviceBvhMethod::kPloc : DeviceBvhMethod::kLbvh,
-                        rad, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), r, err))
+                        rad, ctx->tuning.node_width, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), r, err))
     return fail(ctx, GI_ERR_HIP, err);
   return finish_bvh(ctx, r.nnodes, r.depth);
 }
@@ -2995,7 +3000,19 @@ int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t) {
     return fail(ctx, GI_ERR_INVALID_ARG, "stack_lds must be 8, 16 or 32");
   if (n.target_slots < 0 || n.target_slots > 64 || n.trace_blocks_per_cu > 64 || n.samples_per_pass < 0)
     return fail(ctx, GI_ERR_INVALID_ARG, "target_slots in [0,64], trace_blocks_per_cu <= 64");
+  if (n.node_width == 0) n.node_width = 4;
+  if (n.refill_lanes == 0) n.refill_lanes = 32;
+  if (n.node_width < 2 || n.node_width > 4) return fail(ctx, GI_ERR_INVALID_ARG, "node_width must be 2, 3 or 4");
+  if (n.refill_lanes != 8 && n.refill_lanes != 16 && n.refill_lanes != 32 && n.refill_lanes != 64)
+    return fail(ctx, GI_ERR_INVALID_ARG, "refill_lanes must be 8, 16, 32 or 64");
+  const bool recollapse = n.node_width != ctx->tuning.node_width && ctx->bvh && ctx->have_bvh;
+  if (recollapse && ctx->n_inst > 0)
+    return fail(ctx, GI_ERR_STATE, "node_width changes the world tree: set it before gi_upload_instances");
   ctx->tuning = n;
+  if (recollapse) {
+    GI_HIP(ctx, hipSetDevice(ctx->device));
+    return upload_wide(ctx);
+  }
   return GI_OK;
 }
 

@@ -165,6 +165,14 @@ typedef struct {
   int32_t samples_per_pass;
   int32_t stack_lds;        /* traversal stack entries per lane in LDS: 8, 16 or 32 */
   int32_t target_slots;     /* paths in flight per pass, in units of 2^20; 0 = 64 */
+  int32_t node_width;       /* child slots the collapse fills per BVH node: 2, 3 or 4
+                               (0 = 4).  A build parameter: a host-built world tree
+                               is collapsed again at once (refused while instances
+                               are placed); a device-built one at its next build.
+                               Mesh and instance trees keep 4. */
+  int32_t refill_lanes;     /* compaction threshold of the traversal kernels: a wave
+                               takes new rays once this many of its lanes are idle,
+                               in chunks of this many rays: 8, 16, 32 or 64 (0 = 32) */
 } gi_tuning;
 
 /* Per-material gradient of the adjoint pass. */

@@ -1,7 +1,8 @@
 // Host-only check of the binary -> 4-wide collapse (collapse_wide_nodes):
 // every primitive sits in exactly one leaf, every child box contains the
 // triangles below it, references and the reported depth are consistent,
-// and merged leaves stay within the merge limit.  Trees come from the SAH
+// merged leaves stay within the merge limit, and no node fills more child
+// slots than the width.  Trees come from the SAH
 // builder and from a median-split builder with one triangle per leaf (so
 // the collapse's leaf merging is exercised).  Prints one line per case and
 // exits non-zero on the first violation.
@@ -81,13 +82,19 @@ static void median_tree(const std::vector<gi_tri> &t, gi_bvh &b) {
   rec(0, 0, n);
 }
 
-static int check(const char *what, const std::vector<gi_tri> &t, const gi_bvh &b, int merge_limit) {
+static int check(const char *what, const std::vector<gi_tri> &t, const gi_bvh &b, int merge_limit, int width) {
   const int n = (int)t.size();
   std::vector<WideNode> w;
   int32_t depth = -1;
   collapse_wide_nodes(
       b, [](const Bvh2Node &nd) { return kLeafBit | ((uint32_t)nd.count << 27) | (uint32_t)nd.first; }, 0u, w, depth,
-      merge_limit);
+      merge_limit, width);
+  for (const WideNode &nd : w)
+    for (int k = width; k < 4; ++k)
+      if (nd.child[k] != kEmptyRef) {
+        std::printf("FAIL %s: slot %d used at width %d\n", what, k, width);
+        return 1;
+      }
   std::vector<int> seen(n, 0);
   int maxd = 0, leaves = 0, big = 0;
   bool ok = true;
@@ -149,7 +156,8 @@ static int check(const char *what, const std::vector<gi_tri> &t, const gi_bvh &b
     std::printf("FAIL %s: %d leaves above the merge limit\n", what, big);
     return 1;
   }
-  std::printf("ok %s n=%d merge=%d wide=%zu leaves=%d depth=%d\n", what, n, merge_limit, w.size(), leaves, depth);
+  std::printf("ok %s n=%d merge=%d width=%d wide=%zu leaves=%d depth=%d\n", what, n, merge_limit, width, w.size(),
+              leaves, depth);
   return 0;
 }
 
@@ -160,9 +168,10 @@ int main() {
     build_bvh2(t.data(), n, 4, sah);
     gi_bvh med;
     median_tree(t, med);
-    for (int m : {1, 4, 8}) {
-      if (check("sah", t, sah, m)) return 1;
-      if (check("median", t, med, m)) return 1;
+    const int cases[5][2] = {{1, 4}, {4, 4}, {8, 4}, {4, 2}, {4, 3}};  // (merge limit, width)
+    for (const auto &c : cases) {
+      if (check("sah", t, sah, c[0], c[1])) return 1;
+      if (check("median", t, med, c[0], c[1])) return 1;
     }
   }
   return 0;

@@ -1,0 +1,76 @@
+"""Tuning parameters and the autotuner on the GPU: images do not depend on
+node width (2, 3, 4 child slots per BVH node, host- and device-built trees)
+or on the traversal kernels' refill threshold (8..64 lanes); narrower nodes
+mean more node visits; invalid values and a node-width change under placed
+instances are refused; the autotuner returns a setting from its space,
+leaves it applied, and the image rendered with it is the default one."""
+import numpy as np
+import pytest
+
+from globillum_amd import autotune, scenes
+from globillum_amd._lib import FLAG_BATCHED, GiError
+
+pytestmark = pytest.mark.gpu
+FMAX = np.finfo(np.float32).max
+
+
+@pytest.fixture(scope="module")
+def ts():
+    return scenes.terrain_spheres(cells=(32, 32), n_spheres=40, w=64, h=48, spp=4, depth=4)
+
+
+def test_node_width_and_refill_keep_the_image(ts, gpu_ctx_factory):
+    with gpu_ctx_factory(ts) as ctx:
+        ref = ctx.render(flags=FLAG_BATCHED).image(0).copy()
+        c0 = ctx.counters()
+        nodes = {}
+        for w in (2, 3, 4):
+            ctx.set_tuning(node_width=w)
+            assert ctx.tuning()["node_width"] == w
+            ctx.traversal_stats(True)
+            assert np.array_equal(ctx.render(flags=FLAG_BATCHED).image(0), ref), w
+            s = ctx.traversal_stats()
+            ctx.traversal_stats(False)
+            nodes[w] = s["closest_nodes"] / s["closest_rays"]
+            assert ctx.counters()["rays_closest"] == c0["rays_closest"]
+        assert nodes[2] > nodes[3] > nodes[4]
+        for r in (8, 16, 64, 32):
+            ctx.set_tuning(refill_lanes=r)
+            assert np.array_equal(ctx.render(flags=FLAG_BATCHED).image(0), ref), r
+        # a device-built tree at width 2 and 3
+        for w in (2, 3):
+            ctx.set_tuning(node_width=w)
+            ctx.build_bvh_device(4, "ploc")
+            assert np.array_equal(ctx.render(flags=FLAG_BATCHED).image(0), ref), ("ploc", w)
+
+
+def test_tuning_errors(ts, gpu_ctx_factory):
+    with gpu_ctx_factory(ts) as ctx:
+        for kw in ({"node_width": 1}, {"node_width": 5}, {"refill_lanes": 12}, {"refill_lanes": 128}):
+            with pytest.raises(GiError) as e:
+                ctx.set_tuning(**kw)
+            assert e.value.status == 1, kw
+        assert ctx.tuning()["node_width"] == 4 and ctx.tuning()["refill_lanes"] == 32
+    inst = scenes.instanced_terrain(cells=(16, 16), n_instances=20, w=32, h=24, spp=1, depth=2)
+    with gpu_ctx_factory(inst) as ctx:
+        with pytest.raises(GiError) as e:
+            ctx.set_tuning(node_width=3)
+        assert e.value.status == 4
+        ctx.set_tuning(refill_lanes=16)  # launch parameters stay free
+
+
+def test_autotuner(ts, gpu_ctx_factory):
+    space = {"node_width": [3, 4], "refill_lanes": [16, 32], "trace_blocks_per_cu": [4, 5]}
+    with gpu_ctx_factory(ts) as ctx:
+        ref = ctx.render(flags=FLAG_BATCHED).image(0).copy()
+        opts = ts.opts.copy()
+        opts["flags"] = FLAG_BATCHED
+        res = autotune.tune(ctx, ts.views, opts, space=space, repeats=1)
+        p = res["params"]
+        for k, vals in space.items():
+            assert p[k] in vals
+        assert len(res["trials"]) == 1 + sum(len(v) - 1 for v in space.values())
+        assert all(t["seconds"] > 0 for t in res["trials"]) and res["seconds"] > 0
+        applied = ctx.tuning()
+        assert all(applied[k] == v for k, v in p.items())
+        assert np.array_equal(ctx.render(flags=FLAG_BATCHED).image(0), ref)

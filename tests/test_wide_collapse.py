@@ -2,8 +2,9 @@
 program over slot counts), checked on the CPU by tests/csrc/collapse_check.cpp
 compiled against the library's own bvh_build.cpp: every primitive in exactly
 one leaf, every child box containing the triangles below it, consistent
-references and depth, merged leaves within the limit -- on SAH trees and on
-median-split trees with one triangle per leaf, where the collapse merges."""
+references and depth, merged leaves within the limit, at most `width` child
+slots filled (widths 2, 3, 4) -- on SAH trees and on median-split trees
+with one triangle per leaf, where the collapse merges."""
 import os
 import shutil
 import subprocess
@@ -27,7 +28,11 @@ def test_collapse_invariants(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:]
     lines = r.stdout.splitlines()
-    assert len(lines) == 42 and all(l.startswith("ok ") for l in lines)
-    # the median trees (one triangle per leaf) do get merged leaves at limit 4 / 8
-    med = {(l.split()[2], l.split()[3]): int(l.split()[5].split("=")[1]) for l in lines if l.split()[1] == "median"}
-    assert med[("n=20000", "merge=4")] < med[("n=20000", "merge=1")]
+    assert len(lines) == 70 and all(l.startswith("ok ") for l in lines)
+    f = {tuple(l.split()[1:5]): {kv.split("=")[0]: int(kv.split("=")[1]) for kv in l.split()[5:]} for l in lines}
+    # the median trees (one triangle per leaf) do get merged leaves at limit 4
+    assert f[("median", "n=20000", "merge=4", "width=4")]["leaves"] < f[("median", "n=20000", "merge=1", "width=4")]["leaves"]
+    # narrower nodes: more of them, deeper trees
+    for tree in ("sah", "median"):
+        w2, w3, w4 = (f[(tree, "n=20000", "merge=4", f"width={w}")] for w in (2, 3, 4))
+        assert w2["wide"] > w3["wide"] > w4["wide"] and w2["depth"] > w4["depth"]
