@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--graph", action="store_true", help="replay the render as one hipGraph")
     p.add_argument("--no-batching", action="store_true", help="skip the multi-view batching leg")
-    p.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batching leg")
+    p.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batching and small legs")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     p.add_argument("--no-bvh", action="store_true", help="skip the BVH construction leg")
     p.add_argument("--no-instancing", action="store_true", help="skip the two-level instancing leg")
@@ -88,10 +88,11 @@ def read_bandwidth():
     return out
 
 
-def small_leg(steps: int, with_cpu: bool, bw: dict | None):
+def small_leg(steps: int, with_cpu: bool, bw: dict | None, tune: bool):
     """The small plain configuration (cornell_boxes: 5 walls and 2 boxes, 34
     triangles, 1 rect light of emission 10, Lambert albedo U[0.2, 0.8],
-    256x256, 16 spp, depth 4), rendered as one hipGraph: Mrays/s,
+    256x256, 16 spp, depth 4), rendered as one hipGraph with autotuned
+    launch parameters: Mrays/s,
     samples/s, the relative RMSE of its image against the same scene at
     1024 spp, and the closest-hit kernel's algorithmic bytes (box and
     triangle test counts from the oracle on the same BVH, which runs the
@@ -100,7 +101,7 @@ def small_leg(steps: int, with_cpu: bool, bw: dict | None):
     against the HBM peak."""
     import numpy as np
 
-    from globillum_amd import scenes
+    from globillum_amd import autotune, scenes
     from globillum_amd._lib import FLAG_BATCHED, FLAG_GRAPH
     from globillum_amd.context import Context, build_bvh2
 
@@ -109,8 +110,15 @@ def small_leg(steps: int, with_cpu: bool, bw: dict | None):
     flags = FLAG_BATCHED | FLAG_GRAPH
     tot = {"rays_closest": 0, "rays_shadow": 0, "samples": 0}
     kern = {"rays_closest": 0, "ms_trace_closest": 0.0, "launches_closest": 0}
+    tuned = None
     with Context(0) as ctx:
         ctx.upload(sc, bvh2=bvh2)
+        if tune:
+            o = sc.opts.copy()
+            o["flags"] = flags
+            # launch shape only: the tree stays the exported one the oracle counts on
+            space = {k: v for k, v in autotune.SPACE.items() if k != "node_width"}
+            tuned = autotune.tune(ctx, sc.views, o, space=space, repeats=3)["params"]
         for _ in range(3):
             ctx.render(flags=flags)
         t0 = time.perf_counter()
@@ -136,7 +144,8 @@ def small_leg(steps: int, with_cpu: bool, bw: dict | None):
                        "depth 4 (the small plain configuration), batched + hipGraph",
            "mrays_per_s": round(rays / dt / 1e6, 2), "samples_per_s": round(tot["samples"] / dt, 1),
            "ms_per_frame": round(dt / steps * 1e3, 3),
-           "rmse_rel_vs_1024spp": round(float(np.sqrt(((img - ref) ** 2).mean()) / np.sqrt((ref ** 2).mean())), 5)}
+           "rmse_rel_vs_1024spp": round(float(np.sqrt(((img - ref) ** 2).mean()) / np.sqrt((ref ** 2).mean())), 5),
+           "tuned": tuned}
     if with_cpu:
         base, per_ray = cpu_baseline(sc, bvh2, full=True)
         out["cpu_baseline"] = base
@@ -511,7 +520,7 @@ def main():
     }
     ctx.close()
     if not a.no_small:
-        out["small"] = small_leg(20, not a.no_cpu, bw)
+        out["small"] = small_leg(20, not a.no_cpu, bw, not a.no_tune)
     if not a.no_batching:
         out["batching"] = batching_leg(max(1, min(a.steps, 3)), not a.no_tune)
     if not a.no_adjoint:
