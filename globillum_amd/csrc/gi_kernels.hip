@@ -194,6 +194,12 @@ constexpr float kSlabSlack = 1.0f + 0x1p-21f;
 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
+// Largest scene (wide nodes + intersection triangles) that the traversal
+// kernels copy into each block's LDS: nodes and triangles are then read
+// with ds_read_b128 instead of from L2.  16 KB keeps 5 blocks of 256 lanes
+// (16 KB of stack each) within a CU's 160 KB.
+constexpr size_t kLdsSceneMaxBytes = 16384;
+
 template <int kLdsDepth>
 struct Stack {
   lds_u32 *lds;    // this lane's LDS column: entry k at lds[k << shift]
@@ -243,11 +249,14 @@ __device__ __forceinline__ bool special_ref(uint32_t ref) {
 // tests and instance entries, and the wave adds them to stats[0..3] at the
 // end (traversal statistics; a separate instantiation).
 // cs: log2 of the chunk size (the refill threshold, 8..64 rays).
-template <bool kAny, bool kInst, int kLdsDepth, bool kCount, class Start, class Finish>
+// kLdsScene: the block's LDS holds a copy of the nodes (8 float4 each) at
+// lnodes and of the triangles at ltris; they are read from there.
+template <bool kAny, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene, class Start, class Finish>
 __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, const WideNode *__restrict__ nodes,
-                                                 const float4 *__restrict__ itris, Stack<kLdsDepth> &st,
-                                                 uint32_t *overflow, InstScene IS, unsigned long long *stats,
-                                                 Start start, Finish finish) {
+                                                 const float4 *__restrict__ itris, const float4 *lnodes,
+                                                 const float4 *ltris, Stack<kLdsDepth> &st, uint32_t *overflow,
+                                                 InstScene IS, unsigned long long *stats, Start start,
+                                                 Finish finish) {
   uint32_t c_rays = 0, c_nodes = 0, c_tris = 0, c_inst = 0;
   const uint32_t lane = __lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
@@ -297,9 +306,18 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
     for (;;) {
       if (!(ref & kLeafBit)) {
         if constexpr (kCount) ++c_nodes;
-        const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
-        const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-        const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
+        float4 lox, hix, loy, hiy, loz, hiz;
+        uint4 ch;
+        if constexpr (kLdsScene) {
+          const float4 *np = lnodes + 8u * ref;  // in LDS: derived from the shared base
+          lox = np[0]; hix = np[1]; loy = np[2]; hiy = np[3]; loz = np[4]; hiz = np[5];
+          const float4 c = np[6];
+          ch = make_uint4(__float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.z), __float_as_uint(c.w));
+        } else {
+          const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
+          lox = np[0]; hix = np[1]; loy = np[2]; hiy = np[3]; loz = np[4]; hiz = np[5];
+          ch = *reinterpret_cast<const uint4 *>(np + 6);
+        }
         const float lim = kAny ? tmax : smin(bt, tmax);
         float dist[4];
         uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
@@ -387,7 +405,10 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
     } else if (parked != kDoneRef) {
       // triangle j+1 is loaded while triangle j is tested
       const uint32_t cnt = (parked >> 27) & 15u;
-      const float4 *tp = itris + 3 * (parked & kMaxLeafFirst);
+      const auto tp = [&]() {
+        if constexpr (kLdsScene) return ltris + 3 * (parked & kMaxLeafFirst);
+        else return itris + 3 * (parked & kMaxLeafFirst);
+      }();
       float4 na = make_float4(0.f, 0.f, 0.f, 0.f), n1 = na, n2 = na;
       if (cnt > 0) {
         na = tp[0];
@@ -438,17 +459,37 @@ constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the g
 // Closest-hit kernel over a ray queue.  kHasTmax: max_t in ray_o.w (the
 // primitive entry point); otherwise ray_o.w carries the Halton index and
 // max_t is FLT_MAX (camera and bounce rays, src/hybrid-pt.cpp:558).
-template <bool kHasTmax, bool kInst, int kLdsDepth, bool kCount>
+// kLdsScene: copy the scene (n_nodes wide nodes, n_tris triangles) into
+// LDS after the stack (stack_words 32-bit words), then traverse it there.
+template <bool kLdsScene>
+__device__ __forceinline__ void stage_scene(uint32_t *smem, uint32_t stack_words, const WideNode *nodes,
+                                            uint32_t n_nodes, const float4 *itris, uint32_t n_tris,
+                                            const float4 *&lnodes, const float4 *&ltris) {
+  if constexpr (kLdsScene) {
+    float4 *dst = reinterpret_cast<float4 *>(smem + stack_words);
+    const float4 *srcn = reinterpret_cast<const float4 *>(nodes);
+    for (uint32_t i = threadIdx.x; i < 8u * n_nodes; i += blockDim.x) dst[i] = srcn[i];
+    for (uint32_t i = threadIdx.x; i < 3u * n_tris; i += blockDim.x) dst[8u * n_nodes + i] = itris[i];
+    __syncthreads();
+    lnodes = dst;
+    ltris = dst + 8u * n_nodes;
+  }
+}
+
+template <bool kHasTmax, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_closest(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, float4 *__restrict__ hits, const WideNode *__restrict__ nodes,
     const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow,
-    InstScene IS, unsigned long long *stats, uint32_t chunk_shift) {
+    InstScene IS, unsigned long long *stats, uint32_t chunk_shift, uint32_t stack_words, uint32_t n_nodes,
+    uint32_t n_tris) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
-  trace_persistent<false, kInst, kLdsDepth, kCount>(
-      count, chunk_shift, nodes, itris, st, overflow, IS, stats,
+  const float4 *lnodes = nullptr, *ltris = nullptr;
+  stage_scene<kLdsScene>(smem, stack_words, nodes, n_nodes, itris, n_tris, lnodes, ltris);
+  trace_persistent<false, kInst, kLdsDepth, kCount, kLdsScene>(
+      count, chunk_shift, nodes, itris, lnodes, ltris, st, overflow, IS, stats,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
         const float4 ro = ray_o[i], rd = ray_d[i];
         o = xyz(ro);
@@ -462,18 +503,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WA
 // ray adds its precomputed NEE contribution to its path's colour (the
 // !is.valid() branch of src/hybrid-pt.cpp:355-365).  Otherwise writes the
 // occlusion byte (primitive entry point).
-template <bool kDeposit, bool kInst, int kLdsDepth, bool kCount>
+template <bool kDeposit, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_any(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
     uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes, const float4 *__restrict__ itris,
     uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
-    uint32_t rec_stride, int32_t b, InstScene IS, unsigned long long *stats, uint32_t chunk_shift) {
+    uint32_t rec_stride, int32_t b, InstScene IS, unsigned long long *stats, uint32_t chunk_shift,
+    uint32_t stack_words, uint32_t n_nodes, uint32_t n_tris) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
-  trace_persistent<true, kInst, kLdsDepth, kCount>(
-      count, chunk_shift, nodes, itris, st, overflow, IS, stats,
+  const float4 *lnodes = nullptr, *ltris = nullptr;
+  stage_scene<kLdsScene>(smem, stack_words, nodes, n_nodes, itris, n_tris, lnodes, ltris);
+  trace_persistent<true, kInst, kLdsDepth, kCount, kLdsScene>(
+      count, chunk_shift, nodes, itris, lnodes, ltris, st, overflow, IS, stats,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
         const float4 ro = ray_o[i], rd = ray_d[i];
         o = xyz(ro);
@@ -1701,25 +1745,39 @@ uint32_t chunk_shift_of(const gi_ctx *ctx) {
   return r == 8 ? 3u : r == 16 ? 4u : r == 64 ? 6u : 5u;
 }
 
-template <bool kHasTmax, bool kInst, int kLds, bool kCount = false>
-void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
-                      const float4 *ro, const float4 *rd, float4 *hits) {
-  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kInst, kLds, kCount>), dim3(grid), dim3(block),
-                     trace_lds_bytes(kLds, block), ctx->stream, count_ptr, count_host, ro, rd, hits,
-                     ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
-                     ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), inst_scene(ctx),
-                     kCount ? ctx->d_tstats.as<unsigned long long>() : nullptr, chunk_shift_of(ctx));
+// The world tree and its triangles fit the LDS copy (non-instanced scenes).
+bool lds_scene_of(const gi_ctx *ctx) {
+  return !instanced(ctx) &&
+         (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) <= kLdsSceneMaxBytes;
 }
 
-template <bool kDeposit, bool kInst, int kLds, bool kCount = false>
+template <bool kHasTmax, bool kInst, int kLds, bool kCount = false, bool kLdsScene = false>
+void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
+                      const float4 *ro, const float4 *rd, float4 *hits) {
+  const size_t stack_bytes = trace_lds_bytes(kLds, block);
+  const size_t scene_bytes =
+      kLdsScene ? (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) : 0;
+  hipLaunchKernelGGL((k_trace_closest<kHasTmax, kInst, kLds, kCount, kLdsScene>), dim3(grid), dim3(block),
+                     stack_bytes + scene_bytes, ctx->stream, count_ptr, count_host, ro, rd, hits,
+                     ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
+                     ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), inst_scene(ctx),
+                     kCount ? ctx->d_tstats.as<unsigned long long>() : nullptr, chunk_shift_of(ctx),
+                     (uint32_t)(stack_bytes / 4), (uint32_t)ctx->nnodes, (uint32_t)ctx->ntri);
+}
+
+template <bool kDeposit, bool kInst, int kLds, bool kCount = false, bool kLdsScene = false>
 void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
                   const float4 *ro, const float4 *rd, const float4 *contrib, float4 *pl, uint8_t *occ, float4 *rec,
                   uint32_t rec_stride, int32_t b) {
-  hipLaunchKernelGGL((k_trace_any<kDeposit, kInst, kLds, kCount>), dim3(grid), dim3(block),
-                     trace_lds_bytes(kLds, block), ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ,
+  const size_t stack_bytes = trace_lds_bytes(kLds, block);
+  const size_t scene_bytes =
+      kLdsScene ? (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) : 0;
+  hipLaunchKernelGGL((k_trace_any<kDeposit, kInst, kLds, kCount, kLdsScene>), dim3(grid), dim3(block),
+                     stack_bytes + scene_bytes, ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ,
                      ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
                      ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b, inst_scene(ctx),
-                     kCount ? ctx->d_tstats.as<unsigned long long>() + 4 : nullptr, chunk_shift_of(ctx));
+                     kCount ? ctx->d_tstats.as<unsigned long long>() + 4 : nullptr, chunk_shift_of(ctx),
+                     (uint32_t)(stack_bytes / 4), (uint32_t)ctx->nnodes, (uint32_t)ctx->ntri);
 }
 
 int stack_lds_of(const gi_ctx *ctx) {
@@ -1739,6 +1797,8 @@ int launch_closest(gi_ctx *ctx, bool has_tmax, const uint32_t *count_ptr, uint32
 #define GI_CL(T, II, LL) launch_closest_t<T, II, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, hits)
 #define GI_CL_L(T, II) \
   if (ctx->count_traversal) launch_closest_t<T, II, 16, true>(ctx, grid, block, count_ptr, count_host, ro, rd, hits); \
+  else if (!(II) && L == 16 && lds_scene_of(ctx)) \
+    launch_closest_t<T, false, 16, false, true>(ctx, grid, block, count_ptr, count_host, ro, rd, hits); \
   else if (L == 8) GI_CL(T, II, 8); \
   else if (L == 32) GI_CL(T, II, 32); \
   else GI_CL(T, II, 16)
@@ -1767,6 +1827,9 @@ int launch_any(gi_ctx *ctx, bool deposit, const uint32_t *count_ptr, uint32_t co
   if (ctx->count_traversal) \
     launch_any_t<D, II, 16, true>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, rec_stride, \
                                   b); \
+  else if (!(II) && L == 16 && lds_scene_of(ctx)) \
+    launch_any_t<D, false, 16, false, true>(ctx, grid, block, count_ptr, count_host, ro, rd, contrib, pl, occ, rec, \
+                                            rec_stride, b); \
   else if (L == 8) GI_ANY(D, II, 8); \
   else if (L == 32) GI_ANY(D, II, 32); \
   else GI_ANY(D, II, 16)

