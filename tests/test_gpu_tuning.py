@@ -59,7 +59,10 @@ def test_tuning_errors(ts, gpu_ctx_factory):
         with pytest.raises(GiError) as e:
             ctx.set_tuning(node_width=3)
         assert e.value.status == 4
-        ctx.set_tuning(refill_lanes=16)  # launch parameters stay free
+        ref = ctx.render(flags=FLAG_BATCHED).image(0).copy()
+        for r in (8, 64):  # launch parameters stay free, and the two-level kernels agree
+            ctx.set_tuning(refill_lanes=r)
+            assert np.array_equal(ctx.render(flags=FLAG_BATCHED).image(0), ref), r
 
 
 def test_autotuner(ts, gpu_ctx_factory):
