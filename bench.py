@@ -413,7 +413,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the path shards by view / image "
-              f"tile / spp range, which this build does not run", file=sys.stderr)
+              f"tile / spp range; a multi-GPU run is not implemented", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
