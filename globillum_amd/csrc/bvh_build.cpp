@@ -8,6 +8,7 @@
 #include <cstring>
 #include <limits>
 #include <functional>
+#include <future>
 #include <numeric>
 
 #include "gi_internal.h"
@@ -57,6 +58,10 @@ namespace {
 
 constexpr int kBins = 32;
 constexpr int kSahDepth = 48;  // beyond this depth, split at the object median
+// Subtrees of at least this many primitives, down to kParallelDepth, build
+// their right half on another thread.
+constexpr uint32_t kParallelMin = 1u << 16;
+constexpr int kParallelDepth = 5;
 
 struct Builder {
   const gi_tri *tris;
@@ -67,6 +72,7 @@ struct Builder {
   int32_t max_leaf;
   int32_t max_depth = 0;
   bool sah_leaves = true;  // false: split down to max_leaf primitives per leaf
+  bool parallel = false;   // build large right subtrees on other threads
 
   Builder(const gi_tri *t, int64_t n, int32_t ml, std::vector<uint32_t> &p, std::vector<Bvh2Node> &nd)
       : tris(t), tbox(n), cen(3 * n), perm(p), nodes(nd), max_leaf(ml) {
@@ -87,15 +93,20 @@ struct Builder {
     }
   }
 
-  int32_t make_leaf(int32_t node, uint32_t b, uint32_t e) {
+  static int32_t make_leaf(std::vector<Bvh2Node> &nodes, int32_t node, uint32_t b, uint32_t e) {
     nodes[node].left = nodes[node].right = -1;
     nodes[node].first = (int32_t)b;
     nodes[node].count = (int32_t)(e - b);
     return node;
   }
 
-  void build(int32_t node, uint32_t b, uint32_t e, int depth) {
-    max_depth = std::max(max_depth, depth);
+  void build(int32_t node, uint32_t b, uint32_t e, int depth) { build(nodes, node, b, e, depth, max_depth); }
+
+  // Subtree of node over perm[b, e) into `nodes`; children are appended in
+  // the order of a depth-first build (left subtree complete before the
+  // right), whichever thread builds them.
+  void build(std::vector<Bvh2Node> &nodes, int32_t node, uint32_t b, uint32_t e, int depth, int32_t &maxd) {
+    maxd = std::max(maxd, depth);
     Box bb, cb;
     for (uint32_t i = b; i < e; ++i) {
       bb.grow(tbox[perm[i]]);
@@ -106,7 +117,7 @@ struct Builder {
     pad_box(nodes[node].lo, nodes[node].hi);
     const uint32_t n = e - b;
     if (n <= (uint32_t)max_leaf) {
-      make_leaf(node, b, e);
+      make_leaf(nodes, node, b, e);
       return;
     }
     int axis = -1;
@@ -154,7 +165,7 @@ struct Builder {
       if (axis >= 0) {
         float sah = 1.0f + best / std::max(parent_area, 1e-30f);
         if (sah_leaves && sah >= leaf_cost && n <= (uint32_t)kMaxLeafTris) {
-          make_leaf(node, b, e);
+          make_leaf(nodes, node, b, e);
           return;
         }
         float ext = cb.hi[axis] - cb.lo[axis];
@@ -183,8 +194,32 @@ struct Builder {
     nodes[node].right = l + 1;
     nodes[node].first = 0;
     nodes[node].count = 0;
-    build(l, b, mid, depth + 1);
-    build(l + 1, mid, e, depth + 1);
+    if (!(parallel && n >= kParallelMin && depth < kParallelDepth)) {
+      build(nodes, l, b, mid, depth + 1, maxd);
+      build(nodes, l + 1, mid, e, depth + 1, maxd);
+      return;
+    }
+    // the right subtree on another thread, into its own array (its root at
+    // 0), appended after the left subtree: the same nodes in the same order
+    // as the sequential build
+    std::vector<Bvh2Node> sub(1);
+    sub.reserve(2 * (size_t)(e - mid));
+    int32_t sub_depth = 0;
+    auto right = std::async(std::launch::async, [&] { build(sub, 0, mid, e, depth + 1, sub_depth); });
+    build(nodes, l, b, mid, depth + 1, maxd);
+    right.get();
+    maxd = std::max(maxd, sub_depth);
+    const int32_t base = (int32_t)nodes.size();
+    auto remap = [&](int32_t c) { return c < 0 ? c : (c == 0 ? l + 1 : base + c - 1); };
+    nodes.resize((size_t)base + sub.size() - 1);
+    for (size_t j = 0; j < sub.size(); ++j) {
+      Bvh2Node nd = sub[j];
+      if (nd.count == 0 && nd.left >= 0) {
+        nd.left = remap(nd.left);
+        nd.right = remap(nd.right);
+      }
+      nodes[j == 0 ? (size_t)l + 1 : (size_t)base + j - 1] = nd;
+    }
   }
 };
 
@@ -201,7 +236,7 @@ inline float bitsf(int32_t i) {
 
 }  // namespace
 
-void build_bvh2(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh &out) {
+void build_bvh2(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh &out, int32_t threads) {
   out.nodes.clear();
   out.perm.resize(n);
   std::iota(out.perm.begin(), out.perm.end(), 0u);
@@ -214,11 +249,12 @@ void build_bvh2(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh &out) {
     return;
   }
   Builder bld(tris, n, max_leaf, out.perm, out.nodes);
+  bld.parallel = threads != 1;
   bld.build(0, 0, (uint32_t)n, 0);
   out.max_depth = bld.max_depth;
 }
 
-void build_bvh2_boxes(const float *boxes, int64_t n, int32_t max_leaf, gi_bvh &out) {
+void build_bvh2_boxes(const float *boxes, int64_t n, int32_t max_leaf, gi_bvh &out, int32_t threads) {
   out.nodes.clear();
   out.perm.resize(n);
   std::iota(out.perm.begin(), out.perm.end(), 0u);
@@ -232,6 +268,7 @@ void build_bvh2_boxes(const float *boxes, int64_t n, int32_t max_leaf, gi_bvh &o
   }
   Builder bld(boxes, n, max_leaf, out.perm, out.nodes);
   bld.sah_leaves = false;
+  bld.parallel = threads != 1;
   bld.build(0, 0, (uint32_t)n, 0);
   out.max_depth = bld.max_depth;
 }

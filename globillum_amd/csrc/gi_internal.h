@@ -70,7 +70,10 @@ namespace gi {
 
 // Binned-SAH build over the triangle list (host).  Boxes are padded by a
 // relative 2^-20 so the float slab tests stay conservative.
-void build_bvh2(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh &out);
+// threads == 1 builds on the calling thread only; otherwise subtrees of at
+// least 2^16 primitives near the root build on other threads.  Both give
+// the same nodes in the same order.
+void build_bvh2(const gi_tri *tris, int64_t n, int32_t max_leaf, gi_bvh &out, int32_t threads = 0);
 // Import a binary BVH in the reference's 2 x float4 layout.
 bool import_bvh2(const float *nodes, int64_t n_nodes, const uint32_t *indices, int64_t n_indices,
                  int64_t n_tris, gi_bvh &out, std::string &err);
@@ -97,7 +100,7 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
 void pad_box(float *lo, float *hi);
 // Binned-SAH build over boxes (lo xyz, hi xyz per primitive), split down to
 // max_leaf primitives per leaf.
-void build_bvh2_boxes(const float *boxes, int64_t n, int32_t max_leaf, gi_bvh &out);
+void build_bvh2_boxes(const float *boxes, int64_t n, int32_t max_leaf, gi_bvh &out, int32_t threads = 0);
 
 // Special references (count 0): an empty slot, the end of a traversal, the
 // return from an instance to world space, and instance i (kLeafBit | i + 1).

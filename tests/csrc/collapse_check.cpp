@@ -7,6 +7,7 @@
 // the collapse's leaf merging is exercised).  Prints one line per case and
 // exits non-zero on the first violation.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -161,7 +162,29 @@ static int check(const char *what, const std::vector<gi_tri> &t, const gi_bvh &b
   return 0;
 }
 
+// The threaded SAH build gives the same nodes and permutation as the
+// sequential one.
+static int check_parallel(int n) {
+  const std::vector<gi_tri> t = make_tris(n, 7u);
+  gi_bvh a, b;
+  auto t0 = std::chrono::steady_clock::now();
+  build_bvh2(t.data(), n, 4, a, 1);
+  auto t1 = std::chrono::steady_clock::now();
+  build_bvh2(t.data(), n, 4, b, 0);
+  auto t2 = std::chrono::steady_clock::now();
+  if (a.nodes.size() != b.nodes.size() || a.perm != b.perm || a.max_depth != b.max_depth ||
+      std::memcmp(a.nodes.data(), b.nodes.data(), a.nodes.size() * sizeof(Bvh2Node)) != 0) {
+    std::printf("FAIL parallel build n=%d differs\n", n);
+    return 1;
+  }
+  std::printf("ok parallel n=%d nodes=%zu sequential_ms=%.1f threaded_ms=%.1f\n", n, a.nodes.size(),
+              std::chrono::duration<double, std::milli>(t1 - t0).count(),
+              std::chrono::duration<double, std::milli>(t2 - t1).count());
+  return 0;
+}
+
 int main() {
+  if (check_parallel(300000)) return 1;
   for (int n : {1, 2, 3, 5, 17, 1000, 20000}) {
     const std::vector<gi_tri> t = make_tris(n, (unsigned)n);
     gi_bvh sah;

@@ -4,7 +4,8 @@ compiled against the library's own bvh_build.cpp: every primitive in exactly
 one leaf, every child box containing the triangles below it, consistent
 references and depth, merged leaves within the limit, at most `width` child
 slots filled (widths 2, 3, 4) -- on SAH trees and on median-split trees
-with one triangle per leaf, where the collapse merges."""
+with one triangle per leaf, where the collapse merges.  Also: the threaded
+SAH build produces the sequential build's nodes and permutation exactly."""
 import os
 import shutil
 import subprocess
@@ -22,13 +23,15 @@ def test_collapse_invariants(tmp_path):
     csrc = os.path.join(ROOT, "globillum_amd", "csrc")
     cmd = [hipcc, "-O2", "-std=c++17", "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I" + csrc,
            "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "csrc", "collapse_check.cpp"),
-           os.path.join(csrc, "bvh_build.cpp"), "-o", exe]
+           os.path.join(csrc, "bvh_build.cpp"), "-pthread", "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:]
     lines = r.stdout.splitlines()
-    assert len(lines) == 70 and all(l.startswith("ok ") for l in lines)
+    assert len(lines) == 71 and all(l.startswith("ok ") for l in lines)
+    assert lines[0].startswith("ok parallel n=300000")  # threaded SAH build == sequential, bit for bit
+    lines = lines[1:]
     f = {tuple(l.split()[1:5]): {kv.split("=")[0]: int(kv.split("=")[1]) for kv in l.split()[5:]} for l in lines}
     # the median trees (one triangle per leaf) do get merged leaves at limit 4
     assert f[("median", "n=20000", "merge=4", "width=4")]["leaves"] < f[("median", "n=20000", "merge=1", "width=4")]["leaves"]
