@@ -8,6 +8,7 @@
 #include <cstring>
 #include <limits>
 #include <functional>
+#include <memory>
 #include <future>
 #include <numeric>
 
@@ -335,19 +336,29 @@ void leaf_order_tris(const gi_bvh &b, const gi_tri *tris, std::vector<IsectTri> 
   // Leaf-ordered intersection triangles: e1 = b - a, e2 = c - a in float,
   // the same rounding the oracle's per-test subtraction produces; a.w holds
   // the triangle's index in tris.
-  leaf_tris.resize(b.perm.size());
-  for (size_t i = 0; i < b.perm.size(); ++i) {
-    const gi_tri &t = tris[b.perm[i]];
-    IsectTri &o = leaf_tris[i];
-    for (int k = 0; k < 3; ++k) {
-      o.a[k] = t.a[k];
-      o.e1[k] = t.b[k] - t.a[k];
-      o.e2[k] = t.c[k] - t.a[k];
+  const size_t n = b.perm.size();
+  leaf_tris.resize(n);
+  auto fill = [&](size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) {
+      const gi_tri &t = tris[b.perm[i]];
+      IsectTri &o = leaf_tris[i];
+      for (int k = 0; k < 3; ++k) {
+        o.a[k] = t.a[k];
+        o.e1[k] = t.b[k] - t.a[k];
+        o.e2[k] = t.c[k] - t.a[k];
+      }
+      o.a[3] = bitsf((int32_t)b.perm[i]);
+      o.e1[3] = 0.f;
+      o.e2[3] = 0.f;
     }
-    o.a[3] = bitsf((int32_t)b.perm[i]);
-    o.e1[3] = 0.f;
-    o.e2[3] = 0.f;
-  }
+  };
+  // a gather through the permutation: in chunks on other threads when large
+  constexpr size_t kChunkTris = 1u << 18;
+  std::vector<std::future<void>> parts;
+  for (size_t lo = kChunkTris; lo < n; lo += kChunkTris)
+    parts.push_back(std::async(std::launch::async, fill, lo, std::min(n, lo + kChunkTris)));
+  fill(0, std::min(n, kChunkTris));
+  for (auto &f : parts) f.get();
 }
 
 // Binary -> 4-wide collapse chosen by surface-area cost (the dynamic
@@ -375,10 +386,18 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
   const int W = std::max(2, std::min(4, (int)width));  // slots filled per wide node
   const auto &N = b.nodes;
   const size_t nn = N.size();
-  // post-order of the nodes reachable from the root
+  // post-order of the nodes reachable from the root: reverse index order
+  // when every child follows its parent (the builders here allocate so),
+  // else an explicit walk (imported trees)
   std::vector<int32_t> post;
-  post.reserve(nn);
-  {
+  bool children_after = true;
+  for (size_t i = 0; i < nn && children_after; ++i)
+    if (N[i].left >= 0) children_after = N[i].left > (int32_t)i && N[i].right > (int32_t)i;
+  if (children_after) {
+    post.resize(nn);
+    for (size_t i = 0; i < nn; ++i) post[i] = (int32_t)(nn - 1 - i);
+  } else {
+    post.reserve(nn);
     std::vector<std::pair<int32_t, bool>> st{{0, false}};
     while (!st.empty()) {
       const auto [i, open] = st.back();
@@ -392,12 +411,13 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
       st.push_back({N[i].left, false});
     }
   }
+  // per node, written before it is read (children precede parents in post)
   constexpr double kInf = std::numeric_limits<double>::infinity();
-  std::vector<double> C(5 * nn, 0.0);        // C[5n + i], i = 1..4
-  std::vector<int8_t> split(5 * nn, 0);      // left share of the best split of i slots
-  std::vector<int8_t> opened(5 * nn, 0);     // C[n][i] opens n (else it equals C[n][i-1])
-  std::vector<uint8_t> as_leaf(nn, 0), contiguous(nn, 1);
-  std::vector<int32_t> cnt(nn, 0), first(nn, 0);
+  std::unique_ptr<double[]> C(new double[5 * nn]);      // C[5n + i], i = 1..4
+  std::unique_ptr<int8_t[]> split(new int8_t[5 * nn]);  // left share of the best split of i slots
+  std::unique_ptr<int8_t[]> opened(new int8_t[5 * nn]); // C[n][i] opens n (else it equals C[n][i-1])
+  std::unique_ptr<uint8_t[]> as_leaf(new uint8_t[nn]), contiguous(new uint8_t[nn]);
+  std::unique_ptr<int32_t[]> cnt(new int32_t[nn]), first(new int32_t[nn]);
   for (const int32_t n : post) {
     const Bvh2Node &nd = N[n];
     const double dx = (double)nd.hi[0] - nd.lo[0], dy = (double)nd.hi[1] - nd.lo[1], dz = (double)nd.hi[2] - nd.lo[2];
@@ -408,7 +428,11 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
       cnt[n] = nd.count;
       first[n] = nd.first;
       as_leaf[n] = 1;
-      for (int i = 1; i <= 4; ++i) c[i] = area * nd.count;
+      contiguous[n] = 1;
+      for (int i = 1; i <= 4; ++i) {
+        c[i] = area * nd.count;
+        opened[5 * (size_t)n + i] = 0;
+      }
       continue;
     }
     const int32_t l = nd.left, r = nd.right;
@@ -418,6 +442,7 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
                     (first[r] == first[l] + cnt[l] || first[l] == first[r] + cnt[r]);
     const double *cl = &C[5 * (size_t)l], *cr = &C[5 * (size_t)r];
     double D[5] = {kInf, kInf, kInf, kInf, kInf};
+    for (int j = 0; j < 5; ++j) split[5 * (size_t)n + j] = 1;
     for (int j = 2; j <= W; ++j)
       for (int k = 1; k < j; ++k)
         if (cl[k] + cr[j - k] < D[j]) {

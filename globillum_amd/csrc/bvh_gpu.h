@@ -25,6 +25,12 @@ enum class DeviceBvhMethod { kLbvh = 0, kPloc = 1 };
 // radix tree over the Morton keys; kPloc: clustering with the given search
 // radius (1..64).  width: child slots filled per wide node (2..4).  Synchronises the stream; scratch memory is allocated and
 // freed inside.
+// The 3n float4 of the leaf-ordered intersection triangles (a with the
+// original index in w, b - a, c - a) from the ShadeTri-strided vertex table
+// and a leaf-order permutation already on the device.  Asynchronous.
+hipError_t launch_leaf_tris(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, const uint32_t *order,
+                            float4 *itris);
+
 bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, const float clo[3],
                       const float chi[3], int max_leaf, DeviceBvhMethod method, int radius, int width,
                       WideNode *nodes, float4 *itris, LbvhResult &res, std::string &err);

@@ -513,6 +513,13 @@ inline uint32_t blocks(size_t n) { return (uint32_t)std::max<size_t>(1, (n + kBl
     }                                                                 \
   } while (0)
 
+hipError_t launch_leaf_tris(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, const uint32_t *order,
+                            float4 *itris) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_itris, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, order, itris);
+  return hipGetLastError();
+}
+
 bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_t n, const float clo[3],
                       const float chi[3], int max_leaf, DeviceBvhMethod method, int radius, int width, WideNode *nodes,
                       float4 *itris, LbvhResult &res, std::string &err) {

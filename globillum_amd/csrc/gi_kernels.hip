@@ -1674,14 +1674,25 @@ int finish_bvh(gi_ctx *ctx, int64_t nnodes, int32_t wide_depth) {
 
 int upload_wide(gi_ctx *ctx) {
   std::vector<WideNode> wn;
-  std::vector<IsectTri> it;
   int32_t wd = 0;
-  collapse_wide(*ctx->bvh, ctx->tris.data(), wn, it, wd, ctx->tuning.node_width);
+  collapse_wide_nodes(
+      *ctx->bvh, [](const Bvh2Node &n) { return kLeafBit | ((uint32_t)n.count << 27) | (uint32_t)n.first; }, 0u, wn,
+      wd, kCollapseMergeLimit, ctx->tuning.node_width);
   GI_HIP(ctx, ctx->d_nodes.ensure(wn.size() * sizeof(WideNode)));
-  GI_HIP(ctx, ctx->d_itris.ensure(std::max<size_t>(1, it.size()) * sizeof(IsectTri)));
   GI_HIP(ctx, copy_sync(ctx, ctx->d_nodes.p, wn.data(), wn.size() * sizeof(WideNode), hipMemcpyHostToDevice));
-  if (!it.empty())
-    GI_HIP(ctx, copy_sync(ctx, ctx->d_itris.p, it.data(), it.size() * sizeof(IsectTri), hipMemcpyHostToDevice));
+  // the leaf-ordered intersection triangles are gathered on the device from
+  // the triangle table through the permutation (16 MB up for 4 M triangles,
+  // instead of 192 MB of gathered records)
+  const std::vector<uint32_t> &perm = ctx->bvh->perm;
+  GI_HIP(ctx, ctx->d_itris.ensure(std::max<size_t>(1, perm.size()) * sizeof(IsectTri)));
+  if (!perm.empty()) {
+    DBuf d_perm;
+    GI_HIP(ctx, d_perm.ensure(perm.size() * sizeof(uint32_t)));
+    GI_HIP(ctx, copy_sync(ctx, d_perm.p, perm.data(), perm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    GI_HIP(ctx, launch_leaf_tris(ctx->stream, ctx->d_stris.as<float4>(), (uint32_t)(sizeof(ShadeTri) / sizeof(float4)),
+                                 (uint32_t)perm.size(), d_perm.as<uint32_t>(), ctx->d_itris.as<float4>()));
+    GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
   return finish_bvh(ctx, (int64_t)wn.size(), wd);
 }
 
