@@ -36,13 +36,13 @@ HBM_PEAK_GBS = 8000.0
 # default tuning (one pass, 8 launches per frame, 18.7 M rays per launch),
 # from two rocprofv3 counter runs of `bench.py --steps 1 --warmup 0 --no-cpu
 # --no-batching --no-adjoint --no-bvh --no-instancing --no-small
-# --no-membench --copy-mib 512`: FETCH_SIZE 2,909,255 KiB and WRITE_SIZE
-# 511,091 KiB per dispatch, each scaled by what the counter reads for the
+# --no-membench --copy-mib 512`: FETCH_SIZE 2,914,794 KiB and WRITE_SIZE
+# 510,834 KiB per dispatch, each scaled by what the counter reads for the
 # 512 MiB device copy (FETCH 262,168 KiB -> x2.000, WRITE 524,288 KiB ->
 # x1.000).  FETCH counts L2 misses to the fabric, so Infinity Cache hits
 # are included: an upper bound of the HBM bytes.
-TRACE_CLOSEST_FETCH_BYTES = 2909255.1 * 1.99982 * 1024
-TRACE_CLOSEST_TRAFFIC_BYTES = TRACE_CLOSEST_FETCH_BYTES + 511090.5 * 1.0 * 1024
+TRACE_CLOSEST_FETCH_BYTES = 2914793.7 * 1.99982 * 1024
+TRACE_CLOSEST_TRAFFIC_BYTES = TRACE_CLOSEST_FETCH_BYTES + 510833.6 * 1.0 * 1024
 
 
 def parse():
