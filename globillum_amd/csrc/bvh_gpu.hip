@@ -345,10 +345,12 @@ __device__ __forceinline__ float union_area(float4 alo, float4 ahi, float4 blo, 
 
 // Nearest neighbour of every cluster within `radius` positions.  The block's
 // clusters and `radius` on either side are staged in LDS.
-__global__ void __launch_bounds__(kBlock) k_ploc_nn(const float4 *__restrict__ cbox, uint32_t cnt, int radius,
-                                                    int32_t *__restrict__ nn) {
+__global__ void __launch_bounds__(kBlock) k_ploc_nn(const float4 *__restrict__ cbox, const uint32_t *__restrict__ ctl,
+                                                    int radius, int32_t *__restrict__ nn) {
   __shared__ float4 slo[kBlock + 2 * kPlocMaxRadius], shi[kBlock + 2 * kPlocMaxRadius];
+  const uint32_t cnt = ctl[0];  // clusters this round (the grid covers an upper bound)
   const int start = (int)(blockIdx.x * kBlock);
+  if (start >= (int)cnt) return;  // whole block past the clusters
   const int lo = max(0, start - radius), hi = min((int)cnt, start + kBlock + radius);
   for (int k = lo + (int)threadIdx.x; k < hi; k += kBlock) {
     slo[k - lo] = cbox[2 * (size_t)k];
@@ -375,9 +377,14 @@ __global__ void __launch_bounds__(kBlock) k_ploc_nn(const float4 *__restrict__ c
 
 // flags[i]: low word 1 if cluster i survives the round (it does not merge
 // into a lower position), high word 1 if it starts a merge.
-__global__ void k_ploc_flags(const int32_t *__restrict__ nn, uint32_t cnt, uint64_t *__restrict__ flags) {
+__global__ void k_ploc_flags(const int32_t *__restrict__ nn, const uint32_t *__restrict__ ctl, uint32_t bound,
+                             uint64_t *__restrict__ flags) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cnt) return;
+  if (i >= bound) return;
+  if (i >= ctl[0]) {  // past this round's clusters: nothing for the scan
+    flags[i] = 0;
+    return;
+  }
   const int32_t m = nn[i];
   const bool mutual = m >= 0 && nn[m] == (int32_t)i;
   const uint64_t lead = mutual && (int32_t)i < m, keep = !(mutual && (int32_t)i > m);
@@ -388,23 +395,25 @@ __device__ __forceinline__ uint32_t leaves_of(const uint32_t *__restrict__ ncoun
   return c < 0 ? 1u : ncount[c];
 }
 
-// Merges the mutual pairs into interior nodes node_base + (merge rank) and
-// writes the surviving clusters in order.  totals = {clusters, merges}.
+// Merges the mutual pairs into interior nodes ctl[1] + (merge rank) and
+// writes the surviving clusters in order.  ctl = {clusters, nodes so far}
+// of this round; totals = the same for the next round.
 __global__ void k_ploc_merge(const int32_t *__restrict__ cl, const float4 *__restrict__ cbox,
                              const int32_t *__restrict__ nn, const uint64_t *__restrict__ flags,
-                             const uint64_t *__restrict__ pre, uint32_t cnt, uint32_t node_base,
+                             const uint64_t *__restrict__ pre, const uint32_t *__restrict__ ctl,
                              int32_t *__restrict__ left, int32_t *__restrict__ right, int32_t *__restrict__ parent,
                              int32_t *__restrict__ leaf_parent, uint32_t *__restrict__ ncount,
                              float4 *__restrict__ nodebox, int32_t *__restrict__ cl_out,
                              float4 *__restrict__ cbox_out, uint32_t *__restrict__ totals) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t cnt = ctl[0], node_base = ctl[1];  // clusters and nodes so far, this round
   if (i >= cnt) return;
   const uint64_t f = flags[i], p = pre[i];
   const uint32_t keep = (uint32_t)f, lead = (uint32_t)(f >> 32);
   const uint32_t slot = (uint32_t)p, rank = (uint32_t)(p >> 32);
   if (i == cnt - 1) {
     totals[0] = slot + keep;
-    totals[1] = rank + lead;
+    totals[1] = node_base + rank + lead;
   }
   if (!keep) return;
   float4 lo = cbox[2 * (size_t)i], hi = cbox[2 * (size_t)i + 1];
@@ -639,30 +648,41 @@ bool device_bvh_build(hipStream_t s, const float4 *tri, uint32_t stride, uint32_
                        cbox[0], cl[0]);
     GI_CK(hipGetLastError());
     GI_CK(hipMemsetAsync(parent, 0xFF, 4 * ni, s));  // the root keeps parent -1
-    uint32_t count = n, merged = 0;
+    // the cluster count and the nodes made so far live on the device
+    // (ctl[0..1] for even rounds, ctl[2..3] for odd ones); the host reads
+    // them back every kRoundsPerSync rounds and sizes the launches by the
+    // last count it read, an upper bound
+    constexpr int kRoundsPerSync = 4;
+    uint32_t *ctl = cnt;
+    const uint32_t ctl0[4] = {n, 0u, 0u, 0u};
+    GI_CK(hipMemcpyAsync(ctl, ctl0, 16, hipMemcpyHostToDevice, s));
+    uint32_t bound = n;
     int rounds = 0, cur = 0;
-    while (count > 1) {
-      hipLaunchKernelGGL(k_ploc_nn, dim3(blocks(count)), dim3(kBlock), 0, s, cbox[cur], count, radius, nn);
-      hipLaunchKernelGGL(k_ploc_flags, dim3(blocks(count)), dim3(kBlock), 0, s, nn, count, flags);
-      GI_CK(hipGetLastError());
-      GI_CK(hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, flags, pre, (int)count, s));
-      hipLaunchKernelGGL(k_ploc_merge, dim3(blocks(count)), dim3(kBlock), 0, s, cl[cur], cbox[cur], nn, flags, pre,
-                         count, merged, left, right, parent, leaf_parent, ncount, (float4 *)nodebox, cl[cur ^ 1],
-                         cbox[cur ^ 1], cnt);
-      GI_CK(hipGetLastError());
+    while (bound > 1) {
+      for (int r = 0; r < kRoundsPerSync; ++r) {
+        const uint32_t *c_in = ctl + 2 * (rounds & 1);
+        uint32_t *c_out = ctl + 2 * ((rounds + 1) & 1);
+        hipLaunchKernelGGL(k_ploc_nn, dim3(blocks(bound)), dim3(kBlock), 0, s, cbox[cur], c_in, radius, nn);
+        hipLaunchKernelGGL(k_ploc_flags, dim3(blocks(bound)), dim3(kBlock), 0, s, nn, c_in, bound, flags);
+        GI_CK(hipGetLastError());
+        GI_CK(hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, flags, pre, (int)bound, s));
+        hipLaunchKernelGGL(k_ploc_merge, dim3(blocks(bound)), dim3(kBlock), 0, s, cl[cur], cbox[cur], nn, flags, pre,
+                           c_in, left, right, parent, leaf_parent, ncount, (float4 *)nodebox, cl[cur ^ 1],
+                           cbox[cur ^ 1], c_out);
+        GI_CK(hipGetLastError());
+        cur ^= 1;
+        ++rounds;
+      }
       uint32_t tot[2] = {0, 0};
-      GI_CK(hipMemcpyAsync(tot, cnt, 8, hipMemcpyDeviceToHost, s));
+      GI_CK(hipMemcpyAsync(tot, ctl + 2 * (rounds & 1), 8, hipMemcpyDeviceToHost, s));
       GI_CK(hipStreamSynchronize(s));
       // every round merges at least the globally closest pair
-      if (tot[1] == 0 || tot[0] + tot[1] != count || merged + tot[1] > n - 1) {
+      if (tot[0] >= bound || tot[0] == 0 || tot[0] + tot[1] != n) {
         err = "device bvh: PLOC clustering stalled (non-finite vertex coordinates?)";
         for (void *q : mem) (void)hipFree(q);
         return false;
       }
-      merged += tot[1];
-      count = tot[0];
-      cur ^= 1;
-      ++rounds;
+      bound = tot[0];
     }
     res.rounds = rounds;
     if (n > 1) {
