@@ -51,8 +51,8 @@ extern "C" {
 // Read bandwidth (GB/s, 1e9 bytes) of `passes` launches, each reading a
 // zeroed buffer of `bytes` (a multiple of 16, at least 16 * 1024) on
 // `device` max(1, 2^30 / bytes) times, timed with HIP events after one
-// warm launch.  Returns 0 or the HIP error
-// code.  Allocates and frees its own buffers; synchronises the device.
+// warm launch.  Returns 0 or the HIP error code.  Makes `device` the
+// current device, allocates and frees its own buffers, and synchronises.
 int gi_membench_read(int32_t device, int64_t bytes, int32_t passes, double *gbs) {
   if (bytes < 16 * 1024 || bytes % 16 != 0 || passes < 1 || !gbs) return (int)hipErrorInvalidValue;
   hipError_t e = hipSetDevice(device);
