@@ -460,25 +460,44 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
     }
   }
   // the binary nodes that take the slots of a wide node opened at n
-  std::function<void(int32_t, int, std::vector<int32_t> &)> take = [&](int32_t n, int i, std::vector<int32_t> &out) {
-    while (i > 1 && N[n].left >= 0 && !opened[5 * (size_t)n + i]) --i;
-    if (i == 1 || N[n].left < 0) {
-      out.push_back(n);
-      return;
+  // (at most 4: a fixed array, filled depth first)
+  struct Kids {
+    int32_t id[4];
+    int n = 0;
+  };
+  auto take = [&](int32_t n0, int i0, Kids &out) {
+    int32_t sn[4];
+    int si[4], sp = 0;
+    sn[sp] = n0;
+    si[sp++] = i0;
+    while (sp > 0) {
+      const int32_t n = sn[--sp];
+      int i = si[sp];
+      while (i > 1 && N[n].left >= 0 && !opened[5 * (size_t)n + i]) --i;
+      if (i == 1 || N[n].left < 0) {
+        out.id[out.n++] = n;
+        continue;
+      }
+      const int k = split[5 * (size_t)n + i];
+      sn[sp] = N[n].right;  // right below left: the left share comes out first
+      si[sp++] = i - k;
+      sn[sp] = N[n].left;
+      si[sp++] = k;
     }
-    const int k = split[5 * (size_t)n + i];
-    take(N[n].left, k, out);
-    take(N[n].right, i - k, out);
   };
   std::vector<std::pair<int32_t, int32_t>> work;  // (binary node, wide slot), breadth first
+  wnodes.reserve(nn / 2 + 1);
+  work.reserve(nn / 2 + 1);
+  std::vector<int32_t> wdepth;
+  wdepth.reserve(nn / 2 + 1);
   wnodes.emplace_back();
-  std::vector<int32_t> wdepth{0};
+  wdepth.push_back(0);
   work.push_back({0, 0});
   for (size_t w = 0; w < work.size(); ++w) {
     const int32_t bi = work[w].first, wi = work[w].second;
-    std::vector<int32_t> kids;
+    Kids kids;
     if (N[bi].left < 0) {
-      kids.push_back(bi);  // root is a leaf: one child slot
+      kids.id[kids.n++] = bi;  // root is a leaf: one child slot
     } else {
       const int k = split[5 * (size_t)bi + W];
       take(N[bi].left, k, kids);
@@ -487,11 +506,11 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
     WideNode nd;
     std::memset(&nd, 0, sizeof(nd));
     for (int k = 0; k < 4; ++k) {
-      if (k >= (int)kids.size()) {
+      if (k >= kids.n) {
         set_empty_slot(nd, k);
         continue;
       }
-      const int32_t c = kids[k];
+      const int32_t c = kids.id[k];
       nd.lox[k] = N[c].lo[0]; nd.loy[k] = N[c].lo[1]; nd.loz[k] = N[c].lo[2];
       nd.hix[k] = N[c].hi[0]; nd.hiy[k] = N[c].hi[1]; nd.hiz[k] = N[c].hi[2];
       if (N[c].left < 0 || as_leaf[c]) {
