@@ -93,6 +93,11 @@ struct InstScene {
   const uint32_t *vbase;  // inst[i].vbase, packed for the search by id
   uint32_t tlas_root, n_world;
   int32_t n_inst;
+  // vtab[b]: the instance holding virtual id n_world + (b << vshift); a
+  // block of 2^vshift ids is no longer than the smallest instanced mesh, so
+  // at most the next instance also starts inside it.  Null: search vbase.
+  const uint32_t *vtab;
+  uint32_t vshift;
 };
 
 // Sky map of the scene's sky light and the tables of its Distribution2D.
@@ -777,11 +782,18 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
   const DevInstance *I = nullptr;
   if (tid >= P.is.n_world) {
     // instance triangle: the last instance whose virtual base is <= tid
-    int lo = 0, hi = P.is.n_inst - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (P.is.vbase[mid] <= tid) lo = mid;
-      else hi = mid - 1;
+    int lo;
+    if (P.is.vtab) {
+      lo = (int)P.is.vtab[(tid - P.is.n_world) >> P.is.vshift];
+      if (lo + 1 < P.is.n_inst && P.is.vbase[lo + 1] <= tid) ++lo;
+    } else {  // no table (more than 2^24 blocks): binary search
+      lo = 0;
+      int hi = P.is.n_inst - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P.is.vbase[mid] <= tid) lo = mid;
+        else hi = mid - 1;
+      }
     }
     I = &P.is.inst[lo];
     tid = I->stri + (tid - I->vbase);
@@ -1468,7 +1480,9 @@ struct gi_ctx {
   DBuf d_sky_rgb, d_sky_x, d_sky_y;  // sky map, per-row pdf|cdf, row pdf|cdf
   // instancing: meshes (object space), instances, the instance tree
   std::vector<std::vector<gi_tri>> meshes;
-  DBuf d_inst, d_inst_vbase;
+  DBuf d_inst, d_inst_vbase, d_inst_vtab;
+  uint32_t inst_vshift = 0;
+  bool use_vtab = false;
   // textures: one buffer per texture (level 0 + mip chain), their records
   struct TexHost {
     int32_t w, h, levels;
@@ -1654,7 +1668,8 @@ ShadeTri make_shadetri(const gi_tri &t) {
 
 InstScene inst_scene(const gi_ctx *ctx) {
   return InstScene{ctx->d_inst.as<DevInstance>(), ctx->d_inst_vbase.as<uint32_t>(), ctx->tlas_root,
-                   (uint32_t)ctx->ntri, ctx->n_inst};
+                   (uint32_t)ctx->ntri, ctx->n_inst, ctx->use_vtab ? ctx->d_inst_vtab.as<uint32_t>() : nullptr,
+                   ctx->inst_vshift};
 }
 bool instanced(const gi_ctx *ctx) { return ctx->n_inst > 0; }
 
@@ -2794,6 +2809,31 @@ int gi_upload_instances(gi_ctx *ctx, const gi_instance *inst, int32_t n) {
   for (int32_t i = 0; i < n; ++i) vb[i] = di[i].vbase;
   GI_HIP(ctx, ctx->d_inst_vbase.ensure((size_t)n * sizeof(uint32_t)));
   GI_HIP(ctx, copy_sync(ctx, ctx->d_inst_vbase.p, vb.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  // block table of the virtual ids: blocks of 2^shift ids, 2^shift no more
+  // than the smallest instanced mesh (each mesh has >= 1 triangle), so a
+  // block holds the start of at most one more instance than its entry
+  {
+    const uint64_t vtotal = (uint64_t)vcount - (uint64_t)n_world;
+    size_t min_mesh = SIZE_MAX;
+    for (int32_t i = 0; i < n; ++i) min_mesh = std::min(min_mesh, ctx->meshes[(size_t)inst[i].mesh].size());
+    uint32_t shift = 0;
+    while (shift < 31 && ((size_t)2 << shift) <= min_mesh) ++shift;
+    const size_t nblk = (size_t)((vtotal + ((uint64_t)1 << shift) - 1) >> shift);
+    ctx->use_vtab = nblk <= ((size_t)1 << 24);  // else the shading searches
+    if (ctx->use_vtab) {
+      std::vector<uint32_t> tab(std::max<size_t>(1, nblk));
+      int32_t cur = 0;
+      for (size_t blk = 0; blk < nblk; ++blk) {
+        const uint64_t id = (uint64_t)n_world + ((uint64_t)blk << shift);
+        while (cur + 1 < n && (uint64_t)vb[cur + 1] <= id) ++cur;
+        tab[blk] = (uint32_t)cur;
+      }
+      GI_HIP(ctx, ctx->d_inst_vtab.ensure(tab.size() * sizeof(uint32_t)));
+      GI_HIP(ctx,
+             copy_sync(ctx, ctx->d_inst_vtab.p, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    ctx->inst_vshift = shift;
+  }
   std::swap(ctx->d_nodes.p, nodes.p);
   std::swap(ctx->d_nodes.bytes, nodes.bytes);
   std::swap(ctx->d_itris.p, itris.p);
