@@ -937,12 +937,9 @@ constexpr uint32_t kRecBackground = 8u;
 // Shading of path vertex b.  The NEE contribution travels with the shadow
 // ray; the shadow kernel adds it when unoccluded.  rec != nullptr: also
 // record the vertex for the adjoint (see kRecBackground).
-// 4 waves per SIMD: the textured variant then fits 128 VGPRs with a few
-// spills (4 waves instead of 3 at its natural 153; measurably faster), the
-// untextured one keeps its 127 without any.
 template <bool kTex>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(
-    int32_t b, const uint32_t *__restrict__ count_ptr, ShadeParams P, const float4 *__restrict__ ray_o,
+__device__ __forceinline__ void shade_vertices(
+    int32_t b, const uint32_t *__restrict__ count_ptr, const ShadeParams &P, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, const float4 *__restrict__ tp_in, const float4 *__restrict__ hits,
     float4 *__restrict__ path_l, float4 *__restrict__ nro, float4 *__restrict__ nrd, float4 *__restrict__ ntp,
     uint32_t *__restrict__ next_count, float4 *__restrict__ sho, float4 *__restrict__ shd, float4 *__restrict__ shc,
@@ -1026,6 +1023,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
   }
 }
+
+// The untextured kernel at the compiler's own choice (127 VGPRs, 4 waves per
+// SIMD).  The textured one is held to 4 waves: it then fits 128 VGPRs with a
+// few spills instead of 153 at 3 waves, which measured faster; the same
+// target on the untextured kernel measured slower.
+#define GI_SHADE_ARGS                                                                                              \
+  int32_t b, const uint32_t *__restrict__ count_ptr, ShadeParams P, const float4 *__restrict__ ray_o,             \
+      const float4 *__restrict__ ray_d, const float4 *__restrict__ tp_in, const float4 *__restrict__ hits,        \
+      float4 *__restrict__ path_l, float4 *__restrict__ nro, float4 *__restrict__ nrd, float4 *__restrict__ ntp,  \
+      uint32_t *__restrict__ next_count, float4 *__restrict__ sho, float4 *__restrict__ shd,                      \
+      float4 *__restrict__ shc, uint32_t *__restrict__ shadow_count, float4 *__restrict__ rec, uint32_t rec_stride
+#define GI_SHADE_PASS                                                                                               \
+  b, count_ptr, P, ray_o, ray_d, tp_in, hits, path_l, nro, nrd, ntp, next_count, sho, shd, shc, shadow_count, rec, \
+      rec_stride
+__global__ __launch_bounds__(256) void k_shade(GI_SHADE_ARGS) { shade_vertices<false>(GI_SHADE_PASS); }
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade_tex(GI_SHADE_ARGS) {
+  shade_vertices<true>(GI_SHADE_PASS);
+}
+#undef GI_SHADE_ARGS
+#undef GI_SHADE_PASS
 
 // ------------------------------------------------------------- adjoint ----
 // Derivatives d f / d (color r, g, b, roughness, metallic) of the BRDF value
@@ -2053,7 +2070,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
       ++launches;
       if (timing && (rc = mark(0))) return rc;
       const int sg = (int)std::max<uint64_t>(1, std::min<uint64_t>((bound + sblock - 1) / sblock, (uint64_t)persist));
-      hipLaunchKernelGGL(SP.rdiff ? k_shade<true> : k_shade<false>, dim3(sg), dim3(sblock), 0, ctx->stream, b,
+      hipLaunchKernelGGL(SP.rdiff ? k_shade_tex : k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b,
                          cnt + b, SP, ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(),
                          ctx->q_tp[cur].as<float4>(), ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(),
                          ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(), ctx->q_tp[nxt].as<float4>(),
@@ -2163,7 +2180,7 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
                             ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
         if (rc) return rc;
         if (phase == 0) {
-          hipLaunchKernelGGL(SP.rdiff ? k_shade<true> : k_shade<false>, dim3(sg), dim3(sblock), 0, ctx->stream, b,
+          hipLaunchKernelGGL(SP.rdiff ? k_shade_tex : k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b,
                              cnt + b, SP,
                              ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
                              ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(), ctx->q_o[nxt].as<float4>(),
