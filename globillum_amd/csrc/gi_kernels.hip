@@ -1728,10 +1728,12 @@ int upload_wide(gi_ctx *ctx) {
   return finish_bvh(ctx, (int64_t)wn.size(), wd);
 }
 
-uint32_t read_overflow(gi_ctx *ctx) {
+// GI_OK, or GI_ERR_LIMIT when some traversal stack overflowed since the
+// counter was last cleared, or the copy's own error.
+int check_overflow(gi_ctx *ctx) {
   uint32_t v = 0;
-  (void)copy_sync(ctx, &v, ctx->d_overflow.p, 4, hipMemcpyDeviceToHost);
-  return v;
+  GI_HIP(ctx, copy_sync(ctx, &v, ctx->d_overflow.p, 4, hipMemcpyDeviceToHost));
+  return v != 0 ? fail(ctx, GI_ERR_LIMIT, "traversal stack overflow") : GI_OK;
 }
 
 // Global part of the traversal stacks: one column per thread of the
@@ -2129,8 +2131,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
   ctx->counters.ms_shade += ms_s;
   ctx->counters.launches_closest += launches;
   ctx->counters.passes += (int32_t)npass;
-  if (read_overflow(ctx) != 0) return fail(ctx, GI_ERR_LIMIT, "traversal stack overflow");
-  return GI_OK;
+  return check_overflow(ctx);
 }
 
 // Adjoint: per pass, a forward pass in record mode (the per-vertex
@@ -2212,8 +2213,7 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
       ctx->counters.rays_shadow += hc[k * ncnt + depth + 1 + b];
     }
   ctx->counters.passes += (int32_t)npass;
-  if (read_overflow(ctx) != 0) return fail(ctx, GI_ERR_LIMIT, "traversal stack overflow");
-  return GI_OK;
+  return check_overflow(ctx);
 }
 
 int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_render_opts *opts) {
@@ -2900,8 +2900,7 @@ int gi_trace_closest_device(gi_ctx *ctx, const float *d_orig, const float *d_dir
                       reinterpret_cast<float4 *>(d_out));
   if (rc) return rc;
   GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (read_overflow(ctx) != 0) return fail(ctx, GI_ERR_LIMIT, "traversal stack overflow");
-  return GI_OK;
+  return check_overflow(ctx);
 }
 
 int gi_trace_any_device(gi_ctx *ctx, const float *d_orig, const float *d_dir, const float *d_max_t, int64_t n,
@@ -2923,8 +2922,7 @@ int gi_trace_any_device(gi_ctx *ctx, const float *d_orig, const float *d_dir, co
                   nullptr, nullptr, d_occluded);
   if (rc) return rc;
   GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (read_overflow(ctx) != 0) return fail(ctx, GI_ERR_LIMIT, "traversal stack overflow");
-  return GI_OK;
+  return check_overflow(ctx);
 }
 
 int gi_trace_closest(gi_ctx *ctx, const float *orig, const float *dir, const float *max_t, int64_t n, gi_hit *out) {
