@@ -51,7 +51,9 @@ typedef enum {
   GI_ERR_HIP = 3,           /* HIP runtime error during a call                 */
   GI_ERR_STATE = 4,         /* call order violated (e.g. render before upload) */
   GI_ERR_OUT_OF_MEMORY = 5,
-  GI_ERR_LIMIT = 6,         /* size beyond a documented limit                  */
+  GI_ERR_LIMIT = 6,         /* size beyond a documented limit; also a traversal
+                               stack overflow (impossible for trees this library
+                               builds: stacks are sized from the tree's depth) */
   GI_ERR_UNKNOWN_NAME = 7   /* unknown algorithm name (gi_algorithm::select)   */
 } gi_status;
 
