@@ -33,7 +33,7 @@ OPTS = np.dtype([("max_depth", "<i4"), ("rr_start", "<i4"), ("seed", "<u4"), ("f
 HIT = np.dtype([("t", "<f4"), ("beta", "<f4"), ("gamma", "<f4"), ("ref", "<u4")])
 COUNTERS = np.dtype([("rays_closest", "<u8"), ("rays_shadow", "<u8"), ("samples", "<u8"),
                      ("path_vertices", "<u8"), ("ms_total", "<f8"), ("ms_trace_closest", "<f8"),
-                     ("ms_trace_any", "<f8"), ("ms_shade", "<f8"), ("ms_other", "<f8"),
+                     ("ms_trace_any", "<f8"), ("ms_shade", "<f8"), ("ms_graph_build", "<f8"),
                      ("launches_closest", "<i4"), ("passes", "<i4")])
 TUNING = np.dtype([("trace_block", "<i4"), ("trace_blocks_per_cu", "<i4"), ("shade_block", "<i4"),
                    ("samples_per_pass", "<i4"), ("stack_lds", "<i4"), ("target_slots", "<i4"),

@@ -32,6 +32,9 @@ class Context:
             raise GiError(rc, f"gi_create(device={device})")
         self.h = h
         self.scene = None
+        # bumped by every call that changes what the next gi_adjoint would
+        # differentiate (materials, lights, views, options, geometry)
+        self.generation = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -59,6 +62,7 @@ class Context:
         radius (builder="ploc"), or an external tree in the reference's
         2 x float4 layout (bvh2=(nodes, indices))."""
         tris = np.ascontiguousarray(scene.tris, TRI)
+        self.generation += 1
         self._ck(self.lib.gi_upload_triangles(self.h, ptr(tris), len(tris)))
         for t in getattr(scene, "textures", None) or []:
             self.upload_texture(t)
@@ -176,11 +180,16 @@ class Context:
         o = np.array(self.scene.opts if opts is None else opts, OPTS).reshape(1).copy()
         for k, v in kw.items():
             o[k] = v
-        self._views = views
+        self.generation += 1
+        self._views = None
         self._ck(self.lib.gi_render(self.h, ptr(views), len(views), ptr(o)))
+        self._views = views
+        self._opts = o
         return self
 
     def image(self, view: int = 0) -> np.ndarray:
+        if getattr(self, "_views", None) is None:
+            raise GiError(_lib.GI_ERR_STATE, "no render: render() or progress_begin() first")
         v = self._views[view]
         out = np.zeros((int(v["h"]), int(v["w"]), 3), np.float32)
         self._ck(self.lib.gi_get_image(self.h, view, ptr(out)))
@@ -192,8 +201,11 @@ class Context:
     def progress_begin(self, views=None, opts=None):
         views = np.ascontiguousarray(self.scene.views if views is None else views, VIEW).reshape(-1)
         o = np.array(self.scene.opts if opts is None else opts, OPTS).reshape(1).copy()
-        self._views = views
+        self.generation += 1
+        self._views = None
         self._ck(self.lib.gi_progress_begin(self.h, ptr(views), len(views), ptr(o)))
+        self._views = views
+        self._opts = o
 
     def progress_step(self) -> bool:
         done = C.c_int32(0)
@@ -216,11 +228,13 @@ class Context:
 
     def set_materials(self, mats):
         mats = np.ascontiguousarray(mats, MATERIAL)
+        self.generation += 1
         self._ck(self.lib.gi_upload_materials(self.h, ptr(mats), len(mats)))
         self.scene.mats = mats
 
     def set_lights(self, lights):
         lights = np.ascontiguousarray(lights, LIGHT)
+        self.generation += 1
         self._ck(self.lib.gi_upload_lights(self.h, ptr(lights), len(lights)))
         self.scene.lights = lights
 

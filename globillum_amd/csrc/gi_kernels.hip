@@ -1525,6 +1525,11 @@ struct gi_ctx {
   gi_render_opts opts{};
   std::vector<DBuf> images;
   DBuf d_views, d_passes, d_counters;
+  DBuf d_cam_view, d_cam_pass;  // gi_generate_camera_rays' own view and pass
+  // the executable graph of the last GI_FLAG_GRAPH render and its launch key
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint8_t> graph_key;
+  int graph_launches = 0;
   DBuf d_matgrad, d_rec, d_dl, d_gmats, d_gem;
   DBuf q_o[2], q_d[2], q_tp[2], q_hit, s_o, s_d, s_c, path_l;
   DBuf rdiff;  // 4 planes of float4 per slot: ray differentials (textures in use)
@@ -1796,16 +1801,26 @@ bool lds_scene_of(const gi_ctx *ctx) {
          (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) <= kLdsSceneMaxBytes;
 }
 
+// The global part of the traversal stacks and the overflow counter a launch
+// uses: the context's own, or other buffers with the grid capped to the
+// stack columns they hold.
+struct TraceScratch {
+  uint32_t *gstack = nullptr;
+  uint32_t *overflow = nullptr;
+  int max_grid = 0;  // 0: no cap beyond the persistent grid
+};
+
 template <bool kHasTmax, bool kInst, int kLds, bool kCount = false, bool kLdsScene = false>
 void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, uint32_t count_host,
-                      const float4 *ro, const float4 *rd, float4 *hits) {
+                      const float4 *ro, const float4 *rd, float4 *hits, const TraceScratch &ts) {
   const size_t stack_bytes = trace_lds_bytes(kLds, block);
   const size_t scene_bytes =
       kLdsScene ? (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) : 0;
   hipLaunchKernelGGL((k_trace_closest<kHasTmax, kInst, kLds, kCount, kLdsScene>), dim3(grid), dim3(block),
                      stack_bytes + scene_bytes, ctx->stream, count_ptr, count_host, ro, rd, hits,
-                     ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
-                     ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), inst_scene(ctx),
+                     ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
+                     ts.gstack ? ts.gstack : ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
+                     ts.overflow ? ts.overflow : ctx->d_overflow.as<uint32_t>(), inst_scene(ctx),
                      kCount ? ctx->d_tstats.as<unsigned long long>() : nullptr, chunk_shift_of(ctx),
                      (uint32_t)(stack_bytes / 4), (uint32_t)ctx->nnodes, (uint32_t)ctx->ntri);
 }
@@ -1833,17 +1848,19 @@ int stack_lds_of(const gi_ctx *ctx) {
 bool instanced(const gi_ctx *ctx);
 
 int launch_closest(gi_ctx *ctx, bool has_tmax, const uint32_t *count_ptr, uint32_t count_host, uint64_t n_bound,
-                   const float4 *ro, const float4 *rd, float4 *hits) {
+                   const float4 *ro, const float4 *rd, float4 *hits, const TraceScratch &ts = TraceScratch{}) {
   int grid, block;
   int rc = trace_grid_for(ctx, n_bound, grid, block);
   if (rc) return rc;
+  if (ts.max_grid > 0) grid = std::min(grid, ts.max_grid);
   const int L = stack_lds_of(ctx);
   const bool I = instanced(ctx);
-#define GI_CL(T, II, LL) launch_closest_t<T, II, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, hits)
+#define GI_CL(T, II, LL) launch_closest_t<T, II, LL>(ctx, grid, block, count_ptr, count_host, ro, rd, hits, ts)
 #define GI_CL_L(T, II) \
-  if (ctx->count_traversal) launch_closest_t<T, II, 16, true>(ctx, grid, block, count_ptr, count_host, ro, rd, hits); \
+  if (ctx->count_traversal) \
+    launch_closest_t<T, II, 16, true>(ctx, grid, block, count_ptr, count_host, ro, rd, hits, ts); \
   else if (!(II) && L == 16 && lds_scene_of(ctx)) \
-    launch_closest_t<T, false, 16, false, true>(ctx, grid, block, count_ptr, count_host, ro, rd, hits); \
+    launch_closest_t<T, false, 16, false, true>(ctx, grid, block, count_ptr, count_host, ro, rd, hits, ts); \
   else if (L == 8) GI_CL(T, II, 8); \
   else if (L == 32) GI_CL(T, II, 32); \
   else GI_CL(T, II, 16)
@@ -2010,43 +2027,83 @@ hipEvent_t ev(gi_ctx *ctx, size_t i) {
   return ctx->events[i];
 }
 
-// Run the passes.  sync_counts: read every bounce's queue length back to
-// the host and size the launches exactly (the reference's synchronous,
-// per-view flow); otherwise every launch uses the persistent grid and reads
-// its length from device memory, and nothing synchronises until the end.
-int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_counts, bool use_graph, bool timing) {
+// Everything the launches of enqueue_passes take as arguments or launch
+// shapes: buffer pointers, scalars, tuning, the plan.  A captured graph is
+// replayed only while this key is unchanged (buffer contents -- materials,
+// lights, views, pass descriptors -- are read by the kernels from device
+// memory, so they may change between replays).
+template <class T> void key_put(std::vector<uint8_t> &k, const T &v) {
+  const uint8_t *p = reinterpret_cast<const uint8_t *>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
+std::vector<uint8_t> launch_key(gi_ctx *ctx, const PassPlan &plan, int32_t depth, const ShadeParams &P) {
+  std::vector<uint8_t> k;
+  key_put(k, depth);
+  key_put(k, ctx->tuning);
+  key_put(k, ctx->count_traversal);
+  key_put(k, instanced(ctx));
+  key_put(k, lds_scene_of(ctx));
+  key_put(k, ctx->stack_cap);
+  key_put(k, ctx->nnodes);
+  key_put(k, ctx->ntri);
+  key_put(k, ctx->num_cus);
+  key_put(k, ctx->slot_cap);
+  for (const void *q : {ctx->d_passes.p, ctx->d_views.p, ctx->d_counters.p, ctx->q_o[0].p, ctx->q_o[1].p,
+                        ctx->q_d[0].p, ctx->q_d[1].p, ctx->q_tp[0].p, ctx->q_tp[1].p, ctx->q_hit.p, ctx->s_o.p,
+                        ctx->s_d.p, ctx->s_c.p, ctx->path_l.p, ctx->d_nodes.p, ctx->d_itris.p, ctx->d_gstack.p,
+                        ctx->d_overflow.p, ctx->d_tstats.p})
+    key_put(k, q);
+  // shading parameters, field by field (the struct has padding)
+  for (const void *q : {(const void *)P.tris, (const void *)P.mats, (const void *)P.lights, (const void *)P.H.dims,
+                        (const void *)P.H.chunks, (const void *)P.sky.rgb, (const void *)P.sky.ycdf,
+                        (const void *)P.sky.ypdf, (const void *)P.sky.xcdf, (const void *)P.sky.xpdf,
+                        (const void *)P.is.inst, (const void *)P.is.vbase, (const void *)P.is.vtab,
+                        (const void *)P.tex, (const void *)P.rdiff})
+    key_put(k, q);
+  key_put(k, P.nlights);
+  key_put(k, P.power_total);
+  key_put(k, P.off_b);
+  key_put(k, P.off_s);
+  for (int i = 0; i < 3; ++i) key_put(k, P.bg[i]);
+  key_put(k, P.depth);
+  key_put(k, P.rr_start);
+  key_put(k, P.sky.w);
+  key_put(k, P.sky.h);
+  key_put(k, P.sky.light);
+  key_put(k, P.sky.scale);
+  key_put(k, P.is.tlas_root);
+  key_put(k, P.is.n_world);
+  key_put(k, P.is.n_inst);
+  key_put(k, P.is.vshift);
+  key_put(k, P.rdiff_stride);
+  // launch shapes follow the passes' sizes (PassDesc is zero-initialised)
+  const uint8_t *pp = reinterpret_cast<const uint8_t *>(plan.passes.data());
+  k.insert(k.end(), pp, pp + plan.passes.size() * sizeof(PassDesc));
+  return k;
+}
+
+void drop_graph(gi_ctx *ctx) {
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  ctx->graph_exec = nullptr;
+  ctx->graph_key.clear();
+}
+
+// The launches of the passes on the context stream.  sync_counts: read
+// every bounce's queue length back to the host and size the launches
+// exactly (the reference's synchronous, per-view flow); otherwise every
+// launch uses the persistent grid and reads its length from device memory,
+// and nothing synchronises.  mark(cat) records a timing event after a
+// launch (cat -1 opens an interval), or is null.
+template <class Mark>
+int enqueue_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_counts, const ShadeParams &SP,
+                   int &launches, Mark mark) {
   const int32_t ncnt = 2 * depth + 1;  // ray queue counts [0..depth], shadow counts
   const size_t npass = plan.passes.size();
-  GI_HIP(ctx, ctx->d_passes.ensure(npass * sizeof(PassDesc)));
-  GI_HIP(ctx, hipMemcpyAsync(ctx->d_passes.p, plan.passes.data(), npass * sizeof(PassDesc), hipMemcpyHostToDevice,
-                             ctx->stream));
-  GI_HIP(ctx, ctx->d_counters.ensure(npass * ncnt * sizeof(uint32_t)));
-  int rc = ensure_workspace(ctx, plan.max_slots);
-  if (rc) return rc;
-  if ((rc = ensure_gstack(ctx))) return rc;
-  ShadeParams SP = shade_params(ctx, depth);
   const PassDesc *dpass = ctx->d_passes.as<PassDesc>();
   const DevView *dviews = ctx->d_views.as<DevView>();
   const int sblock = std::min(256, std::max(64, (ctx->tuning.shade_block / 64) * 64));
   const int persist = ctx->num_cus * 8;
-  double ms_c = 0, ms_a = 0, ms_s = 0;
-  int launches = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  if (use_graph) {
-    sync_counts = false;
-    timing = false;
-    GI_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  }
-  size_t evi = 0;
-  std::vector<std::pair<size_t, int>> marks;  // (event index, category)
-  auto mark = [&](int cat) -> int {
-    hipEvent_t e = ev(ctx, evi);
-    if (!e) return fail(ctx, GI_ERR_HIP, "hipEventCreate");
-    GI_HIP(ctx, hipEventRecord(e, ctx->stream));
-    marks.push_back({evi++, cat});
-    return GI_OK;
-  };
+  int rc = GI_OK;
   for (size_t pi = 0; pi < npass; ++pi) {
     const PassDesc &pd = plan.passes[pi];
     const PassDesc *dp = dpass + pi;
@@ -2065,12 +2122,12 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
         if (host_count == 0) break;
       }
       const uint64_t bound = sync_counts ? host_count : pd.total_slots;
-      if (timing && (rc = mark(-1))) return rc;
+      if ((rc = mark(-1))) return rc;
       rc = launch_closest(ctx, false, cnt + b, 0, bound, ctx->q_o[cur].as<float4>(),
                           ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
       if (rc) return rc;
       ++launches;
-      if (timing && (rc = mark(0))) return rc;
+      if ((rc = mark(0))) return rc;
       const int sg = (int)std::max<uint64_t>(1, std::min<uint64_t>((bound + sblock - 1) / sblock, (uint64_t)persist));
       hipLaunchKernelGGL(SP.rdiff ? k_shade_tex : k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b,
                          cnt + b, SP, ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(),
@@ -2079,7 +2136,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
                          cnt + b + 1, ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>(),
                          cnt + depth + 1 + b, (float4 *)nullptr, 0u);
       GI_HIP(ctx, hipGetLastError());
-      if (timing && (rc = mark(2))) return rc;
+      if ((rc = mark(2))) return rc;
       uint32_t shadow_host = 0;
       uint64_t sbound = bound;
       if (sync_counts) {
@@ -2092,22 +2149,79 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
                         ctx->s_c.as<float4>(), ctx->path_l.as<float4>(), nullptr);
         if (rc) return rc;
       }
-      if (timing && (rc = mark(1))) return rc;
+      if ((rc = mark(1))) return rc;
     }
     const int ag = (int)std::max<uint64_t>(1, std::min<uint64_t>((pd.total_pix + 255) / 256, (uint64_t)persist));
     hipLaunchKernelGGL(k_accumulate, dim3(ag), dim3(256), 0, ctx->stream, dp, dviews, ctx->path_l.as<float4>());
     GI_HIP(ctx, hipGetLastError());
   }
+  return GI_OK;
+}
+
+// Run the passes: per-view with host-read queue lengths (sync_counts), or
+// every launch on device-side counts; use_graph: the launches as one
+// hipGraph.  The executable graph is kept and relaunched by later renders
+// whose launches take the same arguments (launch_key); otherwise it is
+// captured and instantiated again (counters.ms_graph_build).
+int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_counts, bool use_graph, bool timing) {
+  const int32_t ncnt = 2 * depth + 1;
+  const size_t npass = plan.passes.size();
+  GI_HIP(ctx, ctx->d_passes.ensure(npass * sizeof(PassDesc)));
+  GI_HIP(ctx, hipMemcpyAsync(ctx->d_passes.p, plan.passes.data(), npass * sizeof(PassDesc), hipMemcpyHostToDevice,
+                             ctx->stream));
+  GI_HIP(ctx, ctx->d_counters.ensure(npass * ncnt * sizeof(uint32_t)));
+  int rc = ensure_workspace(ctx, plan.max_slots);
+  if (rc) return rc;
+  if ((rc = ensure_gstack(ctx))) return rc;
+  // the stack-overflow counter starts at zero for every render (a fill
+  // kernel rather than a memset node, ahead of any capture)
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_overflow.as<uint32_t>(), 1u, 0u);
+  GI_HIP(ctx, hipGetLastError());
+  ShadeParams SP = shade_params(ctx, depth);
+  double ms_c = 0, ms_a = 0, ms_s = 0;
+  int launches = 0;
   if (use_graph) {
-    GI_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
-    hipError_t e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-    if (e == hipSuccess) e = hipGraphLaunch(gexec, ctx->stream);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(ctx, GI_ERR_HIP, std::string("hipGraph: ") + hipGetErrorString(e));
-  }
-  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (timing) {
+    std::vector<uint8_t> key = launch_key(ctx, plan, depth, SP);
+    if (!ctx->graph_exec || key != ctx->graph_key) {
+      drop_graph(ctx);
+      const auto g0 = std::chrono::steady_clock::now();
+      GI_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_passes(ctx, plan, depth, false, SP, launches, [](int) { return GI_OK; });
+      hipGraph_t graph = nullptr;
+      // the capture always ends, so the stream is usable after an error
+      const hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
+      if (rc == GI_OK && ee != hipSuccess)
+        rc = fail(ctx, GI_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+      if (rc == GI_OK) {
+        const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+          ctx->graph_exec = nullptr;
+          rc = fail(ctx, GI_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+      }
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc) return rc;
+      ctx->graph_key = std::move(key);
+      ctx->counters.ms_graph_build +=
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
+    } else {
+      launches = ctx->graph_launches;
+    }
+    ctx->graph_launches = launches;
+    GI_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  } else {
+    size_t evi = 0;
+    std::vector<std::pair<size_t, int>> marks;  // (event index, category)
+    auto mark = [&](int cat) -> int {
+      if (!timing) return GI_OK;
+      hipEvent_t e = ev(ctx, evi);
+      if (!e) return fail(ctx, GI_ERR_HIP, "hipEventCreate");
+      GI_HIP(ctx, hipEventRecord(e, ctx->stream));
+      marks.push_back({evi++, cat});
+      return GI_OK;
+    };
+    if ((rc = enqueue_passes(ctx, plan, depth, sync_counts, SP, launches, mark))) return rc;
+    GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (size_t i = 1; i < marks.size(); ++i) {
       if (marks[i].second < 0) continue;
       float ms = 0.f;
@@ -2117,6 +2231,7 @@ int run_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_count
       else ms_s += ms;
     }
   }
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
   // telemetry
   std::vector<uint32_t> hc(npass * ncnt);
   GI_HIP(ctx, copy_sync(ctx, hc.data(), ctx->d_counters.p, hc.size() * 4, hipMemcpyDeviceToHost));
@@ -2150,6 +2265,8 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
   if (rc) return rc;
   if ((rc = ensure_gstack(ctx))) return rc;
   GI_HIP(ctx, ctx->d_rec.ensure((size_t)depth * plan.max_slots * sizeof(float4)));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_overflow.as<uint32_t>(), 1u, 0u);
+  GI_HIP(ctx, hipGetLastError());
   ShadeParams SP = shade_params(ctx, depth);
   const PassDesc *dpass = ctx->d_passes.as<PassDesc>();
   const DevView *dviews = ctx->d_views.as<DevView>();
@@ -2216,68 +2333,92 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
   return check_overflow(ctx);
 }
 
+// Camera basis and image-plane extent of one view (src/raygen.cu:17-39);
+// false (with the reason in err) for a view that cannot be rendered.  The
+// image pointer, index base and pixel base are left to the caller.
+bool make_devview(const gi_view &w, DevView &d, std::string &err) {
+  std::memset(&d, 0, sizeof(d));
+  if (w.w <= 0 || w.h <= 0 || w.spp <= 0) {
+    err = "view size / spp must be > 0";
+    return false;
+  }
+  if (!(w.fovy > 0.f && w.fovy < 180.f)) {
+    err = "fovy must be in (0, 180)";
+    return false;
+  }
+  const float dl = w.dir[0] * w.dir[0] + w.dir[1] * w.dir[1] + w.dir[2] * w.dir[2];
+  if (!(dl > 0.f) || !std::isfinite(dl)) {
+    err = "view dir";
+    return false;
+  }
+  float half = w.fovy / 2.0f;
+  d.height = (float)std::tan(3.14159265358979323846 * (double)half / 180.0);
+  d.width = ((float)w.w / (float)w.h) * d.height;
+  float vl = std::sqrt(w.dir[0] * w.dir[0] + w.dir[1] * w.dir[1] + w.dir[2] * w.dir[2]);
+  float W[3] = {w.dir[0] / vl, w.dir[1] / vl, w.dir[2] / vl};
+  float txw[3] = {w.up[1] * W[2] - w.up[2] * W[1], w.up[2] * W[0] - w.up[0] * W[2], w.up[0] * W[1] - w.up[1] * W[0]};
+  float tl = std::sqrt(txw[0] * txw[0] + txw[1] * txw[1] + txw[2] * txw[2]);
+  if (!(tl > 0.f)) {
+    err = "view up parallel to dir";
+    return false;
+  }
+  float U[3] = {txw[0] / tl, txw[1] / tl, txw[2] / tl};
+  float V[3] = {W[1] * U[2] - W[2] * U[1], W[2] * U[0] - W[0] * U[2], W[0] * U[1] - W[1] * U[0]};
+  for (int k = 0; k < 3; ++k) {
+    d.U[k] = U[k]; d.V[k] = V[k]; d.W[k] = W[k];
+    d.pos[k] = w.pos[k]; d.vd[k] = w.dir[k];
+  }
+  d.fw = (float)w.w;
+  d.fh = (float)w.h;
+  d.aperture = w.aperture;
+  d.focus = w.focus_distance;
+  d.w = w.w; d.h = w.h; d.spp = w.spp;
+  return true;
+}
+
+// The render state of a new render or progressive sequence: every view is
+// validated before any state changes; the views, options and images are
+// committed only when all of them (and their buffers) are in place.  On
+// failure the context holds no views (gi_adjoint, gi_progress_step and
+// gi_get_image then report the missing render) and no progressive state.
 int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_render_opts *opts) {
+  ctx->views.clear();
+  ctx->prog_active = false;
   if (!views || n_views <= 0 || !opts) return fail(ctx, GI_ERR_INVALID_ARG, "views/opts");
   if (opts->max_depth < 1 || opts->max_depth > kMaxDepth)
     return fail(ctx, GI_ERR_INVALID_ARG, "max_depth must be in [1, 32]");
+  std::vector<DevView> dv(n_views);
   uint64_t total = 0;
+  uint32_t base = 0, pix_base = 0;
+  std::string err;
   for (int32_t v = 0; v < n_views; ++v) {
     const gi_view &w = views[v];
-    if (w.w <= 0 || w.h <= 0 || w.spp <= 0) return fail(ctx, GI_ERR_INVALID_ARG, "view size / spp must be > 0");
-    if (!(w.fovy > 0.f && w.fovy < 180.f)) return fail(ctx, GI_ERR_INVALID_ARG, "fovy must be in (0, 180)");
-    const float dl = w.dir[0] * w.dir[0] + w.dir[1] * w.dir[1] + w.dir[2] * w.dir[2];
-    if (!(dl > 0.f) || !std::isfinite(dl)) return fail(ctx, GI_ERR_INVALID_ARG, "view dir");
+    if (!make_devview(w, dv[v], err)) return fail(ctx, GI_ERR_INVALID_ARG, err);
     total += (uint64_t)w.w * (uint64_t)w.h * (uint64_t)w.spp;
+    if (total >= (1ull << 32)) return fail(ctx, GI_ERR_LIMIT, "more than 2^32 samples in one render");
+    dv[v].index_base = base;
+    base += (uint32_t)((uint64_t)w.w * w.h * w.spp);
+    dv[v].pix_base = pix_base;
+    pix_base += (uint32_t)((uint64_t)w.w * w.h);
   }
-  if (total >= (1ull << 32)) return fail(ctx, GI_ERR_LIMIT, "more than 2^32 samples in one render");
   int rc = ensure_halton(ctx, opts->seed, dims_for_depth(opts->max_depth));
   if (rc) return rc;
-  ctx->views.assign(views, views + n_views);
-  ctx->opts = *opts;
-  ctx->images.resize(n_views);
-  std::vector<DevView> dv(n_views);
-  uint32_t base = 0, pix_base = 0;
+  if (ctx->images.size() < (size_t)n_views) ctx->images.resize(n_views);
   for (int32_t v = 0; v < n_views; ++v) {
-    const gi_view &w = views[v];
-    DevView &d = dv[v];
-    std::memset(&d, 0, sizeof(d));
-    // image-plane extent and camera basis: src/raygen.cu:17-39
-    float half = w.fovy / 2.0f;
-    d.height = (float)std::tan(3.14159265358979323846 * (double)half / 180.0);
-    d.width = ((float)w.w / (float)w.h) * d.height;
-    float vl = std::sqrt(w.dir[0] * w.dir[0] + w.dir[1] * w.dir[1] + w.dir[2] * w.dir[2]);
-    float W[3] = {w.dir[0] / vl, w.dir[1] / vl, w.dir[2] / vl};
-    float txw[3] = {w.up[1] * W[2] - w.up[2] * W[1], w.up[2] * W[0] - w.up[0] * W[2], w.up[0] * W[1] - w.up[1] * W[0]};
-    float tl = std::sqrt(txw[0] * txw[0] + txw[1] * txw[1] + txw[2] * txw[2]);
-    if (!(tl > 0.f)) return fail(ctx, GI_ERR_INVALID_ARG, "view up parallel to dir");
-    float U[3] = {txw[0] / tl, txw[1] / tl, txw[2] / tl};
-    float V[3] = {W[1] * U[2] - W[2] * U[1], W[2] * U[0] - W[0] * U[2], W[0] * U[1] - W[1] * U[0]};
-    for (int k = 0; k < 3; ++k) {
-      d.U[k] = U[k]; d.V[k] = V[k]; d.W[k] = W[k];
-      d.pos[k] = w.pos[k]; d.vd[k] = w.dir[k];
-    }
-    d.fw = (float)w.w;
-    d.fh = (float)w.h;
-    d.aperture = w.aperture;
-    d.focus = w.focus_distance;
-    d.w = w.w; d.h = w.h; d.spp = w.spp;
-    d.index_base = base;
-    base += (uint32_t)((uint64_t)w.w * w.h * w.spp);
-    d.pix_base = pix_base;
-    pix_base += (uint32_t)((uint64_t)w.w * w.h);
-    const size_t npix = (size_t)w.w * w.h;
+    const size_t npix = (size_t)views[v].w * views[v].h;
     GI_HIP(ctx, ctx->images[v].ensure(npix * sizeof(float4)));
-    d.image = ctx->images[v].as<float4>();
+    dv[v].image = ctx->images[v].as<float4>();
     const uint32_t nb = (uint32_t)((npix + 255) / 256);
-    hipLaunchKernelGGL(k_fill4, dim3(nb), dim3(256), 0, ctx->stream, d.image, (uint32_t)npix,
+    hipLaunchKernelGGL(k_fill4, dim3(nb), dim3(256), 0, ctx->stream, dv[v].image, (uint32_t)npix,
                        make_float4(0.f, 0.f, 0.f, 0.f));
     GI_HIP(ctx, hipGetLastError());
   }
   GI_HIP(ctx, ctx->d_views.ensure(dv.size() * sizeof(DevView)));
   GI_HIP(ctx, hipMemcpyAsync(ctx->d_views.p, dv.data(), dv.size() * sizeof(DevView), hipMemcpyHostToDevice,
                              ctx->stream));
-  GI_HIP(ctx, hipMemsetAsync(ctx->d_overflow.p, 0, 4, ctx->stream));
   GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->views.assign(views, views + n_views);
+  ctx->opts = *opts;
   return GI_OK;
 }
 
@@ -2334,6 +2475,7 @@ void gi_destroy(gi_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
   for (auto e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -2893,6 +3035,8 @@ int gi_trace_closest_device(gi_ctx *ctx, const float *d_orig, const float *d_dir
   if ((rc = ensure_gstack(ctx))) return rc;
   GI_HIP(ctx, ctx->p_ro.ensure(n * sizeof(float4)));
   GI_HIP(ctx, ctx->p_rd.ensure(n * sizeof(float4)));
+  // the stack-overflow counter counts this call only
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_overflow.as<uint32_t>(), 1u, 0u);
   hipLaunchKernelGGL(k_pack_rays, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, d_orig, d_dir,
                      d_max_t, (uint32_t)n, ctx->p_ro.as<float4>(), ctx->p_rd.as<float4>());
   GI_HIP(ctx, hipGetLastError());
@@ -2915,6 +3059,8 @@ int gi_trace_any_device(gi_ctx *ctx, const float *d_orig, const float *d_dir, co
   if ((rc = ensure_gstack(ctx))) return rc;
   GI_HIP(ctx, ctx->p_ro.ensure(n * sizeof(float4)));
   GI_HIP(ctx, ctx->p_rd.ensure(n * sizeof(float4)));
+  // the stack-overflow counter counts this call only
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_overflow.as<uint32_t>(), 1u, 0u);
   hipLaunchKernelGGL(k_pack_rays, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, d_orig, d_dir,
                      d_max_t, (uint32_t)n, ctx->p_ro.as<float4>(), ctx->p_rd.as<float4>());
   GI_HIP(ctx, hipGetLastError());
@@ -2969,11 +3115,15 @@ int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint3
                             float *max_t) {
   if (!ctx || !v || !orig || !dir || !max_t) return fail(ctx, GI_ERR_INVALID_ARG, "camera rays");
   if (sample < 0 || sample >= v->spp) return fail(ctx, GI_ERR_INVALID_ARG, "sample index");
+  if ((uint64_t)v->w * (uint64_t)v->h * (uint64_t)v->spp >= (1ull << 32))
+    return fail(ctx, GI_ERR_LIMIT, "more than 2^32 samples in one view");
   GI_HIP(ctx, hipSetDevice(ctx->device));
-  gi_render_opts o{};
-  o.max_depth = 1;
-  o.seed = seed;
-  int rc = setup_views(ctx, v, 1, &o);
+  // the view, pass and Halton state of this call are its own: the last
+  // render's views, options and images stay as they are
+  DevView dv;
+  std::string err;
+  if (!make_devview(*v, dv, err)) return fail(ctx, GI_ERR_INVALID_ARG, err);
+  int rc = ensure_halton(ctx, seed, dims_for_depth(1));
   if (rc) return rc;
   const uint32_t npix = (uint32_t)((uint64_t)v->w * v->h);
   PassDesc pd;
@@ -2982,13 +3132,15 @@ int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint3
   pd.total_slots = npix;
   pd.total_pix = npix;
   pd.chunk[0] = PassChunk{0, sample, 1, (int32_t)npix, 0, 0, {0, 0}};
-  GI_HIP(ctx, ctx->d_passes.ensure(sizeof(PassDesc)));
-  GI_HIP(ctx, copy_sync(ctx, ctx->d_passes.p, &pd, sizeof(pd), hipMemcpyHostToDevice));
+  GI_HIP(ctx, ctx->d_cam_pass.ensure(sizeof(PassDesc)));
+  GI_HIP(ctx, ctx->d_cam_view.ensure(sizeof(DevView)));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_cam_pass.p, &pd, sizeof(pd), hipMemcpyHostToDevice));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_cam_view.p, &dv, sizeof(dv), hipMemcpyHostToDevice));
   rc = ensure_workspace(ctx, npix);
   if (rc) return rc;
   HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>()};
-  hipLaunchKernelGGL(k_raygen, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_passes.as<PassDesc>(),
-                     ctx->d_views.as<DevView>(), H, ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(),
+  hipLaunchKernelGGL(k_raygen, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_cam_pass.as<PassDesc>(),
+                     ctx->d_cam_view.as<DevView>(), H, ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(),
                      ctx->q_tp[0].as<float4>(), ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, 1,
                      (float4 *)nullptr, 0u);
   GI_HIP(ctx, hipGetLastError());
@@ -3048,6 +3200,9 @@ int gi_progress_step(gi_ctx *ctx, int32_t *done) {
   if (!ctx || !done) return fail(ctx, GI_ERR_INVALID_ARG, "progress_step");
   if (!ctx->prog_active) return fail(ctx, GI_ERR_STATE, "gi_progress_begin first");
   GI_HIP(ctx, hipSetDevice(ctx->device));
+  // gi_generate_camera_rays may have re-seeded the tables since
+  int hrc = ensure_halton(ctx, ctx->opts.seed, dims_for_depth(ctx->opts.max_depth));
+  if (hrc) return hrc;
   const int32_t nv = (int32_t)ctx->views.size();
   std::vector<int32_t> sb(nv), se(nv);
   for (int32_t v = 0; v < nv; ++v) {
@@ -3162,6 +3317,10 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
   if ((size_t)(5 * nm + 3 * nl) * sizeof(float) > 48 * 1024)
     return fail(ctx, GI_ERR_LIMIT, "adjoint: 5*materials + 3*lights must fit 48 KiB of LDS");
   GI_HIP(ctx, hipSetDevice(ctx->device));
+  // the replay needs the render's own Halton points (gi_generate_camera_rays
+  // may have re-seeded the tables since)
+  rc = ensure_halton(ctx, ctx->opts.seed, dims_for_depth(ctx->opts.max_depth));
+  if (rc) return rc;
   auto t0 = std::chrono::steady_clock::now();
   std::memset(&ctx->counters, 0, sizeof(ctx->counters));
   const int32_t nv = (int32_t)ctx->views.size();

@@ -33,6 +33,13 @@ def material_params(scene) -> np.ndarray:
 
 
 class RenderFunction(torch.autograd.Function):
+    """forward renders with the given parameters; backward replays exactly
+    that render.  The parameters, views and options of the forward pass are
+    kept, and when the context has changed since (another forward, a
+    set_materials / set_lights or render in between), backward uploads them
+    again and re-renders before gi_adjoint, so the gradient always belongs
+    to the image forward returned."""
+
     @staticmethod
     def forward(fctx, mat_params, emission, ctx: Context):
         mats, lights = apply_params(ctx.scene, mat_params.detach().cpu().numpy().astype(np.float32),
@@ -42,11 +49,20 @@ class RenderFunction(torch.autograd.Function):
         ctx.render()
         img = np.concatenate([im.reshape(-1, 3) for im in ctx.images()])
         fctx.gi = ctx
+        fctx.state = (mats, lights, ctx._views.copy(), ctx._opts.copy())
+        fctx.generation = ctx.generation
         return torch.from_numpy(img).to(mat_params.device)
 
     @staticmethod
     def backward(fctx, grad_img):
-        dm, de = fctx.gi.adjoint(grad_img.detach().cpu().numpy().astype(np.float32))
+        ctx = fctx.gi
+        if ctx.generation != fctx.generation:
+            mats, lights, views, opts = fctx.state
+            ctx.set_materials(mats)
+            ctx.set_lights(lights)
+            ctx.render(views, opts)
+            fctx.generation = ctx.generation
+        dm, de = ctx.adjoint(grad_img.detach().cpu().numpy().astype(np.float32))
         return (torch.from_numpy(dm).to(grad_img.device), torch.from_numpy(de).to(grad_img.device), None)
 
 

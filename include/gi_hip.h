@@ -142,7 +142,13 @@ typedef struct {
 } gi_render_opts;
 
 #define GI_FLAG_BATCHED   0x1u  /* all views share one set of queues       */
-#define GI_FLAG_GRAPH     0x2u  /* bounce loop captured into a hipGraph    */
+#define GI_FLAG_GRAPH     0x2u  /* bounce loop captured into a hipGraph; the
+                                   context keeps the executable graph and
+                                   relaunches it for later renders whose
+                                   launches take the same arguments (same
+                                   view sizes and sample counts, options,
+                                   tuning and scene buffers; camera poses,
+                                   material and light values may change)   */
 
 /* Telemetry of the last gi_render / gi_adjoint call. */
 typedef struct {
@@ -154,7 +160,9 @@ typedef struct {
   double ms_trace_closest;  /* HIP-event time of the closest-hit kernels   */
   double ms_trace_any;
   double ms_shade;
-  double ms_other;
+  double ms_graph_build;    /* hipGraph capture + instantiate (GI_FLAG_GRAPH
+                               renders that could not relaunch the graph the
+                               context keeps), host clock                   */
   int32_t launches_closest; /* kernel launches of the closest-hit kernel   */
   int32_t passes;
 } gi_counters;
@@ -276,7 +284,9 @@ int gi_render(gi_ctx *ctx, const gi_view *views, int32_t n_views,
               const gi_render_opts *opts);
 /* Running-mean image of view v (float rgb[3*w*h], row y*w+x). */
 int gi_get_image(gi_ctx *ctx, int32_t view, float *rgb);
-/* Device pointer of the image of view v (valid until the next gi_render). */
+/* Device pointer of the running-mean image of view v: 4 floats per pixel
+ * (r, g, b, unused), row y*w+x.  Valid until the next gi_render or
+ * gi_progress_begin. */
 int gi_get_image_device(gi_ctx *ctx, int32_t view, float **d_rgb);
 int gi_get_counters(const gi_ctx *ctx, gi_counters *out);
 /* Traversal statistics: per kind of trace (closest hit; any hit), the rays

@@ -879,15 +879,22 @@ template <class R> V3<R> brdf_eval(const MatP<R> &p, int type, V3<R> wo, V3<R> w
   return eval_diff(p, wo, N, wi);
 }
 
-/* sampleGTR2Aniso (src/principledMaterial.h:121-176) with the halfway
- * sample of :46-61 in tangent space (T = x, B = y, N = z). */
-template <class R> R sample_gtr2(R u, R v, R ax, R ay, V3<R> V, V3<R> &L) {
+/* Halfway vector of sampleGTR2AnisoHalfwayVector (src/principledMaterial.h:
+ * 46-61) in tangent space (T = x, B = y, N = z): the GGX half-vector sample
+ * with slopes stretched by (ax, ay). */
+template <class R> V3<R> gtr2_half(R u, R v, R ax, R ay) {
   R s, c;
   sincos2pi(u, s, c);
   R fac = std::sqrt(v / (R(1) - v));
-  V3<R> m = normalize(mk<R>(fac * (ax * c), fac * (ay * s), R(1)));
-  L = V - (R(2) * dot(V, m)) * m; /* reflectR, src/materialBase.h:36-38 */
-  if (L.z < R(0)) L = -L;
+  return normalize(mk<R>(fac * (ax * c), fac * (ay * s), R(1)));
+}
+
+/* The reference's specular "pdf": Walter et al. 2007's sample weight
+ * (eq. 41) with the Smith G1 of eq. 34 (src/principledMaterial.h:76-84,
+ * 157-175) -- as written there: the tangent in G1 is that of the angle
+ * between the direction and m (not n), alpha_g = (ax*ay/2 + 1/2)^2, and the
+ * weight is clamped at 0.01 from below. */
+template <class R> R walter_weight(V3<R> L, V3<R> V, V3<R> m, R ax, R ay) {
   R roughg = sqr(ax * ay * R(0.5) + R(0.5));
   R dlm = dot(L, m), dvm = dot(V, m);
   R tl = std::sqrt(R(1) - dlm * dlm) / dlm;
@@ -901,6 +908,16 @@ template <class R> R sample_gtr2(R u, R v, R ax, R ay, V3<R> V, V3<R> &L) {
   R lw = (std::fabs(dot(L, m)) * gs) / (std::fabs(L.z) * std::fabs(m.z));
   if (lw < R(0.01)) lw = R(0.01);
   return lw;
+}
+
+/* sampleGTR2Aniso (src/principledMaterial.h:121-176): the halfway vector,
+ * the reflection of V about it (flipped into the upper hemisphere), the
+ * weight. */
+template <class R> R sample_gtr2(R u, R v, R ax, R ay, V3<R> V, V3<R> &L) {
+  V3<R> m = gtr2_half(u, v, ax, ay);
+  L = V - (R(2) * dot(V, m)) * m; /* reflectR, src/materialBase.h:36-38 */
+  if (L.z < R(0)) L = -L;
+  return walter_weight(L, V, m, ax, ay);
 }
 
 /* sampleGlass, src/principledMaterial.h:86-118 (ior 1.9, :358). */
@@ -1910,6 +1927,24 @@ void or_brdf_eval_f64(const gi_material *m, int type, const double *wo, const do
   V3<double> r = brdf_eval(p, type, V3<double>{wo[0], wo[1], wo[2]}, V3<double>{wi[0], wi[1], wi[2]},
                            V3<double>{N[0], N[1], N[2]});
   f[0] = r.x; f[1] = r.y; f[2] = r.z;
+}
+
+void or_gtr2_half_f64(double u, double v, double ax, double ay, double *h) {
+  V3<double> m = gtr2_half(u, v, ax, ay);
+  h[0] = m.x; h[1] = m.y; h[2] = m.z;
+}
+double or_walter_weight_f64(const double *L, const double *V, const double *m, double ax, double ay) {
+  return walter_weight(V3<double>{L[0], L[1], L[2]}, V3<double>{V[0], V[1], V[2]}, V3<double>{m[0], m[1], m[2]},
+                       ax, ay);
+}
+double or_gtr1_f64(double ndh, double a) { return gtr1(ndh, a); }
+void or_camera_basis(const gi_view *v, float *U, float *V, float *W, float *wh) {
+  Cam c = make_cam(*v);
+  std::memcpy(U, c.U, 12);
+  std::memcpy(V, c.V, 12);
+  std::memcpy(W, c.W, 12);
+  wh[0] = c.width;
+  wh[1] = c.height;
 }
 
 void or_render(const or_scene *s, const gi_view *views, int32_t n_views, const gi_render_opts *opts, float *images,

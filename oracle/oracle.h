@@ -93,6 +93,16 @@ int or_brdf_sample_f64(const gi_material *m, const double *wo_ts, const double *
 void or_brdf_eval_f64(const gi_material *m, int type, const double *wo, const double *wi,
                       const double *N, double *f);
 
+/* The pieces of the GTR2 specular sample, FP64: the halfway vector
+ * (tangent space) for (u, v) and slope scales (ax, ay); the Walter sample
+ * weight of a reflected pair (L, V) about m; the GTR1 (clearcoat) density. */
+void or_gtr2_half_f64(double u, double v, double ax, double ay, double *h);
+double or_walter_weight_f64(const double *L, const double *V, const double *m, double ax, double ay);
+double or_gtr1_f64(double ndh, double a);
+/* Camera basis U, V, W and image-plane half extents (width, height) of a
+ * view (src/raygen.cu:17-39). */
+void or_camera_basis(const gi_view *v, float *U, float *V, float *W, float *wh);
+
 /* Full renderer.  images: views concatenated, 3 floats per pixel.
  * Renders pixels [p_begin, p_end) of every view (clamped per view; p_end<0 =
  * all) and samples [0, spp).  threads <= 0: all OpenMP threads. */

@@ -80,6 +80,10 @@ def load():
             "or_sky_sample": (None, [P, P, P, P, P]),
             "or_atan2_turns": (C.c_float, [C.c_float, C.c_float]),
             "or_cdf_sample": (C.c_float, [P, P, I32, C.c_float, P]),
+            "or_gtr2_half_f64": (None, [C.c_double, C.c_double, C.c_double, C.c_double, P]),
+            "or_walter_weight_f64": (C.c_double, [P, P, P, C.c_double, C.c_double]),
+            "or_gtr1_f64": (C.c_double, [C.c_double, C.c_double]),
+            "or_camera_basis": (None, [P, P, P, P, P]),
         }
         for k, (res, args) in sig.items():
             f = getattr(lib, k)
@@ -289,6 +293,29 @@ def camera_rays(view, sample, seed, index_base=0):
     t = np.zeros(n, np.float32)
     load().or_camera_rays(_p(v), sample, seed, index_base, _p(o), _p(d), _p(t))
     return o, d, t
+
+
+def camera_basis(view):
+    """(U, V, W, (width, height)) of the view's camera."""
+    v = np.ascontiguousarray(view).reshape(1)
+    U, V, W, wh = (np.zeros(k, np.float32) for k in (3, 3, 3, 2))
+    load().or_camera_basis(_p(v), _p(U), _p(V), _p(W), _p(wh))
+    return U, V, W, wh
+
+
+def gtr2_half(u, v, ax, ay):
+    h = np.zeros(3)
+    load().or_gtr2_half_f64(u, v, ax, ay, _p(h))
+    return h
+
+
+def walter_weight(L, V, m, ax, ay):
+    L, V, m = (np.ascontiguousarray(x, np.float64) for x in (L, V, m))
+    return float(load().or_walter_weight_f64(_p(L), _p(V), _p(m), ax, ay))
+
+
+def gtr1(ndh, a):
+    return float(load().or_gtr1_f64(ndh, a))
 
 
 def tangent_frame(n):

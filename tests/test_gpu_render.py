@@ -289,3 +289,36 @@ def test_sky_light_errors(gpu_ctx_factory):
         ctx.render()
         with pytest.raises(GiError, match="sky"):
             ctx.adjoint(np.ones((16, 16, 3), np.float32))
+
+
+def _reference_materials():
+    """The reference's material assets (materials/*.pbrdf, from the
+    fixture): clearcoat, anisotropic GTR2, specular tint, sheen and glass
+    (opacity 0) lobes; plus a subsurface material, which no asset sets."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_materials.json")
+    d = json.load(open(path))["materials"]
+    keys = ("subsurface", "specular", "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat",
+            "clearcoat_gloss", "opacity", "transmission")
+    mats = [scenes.principled(r["color"], r["roughness"], r["metallic"], **{k: r[k] for k in keys})
+            for _, r in sorted(d.items())]
+    mats.append(scenes.principled((0.8, 0.5, 0.3), 0.6, 0.1, subsurface=1.0, sheen=0.3, sheen_tint=0.5))
+    return np.concatenate(mats)
+
+
+@pytest.mark.parametrize("spp,depth", [(8, 6), (2, 12)])
+def test_reference_material_assets_bit_exact(spp, depth, oracle, gpu_ctx_factory):
+    mats = _reference_materials()
+    assert (mats["clearcoat"] > 0).sum() >= 8 and (mats["anisotropic"] > 0).sum() >= 3
+    assert (mats["opacity"] < 1).sum() >= 1 and (mats["sheen"] > 0).sum() >= 1 and (mats["subsurface"] > 0).any()
+    sc = scenes.terrain_spheres(cells=(48, 48), n_spheres=160, n_mats=len(mats), w=128, h=72, spp=spp, depth=depth)
+    sc.mats = mats
+    ref, st = _oracle(oracle, sc)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render().image(0).reshape(-1, 3)
+        c = ctx.counters()
+        b = ctx.render(flags=FLAG_BATCHED | FLAG_GRAPH).image(0).reshape(-1, 3)
+    _same(img, ref)
+    _same(b, ref)
+    assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]

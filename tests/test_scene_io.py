@@ -169,3 +169,23 @@ def test_reference_assets():
         want = obj_oracle.read_pbrdf(os.path.join(d, f))
         for k, v in want.items():
             assert np.array_equal(m[k], np.float32(v)), (f, k)
+
+
+def test_reference_material_fixture_is_current(tmp_path):
+    """tests/golden/reference_materials.json holds what the reader makes of
+    the reference's material assets (checked where the reference is)."""
+    import json
+    import shutil
+    root = "/root/reference"
+    if not os.path.isdir(os.path.join(root, "materials")):
+        pytest.skip("reference tree not present")
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    committed = json.load(open(os.path.join(here, "reference_materials.json")))
+    shutil.copy(os.path.join(here, "make_reference_materials.py"), tmp_path)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mkref", os.path.join(here, "make_reference_materials.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    mod.HERE = str(tmp_path)
+    mod.main(root)
+    assert json.load(open(tmp_path / "reference_materials.json")) == committed
