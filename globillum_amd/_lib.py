@@ -80,6 +80,7 @@ SIGNATURES = {
     "gi_trace_any": (_I, [_P, _P, _P, _P, _I64, _P]),
     "gi_trace_closest_device": (_I, [_P, _P, _P, _P, _I64, _P]),
     "gi_trace_any_device": (_I, [_P, _P, _P, _P, _I64, _P]),
+    "gi_trace_closest_device_noalloc": (_I, [_P, _P, _P, _P, _I64, _P]),
     "gi_generate_camera_rays": (_I, [_P, _P, _I32, _U32, _P, _P, _P]),
     "gi_render": (_I, [_P, _P, _I32, _P]),
     "gi_get_image": (_I, [_P, _I32, _P]),

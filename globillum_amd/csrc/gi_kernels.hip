@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1468,6 +1469,46 @@ struct DBuf {
 
 }  // namespace
 
+// ======================================================== no-alloc scratch ==
+// Scratch memory of gi_trace_closest_device_noalloc, for callers that cannot
+// allocate device memory: a file-scope device array that every GPU the
+// library runs on holds until the process ends.  Layout (bytes):
+//   [0, 256)                        the traversal-stack overflow counter
+//   [256, 256 + 32 * kNoAllocRays)  the packed rays: ray_o then ray_d, float4
+//   then kNoAllocStackWords words   the global part of the traversal stacks
+// Host code never names the array in a launch or copy (in host code the
+// name is a stand-in, not a device address): gi_create resolves its address
+// on each device once, with hipGetSymbolAddress, and launches use that.
+// Calls that overlap on one GPU share the array and race.
+constexpr uint32_t kNoAllocRays = 1u << 19;                 // 524288 rays per call
+constexpr size_t kNoAllocStackWords = (size_t)4 << 20;      // 16 MiB of stack columns
+constexpr size_t kNoAllocRayBytes = (size_t)kNoAllocRays * 2 * sizeof(float4);
+constexpr size_t kScratchBytes = 256 + kNoAllocRayBytes + kNoAllocStackWords * sizeof(uint32_t);
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on `device` (current), resolved once per
+// device.
+hipError_t scratch_address(int device, void **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 // ================================================================ context ==
 struct gi_ctx {
   int device = 0;
@@ -1542,6 +1583,7 @@ struct gi_ctx {
   gi_counters counters{};
   gi_tuning tuning{256, 5, 256, 0, 16, 0, 4, 32};
   std::vector<hipEvent_t> events;
+  uint8_t *scratch = nullptr;  // this device's scratch array (gi_create)
 };
 
 namespace {
@@ -1802,8 +1844,8 @@ bool lds_scene_of(const gi_ctx *ctx) {
 }
 
 // The global part of the traversal stacks and the overflow counter a launch
-// uses: the context's own, or other buffers with the grid capped to the
-// stack columns they hold.
+// uses: the context's own, or (gi_trace_closest_device_noalloc) the
+// library's scratch array with the grid capped to the stack columns it holds.
 struct TraceScratch {
   uint32_t *gstack = nullptr;
   uint32_t *overflow = nullptr;
@@ -2463,6 +2505,9 @@ int gi_create(gi_ctx **out, int device) {
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return GI_ERR_NO_DEVICE;
   ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return GI_ERR_NO_DEVICE;
+  void *sp = nullptr;
+  if (scratch_address(device, &sp) != hipSuccess) return GI_ERR_HIP;
+  ctx->scratch = static_cast<uint8_t *>(sp);
   if (ctx->d_overflow.ensure(256) != hipSuccess) return GI_ERR_OUT_OF_MEMORY;
   if (hipMemsetAsync(ctx->d_overflow.p, 0, 4, ctx->stream) != hipSuccess ||
       hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -3069,6 +3114,44 @@ int gi_trace_any_device(gi_ctx *ctx, const float *d_orig, const float *d_dir, co
   if (rc) return rc;
   GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return check_overflow(ctx);
+}
+
+int gi_trace_closest_device_noalloc(gi_ctx *ctx, const float *d_orig, const float *d_dir, const float *d_max_t,
+                                    int64_t n, gi_hit *d_out) {
+  int rc = check_scene(ctx, false);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!d_orig || !d_dir || !d_max_t || !d_out))) return fail(ctx, GI_ERR_INVALID_ARG, "rays");
+  if (n == 0) return GI_OK;
+  if (n > (int64_t)kNoAllocRays)
+    return fail(ctx, GI_ERR_LIMIT, "gi_trace_closest_device_noalloc: at most 524288 rays per call");
+  if (!ctx->scratch) return fail(ctx, GI_ERR_STATE, "no scratch array on this device");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  uint32_t *overflow = reinterpret_cast<uint32_t *>(ctx->scratch);
+  float4 *ro = reinterpret_cast<float4 *>(ctx->scratch + 256);
+  float4 *rd = ro + kNoAllocRays;
+  uint32_t *gstack = reinterpret_cast<uint32_t *>(ctx->scratch + 256 + kNoAllocRayBytes);
+  // one global stack column per thread, as deep as the tree needs beyond
+  // the smallest LDS part; the grid is capped to the columns the array holds
+  int grid, block;
+  if ((rc = trace_grid_for(ctx, (uint64_t)n, grid, block))) return rc;
+  const size_t gdepth = ctx->stack_cap > (uint32_t)kMinLdsDepth ? ctx->stack_cap - kMinLdsDepth : 1;
+  const size_t max_grid = kNoAllocStackWords / (gdepth * (size_t)block);
+  if (max_grid < 1) return fail(ctx, GI_ERR_LIMIT, "gi_trace_closest_device_noalloc: BVH too deep for the scratch stack");
+  TraceScratch ts;
+  ts.gstack = gstack;
+  ts.overflow = overflow;
+  ts.max_grid = (int)std::min<size_t>(max_grid, (size_t)INT32_MAX);
+  // the counter keeps the last call's value: this call starts it at zero
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, overflow, 1u, 0u);
+  hipLaunchKernelGGL(k_pack_rays, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, d_orig, d_dir,
+                     d_max_t, (uint32_t)n, ro, rd);
+  GI_HIP(ctx, hipGetLastError());
+  rc = launch_closest(ctx, true, nullptr, (uint32_t)n, (uint64_t)n, ro, rd, reinterpret_cast<float4 *>(d_out), ts);
+  if (rc) return rc;
+  uint32_t ov = 0;
+  GI_HIP(ctx, hipMemcpyAsync(&ov, overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ov != 0 ? fail(ctx, GI_ERR_LIMIT, "traversal stack overflow") : GI_OK;
 }
 
 int gi_trace_closest(gi_ctx *ctx, const float *orig, const float *dir, const float *max_t, int64_t n, gi_hit *out) {

@@ -273,6 +273,19 @@ int gi_trace_closest_device(gi_ctx *ctx, const float *d_orig, const float *d_dir
                             const float *d_max_t, int64_t n, gi_hit *d_out);
 int gi_trace_any_device(gi_ctx *ctx, const float *d_orig, const float *d_dir,
                         const float *d_max_t, int64_t n, uint8_t *d_occluded);
+/* gi_trace_closest_device without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same rays, same results as gi_trace_closest_device.  Its scratch -- the
+ * packed rays, the global part of the traversal stacks and the overflow
+ * counter -- is a fixed 32 MiB device array of the library (one per GPU,
+ * held until the process ends) instead of context buffers.  Limits: at most
+ * 524288 rays per call (more: GI_ERR_LIMIT); the traversal grid is capped
+ * to the stack columns the array holds (a tree too deep for one block of
+ * columns: GI_ERR_LIMIT).  Calls that overlap on one GPU share the array
+ * and race; each call clears what must start at zero. */
+int gi_trace_closest_device_noalloc(gi_ctx *ctx, const float *d_orig, const float *d_dir,
+                                    const float *d_max_t, int64_t n, gi_hit *d_out);
 
 /* ---- camera rays (setup_jittered_shirley, src/raygen.cu:10-59) ------------
  * Writes the w*h primary rays of sample `sample` of view `v` (host arrays). */
