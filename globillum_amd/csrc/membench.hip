@@ -44,9 +44,45 @@ __global__ __launch_bounds__(kBlock) void k_stream_read(const float4 *__restrict
   if (s == 1.0f) out[blockIdx.x * kBlock + threadIdx.x] = s;
 }
 
+// One pass of a plain copy, dst[i] = src[i]: exactly `n4` * 16 bytes read
+// and written, the known-size kernel that scales HBM traffic counters
+// (FETCH_SIZE / WRITE_SIZE) of other kernels in the same profiled run.
+__global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                 uint64_t n4) {
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += step) dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+// One launch of k_copy over `bytes` (a multiple of 16) between two new
+// buffers on `device`, after both were filled by hipMemsetAsync; returns 0
+// or the HIP error code.  Allocates and frees its buffers, synchronises.
+int gi_membench_copy(int32_t device, int64_t bytes) {
+  if (bytes < 16 || bytes % 16 != 0) return (int)hipErrorInvalidValue;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return (int)e;
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return (int)e;
+  const uint32_t G = (uint32_t)prop.multiProcessorCount * 4u;
+  float4 *src = nullptr, *dst = nullptr;
+  hipStream_t s = nullptr;
+  if ((e = hipMalloc(&src, (size_t)bytes)) != hipSuccess) goto done;
+  if ((e = hipMalloc(&dst, (size_t)bytes)) != hipSuccess) goto done;
+  if ((e = hipStreamCreate(&s)) != hipSuccess) goto done;
+  if ((e = hipMemsetAsync(src, 0, (size_t)bytes, s)) != hipSuccess) goto done;
+  if ((e = hipMemsetAsync(dst, 0, (size_t)bytes, s)) != hipSuccess) goto done;
+  hipLaunchKernelGGL(k_copy, dim3(G), dim3(kBlock), 0, s, src, dst, (uint64_t)bytes / 16u);
+  if ((e = hipGetLastError()) != hipSuccess) goto done;
+  e = hipStreamSynchronize(s);
+done:
+  if (s) (void)hipStreamDestroy(s);
+  if (dst) (void)hipFree(dst);
+  if (src) (void)hipFree(src);
+  return (int)e;
+}
 
 // Read bandwidth (GB/s, 1e9 bytes) of `passes` launches, each reading a
 // zeroed buffer of `bytes` (a multiple of 16, at least 16 * 1024) on

@@ -19,6 +19,6 @@ def test_help_lists_the_legs():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0
-    for flag in ("--steps", "--warmup", "--no-cpu", "--no-batching", "--no-adjoint", "--no-bvh", "--no-instancing",
-                 "--no-small", "--no-membench", "--copy-mib"):
+    for flag in ("--steps", "--warmup", "--no-cpu", "--no-traffic", "--no-plain", "--no-batching", "--no-adjoint",
+                 "--no-bvh", "--no-instancing", "--no-small", "--no-membench", "--copy-mib", "--tuning-cache"):
         assert flag in r.stdout, flag

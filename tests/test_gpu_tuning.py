@@ -85,6 +85,27 @@ def test_autotuner(ts, gpu_ctx_factory):
         assert np.array_equal(ctx.render(flags=FLAG_BATCHED).image(0), ref)
 
 
+def test_autotuner_cache_per_workload(ts, gpu_ctx_factory, tmp_path):
+    space = {"refill_lanes": [16, 32], "trace_blocks_per_cu": [4, 5]}
+    cache = str(tmp_path / "tuning.json")
+    opts = ts.opts.copy()
+    opts["flags"] = FLAG_BATCHED
+    key = autotune.workload_key(ts.views, opts, "ts")
+    with gpu_ctx_factory(ts) as ctx:
+        first = autotune.tune(ctx, ts.views, opts, space=space, repeats=3, cache=cache, key=key)
+        assert not first["cached"] and len(first["trials"]) >= 3
+    with gpu_ctx_factory(ts) as ctx:
+        again = autotune.tune(ctx, ts.views, opts, space=space, cache=cache, key=key)
+        assert again["cached"] and again["params"] == first["params"] and again["trials"] == []
+        assert all(ctx.tuning()[k] == v for k, v in first["params"].items())
+        # another workload (view size) is searched again
+        v2 = ts.views.copy()
+        v2["w"] = 48
+        other = autotune.tune(ctx, v2, opts, space=space, repeats=1, cache=cache,
+                              key=autotune.workload_key(v2, opts, "ts"))
+        assert not other["cached"]
+
+
 def test_lds_scene_matches_global_path(oracle, gpu_ctx_factory):
     sc = scenes.cornell_boxes(w=96, h=64, spp=4, depth=4)
     rng = np.random.default_rng(9)
