@@ -115,3 +115,24 @@ def test_context_does_not_alias_the_callers_scene():
     ctx.scene = None
     copy = dataclasses.replace(sc, mats=sc.mats.copy(), lights=sc.lights.copy())
     assert copy.mats is not sc.mats and np.array_equal(copy.mats, sc.mats)
+
+
+def test_integration_record_checks(tmp_path):
+    """The compile-time record checks of the integration binding against two
+    stand-ins for rta's records (tests/csrc/layout_check.cpp): 4-byte-aligned
+    texture coordinates pass straight through, an 8-byte-aligned float2
+    (simple_triangle padded to 104 bytes) is copied field by field; the
+    uploaded gi_tri records hold the same values either way."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not found")
+    exe = str(tmp_path / "layout_check")
+    r = subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-o", exe,
+                        os.path.join(ROOT, "tests", "csrc", "layout_check.cpp")], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    assert r.stdout.split("\n")[:2] == ["vec2f: sizeof 100, cast, ok", "float2: sizeof 104, copy, ok"]
