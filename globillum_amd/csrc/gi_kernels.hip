@@ -188,14 +188,18 @@ __device__ __forceinline__ void block_append2(bool p0, uint32_t *c0, bool p1, ui
 // the box's own coordinates.  Box culling is therefore conservative and
 // the result is the closest hit = minimum (t, id) over all triangles with
 // 0 < t <= max_t; any-hit stops at the first such triangle.
-// Occupancy target of the traversal kernels (waves per SIMD): 7 caps them
-// at 72 VGPRs, which they fit without spilling (the two-level variants too,
-// as their space changes happen outside the interior-node loop).
+// Occupancy target of the traversal kernels (waves per SIMD): 6 caps them
+// at 80 VGPRs, which they fit without spilling (the default launch, 5 blocks
+// of 4 waves per CU, runs 5 per SIMD).  At 7 (72 VGPRs) the two-level
+// closest-hit kernel spills and runs 39 % slower.
 #ifndef GI_TRACE_WAVES_PER_EU
-#define GI_TRACE_WAVES_PER_EU 7
+#define GI_TRACE_WAVES_PER_EU 6
 #endif
 
 constexpr uint32_t kNoRay = 0xffffffffu;
+// The traversal addresses a node's planes by a 32-bit byte offset (index
+// << 7): at most 2^25 wide nodes.
+constexpr uint64_t kMaxWideNodes = (uint64_t)1 << 25;
 constexpr float kSlabSlack = 1.0f + 0x1p-21f;
 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
@@ -221,6 +225,17 @@ struct Stack {
     return sp < kLdsDepth ? lds[(uint32_t)sp << shift] : glb[(uint32_t)(sp - kLdsDepth) * glb_stride];
   }
 };
+
+// Byte offsets, within a wide node, of a ray's near plane on x, y, z
+// (bits 0-7, 8-15, 16-23): lo (lox 0, loy 32, loz 64) where the inverse
+// direction is positive, hi (16 higher) where it is negative; the far plane
+// is the other one (offset ^ 16).  The sign bit decides: rcp_clamped never
+// returns zero, and -0 directions give negative inverses.
+__device__ __forceinline__ uint32_t near_planes(f3 idir) {
+  const uint32_t sx = __float_as_uint(idir.x) >> 31, sy = __float_as_uint(idir.y) >> 31,
+                 sz = __float_as_uint(idir.z) >> 31;
+  return (sx << 4) | ((32u | (sy << 4)) << 8) | ((64u | (sz << 4)) << 16);
+}
 
 // LDS column stride of a block: the next power of two >= blockDim.
 __host__ __device__ __forceinline__ uint32_t lds_shift_for(uint32_t block) {
@@ -272,6 +287,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
   uint32_t chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // wave-uniform: next chunk
   uint32_t ray = kNoRay, ref = kDoneRef, parked = kDoneRef;
   f3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f}, idir{0.f, 0.f, 0.f}, oid{0.f, 0.f, 0.f};
+  uint32_t near = 0;  // the ray's near planes (near_planes)
   float tmax = 0.f, bt = FLT_MAX, bu = 0.f, bv = 0.f;
   uint32_t bid = GI_INVALID_REF;
   uint32_t vbase = 0;  // virtual id base of the current space (0: world)
@@ -290,6 +306,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
           start(ray, o, d, tmax);
           idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
           oid = f3{o.x * idir.x, o.y * idir.y, o.z * idir.z};
+          near = near_planes(idir);
           bt = FLT_MAX;
           bu = bv = 0.0f;
           bid = GI_INVALID_REF;
@@ -312,40 +329,62 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
     for (;;) {
       if (!(ref & kLeafBit)) {
         if constexpr (kCount) ++c_nodes;
-        float4 lox, hix, loy, hiy, loz, hiz;
+        // only the near and the far plane of each axis: which is which
+        // follows from the ray's direction signs, so each load's offset
+        // within the 128-byte node picks it (fma rounding is monotone in
+        // the plane, so these are exactly the min and max of the two
+        // plane distances)
+        float4 nxv, fxv, nyv, fyv, nzv, fzv;
         uint4 ch;
+        const uint32_t base = ref << 7;
+        const uint32_t onx = base | (near & 0xffu), ony = base | ((near >> 8) & 0xffu),
+                       onz = base | (near >> 16);
         if constexpr (kLdsScene) {
-          const float4 *np = lnodes + 8u * ref;  // in LDS: derived from the shared base
-          lox = np[0]; hix = np[1]; loy = np[2]; hiy = np[3]; loz = np[4]; hiz = np[5];
-          const float4 c = np[6];
+          const char *np = reinterpret_cast<const char *>(lnodes);  // in LDS: derived from the shared base
+          nxv = *reinterpret_cast<const float4 *>(np + onx);
+          fxv = *reinterpret_cast<const float4 *>(np + (onx ^ 16u));
+          nyv = *reinterpret_cast<const float4 *>(np + ony);
+          fyv = *reinterpret_cast<const float4 *>(np + (ony ^ 16u));
+          nzv = *reinterpret_cast<const float4 *>(np + onz);
+          fzv = *reinterpret_cast<const float4 *>(np + (onz ^ 16u));
+          const float4 c = *reinterpret_cast<const float4 *>(np + (base | 96u));
           ch = make_uint4(__float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.z), __float_as_uint(c.w));
         } else {
-          const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
-          lox = np[0]; hix = np[1]; loy = np[2]; hiy = np[3]; loz = np[4]; hiz = np[5];
-          ch = *reinterpret_cast<const uint4 *>(np + 6);
+          const char *np = reinterpret_cast<const char *>(nodes);
+          nxv = *reinterpret_cast<const float4 *>(np + onx);
+          fxv = *reinterpret_cast<const float4 *>(np + (onx ^ 16u));
+          nyv = *reinterpret_cast<const float4 *>(np + ony);
+          fyv = *reinterpret_cast<const float4 *>(np + (ony ^ 16u));
+          nzv = *reinterpret_cast<const float4 *>(np + onz);
+          fzv = *reinterpret_cast<const float4 *>(np + (onz ^ 16u));
+          ch = *reinterpret_cast<const uint4 *>(np + (base | 96u));
         }
         const float lim = kAny ? tmax : smin(bt, tmax);
         float dist[4];
         uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
-        const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-        const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-        const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+        const float nx[4] = {nxv.x, nxv.y, nxv.z, nxv.w}, fx[4] = {fxv.x, fxv.y, fxv.z, fxv.w};
+        const float ny[4] = {nyv.x, nyv.y, nyv.z, nyv.w}, fy[4] = {fyv.x, fyv.y, fyv.z, fyv.w};
+        const float nz[4] = {nzv.x, nzv.y, nzv.z, nzv.w}, fz[4] = {fzv.x, fzv.y, fzv.z, fzv.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const float x0 = fmaf(lx[k], idir.x, -oid.x), x1 = fmaf(hx[k], idir.x, -oid.x);
-          const float y0 = fmaf(ly[k], idir.y, -oid.y), y1 = fmaf(hy[k], idir.y, -oid.y);
-          const float z0 = fmaf(lz[k], idir.z, -oid.z), z1 = fmaf(hz[k], idir.z, -oid.z);
-          const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-          const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), lim));
+          const float tn = fmaxf(fmaxf(fmaf(nx[k], idir.x, -oid.x), fmaf(ny[k], idir.y, -oid.y)),
+                                 fmaxf(fmaf(nz[k], idir.z, -oid.z), 0.0f));
+          const float tf = fminf(fminf(fmaf(fx[k], idir.x, -oid.x), fmaf(fy[k], idir.y, -oid.y)),
+                                 fminf(fmaf(fz[k], idir.z, -oid.z), lim));
           const bool hit = tn <= tf * kSlabSlack;  // empty slots have +inf boxes: never hit
           dist[k] = hit ? tn : INFINITY;
         }
+        // the child references are needed only after the sort; consuming
+        // them here (an empty register-only asm) keeps their load issued
+        // with the planes' instead of sunk behind the sort, where it would
+        // add one memory round trip to every node visit
+        asm volatile("" : "+v"(cref[0]), "+v"(cref[1]), "+v"(cref[2]), "+v"(cref[3]));
         // sort (dist, ref) ascending: network (0,1)(2,3)(0,2)(1,3)(1,2)
 #define GI_CSWAP(i, j)                                      \
-  if (dist[j] < dist[i]) {                                  \
-    float td = dist[i]; dist[i] = dist[j]; dist[j] = td;    \
-    uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
-  }
+if (dist[j] < dist[i]) {                                  \
+  float td = dist[i]; dist[i] = dist[j]; dist[j] = td;    \
+  uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
+}
         GI_CSWAP(0, 1) GI_CSWAP(2, 3) GI_CSWAP(0, 2) GI_CSWAP(1, 3) GI_CSWAP(1, 2)
 #undef GI_CSWAP
         if (dist[0] != INFINITY) {
@@ -407,6 +446,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
       }
       idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
       oid = f3{o.x * idir.x, o.y * idir.y, o.z * idir.z};
+      near = near_planes(idir);
       parked = kDoneRef;
     } else if (parked != kDoneRef) {
       // triangle j+1 is loaded while triangle j is tested
@@ -1739,6 +1779,10 @@ bool instanced(const gi_ctx *ctx) { return ctx->n_inst > 0; }
 
 // State that follows a new wide tree on the device.
 int finish_bvh(gi_ctx *ctx, int64_t nnodes, int32_t wide_depth) {
+  if ((uint64_t)nnodes > kMaxWideNodes) {
+    ctx->have_bvh = false;
+    return fail(ctx, GI_ERR_LIMIT, "more than 2^25 BVH nodes");
+  }
   ctx->nnodes = ctx->world_nnodes = nnodes;
   ctx->wide_depth = ctx->world_depth = wide_depth;
   ctx->n_inst = 0;  // a new world tree drops the instances (gi_upload_instances again)
@@ -3038,6 +3082,8 @@ int gi_upload_instances(gi_ctx *ctx, const gi_instance *inst, int32_t n) {
     }
     ctx->inst_vshift = shift;
   }
+  if ((uint64_t)(nw + nx + nt) > kMaxWideNodes)
+    return fail(ctx, GI_ERR_LIMIT, "more than 2^25 BVH nodes (world, meshes and instance tree)");
   std::swap(ctx->d_nodes.p, nodes.p);
   std::swap(ctx->d_nodes.bytes, nodes.bytes);
   std::swap(ctx->d_itris.p, itris.p);
