@@ -39,6 +39,8 @@ __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = 0.31830988618379067154f;
+constexpr float kTwoPi = 6.283185307179586f;
+constexpr float kPhongNorm = 16.07464925228143f;  // (100 + 1) / (2 pi): the Phong lobe's normalisation
 
 // sin(2*pi*v), cos(2*pi*v) for v in [0,1): quadrant q = floor(4v), the
 // remainder f = 4v - q (both exact), then the Taylor series of
@@ -279,6 +281,59 @@ __host__ __device__ __forceinline__ float pow22(float x) {
   float scale;
   __builtin_memcpy(&scale, &sb, 4);
   return (c * p) * scale;
+}
+
+// ----------------------------------------------------------------- Phong --
+// The powers of gpu_pt's Phong lobe (shininess 100, src/gpu-pt-kernels.cu:
+// 121-125,164-187), from basic operations only, so that host and device
+// agree bit for bit: x^100 as a fixed chain of squarings, and the lobe
+// sampler's x^(1/101) as 2^(log2(x) / 101) with log2 from the series of
+// atanh (m in [sqrt(1/2), sqrt(2)), to t^11) and 2^f from its Taylor
+// polynomial (f in [0, 1), to f^9).  Relative error below 1e-7.
+__host__ __device__ __forceinline__ float phong_pow100(float x) {
+  const float x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8, x32 = x16 * x16, x64 = x32 * x32;
+  return (x64 * x32) * x4;
+}
+__host__ __device__ __forceinline__ float phong_root101(float x) {
+  if (!(x > 0.0f)) return 0.0f;
+  if (x >= 1.0f) return 1.0f;
+  uint32_t bits;
+  __builtin_memcpy(&bits, &x, 4);
+  if (bits < 0x00800000u) return 0.4214777f;  // subnormal inputs: x^(1/101) of 2^-126
+  int32_t e = (int32_t)(bits >> 23) - 127;
+  uint32_t mb = (bits & 0x7fffffu) | 0x3f800000u;
+  if (mb >= 0x3fb504f3u) {  // m >= sqrt(2): halve it
+    mb -= 0x00800000u;
+    e += 1;
+  }
+  float m;
+  __builtin_memcpy(&m, &mb, 4);
+  // log2(m) = 2/ln2 * atanh(t), t = (m - 1) / (m + 1)
+  const float t = (m - 1.0f) / (m + 1.0f), t2 = t * t;
+  float a = 0.09090909090909091f;
+  a = a * t2 + 0.1111111111111111f;
+  a = a * t2 + 0.14285714285714285f;
+  a = a * t2 + 0.2f;
+  a = a * t2 + 0.3333333333333333f;
+  a = a * t2 + 1.0f;
+  const float l2 = (float)e + (a * t) * 2.8853900817779268f;
+  const float y = l2 / 101.0f;  // in (-1.26, 0)
+  const float k = floorf(y), f = y - k;
+  float p = 1.0178086009239699e-07f;
+  p = p * f + 1.3215486790144307e-06f;
+  p = p * f + 1.5252733804059841e-05f;
+  p = p * f + 0.00015403530393381609f;
+  p = p * f + 0.0013333558146428443f;
+  p = p * f + 0.009618129107628477f;
+  p = p * f + 0.05550410866482158f;
+  p = p * f + 0.2402265069591007f;
+  p = p * f + 0.6931471805599453f;
+  p = p * f + 1.0f;
+  const int32_t ki = (int32_t)k;  // -2 or -1
+  const uint32_t sb = (uint32_t)(ki + 127) << 23;
+  float scale;
+  __builtin_memcpy(&scale, &sb, 4);
+  return p * scale;
 }
 
 // --------------------------------------------------------------- texture --

@@ -125,6 +125,7 @@ struct ShadeParams {
   const DevTex *tex;       // textures (diffuse, by DevMat::texture)
   float4 *rdiff;           // ray differentials per path slot, or null (no texture in use):
   uint32_t rdiff_stride;   // planes [0..3] = upper origin, upper dir, right origin, right dir
+  const float4 *ks;        // per material: specular colour (GI_FLAG_PHONG renders)
 };
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -792,6 +793,8 @@ __device__ __forceinline__ f3 sky_eval(const SkyDev &S, f3 d) {
 struct VertexEval {
   f3 P, N, org_dir, wi, wo_ts, wi_ts, xp;
   f3 sd, so, psc, fnee;
+  f3 tpmul;       // Phong mode: the throughput factor of the bounce
+  bool absorbed;  // Phong mode: no lobe chosen, or the lobe's direction is below the surface
   float len, gk, pdf;
   int type, light;
   uint32_t mat;
@@ -816,7 +819,78 @@ __device__ __forceinline__ const DevMat &vertex_mat(const ShadeParams &P, const 
   return kTex ? ve.m : P.mats[ve.mat];
 }
 
+// gpu_pt's material model (compute_path_contribution_and_bounce,
+// src/gpu-pt-kernels.cu:89-199): Lambert plus a Phong lobe of shininess 100
+// around the mirror direction R of the incoming ray about the (unflipped)
+// shading normal.  kd = diffuse colour (times the diffuse texture's texel),
+// ks = specular colour; both are divided by their sum of all six channels
+// when it exceeds 1.  NEE weight f = kd / pi + 101 ks / (2 pi) max(R.l, 0)^100
+// (the NEE contribution already holds the cosine).  The bounce picks the
+// diffuse lobe when xi_z <= pd = sum(kd), the specular one when xi_z <=
+// pd + ps (ps = sum(ks); with the reference's renormalisation ps /= pd + ps
+// after pd /= pd + ps when pd + ps > 1), otherwise the path is absorbed; the
+// direction is cos^n distributed around N (n = 1) or R (n = 100) --
+// z = xi_x^(1/(n+1)) -- and a specular direction below the surface is
+// absorbed too.  Throughput: TP * |dir.N| f_lobe / P_lobe * 2 pi / ((n + 1)
+// z^n), with no clamp.
 template <bool kTex>
+__device__ __forceinline__ void phong_vertex(const ShadeParams &P, uint32_t n, int dim, f3 N, f3 org_dir,
+                                             VertexEval &ve) {
+  f3 kd = ld3(P.mats[ve.mat].diffuse);
+  if (kTex && ve.textured) kd = mul(kd, ve.tx);
+  const float4 ks4 = P.ks[ve.mat];
+  f3 ks = f3{ks4.x, ks4.y, ks4.z};
+  const float sum = ((((kd.x + kd.y) + kd.z) + ks.x) + ks.y) + ks.z;
+  if (sum > 1.0f) {
+    kd = divs(kd, sum);
+    ks = divs(ks, sum);
+  }
+  const f3 R = reflect_dir(org_dir, N);
+  const f3 l = normalize(ve.sd);
+  const float cl = phong_pow100(smax(dot(R, l), 0.0f));
+  ve.fnee = add(scl(kd, kInvPi), scl(ks, kPhongNorm * cl));
+  ve.glass = false;
+  ve.N = N;
+  ve.xp = f3{halton(n, dim + 3, P.H), halton(n, dim + 4, P.H), halton(n, dim + 5, P.H)};
+  float pd = (kd.x + kd.y) + kd.z;
+  float ps = (ks.x + ks.y) + ks.z;
+  if (pd + ps > 1.0f) {
+    pd = pd / (pd + ps);
+    ps = ps / (pd + ps);
+  }
+  ve.absorbed = true;
+  ve.tpmul = f3{0.f, 0.f, 0.f};
+  ve.type = kTypeDiffuse;
+  const bool dif = ve.xp.z <= pd;
+  if (!dif && !(ve.xp.z <= pd + ps)) return;
+  const f3 axis = dif ? N : R;
+  f3 Tt, Bt;
+  tangent_frame(axis, Tt, Bt);
+  const float z = dif ? sqrtf(ve.xp.x) : phong_root101(ve.xp.x);
+  const float r = sqrtf(1.0f - z * z);
+  float sn, cs;
+  sincos2pi(ve.xp.y, sn, cs);
+  f3 dir = add(add(scl(Tt, r * cs), scl(Bt, r * sn)), scl(axis, z));
+  const float dn = dot(dir, N);
+  f3 f;
+  float den;
+  if (dif) {
+    f = scl(kd, kInvPi);
+    den = 2.0f * z;
+  } else {
+    if (dn < 0.0f) return;
+    ve.type = kTypeSpecRefl;
+    f = scl(ks, kPhongNorm * phong_pow100(smax(dot(R, dir), 0.0f)));
+    den = 101.0f * phong_pow100(z);
+  }
+  if (!(den > 0.0f)) return;  // xi_x = 0 (or z^100 below the float range): absorbed
+  const float w = (1.0f / (dif ? pd : ps)) * (kTwoPi / den);
+  ve.tpmul = scl(scl(f, fabsf(dn)), w);
+  ve.wi = normalize(dir);
+  ve.absorbed = false;
+}
+
+template <bool kTex, bool kPhong = false>
 __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, int dim, uint32_t slot, f3 org_dir,
                                             float4 h, VertexEval &ve) {
   uint32_t tid = fbits(h.w);
@@ -943,6 +1017,10 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
   ve.psc = scl(contrib, sel);
   ve.gk = factor * sel;
 
+  if constexpr (kPhong) {
+    phong_vertex<kTex>(P, n, dim, N0, org_dir, ve);
+    return;
+  }
   // bounce: tangent frame, normal flip, BRDF sample
   f3 N = N0;
   f3 Tt, Bt;
@@ -978,7 +1056,7 @@ constexpr uint32_t kRecBackground = 8u;
 // Shading of path vertex b.  The NEE contribution travels with the shadow
 // ray; the shadow kernel adds it when unoccluded.  rec != nullptr: also
 // record the vertex for the adjoint (see kRecBackground).
-template <bool kTex>
+template <bool kTex, bool kPhong = false>
 __device__ __forceinline__ void shade_vertices(
     int32_t b, const uint32_t *__restrict__ count_ptr, const ShadeParams &P, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, const float4 *__restrict__ tp_in, const float4 *__restrict__ hits,
@@ -1017,7 +1095,7 @@ __device__ __forceinline__ void shade_vertices(
         }
       } else {
         VertexEval ve;
-        eval_vertex<kTex>(P, n, dim, slot, org_dir, h, ve);
+        eval_vertex<kTex, kPhong>(P, n, dim, slot, org_dir, h, ve);
         const DevMat &m = vertex_mat<kTex>(P, ve);
         if (!ve.glass) {
           const f3 C = mul(ve.fnee, mul(TP, ve.psc));
@@ -1029,10 +1107,17 @@ __device__ __forceinline__ void shade_vertices(
           }
         }
         uint32_t clamp = 0;
-        if (!last) {
-          const f3 f = brdf_eval(m, ve.type, neg(org_dir), ve.wi, ve.N);
-          const f3 np = ve.glass ? add(ve.P, scl(org_dir, P.off_b)) : add(ve.P, scl(ve.N, P.off_b));
-          f3 t = next_throughput(TP, f, ve.pdf, clamp);
+        if (!last && !(kPhong && ve.absorbed)) {
+          f3 np, t;
+          if constexpr (kPhong) {
+            // gpu_pt: offset along the new direction, no throughput clamp
+            np = add(ve.P, scl(ve.wi, P.off_b));
+            t = mul(TP, ve.tpmul);
+          } else {
+            const f3 f = brdf_eval(m, ve.type, neg(org_dir), ve.wi, ve.N);
+            np = ve.glass ? add(ve.P, scl(org_dir, P.off_b)) : add(ve.P, scl(ve.N, P.off_b));
+            t = next_throughput(TP, f, ve.pdf, clamp);
+          }
           bool alive = true;
           if (P.rr_start > 0 && b + 1 >= P.rr_start) {
             const float q = smax(t.x, smax(t.y, t.z));
@@ -1079,6 +1164,9 @@ __device__ __forceinline__ void shade_vertices(
   b, count_ptr, P, ray_o, ray_d, tp_in, hits, path_l, nro, nrd, ntp, next_count, sho, shd, shc, shadow_count, rec, \
       rec_stride
 __global__ __launch_bounds__(256) void k_shade(GI_SHADE_ARGS) { shade_vertices<false>(GI_SHADE_PASS); }
+// gpu_pt's Lambert + Phong model (GI_FLAG_PHONG), untextured and textured
+__global__ __launch_bounds__(256) void k_shade_phong(GI_SHADE_ARGS) { shade_vertices<false, true>(GI_SHADE_PASS); }
+__global__ __launch_bounds__(256) void k_shade_phong_tex(GI_SHADE_ARGS) { shade_vertices<true, true>(GI_SHADE_PASS); }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade_tex(GI_SHADE_ARGS) {
   shade_vertices<true>(GI_SHADE_PASS);
 }
@@ -1566,6 +1654,7 @@ struct gi_ctx {
   float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, diag = 0.f, power_total = 0.f;
   float clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};  // bounds of the triangle centroids
   DBuf d_stris, d_itris, d_nodes, d_mats, d_lights;
+  DBuf d_ks;  // specular colour per material (float4), for GI_FLAG_PHONG
   // Halton tables
   uint32_t hal_seed = 0;
   int32_t hal_dims = 0;
@@ -2101,6 +2190,7 @@ ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
   P.tex = ctx->d_tex.as<DevTex>();
   P.rdiff = ctx->textured ? ctx->rdiff.as<float4>() : nullptr;
   P.rdiff_stride = (uint32_t)ctx->slot_cap;
+  P.ks = ctx->d_ks.as<float4>();
   return P;
 }
 
@@ -2144,8 +2234,9 @@ std::vector<uint8_t> launch_key(gi_ctx *ctx, const PassPlan &plan, int32_t depth
                         (const void *)P.H.chunks, (const void *)P.sky.rgb, (const void *)P.sky.ycdf,
                         (const void *)P.sky.ypdf, (const void *)P.sky.xcdf, (const void *)P.sky.xpdf,
                         (const void *)P.is.inst, (const void *)P.is.vbase, (const void *)P.is.vtab,
-                        (const void *)P.tex, (const void *)P.rdiff})
+                        (const void *)P.tex, (const void *)P.rdiff, (const void *)P.ks})
     key_put(k, q);
+  key_put(k, ctx->opts.flags & GI_FLAG_PHONG);  // picks the shading kernel
   key_put(k, P.nlights);
   key_put(k, P.power_total);
   key_put(k, P.off_b);
@@ -2215,7 +2306,9 @@ int enqueue_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_c
       ++launches;
       if ((rc = mark(0))) return rc;
       const int sg = (int)std::max<uint64_t>(1, std::min<uint64_t>((bound + sblock - 1) / sblock, (uint64_t)persist));
-      hipLaunchKernelGGL(SP.rdiff ? k_shade_tex : k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b,
+      const bool phong = (ctx->opts.flags & GI_FLAG_PHONG) != 0;
+      hipLaunchKernelGGL(phong ? (SP.rdiff ? k_shade_phong_tex : k_shade_phong) : (SP.rdiff ? k_shade_tex : k_shade),
+                         dim3(sg), dim3(sblock), 0, ctx->stream, b,
                          cnt + b, SP, ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(),
                          ctx->q_tp[cur].as<float4>(), ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(),
                          ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(), ctx->q_tp[nxt].as<float4>(),
@@ -2667,6 +2760,11 @@ int gi_upload_materials(gi_ctx *ctx, const gi_material *mats, int32_t n) {
   GI_HIP(ctx, copy_sync(ctx, ctx->d_mats.p, dm.data(), n * sizeof(DevMat), hipMemcpyHostToDevice));
   GI_HIP(ctx, ctx->d_matgrad.ensure(n * sizeof(DevMatGrad)));
   GI_HIP(ctx, copy_sync(ctx, ctx->d_matgrad.p, dg.data(), n * sizeof(DevMatGrad), hipMemcpyHostToDevice));
+  std::vector<float4> ks(n);
+  for (int32_t i = 0; i < n; ++i)
+    ks[i] = make_float4(mats[i].specular_color[0], mats[i].specular_color[1], mats[i].specular_color[2], 0.0f);
+  GI_HIP(ctx, ctx->d_ks.ensure(n * sizeof(float4)));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_ks.p, ks.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   return GI_OK;
 }
 
@@ -3441,6 +3539,8 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
   if (!dL_dimage || !d_mats || !d_emission) return fail(ctx, GI_ERR_INVALID_ARG, "adjoint buffers");
   if (ctx->views.empty()) return fail(ctx, GI_ERR_STATE, "gi_render first");
   if (ctx->opts.rr_start > 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support Russian roulette");
+  if (ctx->opts.flags & GI_FLAG_PHONG)
+    return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support the Phong material model");
   if (ctx->sky_light >= 0) return fail(ctx, GI_ERR_INVALID_ARG, "the adjoint does not support sky lights");
   const int32_t nm = (int32_t)ctx->mats.size(), nl = (int32_t)ctx->lights.size();
   if ((size_t)(5 * nm + 3 * nl) * sizeof(float) > 48 * 1024)

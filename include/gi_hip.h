@@ -142,6 +142,19 @@ typedef struct {
 } gi_render_opts;
 
 #define GI_FLAG_BATCHED   0x1u  /* all views share one set of queues       */
+#define GI_FLAG_PHONG     0x4u  /* gpu_pt's material model for every material
+                                   (compute_path_contribution_and_bounce,
+                                   src/gpu-pt-kernels.cu:89-199): Lambert with
+                                   diffuse_color (times the diffuse texture)
+                                   plus a Phong lobe of shininess 100 with
+                                   specular_color, both divided by their six-
+                                   channel sum when it exceeds 1; the lobe is
+                                   chosen by the colour sums (absorption past
+                                   them, and below the surface); single-cosine
+                                   NEE; no throughput clamp; the next ray starts
+                                   offset_bounce along its direction.  Principled
+                                   parameters are not used.  Not differentiable
+                                   (gi_adjoint refuses it). */
 #define GI_FLAG_GRAPH     0x2u  /* bounce loop captured into a hipGraph; the
                                    context keeps the executable graph and
                                    relaunches it for later renders whose

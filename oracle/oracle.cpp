@@ -1039,6 +1039,7 @@ struct Options {
   int32_t depth, rr_start;
   float off_b, off_s;
   float bg[3];
+  bool phong; /* GI_FLAG_PHONG: gpu_pt's Lambert + Phong material model */
 };
 
 Options resolve(const or_scene &s, const gi_render_opts &o) {
@@ -1049,7 +1050,110 @@ Options resolve(const or_scene &s, const gi_render_opts &o) {
   r.off_b = o.offset_bounce > 0.f ? o.offset_bounce : 1e-4f * diag;
   r.off_s = o.offset_shadow > 0.f ? o.offset_shadow : 1e-4f * diag;
   for (int k = 0; k < 3; ++k) r.bg[k] = o.background[k];
+  r.phong = (o.flags & GI_FLAG_PHONG) != 0;
   return r;
+}
+
+/* ------------------------------------------------------ gpu_pt's Phong lobe
+ * compute_path_contribution_and_bounce, src/gpu-pt-kernels.cu:89-199.  The
+ * powers of the lobe (shininess 100): x^100 as a chain of squarings and the
+ * sampler's x^(1/101) as 2^(log2(x)/101), log2 from the atanh series (m in
+ * [sqrt(1/2), sqrt(2))), 2^f from its Taylor polynomial -- basic operations
+ * in one order, in FP32; FP64 uses the C library's pow. */
+inline float phong_pow100(float x) {
+  const float x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8, x32 = x16 * x16, x64 = x32 * x32;
+  return (x64 * x32) * x4;
+}
+inline double phong_pow100(double x) { return std::pow(x, 100.0); }
+inline float phong_root101(float x) {
+  if (!(x > 0.0f)) return 0.0f;
+  if (x >= 1.0f) return 1.0f;
+  uint32_t bits;
+  std::memcpy(&bits, &x, 4);
+  if (bits < 0x00800000u) return 0.4214777f;
+  int32_t e = (int32_t)(bits >> 23) - 127;
+  uint32_t mb = (bits & 0x7fffffu) | 0x3f800000u;
+  if (mb >= 0x3fb504f3u) { mb -= 0x00800000u; e += 1; }
+  float m;
+  std::memcpy(&m, &mb, 4);
+  const float t = (m - 1.0f) / (m + 1.0f), t2 = t * t;
+  const float ca[6] = {0.09090909090909091f, 0.1111111111111111f, 0.14285714285714285f, 0.2f,
+                       0.3333333333333333f, 1.0f};
+  float a = ca[0];
+  for (int i = 1; i < 6; ++i) a = a * t2 + ca[i];
+  const float l2 = (float)e + (a * t) * 2.8853900817779268f;
+  const float y = l2 / 101.0f;
+  const float k = std::floor(y), f = y - k;
+  const float cp[10] = {1.0178086009239699e-07f, 1.3215486790144307e-06f, 1.5252733804059841e-05f,
+                        0.00015403530393381609f, 0.0013333558146428443f, 0.009618129107628477f,
+                        0.05550410866482158f, 0.2402265069591007f, 0.6931471805599453f, 1.0f};
+  float p = cp[0];
+  for (int i = 1; i < 10; ++i) p = p * f + cp[i];
+  const uint32_t sb = (uint32_t)((int32_t)k + 127) << 23;
+  float scale;
+  std::memcpy(&scale, &sb, 4);
+  return p * scale;
+}
+inline double phong_root101(double x) { return x > 0.0 ? std::pow(x, 1.0 / 101.0) : 0.0; }
+template <class R> struct PhongK;
+template <> struct PhongK<float> {
+  static constexpr float norm = 16.07464925228143f, two_pi = 6.283185307179586f;
+};
+template <> struct PhongK<double> {
+  static constexpr double norm = 101.0 / (2.0 * M_PI), two_pi = 2.0 * M_PI;
+};
+
+/* One Phong-mode vertex: the NEE weight fnee (no cosine: the NEE
+ * contribution holds it) and the bounce -- false if the path is absorbed
+ * (no lobe chosen, a specular direction below the surface, or a zero
+ * density), else the new direction and the throughput factor.  kd is the
+ * diffuse colour times the texel, ks the specular colour; N the
+ * interpolated, normalised, unflipped shading normal. */
+template <class R>
+bool phong_vertex(V3<R> kd, V3<R> ks, V3<R> N, V3<R> org_dir, V3<R> sd, V3<R> xp, V3<R> &fnee, V3<R> &wi,
+                  V3<R> &tpmul) {
+  const R sum = ((((kd.x + kd.y) + kd.z) + ks.x) + ks.y) + ks.z;
+  if (sum > R(1)) {
+    kd = divs(kd, sum);
+    ks = divs(ks, sum);
+  }
+  const R kr = R(2) * dot(org_dir, N);
+  const V3<R> Rr = mk<R>(org_dir.x - N.x * kr, org_dir.y - N.y * kr, org_dir.z - N.z * kr);
+  const V3<R> l = normalize(sd);
+  const R cl = phong_pow100(std::max(dot(Rr, l), R(0)));
+  fnee = kd * K<R>::inv_pi + ks * (PhongK<R>::norm * cl);
+  R pd = (kd.x + kd.y) + kd.z;
+  R ps = (ks.x + ks.y) + ks.z;
+  if (pd + ps > R(1)) {
+    pd = pd / (pd + ps);
+    ps = ps / (pd + ps); /* the reference's order: with the renormalised pd */
+  }
+  const bool dif = xp.z <= pd;
+  if (!dif && !(xp.z <= pd + ps)) return false;
+  const V3<R> axis = dif ? N : Rr;
+  V3<R> Tt, Bt;
+  tangent_frame(axis, Tt, Bt);
+  const R z = dif ? std::sqrt(xp.x) : phong_root101(xp.x);
+  const R r = std::sqrt(R(1) - z * z);
+  R sn, cs;
+  sincos2pi(xp.y, sn, cs);
+  const V3<R> dir = (Tt * (r * cs) + Bt * (r * sn)) + axis * z;
+  const R dn = dot(dir, N);
+  V3<R> f;
+  R den;
+  if (dif) {
+    f = kd * K<R>::inv_pi;
+    den = R(2) * z;
+  } else {
+    if (dn < R(0)) return false;
+    f = ks * (PhongK<R>::norm * phong_pow100(std::max(dot(Rr, dir), R(0))));
+    den = R(101) * phong_pow100(z);
+  }
+  if (!(den > R(0))) return false;
+  const R w = (R(1) / (dif ? pd : ps)) * (PhongK<R>::two_pi / den);
+  tpmul = (f * std::fabs(dn)) * w;
+  wi = normalize(dir);
+  return true;
 }
 
 /* Sky light sample for NEE (pixel_generate_arealight_sample_cpu with
@@ -1131,6 +1235,8 @@ V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H,
     int32_t mat;
     surface<R>(s, id, bu, bv, P, Ni, mat);
     MatP<R> mp(s.mats[mat]);
+    V3<R> texel = mk<R>(1, 1, 1);
+    bool textured = false;
     if (df) {
       const V3<R> Nn = normalize(Ni);
       float tcx, tcy;
@@ -1139,7 +1245,9 @@ V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H,
       const int32_t tex = s.mats[mat].texture;
       if (tex >= 0) {
         const V3<float> tx = tex_sample(s.textures[(size_t)tex], tcx, tcy, diff);
-        mp.apply_texel(mk<R>(tx.x, tx.y, tx.z));
+        texel = mk<R>(tx.x, tx.y, tx.z);
+        textured = true;
+        mp.apply_texel(texel);
       }
     }
 
@@ -1174,6 +1282,35 @@ V3<R> trace_sample(const or_scene &s, const Tracer<R> &tr, const HaltonTable &H,
     }
     V3<R> so = P + sd * R(op.off_s);
     V3<R> psc = contrib * (R(s.power_total) / R(lt.power));
+
+    if (op.phong) {
+      /* gpu_pt's material model: NEE with the whole lobe, then the bounce
+       * (offset along the new direction, no throughput clamp) */
+      V3<R> kd = ld<R>(s.mats[mat].diffuse_color);
+      if (textured) kd = mul(kd, texel);
+      const V3<R> ks = ld<R>(s.mats[mat].specular_color);
+      const V3<R> xp = mk<R>(R(H.sample(n, dim + 3)), R(H.sample(n, dim + 4)), R(H.sample(n, dim + 5)));
+      V3<R> fnee = mk<R>(0, 0, 0), wi = mk<R>(0, 0, 0), tpmul = mk<R>(0, 0, 0);
+      const bool alive = phong_vertex(kd, ks, normalize(Ni), org_dir, sd, xp, fnee, wi, tpmul);
+      const V3<R> C = mul(fnee, mul(TP, psc));
+      if (C.x > R(0) || C.y > R(0) || C.z > R(0)) {
+        ++st.rays_shadow;
+        if (!tr.any(so, sd, len, st.box_shadow, st.tri_shadow)) L = L + C;
+      }
+      if (b + 1 >= op.depth || !alive) break;
+      V3<R> tp = mul(TP, tpmul);
+      if (op.rr_start > 0 && b + 1 >= op.rr_start) {
+        R q = maxc(tp);
+        R xr = R(H.sample(n, dim + 6));
+        if (!(xr < q)) break;
+        tp = divs(tp, q);
+      }
+      TP = tp;
+      o = P + wi * R(op.off_b);
+      d = wi;
+      tmax = R(FLT_MAX);
+      continue;
+    }
 
     /* bounce: getHitDataTriangle + sample_material + evaluate_material,
      * src/hybrid-pt.cpp:330-371,421-448,519-563. */
@@ -1938,6 +2075,18 @@ double or_walter_weight_f64(const double *L, const double *V, const double *m, d
                        ax, ay);
 }
 double or_gtr1_f64(double ndh, double a) { return gtr1(ndh, a); }
+float or_phong_pow100(float x) { return phong_pow100(x); }
+float or_phong_root101(float x) { return phong_root101(x); }
+int or_phong_vertex(const float *kd, const float *ks, const float *N, const float *org_dir, const float *sd,
+                    const float *xp, float *fnee, float *wi, float *tpmul) {
+  V3<float> f = mk<float>(0, 0, 0), w = mk<float>(0, 0, 0), t = mk<float>(0, 0, 0);
+  const bool alive = phong_vertex(ld<float>(kd), ld<float>(ks), ld<float>(N), ld<float>(org_dir), ld<float>(sd),
+                                  ld<float>(xp), f, w, t);
+  std::memcpy(fnee, &f, 12);
+  std::memcpy(wi, &w, 12);
+  std::memcpy(tpmul, &t, 12);
+  return alive ? 1 : 0;
+}
 void or_camera_basis(const gi_view *v, float *U, float *V, float *W, float *wh) {
   Cam c = make_cam(*v);
   std::memcpy(U, c.U, 12);

@@ -99,6 +99,15 @@ void or_brdf_eval_f64(const gi_material *m, int type, const double *wo, const do
 void or_gtr2_half_f64(double u, double v, double ax, double ay, double *h);
 double or_walter_weight_f64(const double *L, const double *V, const double *m, double ax, double ay);
 double or_gtr1_f64(double ndh, double a);
+/* gpu_pt's Phong model (GI_FLAG_PHONG): the lobe's powers x^100 and
+ * x^(1/101), and one vertex -- kd, ks, the normalised shading normal, the
+ * incoming direction, the light direction and the bounce sample in, the
+ * NEE weight, the new direction and the throughput factor out; returns 0
+ * when the path is absorbed. */
+float or_phong_pow100(float x);
+float or_phong_root101(float x);
+int or_phong_vertex(const float *kd, const float *ks, const float *N, const float *org_dir, const float *sd,
+                    const float *xp, float *fnee, float *wi, float *tpmul);
 /* Camera basis U, V, W and image-plane half extents (width, height) of a
  * view (src/raygen.cu:17-39). */
 void or_camera_basis(const gi_view *v, float *U, float *V, float *W, float *wh);

@@ -84,6 +84,9 @@ def load():
             "or_walter_weight_f64": (C.c_double, [P, P, P, C.c_double, C.c_double]),
             "or_gtr1_f64": (C.c_double, [C.c_double, C.c_double]),
             "or_camera_basis": (None, [P, P, P, P, P]),
+            "or_phong_pow100": (C.c_float, [C.c_float]),
+            "or_phong_root101": (C.c_float, [C.c_float]),
+            "or_phong_vertex": (C.c_int, [P, P, P, P, P, P, P, P, P]),
         }
         for k, (res, args) in sig.items():
             f = getattr(lib, k)
@@ -312,6 +315,14 @@ def gtr2_half(u, v, ax, ay):
 def walter_weight(L, V, m, ax, ay):
     L, V, m = (np.ascontiguousarray(x, np.float64) for x in (L, V, m))
     return float(load().or_walter_weight_f64(_p(L), _p(V), _p(m), ax, ay))
+
+
+def phong_vertex(kd, ks, N, org_dir, sd, xp):
+    """(alive, NEE weight, new direction, throughput factor) of one Phong-mode vertex (FP32)."""
+    args = [np.ascontiguousarray(x, np.float32) for x in (kd, ks, N, org_dir, sd, xp)]
+    out = [np.zeros(3, np.float32) for _ in range(3)]
+    alive = load().or_phong_vertex(*[_p(x) for x in args], *[_p(x) for x in out])
+    return bool(alive), out[0], out[1], out[2]
 
 
 def gtr1(ndh, a):

@@ -322,3 +322,54 @@ def test_reference_material_assets_bit_exact(spp, depth, oracle, gpu_ctx_factory
     _same(img, ref)
     _same(b, ref)
     assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
+
+
+def _phong_mats(n, seed):
+    """Materials for gpu_pt's model: diffuse and specular colours (the
+    reference's .mtl assets give several materials both), some with colour
+    sums above 1 (divided by the sum), some without a specular part."""
+    rng = np.random.default_rng(seed)
+    mats = np.concatenate([scenes.lambert(rng.uniform(0.05, 0.7, 3)) for _ in range(n)])
+    mats["specular_color"] = rng.uniform(0.0, 0.45, (n, 3))
+    mats["specular_color"][0] = 0.0
+    mats["diffuse_color"][1] = 0.0
+    return mats
+
+
+@pytest.mark.parametrize("flags,rr", [(0, 0), (FLAG_BATCHED | FLAG_GRAPH, 0), (FLAG_BATCHED, 3)])
+def test_phong_mode_bit_exact(flags, rr, oracle, gpu_ctx_factory):
+    from globillum_amd._lib import FLAG_PHONG
+    sc = scenes.terrain_spheres(cells=(48, 48), n_spheres=120, n_mats=8, w=128, h=72, spp=8, depth=6)
+    sc.mats = _phong_mats(8, 5)
+    assert ((sc.mats["diffuse_color"].sum(1) + sc.mats["specular_color"].sum(1)) > 1).any()
+    opts = sc.opts.copy()
+    opts["flags"] = flags | FLAG_PHONG
+    opts["rr_start"] = rr
+    ref, st = _oracle(oracle, sc, opts=opts)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render(sc.views, opts).image(0).reshape(-1, 3)
+        c = ctx.counters()
+        plain = ctx.render().image(0).reshape(-1, 3)
+    _same(img, ref)
+    assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
+    assert not np.array_equal(img, plain)  # the flag does change the material model
+
+
+def test_phong_mode_textured_bit_exact(oracle, gpu_ctx_factory):
+    from globillum_amd._lib import FLAG_PHONG
+    sc = scenes.textured_scene(w=64, h=48, spp=4, depth=4)
+    rng = np.random.default_rng(2)
+    sc.mats["specular_color"] = rng.uniform(0.0, 0.3, (len(sc.mats), 3))
+    opts = sc.opts.copy()
+    opts["flags"] = FLAG_BATCHED | FLAG_PHONG
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights, textures=sc.textures, meshes=sc.meshes,
+                             instances=sc.instances, sky=sc.sky)
+    ref, st = osc.render(sc.views, opts)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render(sc.views, opts).image(0)
+        c = ctx.counters()
+        with pytest.raises(GiError) as e:
+            ctx.adjoint(np.ones(64 * 48 * 3, np.float32))
+        assert e.value.status == 1
+    _same(img, ref)
+    assert c["rays_closest"] == st["rays_closest"]

@@ -25,7 +25,13 @@ fixes, independently of the oracle's own code:
   straight through (L = -V; at normal incidence L = -n), total internal
   reflection mirrors V, and the transmitted direction is the reference's
   (ior c - sqrt(1 + ior (c^2 - 1))) n - ior V (with ior, not ior^2, under
-  the root -- not Snell's law; reproduced as written)."""
+  the root -- not Snell's law; reproduced as written);
+- gpu_pt's Lambert + Phong model (src/gpu-pt-kernels.cu:89-199): the powers
+  x^100 and x^(1/101) against numpy; the Phong lobe normalised (101/(2 pi)
+  int cos^100 = 1); the cos^n samplers' distribution (CDF mu^(n+1) around N
+  and around the mirror direction) and their throughput factors kd/pd and
+  ks |dir.N|/ps; the lobe choice by the colour sums and the division by the
+  six-channel sum above 1."""
 import math
 
 import numpy as np
@@ -397,3 +403,84 @@ def test_glass_refraction_and_total_internal_reflection(oracle):
     wi = np.array([0.3, 0.1, -0.9])
     assert np.array_equal(oracle.brdf_eval(m, T_SPEC_REFL, (0, 0, 1), wi, (0, 0, 1), f64=True), [1, 1, 1])
     assert np.allclose(oracle.brdf_eval(m, T_SPEC_TRANS, (0, 0, 1), wi, (0, 0, 1), f64=True), 1 / 0.9, rtol=1e-14)
+
+
+# ------------------------------------------------- gpu_pt's Phong model --
+def test_phong_powers(oracle):
+    rng = np.random.default_rng(41)
+    xs = np.concatenate([rng.uniform(0, 1, 5000), [1e-30, 1e-8, 0.5, 0.9999999, 2.0 ** -126]]).astype(np.float32)
+    for x in xs:
+        x = float(x)
+        assert oracle.load().or_phong_root101(x) == pytest.approx(x ** (1 / 101), rel=2e-7)
+        assert oracle.load().or_phong_pow100(x) == pytest.approx(float(np.float32(x)) ** 100, rel=2e-5, abs=1e-38)
+    assert oracle.load().or_phong_root101(0.0) == 0.0 and oracle.load().or_phong_root101(1.0) == 1.0
+
+
+def _phong(oracle, kd, ks, xp, N=(0, 0, 1), org_dir=(0, 0, -1), sd=(0, 0, 1)):
+    return oracle.phong_vertex(kd, ks, N, org_dir, sd, xp)
+
+
+def test_phong_lobe_is_normalised(oracle):
+    # straight down onto N = z: R = z.  The NEE weight of the specular lobe
+    # is 101/(2 pi) cos^100 of the angle to R; over the hemisphere around R
+    # it integrates to 1 (2 pi int mu^100 dmu = 2 pi / 101)
+    from scipy.integrate import quad
+
+    def f(mu):
+        l = (math.sqrt(max(0.0, 1 - mu * mu)), 0.0, mu)
+        _, fnee, _, _ = _phong(oracle, (0, 0, 0), (1, 0, 0), (0.5, 0.5, 0.99), sd=l)
+        return float(fnee[0])
+
+    val = 2 * math.pi * quad(f, 0, 1, points=[0.9, 0.97, 0.99], limit=200, epsrel=1e-7)[0]
+    assert val == pytest.approx(1.0, rel=1e-4)
+    # the diffuse part is kd / pi, whatever the light direction
+    _, fnee, _, _ = _phong(oracle, (0.3, 0.2, 0.1), (0, 0, 0), (0.5, 0.5, 0.1), sd=(0.6, 0.0, 0.8))
+    assert np.allclose(fnee, np.array([0.3, 0.2, 0.1]) / math.pi, rtol=1e-6)
+
+
+@pytest.mark.parametrize("lobe", ["diffuse", "specular"])
+def test_phong_sampler_cos_power_distribution(oracle, lobe):
+    # cos^n lobes: the cosine of the angle to the lobe's axis has CDF mu^(n+1)
+    # (n = 1 around N, n = 100 around the mirror direction R)
+    n = 1 if lobe == "diffuse" else 100
+    kd, ks = ((0.4, 0.3, 0.2), (0.0, 0.0, 0.0)) if lobe == "diffuse" else ((0.0, 0.0, 0.0), (0.4, 0.3, 0.2))
+    org = np.array([0.6, 0.0, -0.8])  # R = (0.6, 0, 0.8)
+    Rr = np.array([0.6, 0.0, 0.8])
+    axis = np.array([0.0, 0.0, 1.0]) if lobe == "diffuse" else Rr
+    rng = np.random.default_rng(7 + n)
+    F, phis = [], []
+    for x, y in rng.uniform(0, 1, (20000, 2)):
+        alive, _, wi, tpm = _phong(oracle, kd, ks, (x, y, 0.3), org_dir=org)
+        if not alive:
+            assert lobe == "specular" and wi @ np.array([0, 0, 1.0]) < 0 or np.all(wi == 0)
+            continue
+        mu = float(np.clip(wi @ axis, -1, 1))
+        F.append(max(mu, 0.0) ** (n + 1))
+        # throughput factor: kd/pd (diffuse), ks |dir.N| / ps (specular)
+        if lobe == "diffuse":
+            assert np.allclose(tpm, np.array(kd) / 0.9, rtol=2e-5)
+        else:
+            assert np.allclose(tpm, np.array(ks) * wi[2] / 0.9, rtol=5e-4)
+    F = np.array(F)
+    assert len(F) > 19000 and _chi2_uniform(F) < CHI2_19_DOF_P1E4
+
+
+def test_phong_lobe_choice_and_energy_normalisation(oracle):
+    # sums below 1: diffuse when xi_z <= sum(kd), specular up to sum(kd) +
+    # sum(ks), absorbed beyond
+    kd, ks = (0.1, 0.1, 0.1), (0.05, 0.05, 0.05)
+    for z, expect in ((0.29, "d"), (0.31, "s"), (0.44, "s"), (0.46, "a")):
+        alive, _, wi, _ = _phong(oracle, kd, ks, (0.5, 0.25, z), org_dir=(0.0, 0.0, -1.0))
+        if expect == "a":
+            assert not alive
+        else:
+            assert alive and (wi[2] > 0)
+    # sums above 1: kd and ks divided by the six-channel sum, no absorption
+    kd, ks = (0.9, 0.8, 0.7), (0.6, 0.5, 0.4)
+    s = sum(kd) + sum(ks)
+    _, fnee, _, _ = _phong(oracle, kd, ks, (0.5, 0.5, 0.5), sd=(0.6, 0.0, 0.8))
+    assert np.allclose(fnee, np.array(kd) / s / math.pi + np.array(ks) / s * 101 / (2 * math.pi) *
+                       max(0.0, np.array([0.6, 0.0, 0.8]) @ np.array([0.0, 0.0, 1.0])) ** 100, rtol=1e-5)
+    for z in np.linspace(0.01, 0.999, 40):
+        alive, _, _, _ = _phong(oracle, kd, ks, (0.5, 0.3, z))
+        assert alive
