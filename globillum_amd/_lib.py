@@ -51,6 +51,7 @@ GI_INVALID_REF = 0xFFFFFFFF
 FLAG_BATCHED = 0x1
 FLAG_GRAPH = 0x2
 FLAG_PHONG = 0x4
+FLAG_PIXEL_CENTRE = 0x8
 STATUS = {0: "ok", 1: "invalid argument", 2: "no HIP device", 3: "HIP runtime error", 4: "invalid call order",
           5: "out of device memory", 6: "size beyond a documented limit", 7: "unknown name"}
 

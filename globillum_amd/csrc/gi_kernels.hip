@@ -47,7 +47,7 @@ struct alignas(16) DevView {
   uint32_t index_base;
   float4 *image;
   uint32_t pix_base;  // offset of the view's pixels in views-concatenated images
-  int32_t pad;
+  int32_t centre;     // 1: primary rays through the pixel centres (GI_FLAG_PIXEL_CENTRE)
 };
 
 constexpr int kMaxChunks = 64;
@@ -715,8 +715,11 @@ __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pas
     const uint32_t p = local % (uint32_t)ck.npix;
     const uint32_t n = v.index_base + p * (uint32_t)v.spp + s;
     f3 o, d;
-    camera_ray(v, (int32_t)(p % (uint32_t)v.w), (int32_t)(p / (uint32_t)v.w), halton(n, kDimJitterX, H),
-               halton(n, kDimJitterY, H), halton(n, kDimLensX, H), halton(n, kDimLensY, H), o, d);
+    // jittered (setup_jittered_shirley / _dof_shirley) or through the pixel
+    // centre (setup_shirley_lens_rays, src/dofrays.cu:21-22)
+    const float jx = v.centre ? 0.5f : halton(n, kDimJitterX, H), jy = v.centre ? 0.5f : halton(n, kDimJitterY, H);
+    camera_ray(v, (int32_t)(p % (uint32_t)v.w), (int32_t)(p / (uint32_t)v.w), jx, jy, halton(n, kDimLensX, H),
+               halton(n, kDimLensY, H), o, d);
     ray_o[slot] = make_float4(o.x, o.y, o.z, bitsf(n));
     ray_d[slot] = make_float4(d.x, d.y, d.z, bitsf(slot));
     tp[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
@@ -2573,6 +2576,7 @@ int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_ren
   for (int32_t v = 0; v < n_views; ++v) {
     const gi_view &w = views[v];
     if (!make_devview(w, dv[v], err)) return fail(ctx, GI_ERR_INVALID_ARG, err);
+    dv[v].centre = (opts->flags & GI_FLAG_PIXEL_CENTRE) ? 1 : 0;
     total += (uint64_t)w.w * (uint64_t)w.h * (uint64_t)w.spp;
     if (total >= (1ull << 32)) return fail(ctx, GI_ERR_LIMIT, "more than 2^32 samples in one render");
     dv[v].index_base = base;

@@ -142,6 +142,14 @@ typedef struct {
 } gi_render_opts;
 
 #define GI_FLAG_BATCHED   0x1u  /* all views share one set of queues       */
+#define GI_FLAG_PIXEL_CENTRE 0x8u /* primary rays through the pixel centres
+                                   instead of jittered positions: with
+                                   aperture > 0 the reference's
+                                   k::setup_shirley_lens_rays (src/dofrays.cu:
+                                   10-68; its unit-disk rejection never
+                                   rejects the [-1/2,1/2]^2 lens jitter used
+                                   here), with aperture 0 its pixel-centre
+                                   pinhole (src/raygen.cu:24-25) */
 #define GI_FLAG_PHONG     0x4u  /* gpu_pt's material model for every material
                                    (compute_path_contribution_and_bounce,
                                    src/gpu-pt-kernels.cu:89-199): Lambert with

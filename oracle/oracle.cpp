@@ -1040,6 +1040,7 @@ struct Options {
   float off_b, off_s;
   float bg[3];
   bool phong; /* GI_FLAG_PHONG: gpu_pt's Lambert + Phong material model */
+  bool centre; /* GI_FLAG_PIXEL_CENTRE: primary rays through the pixel centres */
 };
 
 Options resolve(const or_scene &s, const gi_render_opts &o) {
@@ -1051,6 +1052,7 @@ Options resolve(const or_scene &s, const gi_render_opts &o) {
   r.off_s = o.offset_shadow > 0.f ? o.offset_shadow : 1e-4f * diag;
   for (int k = 0; k < 3; ++k) r.bg[k] = o.background[k];
   r.phong = (o.flags & GI_FLAG_PHONG) != 0;
+  r.centre = (o.flags & GI_FLAG_PIXEL_CENTRE) != 0;
   return r;
 }
 
@@ -1389,7 +1391,8 @@ void render_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_r
       for (int32_t sidx = 0; sidx < v.spp; ++sidx) {
         uint64_t n = base + (uint64_t)p * (uint64_t)v.spp + (uint64_t)sidx;
         V3<float> of, df;
-        cam_ray(cam, x, y, H.sample(n, DIM_JX), H.sample(n, DIM_JY), H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
+        cam_ray(cam, x, y, op.centre ? 0.5f : H.sample(n, DIM_JX), op.centre ? 0.5f : H.sample(n, DIM_JY),
+                H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
         Diffs dfs;
         if (s.textured) cam_diffs(cam, x, y, dfs);
         V3<R> Ls = trace_sample<R>(s, tr, H, op, n, mk<R>(of.x, of.y, of.z), mk<R>(df.x, df.y, df.z), st,
@@ -1755,7 +1758,8 @@ void adjoint_impl(const or_scene &s, const gi_view *views, int32_t nv, const gi_
       for (int32_t si = 0; si < v.spp; ++si) {
         uint64_t n = base + (uint64_t)p * (uint64_t)v.spp + (uint64_t)si;
         V3<float> of, df;
-        cam_ray(cam, x, y, H.sample(n, DIM_JX), H.sample(n, DIM_JY), H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
+        cam_ray(cam, x, y, op.centre ? 0.5f : H.sample(n, DIM_JX), op.centre ? 0.5f : H.sample(n, DIM_JY),
+                H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
         Diffs dfs;
         if (s.textured) cam_diffs(cam, x, y, dfs);
         adjoint_sample<R>(s, tr, H, op, n, mk<R>(of.x, of.y, of.z), mk<R>(df.x, df.y, df.z), delta, gm[th], ge[th],
@@ -1987,15 +1991,17 @@ float or_halton_sample(uint64_t n, int32_t dim, uint32_t seed) {
 int32_t or_prime(int32_t dim) { return primes().p[dim]; }
 void or_halton_perm(int32_t dim, uint32_t seed, uint16_t *perm) { halton_perm(dim, seed, perm); }
 
-void or_camera_rays(const gi_view *v, int32_t sample, uint32_t seed, uint64_t index_base, float *o, float *d,
-                    float *tmax) {
+void or_camera_rays(const gi_view *v, int32_t sample, uint32_t seed, uint64_t index_base, uint32_t flags, float *o,
+                    float *d, float *tmax) {
+  const bool centre = (flags & GI_FLAG_PIXEL_CENTRE) != 0;
   Cam cam = make_cam(*v);
   HaltonTable H(seed, DIM_VTX0);
   int64_t npix = (int64_t)v->w * v->h;
   for (int64_t p = 0; p < npix; ++p) {
     uint64_t n = index_base + (uint64_t)p * (uint64_t)v->spp + (uint64_t)sample;
     V3<float> of, df;
-    cam_ray(cam, (int32_t)(p % v->w), (int32_t)(p / v->w), H.sample(n, DIM_JX), H.sample(n, DIM_JY),
+    cam_ray(cam, (int32_t)(p % v->w), (int32_t)(p / v->w), centre ? 0.5f : H.sample(n, DIM_JX),
+            centre ? 0.5f : H.sample(n, DIM_JY),
             H.sample(n, DIM_LX), H.sample(n, DIM_LY), of, df);
     std::memcpy(o + 3 * p, &of, 12);
     std::memcpy(d + 3 * p, &df, 12);

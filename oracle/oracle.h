@@ -71,8 +71,9 @@ float or_halton_sample(uint64_t n, int32_t dim, uint32_t seed);
 int32_t or_prime(int32_t dim);
 void or_halton_perm(int32_t dim, uint32_t seed, uint16_t *perm);
 
-/* Camera rays for one sample of one view (src/raygen.cu:10-118). */
-void or_camera_rays(const gi_view *v, int32_t sample, uint32_t seed, uint64_t index_base,
+/* Camera rays for one sample of one view (src/raygen.cu:10-118; flags:
+ * GI_FLAG_PIXEL_CENTRE for src/dofrays.cu:10-68). */
+void or_camera_rays(const gi_view *v, int32_t sample, uint32_t seed, uint64_t index_base, uint32_t flags,
                     float *o, float *d, float *tmax);
 
 /* Tangent frame (stand-in for libhyb make_tangent_frame). */

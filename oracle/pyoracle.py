@@ -64,7 +64,7 @@ def load():
             "or_halton_sample": (C.c_float, [U64, I32, U32]),
             "or_prime": (I32, [I32]),
             "or_halton_perm": (None, [I32, U32, P]),
-            "or_camera_rays": (None, [P, I32, U32, U64, P, P, P]),
+            "or_camera_rays": (None, [P, I32, U32, U64, U32, P, P, P]),
             "or_tangent_frame": (None, [P, P, P]),
             "or_nee": (None, [P, P, P, P, P, P, P, P]),
             "or_brdf_sample": (C.c_int, [P, P, P, P, P]),
@@ -288,13 +288,13 @@ def halton_perm(dim, seed):
     return p
 
 
-def camera_rays(view, sample, seed, index_base=0):
+def camera_rays(view, sample, seed, index_base=0, flags=0):
     v = np.ascontiguousarray(view).reshape(1)
     n = int(v["w"][0]) * int(v["h"][0])
     o = np.zeros((n, 3), np.float32)
     d = np.zeros((n, 3), np.float32)
     t = np.zeros(n, np.float32)
-    load().or_camera_rays(_p(v), sample, seed, index_base, _p(o), _p(d), _p(t))
+    load().or_camera_rays(_p(v), sample, seed, index_base, flags, _p(o), _p(d), _p(t))
     return o, d, t
 
 

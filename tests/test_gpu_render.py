@@ -373,3 +373,28 @@ def test_phong_mode_textured_bit_exact(oracle, gpu_ctx_factory):
         assert e.value.status == 1
     _same(img, ref)
     assert c["rays_closest"] == st["rays_closest"]
+
+
+@pytest.mark.parametrize("aperture", [0.0, 0.06])
+def test_pixel_centre_generator_bit_exact(aperture, oracle, gpu_ctx_factory):
+    from globillum_amd._lib import FLAG_PIXEL_CENTRE
+    sc = scenes.cornell_boxes(w=80, h=64, spp=6, depth=4)
+    sc.views["aperture"] = aperture
+    sc.views["focus_distance"] = 1.6
+    opts = sc.opts.copy()
+    opts["flags"] = FLAG_BATCHED | FLAG_PIXEL_CENTRE
+    opts["background"] = 0.0
+    ref, st = _oracle(oracle, sc, opts=opts)
+    dL = np.random.default_rng(4).normal(size=(80 * 64, 3)).astype(np.float32)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render(sc.views, opts).image(0).reshape(-1, 3)
+        c = ctx.counters()
+        gm, ge = ctx.adjoint(dL)
+        jittered = ctx.render(flags=FLAG_BATCHED).image(0).reshape(-1, 3)
+    _same(img, ref)
+    assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
+    assert not np.array_equal(img, jittered)
+    osc = oracle.OracleScene(sc.tris, sc.mats, sc.lights)
+    om, oe = oracle.adjoint(osc, sc.views, opts, dL, f64=False)
+    assert np.allclose(gm, om, rtol=2e-3, atol=2e-4 * np.abs(om).max())
+    assert np.allclose(ge, oe, rtol=2e-3, atol=2e-4 * np.abs(oe).max())

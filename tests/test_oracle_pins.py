@@ -484,3 +484,36 @@ def test_phong_lobe_choice_and_energy_normalisation(oracle):
     for z in np.linspace(0.01, 0.999, 40):
         alive, _, _, _ = _phong(oracle, kd, ks, (0.5, 0.3, z))
         assert alive
+
+
+def test_pixel_centre_generator(oracle):
+    # setup_shirley_lens_rays (src/dofrays.cu:10-68): rays through the pixel
+    # centres; with a lens, every sample of a pixel passes through the one
+    # focal-plane point of the centre's pinhole ray
+    PIXEL_CENTRE = 0x8
+    a, f = 0.2, 2.5
+    for k in range(3):
+        pos, dd, up, fovy, w, h = VIEWS[k]
+        pin = _view(pos, dd, up, fovy, w, h, spp=4)
+        lens = _view(pos, dd, up, fovy, w, h, spp=4, aperture=a, focus=f)
+        U, V, W, wh = (x.astype(np.float64) for x in oracle.camera_basis(pin))
+        P = pin["pos"][0].astype(np.float64)
+        p = np.arange(w * h)
+        x, y = p % w, p // w
+        _, d0, _ = oracle.camera_rays(pin, 0, 5, flags=PIXEL_CENTRE)
+        for sample in (1, 3):
+            _, ds, _ = oracle.camera_rays(pin, sample, 9, flags=PIXEL_CENTRE)
+            assert np.array_equal(ds, d0)  # no jitter: the same ray for every sample and seed
+        d0 = d0.astype(np.float64)
+        tu, tv = (d0 @ U) / (d0 @ W), (d0 @ V) / (d0 @ W)
+        assert np.allclose((tu / wh[0] + 1) * w / 2, x + 0.5, atol=2e-4 * w)
+        assert np.allclose((tv / wh[1] + 1) * h / 2, y + 0.5, atol=2e-4 * h)
+        F = P + d0 * (f / (d0 @ W))[:, None]
+        origins = []
+        for sample in range(4):
+            ol, dl, _ = oracle.camera_rays(lens, sample, 5, flags=PIXEL_CENTRE)
+            ol, dl = ol.astype(np.float64), dl.astype(np.float64)
+            miss = np.linalg.norm(np.cross(F - ol, dl), axis=1)
+            assert (miss <= 2e-6 * np.linalg.norm(F - ol, axis=1)).all()
+            origins.append(ol)
+        assert not np.array_equal(origins[0], origins[1])  # the lens sample moves
