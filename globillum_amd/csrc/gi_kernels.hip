@@ -126,6 +126,7 @@ struct ShadeParams {
   float4 *rdiff;           // ray differentials per path slot, or null (no texture in use):
   uint32_t rdiff_stride;   // planes [0..3] = upper origin, upper dir, right origin, right dir
   const float4 *ks;        // per material: specular colour (GI_FLAG_PHONG renders)
+  const float *lsum;       // running sums of the light powers: lsum[k] = power of lights 0..k-1
 };
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -984,14 +985,16 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
 
   // NEE: power-proportional light choice, uniform point on the rect
   const f3 xl = f3{halton(n, dim + 0, P.H), halton(n, dim + 1, P.H), halton(n, dim + 2, P.H)};
+  // the first light k with choice <= lsum[k + 1], the last one if none:
+  // what the linear scan over running sums picks, by bisection (the sums
+  // only grow)
   const float choice = xl.z * P.power_total;
-  float acc = 0.0f;
-  int k = 0;
-  while (k < P.nlights && choice > acc + P.lights[k].power) {
-    acc += P.lights[k].power;
-    ++k;
+  int k = 0, hi = P.nlights - 1;
+  while (k < hi) {
+    const int mid = (k + hi) >> 1;
+    if (choice > P.lsum[mid + 1]) k = mid + 1;
+    else hi = mid;
   }
-  if (k >= P.nlights) k = P.nlights - 1;
   ve.light = k;
   const DevLight &lt = P.lights[k];
   f3 sd, contrib;
@@ -1658,6 +1661,7 @@ struct gi_ctx {
   float clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};  // bounds of the triangle centroids
   DBuf d_stris, d_itris, d_nodes, d_mats, d_lights;
   DBuf d_ks;  // specular colour per material (float4), for GI_FLAG_PHONG
+  DBuf d_lsum;  // running sums of the light powers (n_lights + 1 floats)
   // Halton tables
   uint32_t hal_seed = 0;
   int32_t hal_dims = 0;
@@ -2194,6 +2198,7 @@ ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
   P.rdiff = ctx->textured ? ctx->rdiff.as<float4>() : nullptr;
   P.rdiff_stride = (uint32_t)ctx->slot_cap;
   P.ks = ctx->d_ks.as<float4>();
+  P.lsum = ctx->d_lsum.as<float>();
   return P;
 }
 
@@ -2237,7 +2242,7 @@ std::vector<uint8_t> launch_key(gi_ctx *ctx, const PassPlan &plan, int32_t depth
                         (const void *)P.H.chunks, (const void *)P.sky.rgb, (const void *)P.sky.ycdf,
                         (const void *)P.sky.ypdf, (const void *)P.sky.xcdf, (const void *)P.sky.xpdf,
                         (const void *)P.is.inst, (const void *)P.is.vbase, (const void *)P.is.vtab,
-                        (const void *)P.tex, (const void *)P.rdiff, (const void *)P.ks})
+                        (const void *)P.tex, (const void *)P.rdiff, (const void *)P.ks, (const void *)P.lsum})
     key_put(k, q);
   key_put(k, ctx->opts.flags & GI_FLAG_PHONG);  // picks the shading kernel
   key_put(k, P.nlights);
@@ -2803,12 +2808,21 @@ int gi_upload_lights(gi_ctx *ctx, const gi_light *lights, int32_t n) {
     host_tangential(ey, l.dir, d.up);
     total += l.power;
   }
+  // running sums of the powers in light order, each the float sum the
+  // linear selection (pixel_generate_arealight_sample_cpu,
+  // src/arealight-sampler-cpu.cpp:33-39) forms: light k is picked when
+  // choice <= sums[k + 1] for the first time
+  std::vector<float> sums(n + 1);
+  sums[0] = 0.0f;
+  for (int32_t i = 0; i < n; ++i) sums[i + 1] = sums[i] + lights[i].power;
   ctx->lights.assign(lights, lights + n);
   ctx->power_total = total;
   if (sky != ctx->sky_light) ctx->sky_w = ctx->sky_h = 0;  // a new sky light needs its map
   ctx->sky_light = sky;
   GI_HIP(ctx, ctx->d_lights.ensure(n * sizeof(DevLight)));
   GI_HIP(ctx, copy_sync(ctx, ctx->d_lights.p, dl.data(), n * sizeof(DevLight), hipMemcpyHostToDevice));
+  GI_HIP(ctx, ctx->d_lsum.ensure((size_t)(n + 1) * sizeof(float)));
+  GI_HIP(ctx, copy_sync(ctx, ctx->d_lsum.p, sums.data(), (size_t)(n + 1) * sizeof(float), hipMemcpyHostToDevice));
   return GI_OK;
 }
 

@@ -398,3 +398,19 @@ def test_pixel_centre_generator_bit_exact(aperture, oracle, gpu_ctx_factory):
     om, oe = oracle.adjoint(osc, sc.views, opts, dL, f64=False)
     assert np.allclose(gm, om, rtol=2e-3, atol=2e-4 * np.abs(om).max())
     assert np.allclose(ge, oe, rtol=2e-3, atol=2e-4 * np.abs(oe).max())
+
+
+def test_many_lights_selection_bit_exact(oracle, gpu_ctx_factory):
+    """37 lights of uneven power (light selection by bisection over the
+    running power sums must pick what the oracle's linear scan picks)."""
+    sc = scenes.terrain_spheres(cells=(32, 32), n_spheres=60, n_lights=37, w=96, h=64, spp=8, depth=3)
+    rng = np.random.default_rng(12)
+    sc.lights["col"] = rng.uniform(0.5, 30.0, (37, 3)).astype(np.float32)
+    sc.lights["col"][5] = sc.lights["col"][6]  # equal powers next to each other
+    sc.lights["power"] = [scenes.rect_light_power(l["wh"][0], l["wh"][1], l["col"]) for l in sc.lights]
+    ref, st = _oracle(oracle, sc)
+    with gpu_ctx_factory(sc) as ctx:
+        img = ctx.render(flags=FLAG_BATCHED).image(0).reshape(-1, 3)
+        c = ctx.counters()
+    _same(img, ref)
+    assert c["rays_shadow"] == st["rays_shadow"]
