@@ -714,8 +714,8 @@ def adjoint_leg(steps: int):
 
 def bvh_leg(steps: int):
     """BVH construction: the host binned-SAH builder against the device
-    linear-BVH and PLOC (radius 16) builders on the ~300k-triangle scene and
-    on a 2048x1024-cell heightfield (4.19M triangles, the large
+    linear-BVH, PLOC (radius 16) and binned-SAH builders on the ~300k-triangle
+    scene and on a 2048x1024-cell heightfield (4.19M triangles, the large
     configuration's terrain), plus the ~300k-triangle frame time traversing
     each tree."""
     import numpy as np
@@ -731,7 +731,7 @@ def bvh_leg(steps: int):
         ctx.upload(sc)  # host SAH (includes the upload of the tables)
         host_ms = (time.perf_counter() - t0) * 1e3
         frames, dev_ms = {}, {}
-        for name, method in (("sah", None), ("lbvh", "device"), ("ploc", "ploc")):
+        for name, method in (("sah", None), ("lbvh", "device"), ("ploc", "ploc"), ("device_sah", "sah")):
             if method is not None:
                 ctx.build_bvh_device(4, method)  # warm (allocations, code objects)
                 t0 = time.perf_counter()
@@ -744,8 +744,10 @@ def bvh_leg(steps: int):
             frames[name] = (time.perf_counter() - t0) / steps * 1e3
     out["terrain_spheres_291k"] = {"host_sah_ms_incl_upload": round(host_ms, 1),
                                    "device_lbvh_ms": round(dev_ms["lbvh"], 2), "device_ploc_ms": round(dev_ms["ploc"], 2),
+                                   "device_sah_ms": round(dev_ms["device_sah"], 2),
                                    "frame_ms_sah": round(frames["sah"], 2), "frame_ms_lbvh": round(frames["lbvh"], 2),
-                                   "frame_ms_ploc": round(frames["ploc"], 2)}
+                                   "frame_ms_ploc": round(frames["ploc"], 2),
+                                   "frame_ms_device_sah": round(frames["device_sah"], 2)}
     rng = np.random.default_rng(5)
     hf, _ = scenes.heightfield(rng, 2048, 1024, 200.0, 100.0, 10.0, 8)
     big = scenes.Scene(hf, sc.mats, sc.lights, sc.views, sc.opts)
@@ -754,13 +756,14 @@ def bvh_leg(steps: int):
         ctx.upload(big)
         host_ms = (time.perf_counter() - t0) * 1e3
         dev_ms = {}
-        for name, method in (("lbvh", "device"), ("ploc", "ploc")):
+        for name, method in (("lbvh", "device"), ("ploc", "ploc"), ("device_sah", "sah")):
             ctx.build_bvh_device(4, method)
             t0 = time.perf_counter()
             ctx.build_bvh_device(4, method)
             dev_ms[name] = (time.perf_counter() - t0) * 1e3
     out["heightfield_4194k"] = {"host_sah_ms_incl_upload": round(host_ms, 1), "device_lbvh_ms": round(dev_ms["lbvh"], 2),
-                                "device_ploc_ms": round(dev_ms["ploc"], 2)}
+                                "device_ploc_ms": round(dev_ms["ploc"], 2),
+                                "device_sah_ms": round(dev_ms["device_sah"], 2)}
     log(f"bvh leg: {out}")
     return out
 

@@ -11,8 +11,8 @@ from . import _lib
 from ._lib import (COUNTERS, HIT, INSTANCE, LIGHT, MATERIAL, OPTS, TRAVERSAL_STATS, TRI, TUNING, VIEW, GiError,
                    ptr)
 
-# device builders of gi_build_bvh_device_ex (GI_BVH_LBVH, GI_BVH_PLOC)
-_DEVICE_METHODS = {"device": 0, "ploc": 1}
+# device builders of gi_build_bvh_device_ex (GI_BVH_LBVH, GI_BVH_PLOC, GI_BVH_SAH)
+_DEVICE_METHODS = {"device": 0, "ploc": 1, "sah": 2}
 
 
 def _check(lib, ctx, rc):
@@ -59,8 +59,9 @@ class Context:
         """Triangles, materials and lights of `scene`, and its BVH: the
         host binned-SAH builder (builder="host"), the device LBVH builder
         (builder="device"), the device PLOC builder with the given search
-        radius (builder="ploc"), or an external tree in the reference's
-        2 x float4 layout (bvh2=(nodes, indices))."""
+        radius (builder="ploc"), the device binned-SAH builder
+        (builder="sah"), or an external tree in the reference's 2 x float4
+        layout (bvh2=(nodes, indices))."""
         tris = np.ascontiguousarray(scene.tris, TRI)
         self.generation += 1
         self._ck(self.lib.gi_upload_triangles(self.h, ptr(tris), len(tris)))
@@ -80,7 +81,7 @@ class Context:
             self._ck(self.lib.gi_build_bvh_device_ex(self.h, max_leaf, _DEVICE_METHODS[builder], radius))
         elif bvh2 is None:
             if builder != "host":
-                raise ValueError(f"builder must be 'host', 'device' or 'ploc', not {builder!r}")
+                raise ValueError(f"builder must be 'host', 'device', 'ploc' or 'sah', not {builder!r}")
             self._ck(self.lib.gi_build_bvh(self.h, max_leaf))
         else:
             nodes, idx = bvh2
@@ -96,10 +97,11 @@ class Context:
 
     def build_bvh_device(self, max_leaf: int = 4, method: str = "device", radius: int = 16):
         """Rebuild the BVH of the uploaded triangles on the device, as a
-        linear BVH (method="device") or by PLOC clustering (method="ploc").
-        A new world tree drops the instances; the scene's are placed again."""
+        linear BVH (method="device"), by PLOC clustering (method="ploc") or
+        by the host builder's binned SAH (method="sah").  A new world tree
+        drops the instances; the scene's are placed again."""
         if method not in _DEVICE_METHODS:
-            raise ValueError(f"method must be 'device' or 'ploc', not {method!r}")
+            raise ValueError(f"method must be 'device', 'ploc' or 'sah', not {method!r}")
         self._ck(self.lib.gi_build_bvh_device_ex(self.h, max_leaf, _DEVICE_METHODS[method], radius))
         sc = self.scene
         if sc is not None and getattr(sc, "instances", None) is not None and len(sc.instances):

@@ -57,8 +57,8 @@ void pad_box(float *lo, float *hi) {
 
 namespace {
 
-constexpr int kBins = 32;
-constexpr int kSahDepth = 48;  // beyond this depth, split at the object median
+constexpr int kBins = kSahBins;
+constexpr int kSahDepth = kSahMaxDepth;  // beyond this depth, split at the object median
 // Subtrees of at least this many primitives, down to kParallelDepth, build
 // their right half on another thread.
 constexpr uint32_t kParallelMin = 1u << 16;
@@ -365,18 +365,12 @@ void leaf_order_tris(const gi_bvh &b, const gi_tri *tris, std::vector<IsectTri> 
 // program of Ylitie, Karras and Laine 2017, "Efficient incoherent ray
 // traversal on GPUs through compressed wide BVHs", for width 4).  C[n][i] is
 // the least cost of binary subtree n when it may take up to i slots of its
-// parent: one slot holds n as a wide node (area(n) * kNodeCost plus the best
+// parent: one slot holds n as a wide node (area(n) * kCollapseNodeCost plus the best
 // split of `width` (2..4) slots between its children) or, when its triangles are
 // contiguous and at most merge_limit, as one leaf (area(n) * count); more
 // slots open n and split them between its children.  Costs are in triangle
 // tests; a wide-node visit (four box tests, one 128-byte fetch) counts as
-// kNodeCost of them.
-#ifndef GI_COLLAPSE_NODE_COST
-#define GI_COLLAPSE_NODE_COST 1.0
-#endif
-namespace {
-constexpr double kNodeCost = GI_COLLAPSE_NODE_COST;
-}  // namespace
+// kCollapseNodeCost of them.
 
 void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh2Node &)> &leaf_ref,
                          uint32_t node_base, std::vector<WideNode> &wnodes, int32_t &wide_depth,
@@ -449,7 +443,7 @@ void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh
           D[j] = cl[k] + cr[j - k];
           split[5 * (size_t)n + j] = (int8_t)k;
         }
-    const double node_cost = area * kNodeCost + D[W];
+    const double node_cost = area * kCollapseNodeCost + D[W];
     const double leaf_cost =
         contiguous[n] && cnt[n] <= merge_limit && cnt[n] <= kMaxLeafTris ? area * cnt[n] : kInf;
     as_leaf[n] = leaf_cost <= node_cost;

@@ -1,5 +1,5 @@
-// Device BVH build (bvh_gpu.hip): a binary tree (linear BVH + refit, or
-// PLOC clustering) collapsed into 4-wide nodes.
+// Device BVH build (bvh_gpu.hip): a binary tree (linear BVH + refit, PLOC
+// clustering or binned SAH) collapsed into 4-wide nodes.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -16,15 +16,16 @@ struct LbvhResult {
   int32_t rounds = 0;   // PLOC clustering rounds (0 for the radix tree)
 };
 
-enum class DeviceBvhMethod { kLbvh = 0, kPloc = 1 };
+enum class DeviceBvhMethod { kLbvh = 0, kPloc = 1, kSah = 2 };
 
 // Builds the 4-wide BVH of n triangles whose vertices a, b, c are float4
 // 0, 1, 2 of records `stride` float4 apart (the ShadeTri table), over the
 // centroid bounds clo..chi.  Writes nodes[0..nnodes) (capacity >= n) and
 // the 3n float4 of the leaf-ordered intersection triangles.  kLbvh: the
 // radix tree over the Morton keys; kPloc: clustering with the given search
-// radius (1..64).  width: child slots filled per wide node (2..4).  Synchronises the stream; scratch memory is allocated and
-// freed inside.
+// radius (1..64); kSah: the host builder's binned SAH (clo, chi and radius
+// unused).  width: child slots filled per wide node (2..4).  Synchronises
+// the stream; scratch memory is allocated and freed inside.
 // The 3n float4 of the leaf-ordered intersection triangles (a with the
 // original index in w, b - a, c - a) from the ShadeTri-strided vertex table
 // and a leaf-order permutation already on the device.  Asynchronous.

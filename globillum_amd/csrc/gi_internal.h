@@ -91,6 +91,16 @@ void collapse_wide(const gi_bvh &b, const gi_tri *tris, std::vector<WideNode> &n
 #define GI_COLLAPSE_MERGE 4
 #endif
 constexpr int32_t kCollapseMergeLimit = GI_COLLAPSE_MERGE;
+// Cost of a wide-node visit (four box tests, one 128-byte fetch) in
+// triangle tests, for the collapse's surface-area cost.
+#ifndef GI_COLLAPSE_NODE_COST
+#define GI_COLLAPSE_NODE_COST 1.0
+#endif
+constexpr double kCollapseNodeCost = GI_COLLAPSE_NODE_COST;
+// Binned SAH (host builder and its device counterpart): centroid bins per
+// axis, and the depth from which nodes split at the object median.
+constexpr int kSahBins = 32;
+constexpr int kSahMaxDepth = 48;
 void leaf_order_tris(const gi_bvh &b, const gi_tri *tris, std::vector<IsectTri> &leaf_tris);
 void collapse_wide_nodes(const gi_bvh &b, const std::function<uint32_t(const Bvh2Node &)> &leaf_ref,
                          uint32_t node_base, std::vector<WideNode> &nodes, int32_t &wide_depth,

@@ -2888,7 +2888,8 @@ int gi_build_bvh_device_ex(gi_ctx *ctx, int32_t max_leaf, int32_t method, int32_
   if (!ctx || ctx->tris.empty()) return fail(ctx, GI_ERR_STATE, "upload triangles first");
   const int ml = max_leaf > 0 ? max_leaf : 4;
   if (ml > kMaxLeafTris) return fail(ctx, GI_ERR_INVALID_ARG, "max_leaf must be at most 15");
-  if (method != GI_BVH_LBVH && method != GI_BVH_PLOC) return fail(ctx, GI_ERR_INVALID_ARG, "unknown BVH method");
+  if (method != GI_BVH_LBVH && method != GI_BVH_PLOC && method != GI_BVH_SAH)
+    return fail(ctx, GI_ERR_INVALID_ARG, "unknown BVH method");
   const int rad = radius == 0 ? 16 : radius;
   if (method == GI_BVH_PLOC && (rad < 1 || rad > 64)) return fail(ctx, GI_ERR_INVALID_ARG, "PLOC radius must be in 1..64");
   GI_HIP(ctx, hipSetDevice(ctx->device));
@@ -2900,7 +2901,10 @@ int gi_build_bvh_device_ex(gi_ctx *ctx, int32_t max_leaf, int32_t method, int32_
   LbvhResult r;
   std::string err;
   if (!device_bvh_build(ctx->stream, ctx->d_stris.as<float4>(), (uint32_t)(sizeof(ShadeTri) / sizeof(float4)), n,
-                        ctx->clo, ctx->chi, ml, method == GI_BVH_PLOC ? DeviceBvhMethod::kPloc : DeviceBvhMethod::kLbvh,
+                        ctx->clo, ctx->chi, ml,
+                        method == GI_BVH_PLOC  ? DeviceBvhMethod::kPloc
+                        : method == GI_BVH_SAH ? DeviceBvhMethod::kSah
+                                               : DeviceBvhMethod::kLbvh,
                         rad, ctx->tuning.node_width, ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), r, err))
     return fail(ctx, GI_ERR_HIP, err);
   return finish_bvh(ctx, r.nnodes, r.depth);

@@ -257,22 +257,24 @@ int gi_upload_sky(gi_ctx *ctx, const float *rgb, int32_t w, int32_t h);
 int gi_build_bvh(gi_ctx *ctx, int32_t max_leaf);
 /* The same tree format built on the device: a linear BVH over 30-bit
  * Morton codes of the triangle centroids (the reference's GPU LBVH,
- * src/main.cpp:218-225), refit level by level, collapsed greedily into
- * 4-wide nodes with leaves of at most max_leaf (<= 15) triangles.  Boxes are
- * padded as the host builder pads them, so closest-hit results are the same
- * with either tree. */
+ * src/main.cpp:218-225), refit level by level, collapsed into 4-wide nodes
+ * by the host builder's surface-area-cost dynamic program, with leaves of
+ * at most max_leaf (<= 15) triangles.  Boxes are padded as the host builder
+ * pads them, so closest-hit results are the same with either tree. */
 int gi_build_bvh_device(gi_ctx *ctx, int32_t max_leaf);
 #define GI_BVH_LBVH 0 /* radix tree over the Morton keys (gi_build_bvh_device) */
 #define GI_BVH_PLOC 1 /* locally-ordered clustering over the Morton order      */
+#define GI_BVH_SAH 2  /* binned SAH, the host builder's splits (gi_build_bvh)  */
 /* gi_build_bvh_device with a choice of binary tree before the collapse.
  * GI_BVH_PLOC merges mutual nearest neighbours (smallest union surface
  * area) within `radius` (1..64, 0 = 16) positions of the Morton order,
- * round by round (Meister and Bittner 2018); its trees trace faster than
- * the radix tree's (about 1.25x the host SAH tree's frame time, against
- * 1.5x) at device build speed.  Same node format, padding and
- * closest-hit results as the other builders.  The reference builds its
- * GPU trees with rta's LBVH (src/main.cpp:218-225); this method has no
- * counterpart there. */
+ * round by round (Meister and Bittner 2018).  GI_BVH_SAH takes the host
+ * builder's binned-SAH split decisions (32 centroid bins per axis, SAH leaf
+ * test, object median from depth 48) level by level on the device: the
+ * host tree's shape at device build speed; radius is unused.  Same node
+ * format, padding and closest-hit results as the other builders.  The
+ * reference builds its GPU trees with rta's LBVH (src/main.cpp:218-225);
+ * PLOC and device SAH have no counterpart there. */
 int gi_build_bvh_device_ex(gi_ctx *ctx, int32_t max_leaf, int32_t method, int32_t radius);
 /* Upload an externally built binary BVH in the reference's 2 x float4 layout
  * (src/tracers.cu:26-36): nodes[8*n_nodes], indices[n_indices]. */

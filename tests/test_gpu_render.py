@@ -252,7 +252,7 @@ def test_obj_scene_with_map_kd_texture_bit_exact(oracle, gpu_ctx_factory, tmp_pa
         assert np.abs(ctx.render().image(0).reshape(-1, 3) - img).max() > 0.01
 
 
-@pytest.mark.parametrize("builder", ["device", "ploc"])
+@pytest.mark.parametrize("builder", ["device", "ploc", "sah"])
 def test_device_bvh_render_bit_exact(builder, oracle, gpu_ctx_factory):
     sc = scenes.terrain_spheres(cells=(64, 64), n_spheres=100, w=80, h=45, spp=4)
     ref, st = _oracle(oracle, sc)

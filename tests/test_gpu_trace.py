@@ -139,7 +139,8 @@ def test_call_order_and_argument_errors(c1, gpu_ctx_factory):
 
 @pytest.mark.parametrize("builder,max_leaf,radius", [("device", 1, 16), ("device", 4, 16), ("device", 15, 16),
                                                      ("ploc", 1, 16), ("ploc", 4, 16), ("ploc", 15, 16),
-                                                     ("ploc", 4, 1), ("ploc", 4, 64)])
+                                                     ("ploc", 4, 1), ("ploc", 4, 64),
+                                                     ("sah", 1, 0), ("sah", 4, 0), ("sah", 15, 0)])
 def test_device_bvh_closest_and_any(builder, max_leaf, radius, terrain, oracle, gpu_ctx_factory):
     sc = terrain
     rng = np.random.default_rng(5 + max_leaf)
@@ -151,6 +152,28 @@ def test_device_bvh_closest_and_any(builder, max_leaf, radius, terrain, oracle, 
     with gpu_ctx_factory(sc, builder=builder, max_leaf=max_leaf, radius=radius) as ctx:
         _assert_hits_equal(ctx.trace_closest(o, d, tmax), ref, osc, o, d, tmax)
         assert np.array_equal(ctx.trace_any(o, d, tmax), occ)
+
+
+def test_device_sah_is_the_host_tree(terrain, gpu_ctx_factory):
+    """The device binned-SAH builder takes the host builder's split decisions,
+    so its tree makes a closest-hit traversal visit the same nodes and test
+    the same triangles (the counts of a primitive call on a fixed ray array
+    repeat exactly; the collapse's costs are float on the device, double on
+    the host, so a near tie may still go the other way).  The radix tree
+    and clustering trees differ from it by tens of percent."""
+    rng = np.random.default_rng(11)
+    o, d = _rays(rng, 100_000, -5, 105, 0, 20)
+    t = np.full(len(o), FMAX, np.float32)
+    counts = {}
+    for builder in ("host", "sah", "ploc"):
+        with gpu_ctx_factory(terrain, builder=builder) as ctx:
+            ctx.traversal_stats(True)
+            ctx.trace_closest(o, d, t)
+            s = ctx.traversal_stats()
+        counts[builder] = np.array([s["closest_nodes"], s["closest_triangles"]], np.float64)
+    print({k: v.tolist() for k, v in counts.items()})
+    assert np.all(np.abs(counts["sah"] / counts["host"] - 1) <= 2e-3)
+    assert np.any(np.abs(counts["ploc"] / counts["host"] - 1) > 0.05)
 
 
 def test_ploc_argument_errors(c1, gpu_ctx_factory):
@@ -166,7 +189,7 @@ def test_ploc_argument_errors(c1, gpu_ctx_factory):
             ctx.build_bvh_device(4, "sah")
 
 
-@pytest.mark.parametrize("builder", ["device", "ploc"])
+@pytest.mark.parametrize("builder", ["device", "ploc", "sah"])
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 17])
 def test_device_bvh_tiny_and_degenerate(n, builder, oracle, gpu_ctx_factory):
     rng = np.random.default_rng(n)
