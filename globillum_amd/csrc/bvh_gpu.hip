@@ -678,6 +678,7 @@ struct SahArgs {
   float4 *bb, *cb;    // node box and centroid box (lo, hi), unpadded
   float *nodebox;     // padded node box, 8 floats per node
   SahDec *dec;
+  uint32_t *leaf_perm;  // triangle index at each position, written when its node becomes a leaf
   int max_leaf;
 };
 
@@ -1031,9 +1032,11 @@ __global__ void __launch_bounds__(64) k_sah_wave(SahArgs S, int buf, uint32_t co
 // merged into the node's global bins (slot = its rank among them).
 __global__ void __launch_bounds__(kBlock) k_sah_bin_chunks(SahArgs S, int buf, uint32_t count,
                                                          const uint64_t *__restrict__ pre,
+                                                         const uint32_t *__restrict__ tot,
                                                          uint32_t *__restrict__ gbins, uint32_t *__restrict__ big_pos) {
   __shared__ uint32_t sb[kSahBinWords];
   const uint32_t chunk = blockIdx.x;
+  if (chunk >= tot[0]) return;  // the grid covers an upper bound
   // the last list position whose first chunk is at or before this one
   uint32_t lo = 0, hi = count;
   while (hi - lo > 1) {
@@ -1070,15 +1073,29 @@ __global__ void __launch_bounds__(kBlock) k_sah_bin_chunks(SahArgs S, int buf, u
   if (c == 0 && threadIdx.x == 0) big_pos[slot] = j;
 }
 
-__global__ void k_sah_bins_identity(uint32_t *__restrict__ gbins, size_t words) {
+// tot = {chunks, nodes of several chunks} of the level, from the scan of want.
+__global__ void k_sah_totals(const uint64_t *__restrict__ pre, const uint64_t *__restrict__ want, uint32_t count,
+                             uint32_t *__restrict__ tot) {
+  if (threadIdx.x == 0) {
+    const uint64_t t = pre[count - 1] + want[count - 1];
+    tot[0] = (uint32_t)t;
+    tot[1] = (uint32_t)(t >> 32);
+  }
+}
+
+// Global bins of the level's several-chunk nodes (tot[1] of them) to identity.
+__global__ void k_sah_bins_identity(uint32_t *__restrict__ gbins, const uint32_t *__restrict__ tot) {
+  const size_t words = (size_t)tot[1] * kSahBinWords;
   for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < words; w += (size_t)gridDim.x * blockDim.x)
     gbins[w] = ((w % kSahFields) < 3 || ((w % kSahFields) >= 6 && (w % kSahFields) < 9)) ? 0xFFFFFFFFu : 0u;
 }
 
 // Decisions of the several-chunk nodes from their merged bins (one wave each).
-__global__ void __launch_bounds__(64) k_sah_big(SahArgs S, const uint32_t *__restrict__ gbins,
+__global__ void __launch_bounds__(64) k_sah_big(SahArgs S, const uint32_t *__restrict__ tot,
+                                              const uint32_t *__restrict__ gbins,
                                               const uint32_t *__restrict__ big_pos) {
   __shared__ uint32_t sb[kSahBinWords];
+  if (blockIdx.x >= tot[1]) return;  // the grid covers an upper bound
   const uint32_t j = big_pos[blockIdx.x];
   const uint32_t *g = gbins + (size_t)blockIdx.x * kSahBinWords;
   for (int w = threadIdx.x; w < kSahBinWords; w += blockDim.x) sb[w] = g[w];
@@ -1155,10 +1172,12 @@ __global__ void k_sah_flags(const SahArgs S, int buf, uint32_t n, uint32_t *__re
   flags[i] = f;
 }
 
-// Every triangle into the other buffer: split nodes' triangles to their
-// child's range (a stable partition by the scan of flags), the others in
-// place; median nodes' triangles get their sort keys (centroid on the
-// axis, -0 as +0, then triangle index).
+// Every triangle of the level into the other buffer: split nodes'
+// triangles to their child's range (a stable partition by the scan of
+// flags); a new leaf's triangles stay where they are for good (leaf_perm),
+// and later levels only carry their "in a leaf" mark; median nodes'
+// triangles get their sort keys (centroid on the axis, -0 as +0, then
+// triangle index) for k_sah_median, which moves them.
 __global__ void k_sah_scatter(SahArgs S, int buf, uint32_t n, const uint32_t *__restrict__ flags,
                               const uint32_t *__restrict__ scan, uint64_t *__restrict__ keys,
                               uint32_t *__restrict__ vals) {
@@ -1166,30 +1185,40 @@ __global__ void k_sah_scatter(SahArgs S, int buf, uint32_t n, const uint32_t *__
   if (i >= n) return;
   const int out = buf ^ 1;
   const int32_t j = S.pos[buf][i];
-  uint32_t dst = i;
-  int32_t pn = -1;
-  if (j >= 0) {
-    const SahDec &d = S.dec[j];
-    if (d.kind == kSahSplit) {
-      const uint32_t b = S.nb[d.node], lb = scan[b], li = scan[i] - lb;
-      if (flags[i]) {
-        dst = b + li;
-        pn = (int32_t)d.child;
-      } else {
-        dst = b + d.nleft + (i - b) - li;
-        pn = (int32_t)d.child + 1;
-      }
-    } else if (d.kind == kSahMedian) {
-      const float c = 0.5f * (comp(S.box[buf][2 * (size_t)i], d.axis) + comp(S.box[buf][2 * (size_t)i + 1], d.axis));
-      keys[i] = ((uint64_t)okey(c + 0.0f) << 32) | S.perm[buf][i];
-      vals[i] = i;
-      pn = (int32_t)d.child;  // replaced after the sort
-    }
+  if (j < 0) {
+    S.pos[out][i] = -1;
+    return;
   }
-  S.box[out][2 * (size_t)dst] = S.box[buf][2 * (size_t)i];
-  S.box[out][2 * (size_t)dst + 1] = S.box[buf][2 * (size_t)i + 1];
-  S.perm[out][dst] = S.perm[buf][i];
-  S.pos[out][dst] = pn;
+  const SahDec &d = S.dec[j];
+  if (d.kind == kSahSplit) {
+    const uint32_t b = S.nb[d.node], lb = scan[b], li = scan[i] - lb;
+    uint32_t dst;
+    int32_t pn;
+    if (flags[i]) {
+      dst = b + li;
+      pn = (int32_t)d.child;
+    } else {
+      dst = b + d.nleft + (i - b) - li;
+      pn = (int32_t)d.child + 1;
+    }
+    S.box[out][2 * (size_t)dst] = S.box[buf][2 * (size_t)i];
+    S.box[out][2 * (size_t)dst + 1] = S.box[buf][2 * (size_t)i + 1];
+    S.perm[out][dst] = S.perm[buf][i];
+    S.pos[out][dst] = pn;
+  } else if (d.kind == kSahMedian) {
+    const float c = 0.5f * (comp(S.box[buf][2 * (size_t)i], d.axis) + comp(S.box[buf][2 * (size_t)i + 1], d.axis));
+    keys[i] = ((uint64_t)okey(c + 0.0f) << 32) | S.perm[buf][i];
+    vals[i] = i;
+  } else {  // a leaf
+    S.leaf_perm[i] = S.perm[buf][i];
+    S.pos[out][i] = -1;
+  }
+}
+
+// The triangles of the last level's nodes (all leaves) into leaf_perm.
+__global__ void k_sah_final(SahArgs S, int buf, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && S.pos[buf][i] >= 0) S.leaf_perm[i] = S.perm[buf][i];
 }
 
 // Median nodes (one workgroup each): their triangles in sorted order, the
@@ -1325,6 +1354,7 @@ static bool device_sah_build(hipStream_t s, const float4 *tri, uint32_t stride, 
   S.cb = (float4 *)alloc(32 * nn);
   S.nodebox = (float *)alloc(32 * nn);
   S.dec = (SahDec *)alloc(sizeof(SahDec) * lcap);
+  S.leaf_perm = (uint32_t *)alloc(4 * (size_t)n);
   S.max_leaf = max_leaf;
   int32_t *lvl = (int32_t *)alloc(4 * nn);  // every level's list, one after another
   uint64_t *want = (uint64_t *)alloc(8 * lcap), *pre = (uint64_t *)alloc(8 * lcap);
@@ -1339,6 +1369,7 @@ static bool device_sah_build(hipStream_t s, const float4 *tri, uint32_t stride, 
   uint32_t *meta = (uint32_t *)alloc(4 * nn);
   int32_t *front[2] = {(int32_t *)alloc(4 * (size_t)n), (int32_t *)alloc(4 * (size_t)n)};
   uint32_t *cnt = (uint32_t *)alloc(64);  // the collapse's counter; first the root's 12 box keys
+  uint32_t *tot = (uint32_t *)alloc(16);  // a level's chunk and several-chunk node counts
   for (void *p : mem)
     if (!p) {
       err = "device bvh: out of device memory";
@@ -1383,25 +1414,19 @@ static bool device_sah_build(hipStream_t s, const float4 *tri, uint32_t stride, 
     hipLaunchKernelGGL(k_sah_plan, dim3(blocks(count)), dim3(kBlock), 0, s, S, list, count, depth, want);
     GI_CK(hipGetLastError());
     GI_CK(hipcub::DeviceScan::ExclusiveSum(scan_temp, scan64_bytes, want, pre, (int)count, s));
-    uint64_t tail[2] = {0, 0};
-    GI_CK(hipMemcpyAsync(&tail[0], pre + count - 1, 8, hipMemcpyDeviceToHost, s));
-    GI_CK(hipMemcpyAsync(&tail[1], want + count - 1, 8, hipMemcpyDeviceToHost, s));
-    GI_CK(hipStreamSynchronize(s));
-    const uint64_t tot = tail[0] + tail[1];
-    const uint32_t chunks = (uint32_t)tot, big = (uint32_t)(tot >> 32);
-    if (big > max_big) {
-      err = "device bvh: SAH bin slots exceeded";
-      for (void *q : mem) (void)hipFree(q);
-      return false;
+    // the chunk and several-chunk node counts stay on the device: the
+    // launches cover their upper bounds (a several-chunk node has more than
+    // kSahChunk triangles, and at most one partial chunk)
+    hipLaunchKernelGGL(k_sah_totals, dim3(1), dim3(64), 0, s, pre, want, count, tot);
+    const size_t big_bound = std::min<size_t>(max_big, count), chunk_bound = (size_t)n / kSahChunk + big_bound;
+    if (n > kSahChunk) {
+      hipLaunchKernelGGL(k_sah_bins_identity,
+                         dim3((uint32_t)std::min<size_t>(blocks(big_bound * kSahBinWords), 4096)), dim3(kBlock), 0, s,
+                         gbins, tot);
+      hipLaunchKernelGGL(k_sah_bin_chunks, dim3((uint32_t)chunk_bound), dim3(kBlock), 0, s, S, buf, count, pre, tot,
+                         gbins, big_pos);
+      hipLaunchKernelGGL(k_sah_big, dim3((uint32_t)big_bound), dim3(64), 0, s, S, tot, gbins, big_pos);
     }
-    if (big) {
-      const size_t words = (size_t)big * kSahBinWords;
-      hipLaunchKernelGGL(k_sah_bins_identity, dim3((uint32_t)std::min<size_t>(blocks(words), 4096)), dim3(kBlock), 0,
-                         s, gbins, words);
-    }
-    if (chunks)
-      hipLaunchKernelGGL(k_sah_bin_chunks, dim3(chunks), dim3(kBlock), 0, s, S, buf, count, pre, gbins, big_pos);
-    if (big) hipLaunchKernelGGL(k_sah_big, dim3(big), dim3(64), 0, s, S, gbins, big_pos);
     hipLaunchKernelGGL(k_sah_wave, dim3(count), dim3(64), 0, s, S, buf, count);
     hipLaunchKernelGGL(k_sah_flags, dim3(blocks(n)), dim3(kBlock), 0, s, S, buf, n, flags);
     GI_CK(hipGetLastError());
@@ -1409,6 +1434,7 @@ static bool device_sah_build(hipStream_t s, const float4 *tri, uint32_t stride, 
     hipLaunchKernelGGL(k_sah_count, dim3(blocks(count)), dim3(kBlock), 0, s, S, count, want);
     GI_CK(hipGetLastError());
     GI_CK(hipcub::DeviceScan::ExclusiveSum(scan_temp, scan64_bytes, want, pre, (int)count, s));
+    uint64_t tail[2] = {0, 0};
     GI_CK(hipMemcpyAsync(&tail[0], pre + count - 1, 8, hipMemcpyDeviceToHost, s));
     GI_CK(hipMemcpyAsync(&tail[1], want + count - 1, 8, hipMemcpyDeviceToHost, s));
     GI_CK(hipStreamSynchronize(s));
@@ -1459,8 +1485,9 @@ static bool device_sah_build(hipStream_t s, const float4 *tri, uint32_t stride, 
     nodes_made += 2 * splits;
     off.push_back(off.back() + 2 * splits);
   }
-  // leaf order: the final buffer's triangle order
-  hipLaunchKernelGGL(k_itris, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, S.perm[buf], itris);
+  // leaf order
+  hipLaunchKernelGGL(k_sah_final, dim3(blocks(n)), dim3(kBlock), 0, s, S, buf, n);
+  hipLaunchKernelGGL(k_itris, dim3(blocks(n)), dim3(kBlock), 0, s, tri, stride, n, S.leaf_perm, itris);
   hipLaunchKernelGGL(k_sah_last, dim3(blocks(nodes_made)), dim3(kBlock), 0, s, S.ne, nodes_made);
   GI_CK(hipGetLastError());
   const CollapseArgs A{S.left, S.right, S.nb, S.ne, S.nodebox, S.nodebox, cost, meta, kCollapseMergeLimit, W};
