@@ -183,10 +183,10 @@ def test_ploc_argument_errors(c1, gpu_ctx_factory):
                 ctx.build_bvh_device(4, method, radius)
             assert e.value.status == 1
         with pytest.raises(GiError) as e:
-            ctx._ck(ctx.lib.gi_build_bvh_device_ex(ctx.h, 4, 2, 16))
+            ctx._ck(ctx.lib.gi_build_bvh_device_ex(ctx.h, 4, 3, 16))  # 0..2 are the builders
         assert e.value.status == 1
         with pytest.raises(ValueError):
-            ctx.build_bvh_device(4, "sah")
+            ctx.build_bvh_device(4, "host")
 
 
 @pytest.mark.parametrize("builder", ["device", "ploc", "sah"])
