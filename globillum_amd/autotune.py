@@ -30,10 +30,10 @@ SPACE = {
     "node_width": [2, 3, 4],
     "refill_lanes": [8, 16, 32, 64],
     "shade_block": [64, 128, 256],
-    "target_slots": [16, 32, 64],
+    "target_slots": [32, 64, 128],
 }
 DEFAULT = {"trace_block": 256, "trace_blocks_per_cu": 5, "stack_lds": 16, "node_width": 4, "refill_lanes": 32,
-           "shade_block": 256, "target_slots": 64}
+           "shade_block": 256, "target_slots": 128}
 
 
 def _time(ctx, views, opts, repeats: int) -> float:

@@ -203,6 +203,14 @@ constexpr uint32_t kNoRay = 0xffffffffu;
 // << 7): at most 2^25 wide nodes.
 constexpr uint64_t kMaxWideNodes = (uint64_t)1 << 25;
 constexpr float kSlabSlack = 1.0f + 0x1p-21f;
+// The interior-node loop hands over to the leaf tests once at most this
+// many lanes of the wave are still looking for their first leaf (and some
+// lane holds one): the last searchers would otherwise keep the whole wave
+// in the loop at a few active lanes while the others' parked leaves wait.
+#ifndef GI_SPEC_LANES
+#define GI_SPEC_LANES 12
+#endif
+constexpr uint32_t kSpecLanes = GI_SPEC_LANES;
 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
@@ -327,7 +335,8 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
     // ---- interior nodes.  A lane that reaches a leaf parks it (one slot)
     // and goes on with its stack, so it keeps doing interior work while the
     // other lanes look for their leaves; the loop ends once every lane with
-    // interior work left has a parked leaf (or none has interior work).
+    // interior work left has a parked leaf (or none has interior work), or
+    // once at most kSpecLanes lanes still look for one.
     for (;;) {
       if (!(ref & kLeafBit)) {
         if constexpr (kCount) ++c_nodes;
@@ -414,7 +423,13 @@ if (dist[j] < dist[i]) {                                  \
         // return marker changes the ray's space, so the lane holds still
         ref = (kInst && ref == kReturnRef) ? kDoneRef : (st.sp > 0 ? st.take() : kDoneRef);
       }
-      if (__ballot(!(ref & kLeafBit) && parked == kDoneRef) == 0) break;
+      // lanes still looking for a leaf: the loop runs while some do, until
+      // few are left and some lane has a parked leaf to test (the searchers
+      // go on from their stacks after the leaf tests)
+      const uint64_t searching = __ballot(!(ref & kLeafBit) && parked == kDoneRef);
+      if (searching == 0 ||
+          ((uint32_t)__popcll(searching) <= kSpecLanes && __ballot(parked != kDoneRef) != 0))
+        break;
     }
     // ---- the parked leaf, instance or return marker
     if (kInst && special_ref(parked)) {
@@ -2612,9 +2627,10 @@ int setup_views(gi_ctx *ctx, const gi_view *views, int32_t n_views, const gi_ren
 
 size_t target_slots(gi_ctx *ctx) {
   const int t = ctx->tuning.target_slots;
-  // default: 64 Mi paths in flight (10.7 GB of path state), so a
-  // 1280x720x64 render is one pass and every launch is as wide as it gets
-  return (size_t)(t > 0 ? std::min(t, 64) : 64) << 20;
+  // default: 128 Mi paths in flight (22 GB of path state, 30 GB with ray
+  // differentials), so a 1280x720x64 render or a 3840x2160x16 view is one
+  // pass and every launch is as wide as it gets
+  return (size_t)(t > 0 ? std::min(t, 256) : 128) << 20;
 }
 
 }  // namespace
@@ -3531,8 +3547,8 @@ int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t) {
     return fail(ctx, GI_ERR_INVALID_ARG, "block sizes must be multiples of 64, at most 256");
   if (n.stack_lds != 8 && n.stack_lds != 16 && n.stack_lds != 32)
     return fail(ctx, GI_ERR_INVALID_ARG, "stack_lds must be 8, 16 or 32");
-  if (n.target_slots < 0 || n.target_slots > 64 || n.trace_blocks_per_cu > 64 || n.samples_per_pass < 0)
-    return fail(ctx, GI_ERR_INVALID_ARG, "target_slots in [0,64], trace_blocks_per_cu <= 64");
+  if (n.target_slots < 0 || n.target_slots > 256 || n.trace_blocks_per_cu > 64 || n.samples_per_pass < 0)
+    return fail(ctx, GI_ERR_INVALID_ARG, "target_slots in [0,256], trace_blocks_per_cu <= 64");
   if (n.node_width == 0) n.node_width = 4;
   if (n.refill_lanes == 0) n.refill_lanes = 32;
   if (n.node_width < 2 || n.node_width > 4) return fail(ctx, GI_ERR_INVALID_ARG, "node_width must be 2, 3 or 4");

@@ -195,7 +195,8 @@ typedef struct {
   int32_t shade_block;
   int32_t samples_per_pass;
   int32_t stack_lds;        /* traversal stack entries per lane in LDS: 8, 16 or 32 */
-  int32_t target_slots;     /* paths in flight per pass, in units of 2^20; 0 = 64 */
+  int32_t target_slots;     /* paths in flight per pass, in units of 2^20: 0..256,
+                               0 = 128 (about 30 GB of path state with textures) */
   int32_t node_width;       /* child slots the collapse fills per BVH node: 2, 3 or 4
                                (0 = 4).  A build parameter: a host-built world tree
                                is collapsed again at once (refused while instances
