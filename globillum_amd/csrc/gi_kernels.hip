@@ -599,7 +599,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WA
             l.z = l.z + c.z;
             path_l[slot] = l;
             if (rec) {
-              float4 &r = rec[(uint32_t)b * rec_stride + slot];
+              float4 &r = rec[(size_t)b * rec_stride + slot];
               r = make_float4(c.x, c.y, c.z, r.w);
             }
           }
@@ -741,7 +741,7 @@ __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pas
     tp[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
     path_l[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (rec)
-      for (int32_t b = 0; b < depth; ++b) rec[(uint32_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int32_t b = 0; b < depth; ++b) rec[(size_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (rdiff) {
       // ray differentials: pinhole rays from the camera position through the
       // centres of pixels (x, y+1) ("upper") and (x+1, y) ("right"); rta's
@@ -1096,6 +1096,9 @@ __device__ __forceinline__ void shade_vertices(
       const uint32_t n = fbits(ro.w), slot = fbits(rd.w), id = fbits(h.w);
       const f3 TP = xyz(tpv);
       const f3 org_dir = normalize(xyz(rd));
+      // the adjoint's record also keeps the vertex's hit (its second half),
+      // so that the replay need not trace the same ray again
+      if (rec) rec[(size_t)(P.depth + b) * rec_stride + slot] = h;
       if (id == GI_INVALID_REF) {
         // handle_no_hit with NO_BACKGROUND (src/hybrid-pt.cpp:196-206,567-574)
         if (TP.x == 1.0f && TP.y == 1.0f && TP.z == 1.0f) {
@@ -1105,7 +1108,7 @@ __device__ __forceinline__ void shade_vertices(
           l.y = l.y + by;
           l.z = l.z + bz;
           path_l[slot] = l;
-          if (rec) rec[(uint32_t)b * rec_stride + slot] = make_float4(bx, by, bz, bitsf(kRecBackground));
+          if (rec) rec[(size_t)b * rec_stride + slot] = make_float4(bx, by, bz, bitsf(kRecBackground));
         } else if (P.sky.light >= 0) {
           const f3 Ls = sky_eval(P.sky, org_dir);
           float4 l = path_l[slot];
@@ -1153,7 +1156,7 @@ __device__ __forceinline__ void shade_vertices(
             tp_next = make_float4(t.x, t.y, t.z, 0.0f);
           }
         }
-        if (rec) rec[(uint32_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, bitsf(clamp));
+        if (rec) rec[(size_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, bitsf(clamp));
       }
     }
     uint32_t qn, qs;
@@ -1389,8 +1392,7 @@ template <bool kTex>
 __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t *__restrict__ count_ptr,
                                                        ShadeParams P, AdjParams A, const float4 *__restrict__ ray_o,
                                                        const float4 *__restrict__ ray_d,
-                                                       const float4 *__restrict__ tp_in,
-                                                       const float4 *__restrict__ hits, float4 *__restrict__ nro,
+                                                       const float4 *__restrict__ tp_in, float4 *__restrict__ nro,
                                                        float4 *__restrict__ nrd, float4 *__restrict__ ntp,
                                                        uint32_t *__restrict__ next_count) {
   extern __shared__ float sg[];  // 5*nmats + 3*nlights
@@ -1409,8 +1411,12 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
     float *tex_a[4] = {nullptr, nullptr, nullptr, nullptr};
     float tex_w[4] = {0.f, 0.f, 0.f, 0.f}, tex_g[3] = {0.f, 0.f, 0.f};
     if (i < count) {
-      const float4 ro = ray_o[i], rd = ray_d[i], tpv = tp_in[i], h = hits[i];
-      const uint32_t n = fbits(ro.w), slot = fbits(rd.w), id = fbits(h.w);
+      const float4 ro = ray_o[i], rd = ray_d[i], tpv = tp_in[i];
+      const uint32_t n = fbits(ro.w), slot = fbits(rd.w);
+      // the hit the forward pass recorded for this path's vertex b (the
+      // replay's rays are the forward pass's, bit for bit)
+      const float4 h = A.rec[(size_t)(P.depth + b) * A.rec_stride + slot];
+      const uint32_t id = fbits(h.w);
       const f3 TP = xyz(tpv);
       const f3 org_dir = normalize(xyz(rd));
       if (id != GI_INVALID_REF) {
@@ -1438,12 +1444,12 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
         const f3 delta = f3{w[0] * inv_spp, w[1] * inv_spp, w[2] * inv_spp};
         // recorded contributions from this vertex on: S (all) and R (after
         // this vertex, up to the next throughput clamp per channel)
-        const float4 r0 = A.rec[(uint32_t)b * A.rec_stride + slot];
+        const float4 r0 = A.rec[(size_t)b * A.rec_stride + slot];
         f3 S = xyz(r0);
         f3 R = f3{0.f, 0.f, 0.f};
         uint32_t open = 7u & ~fbits(r0.w);  // channels whose chain continues past b
         for (int bb = b + 1; bb < P.depth; ++bb) {
-          const float4 r = A.rec[(uint32_t)bb * A.rec_stride + slot];
+          const float4 r = A.rec[(size_t)bb * A.rec_stride + slot];
           S = add(S, xyz(r));
           const uint32_t fl = fbits(r.w);
           if (!(fl & kRecBackground)) {
@@ -2466,7 +2472,8 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
   int rc = ensure_workspace(ctx, plan.max_slots);
   if (rc) return rc;
   if ((rc = ensure_gstack(ctx))) return rc;
-  GI_HIP(ctx, ctx->d_rec.ensure((size_t)depth * plan.max_slots * sizeof(float4)));
+  // per path and vertex: the radiance record, then the hit
+  GI_HIP(ctx, ctx->d_rec.ensure((size_t)2 * depth * plan.max_slots * sizeof(float4)));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_overflow.as<uint32_t>(), 1u, 0u);
   GI_HIP(ctx, hipGetLastError());
   ShadeParams SP = shade_params(ctx, depth);
@@ -2496,10 +2503,11 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
                          SP.rdiff, SP.rdiff_stride);
       for (int32_t b = 0; b < depth; ++b) {
         const int cur = b & 1, nxt = cur ^ 1;
-        rc = launch_closest(ctx, false, cnt + b, 0, pd.total_slots, ctx->q_o[cur].as<float4>(),
-                            ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
-        if (rc) return rc;
         if (phase == 0) {
+          // the replay (phase 1) reads these hits from the record
+          rc = launch_closest(ctx, false, cnt + b, 0, pd.total_slots, ctx->q_o[cur].as<float4>(),
+                              ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
+          if (rc) return rc;
           hipLaunchKernelGGL(SP.rdiff ? k_shade_tex : k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b,
                              cnt + b, SP,
                              ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
@@ -2516,7 +2524,7 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
           hipLaunchKernelGGL(SP.rdiff ? k_shade_adjoint<true> : k_shade_adjoint<false>, dim3(sg), dim3(sblock),
                              lds_adj, ctx->stream, b, cnt + b, SP, A,
                              ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
-                             ctx->q_hit.as<float4>(), ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(),
+                             ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(),
                              ctx->q_tp[nxt].as<float4>(), cnt + b + 1);
           GI_HIP(ctx, hipGetLastError());
         }
@@ -2526,7 +2534,7 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
   GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<uint32_t> hc(2 * npass * ncnt);
   GI_HIP(ctx, copy_sync(ctx, hc.data(), ctx->d_counters.p, hc.size() * 4, hipMemcpyDeviceToHost));
-  for (size_t k = 0; k < 2 * npass; ++k)
+  for (size_t k = 0; k < 2 * npass; k += 2)  // the forward (recording) phases trace; the replays do not
     for (int32_t b = 0; b < depth; ++b) {
       ctx->counters.rays_closest += hc[k * ncnt + b];
       ctx->counters.rays_shadow += hc[k * ncnt + depth + 1 + b];
@@ -3606,7 +3614,11 @@ int gi_adjoint(gi_ctx *ctx, const float *dL_dimage, gi_material_grad *d_mats, fl
   GI_HIP(ctx, hipMemsetAsync(ctx->d_gem.p, 0, (size_t)3 * nl * sizeof(float), ctx->stream));
   const int32_t spp_pp = ctx->opts.samples_per_pass > 0 ? ctx->opts.samples_per_pass : ctx->tuning.samples_per_pass;
   PassPlan plan;
-  plan_passes(ctx->views, sb, se, true, target_slots(ctx), spp_pp, plan);
+  // the record holds 2 float4 per path and vertex: passes of at most
+  // 32 GiB of record (128 Mi paths up to depth 8)
+  const size_t rec_cap = ((size_t)32 << 30) / ((size_t)2 * ctx->opts.max_depth * sizeof(float4));
+  plan_passes(ctx->views, sb, se, true, std::min(target_slots(ctx), std::max<size_t>(rec_cap, (size_t)1 << 20)),
+              spp_pp, plan);
   AdjParams A;
   std::memset(&A, 0, sizeof(A));
   A.dL = ctx->d_dl.as<float>();
