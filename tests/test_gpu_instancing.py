@@ -2,8 +2,8 @@
 gi_upload_instances): closest and any hits against the oracle's
 instanced traversal (ids, t and barycentrics bit-exact, any mismatch
 adjudicated by the FP64 brute force), closed-form box hits, renders
-bit-exact against the oracle with both BVH builders and the batched and
-graph schedules, the adjoint against the oracle's, and the error paths."""
+bit-exact against the oracle with the host and device BVH builders and the
+batched and graph schedules, the adjoint against the oracle's, and the error paths."""
 import dataclasses
 
 import numpy as np
@@ -45,7 +45,7 @@ def _rays(rng, n, size):
     return o, d
 
 
-@pytest.mark.parametrize("builder", ["host", "device", "ploc"])
+@pytest.mark.parametrize("builder", ["host", "device", "ploc", "sah"])
 def test_closest_and_any_parity(builder, inst, oracle, gpu_ctx_factory):
     rng = np.random.default_rng(21)
     o, d = _rays(rng, 200_000, 40.0)
@@ -93,7 +93,8 @@ def test_closed_form_box_hits(oracle, gpu_ctx_factory):
         assert ctx.trace_any(o, d, tmax).tolist() == [1, 1, 1, 0, 0]
 
 
-@pytest.mark.parametrize("builder,flags", [("host", 0), ("device", FLAG_BATCHED), ("host", FLAG_BATCHED | FLAG_GRAPH)])
+@pytest.mark.parametrize("builder,flags", [("host", 0), ("device", FLAG_BATCHED), ("host", FLAG_BATCHED | FLAG_GRAPH),
+                                           ("sah", FLAG_BATCHED)])
 def test_render_bit_exact(builder, flags, inst, oracle, gpu_ctx_factory):
     ref, st = _osc(oracle, inst).render(inst.views, inst.opts)
     with gpu_ctx_factory(inst, builder=builder) as ctx:
