@@ -154,6 +154,37 @@ def test_device_bvh_closest_and_any(builder, max_leaf, radius, terrain, oracle, 
         assert np.array_equal(ctx.trace_any(o, d, tmax), occ)
 
 
+@pytest.mark.parametrize("builder", ["sah", "ploc", "device"])
+def test_device_bvh_many_equal_centroids(builder, oracle, gpu_ctx_factory):
+    """6000 coplanar triangles scaled about one centroid: no axis has a
+    centroid extent, so every binned-SAH node above the leaves is an
+    object-median split (the root's segment spans several binning chunks),
+    and rays hit many triangles at the same t (ties go to the lower id)."""
+    rng = np.random.default_rng(3)
+    sc = scenes.cornell_boxes(w=8, h=8, spp=1)
+    n = 6000
+    tris = np.zeros(n, sc.tris.dtype)
+    tris[:] = sc.tris[0]
+    c = np.array([0.5, 0.5, 0.5], np.float32)
+    base = np.array([[-0.3, -0.2, 0.0], [0.35, -0.25, 0.0], [-0.05, 0.45, 0.0]], np.float32)
+    s_ = rng.uniform(0.05, 1.0, n).astype(np.float32)
+    for k, name in enumerate("abc"):
+        tris[name] = c + s_[:, None] * base[k]
+    o = np.stack([rng.uniform(0.1, 0.9, 20_000), rng.uniform(0.1, 0.9, 20_000), np.full(20_000, -1.0)],
+                 1).astype(np.float32)
+    d = np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (len(o), 1))
+    d[:, :2] = rng.normal(0, 0.05, (len(o), 2))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    t = np.full(len(o), FMAX, np.float32)
+    osc = oracle.OracleScene(tris, sc.mats, sc.lights)
+    ref = osc.trace_closest(o, d, t)
+    assert (ref["ref"] != 0xFFFFFFFF).mean() > 0.3
+    one = scenes.Scene(tris, sc.mats, sc.lights, sc.views, sc.opts)
+    with gpu_ctx_factory(one, builder=builder) as ctx:
+        _assert_hits_equal(ctx.trace_closest(o, d, t), ref, osc, o, d, t)
+        assert np.array_equal(ctx.trace_any(o, d, t), osc.trace_any(o, d, t))
+
+
 def test_device_sah_is_the_host_tree(terrain, gpu_ctx_factory):
     """The device binned-SAH builder takes the host builder's split decisions,
     so its tree makes a closest-hit traversal visit the same nodes and test
