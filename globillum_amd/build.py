@@ -17,6 +17,8 @@ LIB = os.path.join(PKG, "libgi_hip.so")
 SOURCES = ["gi_kernels.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene_io.cpp", "image_io.cpp"]
 # the memory-hierarchy read benchmark used by bench.py (not part of the path)
 MEMBENCH = os.path.join(PKG, "libgi_membench.so")
+# exhaustive check of the traversal's reciprocal against the IEEE division
+RCP_CHECK = os.path.join(PKG, "gi_rcp_check")
 
 # -ffp-contract=off: hit ids, sample directions and image values are
 # reproduced bit-for-bit on the host, so no a*b+c may be fused.
@@ -63,8 +65,8 @@ def build_host(verbose: bool = False) -> None:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every source into libgi_hip.so and the host binaries (always
-    recompiles)."""
+    """Compile every source into libgi_hip.so, the reciprocal check and the
+    host binaries (always recompiles)."""
     tmp = LIB + ".tmp"
     cmd = [hipcc(), *HIP_FLAGS, "-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lz"]
     if verbose:
@@ -83,6 +85,13 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libgi_membench.so")
     os.replace(tmp, MEMBENCH)
+    tmp = RCP_CHECK + ".tmp"
+    r = subprocess.run([hipcc(), *[f for f in HIP_FLAGS if f != "-fPIC"], "-o", tmp,
+                        os.path.join(CSRC, "rcp_check.hip")], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building gi_rcp_check")
+    os.replace(tmp, RCP_CHECK)
     build_host(verbose)
     return LIB
 

@@ -188,6 +188,22 @@ __device__ __forceinline__ f3 tangential(f3 v, f3 n) {
 // ---------------------------------------------------------- intersect ----
 // Moller-Trumbore 1997, no culling: hit iff det != 0, u,v >= 0, u+v <= 1,
 // t > 0.  e1 = b - a and e2 = c - a come precomputed (same float rounding).
+// Correctly rounded 1/x.  For |x| in [2^-126, 2^126): the hardware
+// reciprocal (within 1 ulp) refined by one Newton-Raphson step with fused
+// multiply-adds (Markstein, IBM J. Res. Dev. 34(1), 1990): 4 instructions
+// instead of the 11 of the IEEE division sequence.  rcp_check.hip compares
+// it bit for bit with the IEEE division over all 2^32 floats.  Elsewhere
+// (zero, subnormals, |x| >= 2^126, infinities, NaN) the IEEE division
+// itself.
+__device__ __forceinline__ float rcp_rn(float x) {
+  const float a = fabsf(x);
+  if (__builtin_expect(a >= 0x1p-126f && a < 0x1p126f, 1)) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+  }
+  return 1.0f / x;
+}
+
 __device__ __forceinline__ bool intersect_tri(f3 a, f3 e1, f3 e2, f3 o, f3 d, float &t, float &u,
                                               float &v) {
   // evaluated without early exits (one wave runs every lane's test anyway);
@@ -197,7 +213,7 @@ __device__ __forceinline__ bool intersect_tri(f3 a, f3 e1, f3 e2, f3 o, f3 d, fl
   const float det = dot(e1, pvec);
   const f3 tvec = sub(o, a);
   const f3 qvec = cross(tvec, e1);
-  const float inv_det = 1.0f / det;
+  const float inv_det = rcp_rn(det);  // = 1.0f / det, bit for bit
   u = dot(tvec, pvec) * inv_det;
   v = dot(d, qvec) * inv_det;
   t = dot(e2, qvec) * inv_det;

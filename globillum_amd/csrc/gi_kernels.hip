@@ -466,25 +466,26 @@ if (dist[j] < dist[i]) {                                  \
       near = near_planes(idir);
       parked = kDoneRef;
     } else if (parked != kDoneRef) {
-      // triangle j+1 is loaded while triangle j is tested
+      // two triangle buffers, refilled alternately: triangle j+2 is loaded
+      // into j's registers once j is tested, so no register copies
       const uint32_t cnt = (parked >> 27) & 15u;
       const auto tp = [&]() {
         if constexpr (kLdsScene) return ltris + 3 * (parked & kMaxLeafFirst);
         else return itris + 3 * (parked & kMaxLeafFirst);
       }();
-      float4 na = make_float4(0.f, 0.f, 0.f, 0.f), n1 = na, n2 = na;
+      float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), b0 = a0, c0 = a0, a1 = a0, b1 = a0, c1 = a0;
       if (cnt > 0) {
-        na = tp[0];
-        n1 = tp[1];
-        n2 = tp[2];
+        a0 = tp[0];
+        b0 = tp[1];
+        c0 = tp[2];
       }
-      for (uint32_t j = 0; j < cnt; ++j) {
-        const float4 ta = na, t1 = n1, t2 = n2;
-        if (j + 1 < cnt) {
-          na = tp[3 * j + 3];
-          n1 = tp[3 * j + 4];
-          n2 = tp[3 * j + 5];
-        }
+      if (cnt > 1) {
+        a1 = tp[3];
+        b1 = tp[4];
+        c1 = tp[5];
+      }
+      // tests one triangle; true: an any-hit ray is done
+      const auto test = [&](const float4 &ta, const float4 &t1, const float4 &t2) -> bool {
         float t, u, v;
         if constexpr (kCount) ++c_tris;
         if (intersect_tri(xyz(ta), xyz(t1), xyz(t2), o, d, t, u, v) && t <= tmax) {
@@ -492,11 +493,27 @@ if (dist[j] < dist[i]) {                                  \
           if (kAny) {
             bt = t; bu = u; bv = v; bid = id;
             ref = kDoneRef;  // any hit ends the ray
-            break;
+            return true;
           }
           if (t < bt || (t == bt && id < bid)) {
             bt = t; bu = u; bv = v; bid = id;
           }
+        }
+        return false;
+      };
+      for (uint32_t j = 0; j < cnt; j += 2) {
+        if (test(a0, b0, c0)) break;
+        if (j + 2 < cnt) {
+          a0 = tp[3 * j + 6];
+          b0 = tp[3 * j + 7];
+          c0 = tp[3 * j + 8];
+        }
+        if (j + 1 >= cnt) break;
+        if (test(a1, b1, c1)) break;
+        if (j + 3 < cnt) {
+          a1 = tp[3 * j + 9];
+          b1 = tp[3 * j + 10];
+          c1 = tp[3 * j + 11];
         }
       }
       parked = kDoneRef;

@@ -241,3 +241,20 @@ def test_device_bvh_tiny_and_degenerate(n, builder, oracle, gpu_ctx_factory):
     one = scenes.Scene(tris, sc.mats, sc.lights, sc.views, sc.opts)
     with gpu_ctx_factory(one, builder=builder) as ctx:
         _assert_hits_equal(ctx.trace_closest(o, d, t), ref, osc, o, d, t)
+
+
+def test_triangle_reciprocal_exhaustive():
+    """The triangle test's reciprocal (hardware rcp + one Newton step) equals
+    the IEEE division 1/x bit for bit for all 2^32 floats, so t, u and v keep
+    the oracle's bits (globillum_amd/csrc/rcp_check.hip)."""
+    import os
+    import subprocess
+
+    import globillum_amd
+
+    exe = os.path.join(os.path.dirname(globillum_amd.__file__), "gi_rcp_check")
+    assert os.path.exists(exe), "gi_rcp_check not built (python -m globillum_amd.build)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "total mismatches 0" in r.stdout
