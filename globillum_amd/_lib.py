@@ -104,6 +104,7 @@ SIGNATURES = {
     "gi_build_bvh_device_ex": (_I, [_P, _I32, _I32, _I32]),
     "gi_set_traversal_stats": (_I, [_P, _I32]),
     "gi_get_traversal_stats": (_I, [_P, _P]),
+    "gi_check_half_nodes": (_I, [_P, C.POINTER(_I64)]),
     "gi_upload_mesh": (_I, [_P, _P, _I64, C.POINTER(C.c_int32)]),
     "gi_upload_texture": (_I, [_P, _P, _I32, _I32, C.POINTER(C.c_int32)]),
     "gi_texture_info": (_I, [_P, _I32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),

@@ -245,6 +245,13 @@ class Context:
         self._ck(self.lib.gi_get_counters(self.h, ptr(c)))
         return {k: c[k][0].item() for k in COUNTERS.names}
 
+    def check_half_nodes(self) -> int:
+        """Child slots of the current tree whose half-precision traversal box
+        fails to enclose the f32 box widened by its margins (0 expected)."""
+        n = C.c_int64(0)
+        self._ck(self.lib.gi_check_half_nodes(self.h, C.byref(n)))
+        return n.value
+
     def traversal_stats(self, enable: bool | None = None) -> dict | None:
         """enable=True zeroes the traversal statistics and counts every
         later trace; enable=False stops counting; with no argument returns

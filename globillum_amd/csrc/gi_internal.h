@@ -32,6 +32,23 @@ struct alignas(128) WideNode {
   uint32_t pad[4];
 };
 static_assert(sizeof(WideNode) == 128, "wide node must be one cache line");
+
+// The traversal's copy of a WideNode, with the child planes in half
+// precision: five 16-byte loads per visit instead of seven.
+//   org, scale: the node box's min corner (f32) and a power of two s;
+//   q[axis]: the four children's lo and hi planes on that axis as f16
+//     offsets from org in units of s, packed lo0|lo1, lo2|lo3, hi0|hi1,
+//     hi2|hi3 (low half first), rounded outward past a margin; empty slots
+//     +inf (their box stays [+inf, +inf]);
+//   child: the WideNode's child references.
+// 80 bytes used of one 128-byte line, so nodes keep the WideNode indexing.
+struct alignas(128) HalfNode {
+  float org[3], scale;
+  uint32_t q[3][4];
+  uint32_t child[4];
+  uint32_t pad[12];
+};
+static_assert(sizeof(HalfNode) == sizeof(WideNode), "half node keeps the wide node's stride");
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmptyRef = 0x80000000u;  // a leaf with zero triangles
 constexpr int kMaxLeafTris = 15;

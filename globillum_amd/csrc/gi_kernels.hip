@@ -2,6 +2,7 @@
 //
 // Data layout in HBM (all arrays 16-byte aligned, structure of arrays):
 //   WideNode   nodes[]      128 B per 4-wide node (one cache line)
+//   HalfNode   hnodes[]     the traversal's copy: f16 child planes, 80 of 128 B
 //   IsectTri   itris[]      48 B per triangle, BVH leaf order
 //   ShadeTri   stris[]      128 B per triangle, original order (shading)
 //   ray queue  ray_o[], ray_d[], tp[]  float4 each: (o, halton index),
@@ -183,13 +184,19 @@ __device__ __forceinline__ void block_append2(bool p0, uint32_t *c0, bool p1, ui
 // per level of the tree; an overflow (impossible by construction) is
 // counted and turns into GI_ERR_LIMIT on the host.
 //
-// Box test: slab distances fma(bound, 1/d, -o/d), entry <= exit * (1 +
-// 2^-21).  The relative slack covers the rounding of the reciprocal, of
-// o/d and of the fma relative to the distance; the boxes' padding (2^-20
-// of their largest coordinate magnitude) covers the part proportional to
-// the box's own coordinates.  Box culling is therefore conservative and
-// the result is the closest hit = minimum (t, id) over all triangles with
-// 0 < t <= max_t; any-hit stops at the first such triangle.
+// Box test on the half-precision nodes (HalfNode): per node and axis
+// b = fma(org, 1/d, -o/d) and s' = scale/d (exact: a power of two), then
+// each child plane's distance is fma(q, s', b) with the f16 offset q
+// (v_fma_mix_f32: one rounding); entry <= exit * (1 + 2^-21).  The relative
+// slack covers the rounding of the reciprocal, of o/d and of the fmas
+// relative to the distance.  The parts proportional to coordinates are
+// covered by padding: the f32 boxes are padded by 2^-20 of their largest
+// coordinate magnitude, and the f16 planes are rounded outward past a
+// further 2^-18 of the node's coordinate magnitude, which covers the
+// rounding of b (2^-24 of |org - o| / |d|).  Box culling is therefore
+// conservative and the result is the closest hit = minimum (t, id) over
+// all triangles with 0 < t <= max_t; any-hit stops at the first such
+// triangle.
 // Occupancy target of the traversal kernels (waves per SIMD): 6 caps them
 // at 80 VGPRs, which they fit without spilling (the default launch, 5 blocks
 // of 4 waves per CU, runs 5 per SIMD).  At 7 (72 VGPRs) the two-level
@@ -236,15 +243,41 @@ struct Stack {
   }
 };
 
-// Byte offsets, within a wide node, of a ray's near plane on x, y, z
-// (bits 0-7, 8-15, 16-23): lo (lox 0, loy 32, loz 64) where the inverse
-// direction is positive, hi (16 higher) where it is negative; the far plane
-// is the other one (offset ^ 16).  The sign bit decides: rcp_clamped never
-// returns zero, and -0 directions give negative inverses.
-__device__ __forceinline__ uint32_t near_planes(f3 idir) {
-  const uint32_t sx = __float_as_uint(idir.x) >> 31, sy = __float_as_uint(idir.y) >> 31,
-                 sz = __float_as_uint(idir.z) >> 31;
-  return (sx << 4) | ((32u | (sy << 4)) << 8) | ((64u | (sz << 4)) << 16);
+// Sign bits of the inverse direction (bit 0 x, 1 y, 2 z): on an axis with
+// a positive inverse the near plane is lo and the far plane hi, with a
+// negative one the other way round (fma rounding is monotone in the plane,
+// so these are exactly the min and max of the two plane distances).
+// rcp_clamped never returns zero, and -0 directions give negative inverses.
+__device__ __forceinline__ uint32_t dir_signs(f3 idir) {
+  return (__float_as_uint(idir.x) >> 31) | ((__float_as_uint(idir.y) >> 31) << 1) |
+         ((__float_as_uint(idir.z) >> 31) << 2);
+}
+
+// v_fma_mix_f32: the f16 in the low (mix_lo) or high (mix_hi) half of h2,
+// converted exactly, times s plus b, rounded once in f32.
+__device__ __forceinline__ float mix_lo(uint32_t h2, float s, float b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h2), "v"(s), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float mix_hi(uint32_t h2, float s, float b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h2), "v"(s), "v"(b));
+  return r;
+}
+
+// v_max3_f32 / v_min3_f32 on plane distances (never NaN: an empty slot's
+// +inf offset times a finite nonzero s' gives an infinity), without the
+// quieting the compiler adds to values it did not produce itself.
+__device__ __forceinline__ float max3_raw(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+__device__ __forceinline__ float min3_raw(float a, float b, float c) {
+  float r;
+  asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
 }
 
 // LDS column stride of a block: the next power of two >= blockDim.
@@ -283,7 +316,7 @@ __device__ __forceinline__ bool special_ref(uint32_t ref) {
 // kLdsScene: the block's LDS holds a copy of the nodes (8 float4 each) at
 // lnodes and of the triangles at ltris; they are read from there.
 template <bool kAny, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene, class Start, class Finish>
-__device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, const WideNode *__restrict__ nodes,
+__device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, const HalfNode *__restrict__ nodes,
                                                  const float4 *__restrict__ itris, const float4 *lnodes,
                                                  const float4 *ltris, Stack<kLdsDepth> &st, uint32_t *overflow,
                                                  InstScene IS, unsigned long long *stats, Start start,
@@ -297,7 +330,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
   uint32_t chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // wave-uniform: next chunk
   uint32_t ray = kNoRay, ref = kDoneRef, parked = kDoneRef;
   f3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f}, idir{0.f, 0.f, 0.f}, oid{0.f, 0.f, 0.f};
-  uint32_t near = 0;  // the ray's near planes (near_planes)
+  uint32_t neg = 0;  // dir_signs of the ray
   float tmax = 0.f, bt = FLT_MAX, bu = 0.f, bv = 0.f;
   uint32_t bid = GI_INVALID_REF;
   uint32_t vbase = 0;  // virtual id base of the current space (0: world)
@@ -316,7 +349,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
           start(ray, o, d, tmax);
           idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
           oid = f3{o.x * idir.x, o.y * idir.y, o.z * idir.z};
-          near = near_planes(idir);
+          neg = dir_signs(idir);
           bt = FLT_MAX;
           bu = bv = 0.0f;
           bid = GI_INVALID_REF;
@@ -340,62 +373,69 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
     for (;;) {
       if (!(ref & kLeafBit)) {
         if constexpr (kCount) ++c_nodes;
-        // only the near and the far plane of each axis: which is which
-        // follows from the ray's direction signs, so each load's offset
-        // within the 128-byte node picks it (fma rounding is monotone in
-        // the plane, so these are exactly the min and max of the two
-        // plane distances)
-        float4 nxv, fxv, nyv, fyv, nzv, fzv;
-        uint4 ch;
+        // origin and scale, the f16 planes of each axis, the references
+        float4 org;
+        uint4 qx, qy, qz, ch;
         const uint32_t base = ref << 7;
-        const uint32_t onx = base | (near & 0xffu), ony = base | ((near >> 8) & 0xffu),
-                       onz = base | (near >> 16);
         if constexpr (kLdsScene) {
           const char *np = reinterpret_cast<const char *>(lnodes);  // in LDS: derived from the shared base
-          nxv = *reinterpret_cast<const float4 *>(np + onx);
-          fxv = *reinterpret_cast<const float4 *>(np + (onx ^ 16u));
-          nyv = *reinterpret_cast<const float4 *>(np + ony);
-          fyv = *reinterpret_cast<const float4 *>(np + (ony ^ 16u));
-          nzv = *reinterpret_cast<const float4 *>(np + onz);
-          fzv = *reinterpret_cast<const float4 *>(np + (onz ^ 16u));
-          const float4 c = *reinterpret_cast<const float4 *>(np + (base | 96u));
-          ch = make_uint4(__float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.z), __float_as_uint(c.w));
+          const auto u4 = [](float4 f) {
+            return make_uint4(__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z), __float_as_uint(f.w));
+          };
+          org = *reinterpret_cast<const float4 *>(np + base);
+          qx = u4(*reinterpret_cast<const float4 *>(np + (base | 16u)));
+          qy = u4(*reinterpret_cast<const float4 *>(np + (base | 32u)));
+          qz = u4(*reinterpret_cast<const float4 *>(np + (base | 48u)));
+          ch = u4(*reinterpret_cast<const float4 *>(np + (base | 64u)));
         } else {
           const char *np = reinterpret_cast<const char *>(nodes);
-          nxv = *reinterpret_cast<const float4 *>(np + onx);
-          fxv = *reinterpret_cast<const float4 *>(np + (onx ^ 16u));
-          nyv = *reinterpret_cast<const float4 *>(np + ony);
-          fyv = *reinterpret_cast<const float4 *>(np + (ony ^ 16u));
-          nzv = *reinterpret_cast<const float4 *>(np + onz);
-          fzv = *reinterpret_cast<const float4 *>(np + (onz ^ 16u));
-          ch = *reinterpret_cast<const uint4 *>(np + (base | 96u));
+          org = *reinterpret_cast<const float4 *>(np + base);
+          qx = *reinterpret_cast<const uint4 *>(np + (base | 16u));
+          qy = *reinterpret_cast<const uint4 *>(np + (base | 32u));
+          qz = *reinterpret_cast<const uint4 *>(np + (base | 48u));
+          ch = *reinterpret_cast<const uint4 *>(np + (base | 64u));
         }
         const float lim = kAny ? tmax : smin(bt, tmax);
         float dist[4];
         uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
-        const float nx[4] = {nxv.x, nxv.y, nxv.z, nxv.w}, fx[4] = {fxv.x, fxv.y, fxv.z, fxv.w};
-        const float ny[4] = {nyv.x, nyv.y, nyv.z, nyv.w}, fy[4] = {fyv.x, fyv.y, fyv.z, fyv.w};
-        const float nz[4] = {nzv.x, nzv.y, nzv.z, nzv.w}, fz[4] = {fzv.x, fzv.y, fzv.z, fzv.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const float tn = fmaxf(fmaxf(fmaf(nx[k], idir.x, -oid.x), fmaf(ny[k], idir.y, -oid.y)),
-                                 fmaxf(fmaf(nz[k], idir.z, -oid.z), 0.0f));
-          const float tf = fminf(fminf(fmaf(fx[k], idir.x, -oid.x), fmaf(fy[k], idir.y, -oid.y)),
-                                 fminf(fmaf(fz[k], idir.z, -oid.z), lim));
-          const bool hit = tn <= tf * kSlabSlack;  // empty slots have +inf boxes: never hit
-          dist[k] = hit ? tn : INFINITY;
-        }
+        const float sx = idir.x * org.w, sy = idir.y * org.w, sz = idir.z * org.w;
+        const float bx = fmaf(org.x, idir.x, -oid.x), by = fmaf(org.y, idir.y, -oid.y),
+                    bz = fmaf(org.z, idir.z, -oid.z);
+        // near / far plane pairs (children 0|1 and 2|3) of each axis
+        const bool gx = neg & 1u, gy = neg & 2u, gz = neg & 4u;
+        const uint32_t nx0 = gx ? qx.z : qx.x, nx1 = gx ? qx.w : qx.y, fx0 = gx ? qx.x : qx.z,
+                       fx1 = gx ? qx.y : qx.w;
+        const uint32_t ny0 = gy ? qy.z : qy.x, ny1 = gy ? qy.w : qy.y, fy0 = gy ? qy.x : qy.z,
+                       fy1 = gy ? qy.y : qy.w;
+        const uint32_t nz0 = gz ? qz.z : qz.x, nz1 = gz ? qz.w : qz.y, fz0 = gz ? qz.x : qz.z,
+                       fz1 = gz ? qz.y : qz.w;
+#define GI_SLAB(k, NX, NY, NZ, FX, FY, FZ, MIX)                                                          \
+  {                                                                                                  \
+    const float tn = max3_raw(MIX(NX, sx, bx), MIX(NY, sy, by), max3_raw(MIX(NZ, sz, bz), 0.0f, 0.0f)); \
+    const float tf = min3_raw(MIX(FX, sx, bx), MIX(FY, sy, by), min3_raw(MIX(FZ, sz, bz), lim, lim));  \
+    dist[k] = tn <= tf * kSlabSlack ? tn : INFINITY; /* empty slots: +inf boxes, never hit */       \
+  }
+        GI_SLAB(0, nx0, ny0, nz0, fx0, fy0, fz0, mix_lo)
+        GI_SLAB(1, nx0, ny0, nz0, fx0, fy0, fz0, mix_hi)
+        GI_SLAB(2, nx1, ny1, nz1, fx1, fy1, fz1, mix_lo)
+        GI_SLAB(3, nx1, ny1, nz1, fx1, fy1, fz1, mix_hi)
+#undef GI_SLAB
         // the child references are needed only after the sort; consuming
         // them here (an empty register-only asm) keeps their load issued
         // with the planes' instead of sunk behind the sort, where it would
         // add one memory round trip to every node visit
         asm volatile("" : "+v"(cref[0]), "+v"(cref[1]), "+v"(cref[2]), "+v"(cref[3]));
         // sort (dist, ref) ascending: network (0,1)(2,3)(0,2)(1,3)(1,2)
-#define GI_CSWAP(i, j)                                      \
-if (dist[j] < dist[i]) {                                  \
-  float td = dist[i]; dist[i] = dist[j]; dist[j] = td;    \
-  uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
-}
+#define GI_CSWAP(i, j)                                \
+  {                                                 \
+    const bool sw = dist[j] < dist[i];              \
+    const float di = dist[i], dj = dist[j];         \
+    const uint32_t ri = cref[i], rj = cref[j];      \
+    dist[i] = sw ? dj : di;                         \
+    dist[j] = sw ? di : dj;                         \
+    cref[i] = sw ? rj : ri;                         \
+    cref[j] = sw ? ri : rj;                         \
+  }
         GI_CSWAP(0, 1) GI_CSWAP(2, 3) GI_CSWAP(0, 2) GI_CSWAP(1, 3) GI_CSWAP(1, 2)
 #undef GI_CSWAP
         if (dist[0] != INFINITY) {
@@ -463,7 +503,7 @@ if (dist[j] < dist[i]) {                                  \
       }
       idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
       oid = f3{o.x * idir.x, o.y * idir.y, o.z * idir.z};
-      near = near_planes(idir);
+      neg = dir_signs(idir);
       parked = kDoneRef;
     } else if (parked != kDoneRef) {
       // two triangle buffers, refilled alternately: triangle j+2 is loaded
@@ -542,7 +582,7 @@ constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the g
 // kLdsScene: copy the scene (n_nodes wide nodes, n_tris triangles) into
 // LDS after the stack (stack_words 32-bit words), then traverse it there.
 template <bool kLdsScene>
-__device__ __forceinline__ void stage_scene(uint32_t *smem, uint32_t stack_words, const WideNode *nodes,
+__device__ __forceinline__ void stage_scene(uint32_t *smem, uint32_t stack_words, const HalfNode *nodes,
                                             uint32_t n_nodes, const float4 *itris, uint32_t n_tris,
                                             const float4 *&lnodes, const float4 *&ltris) {
   if constexpr (kLdsScene) {
@@ -559,7 +599,7 @@ __device__ __forceinline__ void stage_scene(uint32_t *smem, uint32_t stack_words
 template <bool kHasTmax, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_closest(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
-    const float4 *__restrict__ ray_d, float4 *__restrict__ hits, const WideNode *__restrict__ nodes,
+    const float4 *__restrict__ ray_d, float4 *__restrict__ hits, const HalfNode *__restrict__ nodes,
     const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow,
     InstScene IS, unsigned long long *stats, uint32_t chunk_shift, uint32_t stack_words, uint32_t n_nodes,
     uint32_t n_tris) {
@@ -587,7 +627,7 @@ template <bool kDeposit, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_any(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
-    uint8_t *__restrict__ occluded, const WideNode *__restrict__ nodes, const float4 *__restrict__ itris,
+    uint8_t *__restrict__ occluded, const HalfNode *__restrict__ nodes, const float4 *__restrict__ itris,
     uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
     uint32_t rec_stride, int32_t b, InstScene IS, unsigned long long *stats, uint32_t chunk_shift,
     uint32_t stack_words, uint32_t n_nodes, uint32_t n_tris) {
@@ -1697,7 +1737,7 @@ struct gi_ctx {
   bool have_bvh = false;
   float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, diag = 0.f, power_total = 0.f;
   float clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};  // bounds of the triangle centroids
-  DBuf d_stris, d_itris, d_nodes, d_mats, d_lights;
+  DBuf d_stris, d_itris, d_nodes, d_hnodes, d_mats, d_lights;  // d_hnodes: HalfNode copy of d_nodes
   DBuf d_ks;  // specular colour per material (float4), for GI_FLAG_PHONG
   DBuf d_lsum;  // running sums of the light powers (n_lights + 1 floats)
   // Halton tables
@@ -1911,6 +1951,132 @@ InstScene inst_scene(const gi_ctx *ctx) {
 }
 bool instanced(const gi_ctx *ctx) { return ctx->n_inst > 0; }
 
+// ---- half-precision traversal nodes (HalfNode) from the wide nodes.
+// Per node: the origin is the min corner of its children's boxes, the
+// margin on each axis 2^-18 of that axis's largest coordinate magnitude,
+// and the scale s the power of two that brings the node's extent plus
+// margins below 2^14 (f16's largest finite value is 65504; being floating
+// point, f16 keeps small offsets precise at any s).  Nodes keep their
+// index, so child references and the 128-byte addressing carry over.  Each plane offset is
+// computed in double ((bound -/+ margin - origin) / s, exact up to 2^-53 of
+// the coordinates, far inside the margin) and rounded outward to f16.
+__device__ __forceinline__ uint16_t f16_step(uint16_t b, bool up) {
+  if ((b & 0x7fffu) == 0u) return up ? (uint16_t)0x0001u : (uint16_t)0x8001u;  // from +-0
+  if ((b & 0x7fffu) == 0x7c00u) return b;                                       // infinities stay
+  return (up != (bool)(b & 0x8000u)) ? (uint16_t)(b + 1u) : (uint16_t)(b - 1u);
+}
+__device__ __forceinline__ uint16_t f16_outward(double x, bool up) {
+  const _Float16 h = (_Float16)(float)x;
+  const uint16_t b = __builtin_bit_cast(uint16_t, h);
+  const double v = (double)(float)h;
+  return (up ? v < x : v > x) ? f16_step(b, up) : b;
+}
+__device__ __forceinline__ double f16_value(uint32_t h2, int half) {
+  return (double)(float)__builtin_bit_cast(_Float16, (uint16_t)(half ? h2 >> 16 : h2 & 0xffffu));
+}
+
+struct HalfFrame {
+  double org[3], margin[3], scale;
+};
+__device__ __forceinline__ HalfFrame half_frame(const WideNode &w) {
+  const float *lo[3] = {w.lox, w.loy, w.loz}, *hi[3] = {w.hix, w.hiy, w.hiz};
+  HalfFrame f;
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    double mn = INFINITY, mx = -INFINITY;
+    for (int k = 0; k < 4; ++k)
+      if (lo[a][k] != INFINITY) {
+        mn = fmin(mn, (double)lo[a][k]);
+        mx = fmax(mx, (double)hi[a][k]);
+      }
+    if (!(mn <= mx)) mn = mx = 0.0;  // no child (a lone root without children)
+    f.org[a] = mn;                   // a float: the origin is stored exactly
+    f.margin[a] = ldexp(fmax(fabs(mn), fabs(mx)), -18);
+    ext = fmax(ext, mx - mn + 2.0 * f.margin[a]);
+  }
+  int e = 0;
+  if (ext > 0.0) frexp(ext, &e);  // ext < 2^e
+  // 2^-60..2^40: scale/d then neither overflows nor underflows for the
+  // inverse directions rcp_clamped gives (|1/d| <= 2^80) and any |d| below
+  // 2^89; a node wider than 2^54 gets +inf hi planes (still conservative)
+  f.scale = ldexp(1.0, min(max(e - 14, -60), 40));
+  return f;
+}
+
+__global__ void k_half_nodes(const WideNode *__restrict__ in, HalfNode *__restrict__ out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const WideNode w = in[i];
+  const HalfFrame f = half_frame(w);
+  const float *lo[3] = {w.lox, w.loy, w.loz}, *hi[3] = {w.hix, w.hiy, w.hiz};
+  uint32_t q[3][4];
+  for (int a = 0; a < 3; ++a) {
+    uint32_t ql[4], qh[4];
+    for (int k = 0; k < 4; ++k) {
+      if (lo[a][k] == INFINITY) {  // empty slot
+        ql[k] = qh[k] = 0x7c00u;
+      } else {
+        ql[k] = f16_outward(((double)lo[a][k] - f.margin[a] - f.org[a]) / f.scale, false);
+        qh[k] = f16_outward(((double)hi[a][k] + f.margin[a] - f.org[a]) / f.scale, true);
+      }
+    }
+    q[a][0] = ql[0] | (ql[1] << 16);
+    q[a][1] = ql[2] | (ql[3] << 16);
+    q[a][2] = qh[0] | (qh[1] << 16);
+    q[a][3] = qh[2] | (qh[3] << 16);
+  }
+  float4 *o = reinterpret_cast<float4 *>(out + i);
+  o[0] = make_float4((float)f.org[0], (float)f.org[1], (float)f.org[2], (float)f.scale);
+  for (int a = 0; a < 3; ++a)
+    o[1 + a] = make_float4(__uint_as_float(q[a][0]), __uint_as_float(q[a][1]), __uint_as_float(q[a][2]),
+                           __uint_as_float(q[a][3]));
+  o[4] = make_float4(__uint_as_float(w.child[0]), __uint_as_float(w.child[1]), __uint_as_float(w.child[2]),
+                     __uint_as_float(w.child[3]));
+  o[5] = o[6] = o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Checks every half node against its wide node: same references; an empty
+// slot is +inf on every plane; otherwise the decoded planes org + q * s
+// (exact in double) enclose the f32 box widened by the node's margins.
+// Counts the failing child slots.
+__global__ void k_check_half_nodes(const WideNode *__restrict__ in, const HalfNode *__restrict__ hn, uint32_t n,
+                                   unsigned long long *bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const WideNode w = in[i];
+  const HalfNode h = hn[i];
+  const HalfFrame f = half_frame(w);
+  const float *lo[3] = {w.lox, w.loy, w.loz}, *hi[3] = {w.hix, w.hiy, w.hiz};
+  uint32_t nbad = 0;
+  for (int k = 0; k < 4; ++k) {
+    bool ok = h.child[k] == w.child[k];
+    for (int a = 0; a < 3; ++a) {
+      const double ql = f16_value(h.q[a][k >> 1], k & 1), qh = f16_value(h.q[a][2 + (k >> 1)], k & 1);
+      if (lo[a][k] == INFINITY) {
+        ok = ok && ql == INFINITY && qh == INFINITY;
+      } else {
+        const double dl = (double)h.org[a] + ql * (double)h.scale, dh = (double)h.org[a] + qh * (double)h.scale;
+        ok = ok && (double)h.scale == f.scale && dl <= (double)lo[a][k] - f.margin[a] &&
+             dh >= (double)hi[a][k] + f.margin[a];
+      }
+    }
+    nbad += ok ? 0u : 1u;
+  }
+  if (nbad) atomicAdd(bad, (unsigned long long)nbad);
+}
+
+// Builds the traversal's half-node copy of the current wide tree.
+int make_half_nodes(gi_ctx *ctx) {
+  GI_HIP(ctx, ctx->d_hnodes.ensure((size_t)std::max<int64_t>(1, ctx->nnodes) * sizeof(HalfNode)));
+  if (ctx->nnodes > 0) {
+    hipLaunchKernelGGL(k_half_nodes, dim3((unsigned)((ctx->nnodes + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_nodes.as<WideNode>(), ctx->d_hnodes.as<HalfNode>(), (uint32_t)ctx->nnodes);
+    GI_HIP(ctx, hipGetLastError());
+  }
+  GI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GI_OK;
+}
+
 // State that follows a new wide tree on the device.
 int finish_bvh(gi_ctx *ctx, int64_t nnodes, int32_t wide_depth) {
   if ((uint64_t)nnodes > kMaxWideNodes) {
@@ -1924,6 +2090,8 @@ int finish_bvh(gi_ctx *ctx, int64_t nnodes, int32_t wide_depth) {
   ctx->stack_cap = (uint32_t)(3 * (wide_depth + 1) + 4);
   ctx->trace_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
   int rc = ensure_gstack(ctx);
+  if (rc) return rc;
+  rc = make_half_nodes(ctx);
   if (rc) return rc;
   ctx->have_bvh = true;
   return GI_OK;
@@ -2038,7 +2206,7 @@ void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_pt
       kLdsScene ? (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) : 0;
   hipLaunchKernelGGL((k_trace_closest<kHasTmax, kInst, kLds, kCount, kLdsScene>), dim3(grid), dim3(block),
                      stack_bytes + scene_bytes, ctx->stream, count_ptr, count_host, ro, rd, hits,
-                     ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
+                     ctx->d_hnodes.as<HalfNode>(), ctx->d_itris.as<float4>(),
                      ts.gstack ? ts.gstack : ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
                      ts.overflow ? ts.overflow : ctx->d_overflow.as<uint32_t>(), inst_scene(ctx),
                      kCount ? ctx->d_tstats.as<unsigned long long>() : nullptr, chunk_shift_of(ctx),
@@ -2054,7 +2222,7 @@ void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, u
       kLdsScene ? (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) : 0;
   hipLaunchKernelGGL((k_trace_any<kDeposit, kInst, kLds, kCount, kLdsScene>), dim3(grid), dim3(block),
                      stack_bytes + scene_bytes, ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ,
-                     ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
+                     ctx->d_hnodes.as<HalfNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
                      ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b, inst_scene(ctx),
                      kCount ? ctx->d_tstats.as<unsigned long long>() + 4 : nullptr, chunk_shift_of(ctx),
                      (uint32_t)(stack_bytes / 4), (uint32_t)ctx->nnodes, (uint32_t)ctx->ntri);
@@ -2272,7 +2440,7 @@ std::vector<uint8_t> launch_key(gi_ctx *ctx, const PassPlan &plan, int32_t depth
   key_put(k, ctx->slot_cap);
   for (const void *q : {ctx->d_passes.p, ctx->d_views.p, ctx->d_counters.p, ctx->q_o[0].p, ctx->q_o[1].p,
                         ctx->q_d[0].p, ctx->q_d[1].p, ctx->q_tp[0].p, ctx->q_tp[1].p, ctx->q_hit.p, ctx->s_o.p,
-                        ctx->s_d.p, ctx->s_c.p, ctx->path_l.p, ctx->d_nodes.p, ctx->d_itris.p, ctx->d_gstack.p,
+                        ctx->s_d.p, ctx->s_c.p, ctx->path_l.p, ctx->d_hnodes.p, ctx->d_itris.p, ctx->d_gstack.p,
                         ctx->d_overflow.p, ctx->d_tstats.p})
     key_put(k, q);
   // shading parameters, field by field (the struct has padding)
@@ -3260,7 +3428,9 @@ int gi_upload_instances(gi_ctx *ctx, const gi_instance *inst, int32_t n) {
   // a kept reference and the return marker, mesh levels (three per level)
   ctx->wide_depth = tlas_depth + 1 + ctx->world_depth + 1 + mesh_depth;
   ctx->stack_cap = (uint32_t)(3 * (ctx->wide_depth + 1) + 4 + 2);
-  return ensure_gstack(ctx);
+  const int rc = ensure_gstack(ctx);
+  if (rc) return rc;
+  return make_half_nodes(ctx);
 }
 
 int gi_upload_bvh2(gi_ctx *ctx, const float *nodes, int64_t n_nodes, const uint32_t *indices, int64_t n_indices) {
@@ -3549,6 +3719,26 @@ int gi_set_traversal_stats(gi_ctx *ctx, int32_t enable) {
     GI_HIP(ctx, copy_sync(ctx, ctx->d_tstats.p, &zero, sizeof(zero), hipMemcpyHostToDevice));
   }
   ctx->count_traversal = enable != 0;
+  return GI_OK;
+}
+
+int gi_check_half_nodes(gi_ctx *ctx, int64_t *bad_slots) {
+  if (!ctx || !bad_slots) return fail(ctx, GI_ERR_INVALID_ARG, "check_half_nodes");
+  if (!ctx->have_bvh) return fail(ctx, GI_ERR_STATE, "no BVH");
+  GI_HIP(ctx, hipSetDevice(ctx->device));
+  DBuf cnt;
+  GI_HIP(ctx, cnt.ensure(sizeof(unsigned long long)));
+  const unsigned long long zero = 0;
+  GI_HIP(ctx, copy_sync(ctx, cnt.p, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  if (ctx->nnodes > 0) {
+    hipLaunchKernelGGL(k_check_half_nodes, dim3((unsigned)((ctx->nnodes + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_nodes.as<WideNode>(), ctx->d_hnodes.as<HalfNode>(), (uint32_t)ctx->nnodes,
+                       cnt.as<unsigned long long>());
+    GI_HIP(ctx, hipGetLastError());
+  }
+  unsigned long long n = 0;
+  GI_HIP(ctx, copy_sync(ctx, &n, cnt.p, sizeof(n), hipMemcpyDeviceToHost));
+  *bad_slots = (int64_t)n;
   return GI_OK;
 }
 

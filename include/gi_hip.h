@@ -343,6 +343,13 @@ typedef struct {
 int gi_set_traversal_stats(gi_ctx *ctx, int32_t enable);
 /* Totals since the last gi_set_traversal_stats(ctx, 1). */
 int gi_get_traversal_stats(gi_ctx *ctx, gi_traversal_stats *out);
+/* Consistency check of the traversal's half-precision node copy (no
+ * counterpart in the reference, whose traversal reads f32 boxes,
+ * src/tracers.cu:70-73): the kernels test each child's box as f16 plane
+ * offsets from a per-node f32 origin, rounded outward.  Counts the child
+ * slots of the current tree whose decoded box does not enclose the f32
+ * box widened by its margins, or whose reference differs; 0 expected. */
+int gi_check_half_nodes(gi_ctx *ctx, int64_t *bad_slots);
 int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);
 int gi_get_tuning(const gi_ctx *ctx, gi_tuning *t);
 

@@ -54,6 +54,9 @@ def test_closest_and_any_parity(builder, inst, oracle, gpu_ctx_factory):
     ref = osc.trace_closest(o, d, tmax)
     assert (ref["ref"][ref["ref"] != MISS] >= len(inst.tris)).sum() > 10_000
     with gpu_ctx_factory(inst, builder=builder) as ctx:
+        # the traversal's half-precision boxes (world, mesh and instance
+        # trees) enclose the f32 ones
+        assert ctx.check_half_nodes() == 0
         _assert_hits_equal(ctx.trace_closest(o, d, tmax), ref, osc, o, d, tmax)
         assert np.array_equal(ctx.trace_any(o, d, tmax).astype(bool), ref["ref"] != MISS)
         # camera rays

@@ -243,6 +243,38 @@ def test_device_bvh_tiny_and_degenerate(n, builder, oracle, gpu_ctx_factory):
         _assert_hits_equal(ctx.trace_closest(o, d, t), ref, osc, o, d, t)
 
 
+@pytest.mark.parametrize("builder", ["host", "device", "ploc", "sah"])
+def test_half_nodes_enclose_wide_boxes(builder, terrain, gpu_ctx_factory):
+    """The traversal reads each node as f16 plane offsets from an f32 origin
+    (HalfNode): every child box it decodes encloses the f32 box widened by
+    the node's margins, and the child references are the same."""
+    with gpu_ctx_factory(terrain, builder=builder) as ctx:
+        assert ctx.check_half_nodes() == 0
+
+
+@pytest.mark.parametrize("scale,offset", [(1e-5, 0.0), (1e-4, 30.0), (250.0, -2e6), (3e4, 0.0)])
+def test_extreme_coordinates(scale, offset, oracle, gpu_ctx_factory):
+    """Scenes far from the origin or of very small or large size: the f16
+    node planes (scale and offsets relative to each node's origin) stay
+    conservative, so hits still match the oracle bit for bit."""
+    sc = scenes.terrain_spheres(cells=(48, 48), n_spheres=60, w=32, h=18, spp=1)
+    tris = sc.tris.copy()
+    for k in "abc":
+        tris[k] = (tris[k].astype(np.float64) * scale + offset).astype(np.float32)
+    one = scenes.Scene(tris, sc.mats, sc.lights, sc.views, sc.opts)
+    rng = np.random.default_rng(17)
+    o, d = _rays(rng, 100_000, -5, 105, 0, 20)
+    o = (o.astype(np.float64) * scale + offset).astype(np.float32)
+    t = np.full(len(o), FMAX, np.float32)
+    osc = oracle.OracleScene(tris, sc.mats, sc.lights)
+    ref = osc.trace_closest(o, d, t)
+    assert (ref["ref"] != 0xFFFFFFFF).mean() > 0.2
+    with gpu_ctx_factory(one) as ctx:
+        assert ctx.check_half_nodes() == 0
+        _assert_hits_equal(ctx.trace_closest(o, d, t), ref, osc, o, d, t)
+        assert np.array_equal(ctx.trace_any(o, d, t), osc.trace_any(o, d, t))
+
+
 def test_triangle_reciprocal_exhaustive():
     """The triangle test's reciprocal (hardware rcp + one Newton step) equals
     the IEEE division 1/x bit for bit for all 2^32 floats, so t, u and v keep
