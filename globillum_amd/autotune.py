@@ -25,14 +25,14 @@ import time
 
 SPACE = {
     "trace_block": [64, 128, 256],
-    "trace_blocks_per_cu": [4, 5, 6, 8],
+    "trace_blocks_per_cu": [4, 5, 6],
     "stack_lds": [8, 16, 32],
     "node_width": [2, 3, 4],
     "refill_lanes": [8, 16, 32, 64],
     "shade_block": [64, 128, 256],
     "target_slots": [32, 64, 128],
 }
-DEFAULT = {"trace_block": 256, "trace_blocks_per_cu": 5, "stack_lds": 16, "node_width": 4, "refill_lanes": 32,
+DEFAULT = {"trace_block": 256, "trace_blocks_per_cu": 6, "stack_lds": 16, "node_width": 4, "refill_lanes": 32,
            "shade_block": 256, "target_slots": 128}
 
 

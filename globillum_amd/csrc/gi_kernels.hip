@@ -198,9 +198,9 @@ __device__ __forceinline__ void block_append2(bool p0, uint32_t *c0, bool p1, ui
 // all triangles with 0 < t <= max_t; any-hit stops at the first such
 // triangle.
 // Occupancy target of the traversal kernels (waves per SIMD): 6 caps them
-// at 80 VGPRs, which they fit without spilling (the default launch, 5 blocks
-// of 4 waves per CU, runs 5 per SIMD).  At 7 (72 VGPRs) the two-level
-// closest-hit kernel spills and runs 39 % slower.
+// at 80 VGPRs, which they fit without spilling (the default launch, 6 blocks
+// of 4 waves per CU, runs 6 per SIMD).  At 7 (72 VGPRs) the two-level
+// kernels spill.
 #ifndef GI_TRACE_WAVES_PER_EU
 #define GI_TRACE_WAVES_PER_EU 6
 #endif
@@ -223,9 +223,9 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 // Largest scene (wide nodes + intersection triangles) that the traversal
 // kernels copy into each block's LDS: nodes and triangles are then read
-// with ds_read_b128 instead of from L2.  16 KB keeps 5 blocks of 256 lanes
+// with ds_read_b128 instead of from L2.  10 KB keeps 6 blocks of 256 lanes
 // (16 KB of stack each) within a CU's 160 KB.
-constexpr size_t kLdsSceneMaxBytes = 16384;
+constexpr size_t kLdsSceneMaxBytes = 10240;
 
 template <int kLdsDepth>
 struct Stack {
@@ -313,7 +313,7 @@ __device__ __forceinline__ bool special_ref(uint32_t ref) {
 // tests and instance entries, and the wave adds them to stats[0..3] at the
 // end (traversal statistics; a separate instantiation).
 // cs: log2 of the chunk size (the refill threshold, 8..64 rays).
-// kLdsScene: the block's LDS holds a copy of the nodes (8 float4 each) at
+// kLdsScene: the block's LDS holds a copy of the f32 wide nodes (8 float4 each) at
 // lnodes and of the triangles at ltris; they are read from there.
 template <bool kAny, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene, class Start, class Finish>
 __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, const HalfNode *__restrict__ nodes,
@@ -373,53 +373,73 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
     for (;;) {
       if (!(ref & kLeafBit)) {
         if constexpr (kCount) ++c_nodes;
-        // origin and scale, the f16 planes of each axis, the references
-        float4 org;
-        uint4 qx, qy, qz, ch;
-        const uint32_t base = ref << 7;
-        if constexpr (kLdsScene) {
-          const char *np = reinterpret_cast<const char *>(lnodes);  // in LDS: derived from the shared base
-          const auto u4 = [](float4 f) {
-            return make_uint4(__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z), __float_as_uint(f.w));
-          };
-          org = *reinterpret_cast<const float4 *>(np + base);
-          qx = u4(*reinterpret_cast<const float4 *>(np + (base | 16u)));
-          qy = u4(*reinterpret_cast<const float4 *>(np + (base | 32u)));
-          qz = u4(*reinterpret_cast<const float4 *>(np + (base | 48u)));
-          ch = u4(*reinterpret_cast<const float4 *>(np + (base | 64u)));
-        } else {
-          const char *np = reinterpret_cast<const char *>(nodes);
-          org = *reinterpret_cast<const float4 *>(np + base);
-          qx = *reinterpret_cast<const uint4 *>(np + (base | 16u));
-          qy = *reinterpret_cast<const uint4 *>(np + (base | 32u));
-          qz = *reinterpret_cast<const uint4 *>(np + (base | 48u));
-          ch = *reinterpret_cast<const uint4 *>(np + (base | 64u));
-        }
         const float lim = kAny ? tmax : smin(bt, tmax);
         float dist[4];
-        uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
-        const float sx = idir.x * org.w, sy = idir.y * org.w, sz = idir.z * org.w;
-        const float bx = fmaf(org.x, idir.x, -oid.x), by = fmaf(org.y, idir.y, -oid.y),
-                    bz = fmaf(org.z, idir.z, -oid.z);
-        // near / far plane pairs (children 0|1 and 2|3) of each axis
+        uint32_t cref[4];
+        const uint32_t base = ref << 7;
         const bool gx = neg & 1u, gy = neg & 2u, gz = neg & 4u;
-        const uint32_t nx0 = gx ? qx.z : qx.x, nx1 = gx ? qx.w : qx.y, fx0 = gx ? qx.x : qx.z,
-                       fx1 = gx ? qx.y : qx.w;
-        const uint32_t ny0 = gy ? qy.z : qy.x, ny1 = gy ? qy.w : qy.y, fy0 = gy ? qy.x : qy.z,
-                       fy1 = gy ? qy.y : qy.w;
-        const uint32_t nz0 = gz ? qz.z : qz.x, nz1 = gz ? qz.w : qz.y, fz0 = gz ? qz.x : qz.z,
-                       fz1 = gz ? qz.y : qz.w;
+        if constexpr (kLdsScene) {
+          // the small-scene copy in LDS holds the f32 wide nodes: LDS reads
+          // are cheap, so only the near and the far plane of each axis are
+          // read (at offsets by the direction signs), with no conversion
+          const char *np = reinterpret_cast<const char *>(lnodes);  // in LDS: derived from the shared base
+          const uint32_t onx = base | (gx ? 16u : 0u), ony = base | (gy ? 48u : 32u), onz = base | (gz ? 80u : 64u);
+          const float4 nxv = *reinterpret_cast<const float4 *>(np + onx);
+          const float4 fxv = *reinterpret_cast<const float4 *>(np + (onx ^ 16u));
+          const float4 nyv = *reinterpret_cast<const float4 *>(np + ony);
+          const float4 fyv = *reinterpret_cast<const float4 *>(np + (ony ^ 16u));
+          const float4 nzv = *reinterpret_cast<const float4 *>(np + onz);
+          const float4 fzv = *reinterpret_cast<const float4 *>(np + (onz ^ 16u));
+          const float4 c = *reinterpret_cast<const float4 *>(np + (base | 96u));
+          cref[0] = __float_as_uint(c.x);
+          cref[1] = __float_as_uint(c.y);
+          cref[2] = __float_as_uint(c.z);
+          cref[3] = __float_as_uint(c.w);
+          const float nx[4] = {nxv.x, nxv.y, nxv.z, nxv.w}, fx[4] = {fxv.x, fxv.y, fxv.z, fxv.w};
+          const float ny[4] = {nyv.x, nyv.y, nyv.z, nyv.w}, fy[4] = {fyv.x, fyv.y, fyv.z, fyv.w};
+          const float nz[4] = {nzv.x, nzv.y, nzv.z, nzv.w}, fz[4] = {fzv.x, fzv.y, fzv.z, fzv.w};
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const float tn = fmaxf(fmaxf(fmaf(nx[k], idir.x, -oid.x), fmaf(ny[k], idir.y, -oid.y)),
+                                   fmaxf(fmaf(nz[k], idir.z, -oid.z), 0.0f));
+            const float tf = fminf(fminf(fmaf(fx[k], idir.x, -oid.x), fmaf(fy[k], idir.y, -oid.y)),
+                                   fminf(fmaf(fz[k], idir.z, -oid.z), lim));
+            dist[k] = tn <= tf * kSlabSlack ? tn : INFINITY;  // empty slots: +inf boxes, never hit
+          }
+        } else {
+          // origin and scale, the f16 planes of each axis, the references
+          const char *np = reinterpret_cast<const char *>(nodes);
+          const float4 org = *reinterpret_cast<const float4 *>(np + base);
+          const uint4 qx = *reinterpret_cast<const uint4 *>(np + (base | 16u));
+          const uint4 qy = *reinterpret_cast<const uint4 *>(np + (base | 32u));
+          const uint4 qz = *reinterpret_cast<const uint4 *>(np + (base | 48u));
+          const uint4 ch = *reinterpret_cast<const uint4 *>(np + (base | 64u));
+          cref[0] = ch.x;
+          cref[1] = ch.y;
+          cref[2] = ch.z;
+          cref[3] = ch.w;
+          const float sx = idir.x * org.w, sy = idir.y * org.w, sz = idir.z * org.w;
+          const float bx = fmaf(org.x, idir.x, -oid.x), by = fmaf(org.y, idir.y, -oid.y),
+                      bz = fmaf(org.z, idir.z, -oid.z);
+          // near / far plane pairs (children 0|1 and 2|3) of each axis
+          const uint32_t nx0 = gx ? qx.z : qx.x, nx1 = gx ? qx.w : qx.y, fx0 = gx ? qx.x : qx.z,
+                         fx1 = gx ? qx.y : qx.w;
+          const uint32_t ny0 = gy ? qy.z : qy.x, ny1 = gy ? qy.w : qy.y, fy0 = gy ? qy.x : qy.z,
+                         fy1 = gy ? qy.y : qy.w;
+          const uint32_t nz0 = gz ? qz.z : qz.x, nz1 = gz ? qz.w : qz.y, fz0 = gz ? qz.x : qz.z,
+                         fz1 = gz ? qz.y : qz.w;
 #define GI_SLAB(k, NX, NY, NZ, FX, FY, FZ, MIX)                                                          \
   {                                                                                                  \
     const float tn = max3_raw(MIX(NX, sx, bx), MIX(NY, sy, by), max3_raw(MIX(NZ, sz, bz), 0.0f, 0.0f)); \
     const float tf = min3_raw(MIX(FX, sx, bx), MIX(FY, sy, by), min3_raw(MIX(FZ, sz, bz), lim, lim));  \
     dist[k] = tn <= tf * kSlabSlack ? tn : INFINITY; /* empty slots: +inf boxes, never hit */       \
   }
-        GI_SLAB(0, nx0, ny0, nz0, fx0, fy0, fz0, mix_lo)
-        GI_SLAB(1, nx0, ny0, nz0, fx0, fy0, fz0, mix_hi)
-        GI_SLAB(2, nx1, ny1, nz1, fx1, fy1, fz1, mix_lo)
-        GI_SLAB(3, nx1, ny1, nz1, fx1, fy1, fz1, mix_hi)
+          GI_SLAB(0, nx0, ny0, nz0, fx0, fy0, fz0, mix_lo)
+          GI_SLAB(1, nx0, ny0, nz0, fx0, fy0, fz0, mix_hi)
+          GI_SLAB(2, nx1, ny1, nz1, fx1, fy1, fz1, mix_lo)
+          GI_SLAB(3, nx1, ny1, nz1, fx1, fy1, fz1, mix_hi)
 #undef GI_SLAB
+        }
         // the child references are needed only after the sort; consuming
         // them here (an empty register-only asm) keeps their load issued
         // with the planes' instead of sunk behind the sort, where it would
@@ -582,7 +602,7 @@ constexpr int kMinLdsDepth = 8;  // smallest instantiated LDS stack; sizes the g
 // kLdsScene: copy the scene (n_nodes wide nodes, n_tris triangles) into
 // LDS after the stack (stack_words 32-bit words), then traverse it there.
 template <bool kLdsScene>
-__device__ __forceinline__ void stage_scene(uint32_t *smem, uint32_t stack_words, const HalfNode *nodes,
+__device__ __forceinline__ void stage_scene(uint32_t *smem, uint32_t stack_words, const WideNode *nodes,
                                             uint32_t n_nodes, const float4 *itris, uint32_t n_tris,
                                             const float4 *&lnodes, const float4 *&ltris) {
   if constexpr (kLdsScene) {
@@ -600,14 +620,14 @@ template <bool kHasTmax, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_closest(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, float4 *__restrict__ hits, const HalfNode *__restrict__ nodes,
-    const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow,
+    const WideNode *__restrict__ wnodes, const float4 *__restrict__ itris, uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow,
     InstScene IS, unsigned long long *stats, uint32_t chunk_shift, uint32_t stack_words, uint32_t n_nodes,
     uint32_t n_tris) {
   extern __shared__ uint32_t smem[];
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
   const float4 *lnodes = nullptr, *ltris = nullptr;
-  stage_scene<kLdsScene>(smem, stack_words, nodes, n_nodes, itris, n_tris, lnodes, ltris);
+  stage_scene<kLdsScene>(smem, stack_words, wnodes, n_nodes, itris, n_tris, lnodes, ltris);
   trace_persistent<false, kInst, kLdsDepth, kCount, kLdsScene>(
       count, chunk_shift, nodes, itris, lnodes, ltris, st, overflow, IS, stats,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
@@ -627,7 +647,8 @@ template <bool kDeposit, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WAVES_PER_EU))) void k_trace_any(
     const uint32_t *__restrict__ count_ptr, uint32_t count_host, const float4 *__restrict__ ray_o,
     const float4 *__restrict__ ray_d, const float4 *__restrict__ contrib, float4 *__restrict__ path_l,
-    uint8_t *__restrict__ occluded, const HalfNode *__restrict__ nodes, const float4 *__restrict__ itris,
+    uint8_t *__restrict__ occluded, const HalfNode *__restrict__ nodes, const WideNode *__restrict__ wnodes,
+    const float4 *__restrict__ itris,
     uint32_t *__restrict__ gstack, uint32_t stack_cap, uint32_t *overflow, float4 *__restrict__ rec,
     uint32_t rec_stride, int32_t b, InstScene IS, unsigned long long *stats, uint32_t chunk_shift,
     uint32_t stack_words, uint32_t n_nodes, uint32_t n_tris) {
@@ -635,7 +656,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GI_TRACE_WA
   const uint32_t count = count_ptr ? *count_ptr : count_host;
   Stack<kLdsDepth> st = make_stack<kLdsDepth>(smem, gstack, stack_cap);
   const float4 *lnodes = nullptr, *ltris = nullptr;
-  stage_scene<kLdsScene>(smem, stack_words, nodes, n_nodes, itris, n_tris, lnodes, ltris);
+  stage_scene<kLdsScene>(smem, stack_words, wnodes, n_nodes, itris, n_tris, lnodes, ltris);
   trace_persistent<true, kInst, kLdsDepth, kCount, kLdsScene>(
       count, chunk_shift, nodes, itris, lnodes, ltris, st, overflow, IS, stats,
       [&](uint32_t i, f3 &o, f3 &d, float &tmax) {
@@ -1725,6 +1746,7 @@ hipError_t scratch_address(int device, void **out) {
 struct gi_ctx {
   int device = 0;
   int num_cus = 256;
+  size_t lds_per_cu = 160 * 1024;
   hipStream_t stream = nullptr;
   std::string err;
   // scene
@@ -1795,7 +1817,7 @@ struct gi_ctx {
   int32_t prog_sample = 0, prog_max = 0;
   bool prog_active = false;
   gi_counters counters{};
-  gi_tuning tuning{256, 5, 256, 0, 16, 0, 4, 32};
+  gi_tuning tuning{256, 6, 256, 0, 16, 0, 4, 32};
   std::vector<hipEvent_t> events;
   uint8_t *scratch = nullptr;  // this device's scratch array (gi_create)
 };
@@ -2140,9 +2162,28 @@ int ensure_gstack(gi_ctx *ctx) {
   return GI_OK;
 }
 
+int stack_lds_of(const gi_ctx *ctx);
+bool lds_scene_of(const gi_ctx *ctx);
+size_t trace_lds_bytes(int lds_depth, int block);
+
+// Traversal blocks per CU that are resident at once: the tuned count,
+// capped by the waves the register budget allows (GI_TRACE_WAVES_PER_EU per
+// SIMD, 4 SIMDs) and by the LDS a block takes (its stack, plus the scene
+// copy on the small-scene path), so that the persistent grid never exceeds
+// what the chip holds at once (a block that had to wait for another to
+// finish would leave its chunks of rays to the end of the launch).
+int resident_trace_blocks(const gi_ctx *ctx, int block) {
+  const int by_waves = GI_TRACE_WAVES_PER_EU * 4 / std::max(1, block / 64);
+  const bool lds_scene = !ctx->count_traversal && stack_lds_of(ctx) == 16 && lds_scene_of(ctx);
+  const size_t lds = trace_lds_bytes(ctx->count_traversal ? 16 : stack_lds_of(ctx), block) +
+                     (lds_scene ? (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) : 0);
+  const int by_lds = (int)std::min<size_t>(64, ctx->lds_per_cu / std::max<size_t>(1, lds));
+  return std::max(1, std::min({ctx->tuning.trace_blocks_per_cu, by_waves, by_lds}));
+}
+
 int trace_grid_for(gi_ctx *ctx, uint64_t n, int &grid, int &block) {
   block = std::max(64, (std::min(256, ctx->tuning.trace_block) / 64) * 64);
-  const int max_grid = ctx->num_cus * std::max(1, ctx->tuning.trace_blocks_per_cu);
+  const int max_grid = ctx->num_cus * resident_trace_blocks(ctx, block);
   uint64_t need = (n + (uint64_t)block - 1) / (uint64_t)block;
   grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(need, (uint64_t)max_grid));
   return GI_OK;
@@ -2206,7 +2247,7 @@ void launch_closest_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_pt
       kLdsScene ? (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) : 0;
   hipLaunchKernelGGL((k_trace_closest<kHasTmax, kInst, kLds, kCount, kLdsScene>), dim3(grid), dim3(block),
                      stack_bytes + scene_bytes, ctx->stream, count_ptr, count_host, ro, rd, hits,
-                     ctx->d_hnodes.as<HalfNode>(), ctx->d_itris.as<float4>(),
+                     ctx->d_hnodes.as<HalfNode>(), ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
                      ts.gstack ? ts.gstack : ctx->d_gstack.as<uint32_t>(), ctx->stack_cap,
                      ts.overflow ? ts.overflow : ctx->d_overflow.as<uint32_t>(), inst_scene(ctx),
                      kCount ? ctx->d_tstats.as<unsigned long long>() : nullptr, chunk_shift_of(ctx),
@@ -2222,7 +2263,8 @@ void launch_any_t(gi_ctx *ctx, int grid, int block, const uint32_t *count_ptr, u
       kLdsScene ? (size_t)ctx->nnodes * sizeof(WideNode) + (size_t)ctx->ntri * sizeof(IsectTri) : 0;
   hipLaunchKernelGGL((k_trace_any<kDeposit, kInst, kLds, kCount, kLdsScene>), dim3(grid), dim3(block),
                      stack_bytes + scene_bytes, ctx->stream, count_ptr, count_host, ro, rd, contrib, pl, occ,
-                     ctx->d_hnodes.as<HalfNode>(), ctx->d_itris.as<float4>(), ctx->d_gstack.as<uint32_t>(),
+                     ctx->d_hnodes.as<HalfNode>(), ctx->d_nodes.as<WideNode>(), ctx->d_itris.as<float4>(),
+                     ctx->d_gstack.as<uint32_t>(),
                      ctx->stack_cap, ctx->d_overflow.as<uint32_t>(), rec, rec_stride, b, inst_scene(ctx),
                      kCount ? ctx->d_tstats.as<unsigned long long>() + 4 : nullptr, chunk_shift_of(ctx),
                      (uint32_t)(stack_bytes / 4), (uint32_t)ctx->nnodes, (uint32_t)ctx->ntri);
@@ -2440,7 +2482,7 @@ std::vector<uint8_t> launch_key(gi_ctx *ctx, const PassPlan &plan, int32_t depth
   key_put(k, ctx->slot_cap);
   for (const void *q : {ctx->d_passes.p, ctx->d_views.p, ctx->d_counters.p, ctx->q_o[0].p, ctx->q_o[1].p,
                         ctx->q_d[0].p, ctx->q_d[1].p, ctx->q_tp[0].p, ctx->q_tp[1].p, ctx->q_hit.p, ctx->s_o.p,
-                        ctx->s_d.p, ctx->s_c.p, ctx->path_l.p, ctx->d_hnodes.p, ctx->d_itris.p, ctx->d_gstack.p,
+                        ctx->s_d.p, ctx->s_c.p, ctx->path_l.p, ctx->d_hnodes.p, ctx->d_nodes.p, ctx->d_itris.p, ctx->d_gstack.p,
                         ctx->d_overflow.p, ctx->d_tstats.p})
     key_put(k, q);
   // shading parameters, field by field (the struct has padding)
@@ -2859,6 +2901,7 @@ int gi_create(gi_ctx **out, int device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return GI_ERR_NO_DEVICE;
   ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  if (prop.maxSharedMemoryPerMultiProcessor > 0) ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return GI_ERR_NO_DEVICE;
   void *sp = nullptr;
   if (scratch_address(device, &sp) != hipSuccess) return GI_ERR_HIP;
@@ -3754,7 +3797,7 @@ int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t) {
   if (!ctx || !t) return fail(ctx, GI_ERR_INVALID_ARG, "tuning");
   gi_tuning n = *t;
   if (n.trace_block <= 0) n.trace_block = 256;
-  if (n.trace_blocks_per_cu <= 0) n.trace_blocks_per_cu = 5;
+  if (n.trace_blocks_per_cu <= 0) n.trace_blocks_per_cu = 6;
   if (n.shade_block <= 0) n.shade_block = 256;
   if (n.stack_lds <= 0) n.stack_lds = 16;
   // every kernel is compiled with __launch_bounds__(256)

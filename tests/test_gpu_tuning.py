@@ -66,7 +66,7 @@ def test_tuning_errors(ts, gpu_ctx_factory):
 
 
 def test_autotuner(ts, gpu_ctx_factory):
-    space = {"node_width": [3, 4], "refill_lanes": [16, 32], "trace_blocks_per_cu": [4, 5]}
+    space = {"node_width": [3, 4], "refill_lanes": [16, 32], "trace_blocks_per_cu": [5, 6]}
     with gpu_ctx_factory(ts) as ctx:
         ref = ctx.render(flags=FLAG_BATCHED).image(0).copy()
         opts = ts.opts.copy()
@@ -86,7 +86,7 @@ def test_autotuner(ts, gpu_ctx_factory):
 
 
 def test_autotuner_cache_per_workload(ts, gpu_ctx_factory, tmp_path):
-    space = {"refill_lanes": [16, 32], "trace_blocks_per_cu": [4, 5]}
+    space = {"refill_lanes": [16, 32], "trace_blocks_per_cu": [5, 6]}
     cache = str(tmp_path / "tuning.json")
     opts = ts.opts.copy()
     opts["flags"] = FLAG_BATCHED
