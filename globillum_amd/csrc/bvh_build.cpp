@@ -614,6 +614,8 @@ void halton_tables(uint32_t seed, int32_t ndims, std::vector<HaltonDim> &dims, s
     h.P = P;
     h.perm0 = perm[0];
     h.magic = (uint64_t)(~0ULL / P) + 1ULL;  // exact floor(n / P) = mulhi(n, magic) for n < 2^32
+    h.kmax = 0;
+    for (uint64_t pk = 1; pk < ((uint64_t)1 << 32); pk *= P) ++h.kmax;
     h.pad = 0;
     chunks.resize(chunks.size() + P);
     for (uint32_t c = 0; c < P; ++c) {

@@ -130,40 +130,48 @@ struct HaltonDev {
 // digits d_k (K of them) map to perm[d_k] and the infinitely many leading
 // zeros to perm[0], which is the exact rational
 //   A / B,  A = rev*(p-1) + perm[0],  B = p^K (p-1),  rev = sum perm[d_k] p^(K-1-k).
-// Digits are consumed m at a time through the chunk table (P = p^m <= 1024);
-// a partial last chunk adds leading zeros, which scales A and B by the same
-// power of p.  A, B < 2^53 for n < 2^32, so the value is one correctly
-// rounded FP64 division and one rounding to FP32.
-// Every dimension has P >= 37, so n < 2^32 has at most 7 chunks: all 7 are
-// split off and looked up at once (independent loads), and the ones above
-// the leading chunk are then dropped.
-constexpr int kHaltonMaxChunks = 7;
+// Digits are consumed m at a time through the chunk table (P = p^m <= 1024).
+// Leading zero digits may be counted as digits: each adds perm[0] to rev and
+// a factor p to p^K, which multiplies A and B by p and leaves A / B as it
+// is.  So every n is split into the same kmax chunks of its dimension (the
+// least K with P^K >= 2^32), zero chunks above its leading one included:
+// A, B <= 2^42 p stay exact in FP64 (every partial value is an integer below
+// 2^53), and the value is one correctly rounded FP64 division and one
+// rounding to FP32, as if only n's own digits were used.  kmax is the same
+// for every lane, so the chunks are taken with no per-lane selection.
+// floor(m / P) is the top 32 bits of the 96-bit m * magic (one 32-bit
+// mulhi and one 32x32+32 multiply-add: exact as the 64-bit mulhi); no
+// 64-bit integer multiplies, the quarter-rate part of this function.
+constexpr int kHaltonMaxChunks = 7;  // kmax <= 7: every dimension has P >= 37
 __device__ __forceinline__ float halton(uint32_t n, int dim, HaltonDev H) {
   const HaltonDim hd = H.dims[dim];
   const uint16_t *chunk = H.chunks + hd.chunk_off;
+  const uint32_t mlo = (uint32_t)hd.magic, mhi = (uint32_t)(hd.magic >> 32);
   uint32_t c[kHaltonMaxChunks];
-  uint32_t m = n, K = 0;
+  uint32_t m = n;
 #pragma unroll
   for (int k = 0; k < kHaltonMaxChunks; ++k) {
-    const uint32_t q = (uint32_t)__umul64hi((uint64_t)m, hd.magic);
+    // (m * mhi + hi32(m * mlo)) >> 32 = hi64(m * magic): the low word of
+    // m * mlo cannot carry into the top 32 bits of the 96-bit product
+    const uint32_t q = (uint32_t)(((uint64_t)m * mhi + __umulhi(m, mlo)) >> 32);
     c[k] = m - q * hd.P;
-    K += m > 0u ? 1u : 0u;
     m = q;
   }
   uint32_t v[kHaltonMaxChunks];
 #pragma unroll
   for (int k = 0; k < kHaltonMaxChunks; ++k) v[k] = chunk[c[k]];
-  uint64_t rev = 0, pk = 1;
+  const double Pd = (double)hd.P;
+  double rev = 0.0, pk = 1.0;
 #pragma unroll
   for (int k = 0; k < kHaltonMaxChunks; ++k) {
-    if ((uint32_t)k < K) {
-      rev = rev * hd.P + v[k];
-      pk *= hd.P;
+    if ((uint32_t)k < hd.kmax) {
+      rev = fma(rev, Pd, (double)v[k]);
+      pk = pk * Pd;
     }
   }
-  const uint64_t A = rev * (uint64_t)(hd.prime - 1) + hd.perm0;
-  const uint64_t B = pk * (uint64_t)(hd.prime - 1);
-  const float f = (float)((double)A / (double)B);
+  const double p1 = (double)(hd.prime - 1);
+  const double A = fma(rev, p1, (double)hd.perm0), B = pk * p1;
+  const float f = (float)(A / B);
   return f >= 1.0f ? 0x1.fffffep-1f : f;
 }
 

@@ -146,7 +146,8 @@ struct HaltonDim {
   uint32_t P;           // p^m
   uint32_t perm0;       // perm[0]
   uint64_t magic;       // floor(2^64 / P) + 1
-  uint64_t pad;
+  uint32_t kmax;        // chunks of any n < 2^32: the least K with P^K >= 2^32
+  uint32_t pad;
 };
 static_assert(sizeof(HaltonDim) == 32, "halton dim layout");
 

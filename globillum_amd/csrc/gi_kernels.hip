@@ -1252,10 +1252,10 @@ __device__ __forceinline__ void shade_vertices(
   }
 }
 
-// The untextured kernel at the compiler's own choice (127 VGPRs, 4 waves per
-// SIMD).  The textured one is held to 4 waves: it then fits 128 VGPRs with a
-// few spills instead of 153 at 3 waves, which measured faster; the same
-// target on the untextured kernel measured slower.
+// Both shading kernels are held to 4 waves per SIMD (128 VGPRs): the
+// untextured one fits without spilling, the textured one spills two
+// registers; at the compiler's own choice they take 137 and 177
+// VGPRs (3 waves), which measured slower.
 #define GI_SHADE_ARGS                                                                                              \
   int32_t b, const uint32_t *__restrict__ count_ptr, ShadeParams P, const float4 *__restrict__ ray_o,             \
       const float4 *__restrict__ ray_d, const float4 *__restrict__ tp_in, const float4 *__restrict__ hits,        \
@@ -1265,7 +1265,9 @@ __device__ __forceinline__ void shade_vertices(
 #define GI_SHADE_PASS                                                                                               \
   b, count_ptr, P, ray_o, ray_d, tp_in, hits, path_l, nro, nrd, ntp, next_count, sho, shd, shc, shadow_count, rec, \
       rec_stride
-__global__ __launch_bounds__(256) void k_shade(GI_SHADE_ARGS) { shade_vertices<false>(GI_SHADE_PASS); }
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(GI_SHADE_ARGS) {
+  shade_vertices<false>(GI_SHADE_PASS);
+}
 // gpu_pt's Lambert + Phong model (GI_FLAG_PHONG), untextured and textured
 __global__ __launch_bounds__(256) void k_shade_phong(GI_SHADE_ARGS) { shade_vertices<false, true>(GI_SHADE_PASS); }
 __global__ __launch_bounds__(256) void k_shade_phong_tex(GI_SHADE_ARGS) { shade_vertices<true, true>(GI_SHADE_PASS); }
