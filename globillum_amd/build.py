@@ -19,6 +19,8 @@ SOURCES = ["gi_kernels.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene_io.cpp", "im
 MEMBENCH = os.path.join(PKG, "libgi_membench.so")
 # exhaustive check of the traversal's reciprocal against the IEEE division
 RCP_CHECK = os.path.join(PKG, "gi_rcp_check")
+# device evaluation of the sampler for given (index, dimension) pairs
+HALTON_CHECK = os.path.join(PKG, "gi_halton_check")
 
 # -ffp-contract=off: hit ids, sample directions and image values are
 # reproduced bit-for-bit on the host, so no a*b+c may be fused.
@@ -65,8 +67,8 @@ def build_host(verbose: bool = False) -> None:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every source into libgi_hip.so, the reciprocal check and the
-    host binaries (always recompiles)."""
+    """Compile every source into libgi_hip.so, the reciprocal and sampler
+    checks and the host binaries (always recompiles)."""
     tmp = LIB + ".tmp"
     cmd = [hipcc(), *HIP_FLAGS, "-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lz"]
     if verbose:
@@ -92,6 +94,14 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building gi_rcp_check")
     os.replace(tmp, RCP_CHECK)
+    tmp = HALTON_CHECK + ".tmp"
+    r = subprocess.run([hipcc(), *[f for f in HIP_FLAGS if f != "-fPIC"], "-o", tmp,
+                        os.path.join(CSRC, "halton_check.hip"), os.path.join(CSRC, "bvh_build.cpp"), "-pthread"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building gi_halton_check")
+    os.replace(tmp, HALTON_CHECK)
     build_host(verbose)
     return LIB
 

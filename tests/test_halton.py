@@ -101,3 +101,47 @@ def test_every_dimension_needs_at_most_seven_chunks():
         while P * p <= 1024:
             P *= p
         assert P ** 7 >= 2 ** 32, (dim, p, P)
+
+
+def test_fixed_chunk_count_keeps_the_rational(oracle):
+    # the kernels split every index n < 2^32 into the same kmax chunks of its
+    # dimension (the least K with P^K >= 2^32), zero chunks above the leading
+    # one included: each zero chunk multiplies A and B by P, so A / B is the
+    # digit loop's rational, and A, B stay below 2^53 (exact in FP64) for
+    # every dimension of a depth-32 render
+    from fractions import Fraction
+
+    lib = _lib.load()
+    rng = np.random.default_rng(5)
+    seed = 7
+    for dim in range(4 + 7 * 32):
+        p = lib.gi_halton_prime(dim)
+        P, m = p, 1
+        while P * p <= 1024:
+            P *= p
+            m += 1
+        kmax, pk = 0, 1
+        while pk < 2 ** 32:
+            pk *= P
+            kmax += 1
+        assert pk * p < 2 ** 53, (dim, p, P, kmax)
+        perm = oracle.halton_perm(dim, seed).astype(np.int64)
+        for n in [0, 1, P - 1, P, P * P, 2 ** 32 - 1] + [int(x) for x in rng.integers(0, 2 ** 32, 4)]:
+            rev, pk_d, x = 0, 1, n
+            while x:
+                rev = rev * p + int(perm[x % p])
+                pk_d *= p
+                x //= p
+            exact = Fraction(rev * (p - 1) + int(perm[0]), pk_d * (p - 1))
+            rev, x = 0, n
+            for _ in range(kmax):
+                c = x % P
+                x //= P
+                v, y = 0, c
+                for _ in range(m):  # the chunk table entry: least significant digit first
+                    v = v * p + int(perm[y % p])
+                    y //= p
+                rev = rev * P + v
+            assert x == 0
+            A, B = rev * (p - 1) + int(perm[0]), pk * (p - 1)
+            assert A < 2 ** 53 and B < 2 ** 53 and Fraction(A, B) == exact, (dim, n)
