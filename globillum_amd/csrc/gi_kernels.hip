@@ -52,6 +52,8 @@ struct alignas(16) DevView {
 };
 
 constexpr int kMaxChunks = 64;
+// A pass holds chunks of path slots: samples s0 .. s0+k-1 of every pixel of
+// one view, pixel-major (slot_base + p * k + j is sample s0 + j of pixel p).
 struct PassChunk {
   int32_t view, s0, k, npix;
   uint32_t slot_base, pix_base;
@@ -805,8 +807,11 @@ __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pas
     const PassChunk &ck = pass->chunk[find_chunk(pass, slot)];
     const DevView &v = views[ck.view];
     const uint32_t local = slot - ck.slot_base;
-    const uint32_t s = (uint32_t)ck.s0 + local / (uint32_t)ck.npix;
-    const uint32_t p = local % (uint32_t)ck.npix;
+    // pixel-major: the chunk's k samples of a pixel are consecutive slots,
+    // so a wave's rays start in a few pixels (coherent first bounces, and
+    // the adjoint's texel gradients of one pixel meet in one wave)
+    const uint32_t s = (uint32_t)ck.s0 + local % (uint32_t)ck.k;
+    const uint32_t p = local / (uint32_t)ck.k;
     const uint32_t n = v.index_base + p * (uint32_t)v.spp + s;
     f3 o, d;
     // jittered (setup_jittered_shirley / _dof_shirley) or through the pixel
@@ -1430,22 +1435,24 @@ __device__ __forceinline__ float wave_sum(float v) {
 // times each of the four bilinear weights, added at texel gi[k] (index into
 // g_tex in texels).  Called by the whole wave.  Lanes whose texel equals the
 // first pending lane's are summed across the wave first and added by one
-// lane: coarse mip levels, which far and secondary footprints reach, are
-// shared by many lanes, and atomics to one address serialise.  ta[k] is
-// where texel gi[k]'s RGB gradient is added (a coarse texel's replica).
+// lane, for up to 8 such groups of at least 2 lanes: the samples of a pixel
+// share a wave (pixel-major slots) and mostly its texels, coarse mip levels
+// are shared by many lanes, and global atomics are what this pass spends
+// its time on.  ta[k] is where texel gi[k]'s RGB gradient is added (a
+// coarse texel's replica).
 __device__ __forceinline__ void scatter_texel_grads(bool has, float *const ta[4], const uint32_t gi[4],
                                                     const float w[4], const float g[3]) {
   const uint32_t lane = __lane_id();
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     bool pend = has;
-    for (int round = 0; round < 2; ++round) {
+    for (int round = 0; round < 8; ++round) {
       const uint64_t act = __ballot(pend);
       if (act == 0) break;
       const int leader = __ffsll((unsigned long long)act) - 1;
       const uint32_t key = (uint32_t)__shfl((int)gi[k], leader, 64);
       const bool mine = pend && gi[k] == key;
-      if (__popcll(__ballot(mine)) < 4) break;
+      if (__popcll(__ballot(mine)) < 2) break;
       const float r = wave_sum(mine ? g[0] * w[k] : 0.0f);
       const float gg = wave_sum(mine ? g[1] * w[k] : 0.0f);
       const float bb = wave_sum(mine ? g[2] * w[k] : 0.0f);
@@ -1518,7 +1525,7 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
         // adjoint weight of the path's pixel: dL/dI / spp
         const PassChunk &ck = A.pass->chunk[find_chunk(A.pass, slot)];
         const DevView &v = A.views[ck.view];
-        const uint32_t p = (slot - ck.slot_base) % (uint32_t)ck.npix;
+        const uint32_t p = (slot - ck.slot_base) / (uint32_t)ck.k;  // pixel-major slots
         const float *w = A.dL + 3 * ((size_t)v.pix_base + p);
         const float inv_spp = 1.0f / (float)v.spp;
         const f3 delta = f3{w[0] * inv_spp, w[1] * inv_spp, w[2] * inv_spp};
@@ -1638,7 +1645,7 @@ __global__ __launch_bounds__(256) void k_accumulate(const PassDesc *__restrict__
     float4 *img = views[ck.view].image;
     float4 acc = img[p];
     for (int j = 0; j < ck.k; ++j) {
-      const float4 l = path_l[ck.slot_base + (uint32_t)j * (uint32_t)ck.npix + p];
+      const float4 l = path_l[ck.slot_base + p * (uint32_t)ck.k + (uint32_t)j];  // pixel-major slots
       const float s = float(ck.s0 + j), s1 = float(ck.s0 + j + 1);
       acc.x = (s * acc.x + l.x) / s1;
       acc.y = (s * acc.y + l.y) / s1;
