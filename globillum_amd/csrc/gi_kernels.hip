@@ -1632,26 +1632,56 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
 }
 
 // Running mean over the pass's samples in sample order
-// (combine_color_samples, src/util-kernels.cu:44-51).
+// (combine_color_samples, src/util-kernels.cu:44-51).  A block takes 256
+// pixels, one per thread.  The slots are pixel-major, so the block's samples
+// are read kAccJ at a time into LDS with line-sized loads (consecutive lanes
+// read a pixel's consecutive samples) and each thread then folds its pixel's
+// kAccJ samples in order from LDS (rows padded by one float4 against bank
+// conflicts).  Blocks must be 256 threads.
+constexpr int kAccJ = 8;
 __global__ __launch_bounds__(256) void k_accumulate(const PassDesc *__restrict__ pass,
                                                     const DevView *__restrict__ views,
                                                     const float4 *__restrict__ path_l) {
-  const uint32_t total = pass->total_pix;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+  __shared__ float4 tile[256 * (kAccJ + 1)];
+  __shared__ uint32_t s_slot[256];
+  __shared__ int32_t s_k[256];
+  __shared__ int32_t s_kmax;
+  const uint32_t total = pass->total_pix, t = threadIdx.x;
+  for (uint32_t g0 = blockIdx.x * 256u; g0 < total; g0 += gridDim.x * 256u) {
+    const uint32_t g = g0 + t;
+    const bool live = g < total;
     int c = 0;
-    while (c + 1 < pass->nchunks && g >= pass->chunk[c + 1].pix_base) ++c;
+    if (live)
+      while (c + 1 < pass->nchunks && g >= pass->chunk[c + 1].pix_base) ++c;
     const PassChunk &ck = pass->chunk[c];
-    const uint32_t p = g - ck.pix_base;
-    float4 *img = views[ck.view].image;
-    float4 acc = img[p];
-    for (int j = 0; j < ck.k; ++j) {
-      const float4 l = path_l[ck.slot_base + p * (uint32_t)ck.k + (uint32_t)j];  // pixel-major slots
-      const float s = float(ck.s0 + j), s1 = float(ck.s0 + j + 1);
-      acc.x = (s * acc.x + l.x) / s1;
-      acc.y = (s * acc.y + l.y) / s1;
-      acc.z = (s * acc.z + l.z) / s1;
+    const uint32_t p = live ? g - ck.pix_base : 0u;
+    if (t == 0) s_kmax = 0;
+    s_slot[t] = live ? ck.slot_base + p * (uint32_t)ck.k : 0u;  // pixel-major slots
+    s_k[t] = live ? ck.k : 0;
+    __syncthreads();
+    if (live) atomicMax(&s_kmax, ck.k);
+    float4 acc = live ? views[ck.view].image[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const int kmax = s_kmax;
+    for (int j0 = 0; j0 < kmax; j0 += kAccJ) {
+#pragma unroll
+      for (int i = 0; i < kAccJ; ++i) {
+        const uint32_t e = (uint32_t)i * 256u + t, q = e / kAccJ, j = e % kAccJ;
+        if (j0 + (int)j < s_k[q]) tile[q * (kAccJ + 1) + j] = path_l[s_slot[q] + (uint32_t)j0 + j];
+      }
+      __syncthreads();
+      if (live) {
+        for (int j = j0; j < min(j0 + kAccJ, ck.k); ++j) {
+          const float4 l = tile[t * (kAccJ + 1) + (uint32_t)(j - j0)];
+          const float sj = float(ck.s0 + j), s1 = float(ck.s0 + j + 1);
+          acc.x = (sj * acc.x + l.x) / s1;
+          acc.y = (sj * acc.y + l.y) / s1;
+          acc.z = (sj * acc.z + l.z) / s1;
+        }
+      }
+      __syncthreads();
     }
-    img[p] = acc;
+    if (live) views[ck.view].image[p] = acc;
   }
 }
 
