@@ -130,6 +130,8 @@ struct ShadeParams {
   uint32_t rdiff_stride;   // planes [0..3] = upper origin, upper dir, right origin, right dir
   const float4 *ks;        // per material: specular colour (GI_FLAG_PHONG renders)
   const float *lsum;       // running sums of the light powers: lsum[k] = power of lights 0..k-1
+  const PassDesc *pass;    // the pass being shaded and the views: the first vertex's ray
+  const DevView *views;    // differentials are the camera's, computed from the slot's pixel
 };
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -795,13 +797,15 @@ __global__ __launch_bounds__(256) void k_tex_grad_down(float *__restrict__ g, ui
   }
 }
 
-// Primary rays of every path slot of a pass (new_path: generate_rays and
-// the throughput / colour reset, src/gpu-pt.h:110-118).
+// Primary rays of every path slot of a pass and the colour reset (new_path:
+// generate_rays and the throughput / colour reset, src/gpu-pt.h:110-118).
+// The first vertex's throughput (1) and ray differentials (the camera's) are
+// not stored: the shading of vertex 0 takes them as known, which spares 80
+// of the 128 bytes per slot this kernel would write.
 __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pass, const DevView *__restrict__ views,
                                                 HaltonDev H, float4 *__restrict__ ray_o, float4 *__restrict__ ray_d,
-                                                float4 *__restrict__ tp, float4 *__restrict__ path_l,
-                                                float4 *__restrict__ rec, uint32_t rec_stride, int32_t depth,
-                                                float4 *__restrict__ rdiff, uint32_t rdiff_stride) {
+                                                float4 *__restrict__ path_l, float4 *__restrict__ rec,
+                                                uint32_t rec_stride, int32_t depth) {
   const uint32_t total = pass->total_slots;
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
     const PassChunk &ck = pass->chunk[find_chunk(pass, slot)];
@@ -821,27 +825,25 @@ __global__ __launch_bounds__(256) void k_raygen(const PassDesc *__restrict__ pas
                halton(n, kDimLensY, H), o, d);
     ray_o[slot] = make_float4(o.x, o.y, o.z, bitsf(n));
     ray_d[slot] = make_float4(d.x, d.y, d.z, bitsf(slot));
-    tp[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
     path_l[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (rec)
       for (int32_t b = 0; b < depth; ++b) rec[(size_t)b * rec_stride + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (rdiff) {
-      // ray differentials: pinhole rays from the camera position through the
-      // centres of pixels (x, y+1) ("upper") and (x+1, y) ("right"); rta's
-      // camera_ray_generator_shirley, which makes them in the reference, is
-      // external (src/raygen.h:122-130)
-      DevView pv = v;
-      pv.aperture = 0.0f;
-      const int32_t x = (int32_t)(p % (uint32_t)v.w), y = (int32_t)(p / (uint32_t)v.w);
-      f3 uo, ud, ro, rd;
-      camera_ray(pv, x, y + 1, 0.5f, 0.5f, 0.5f, 0.5f, uo, ud);
-      camera_ray(pv, x + 1, y, 0.5f, 0.5f, 0.5f, 0.5f, ro, rd);
-      rdiff[slot] = make_float4(uo.x, uo.y, uo.z, 0.f);
-      rdiff[rdiff_stride + slot] = make_float4(ud.x, ud.y, ud.z, 0.f);
-      rdiff[2 * rdiff_stride + slot] = make_float4(ro.x, ro.y, ro.z, 0.f);
-      rdiff[3 * rdiff_stride + slot] = make_float4(rd.x, rd.y, rd.z, 0.f);
-    }
   }
+}
+
+// The ray differentials of a path's primary ray: pinhole rays from the
+// camera position through the centres of pixels (x, y+1) ("upper") and
+// (x+1, y) ("right"); rta's camera_ray_generator_shirley, which makes them
+// in the reference, is external (src/raygen.h:122-130).
+__device__ __forceinline__ void primary_differentials(const PassDesc *pass, const DevView *views, uint32_t slot,
+                                                      f3 &uo, f3 &ud, f3 &ro, f3 &rd) {
+  const PassChunk &ck = pass->chunk[find_chunk(pass, slot)];
+  DevView pv = views[ck.view];
+  const uint32_t p = (slot - ck.slot_base) / (uint32_t)ck.k;  // pixel-major slots
+  pv.aperture = 0.0f;
+  const int32_t x = (int32_t)(p % (uint32_t)pv.w), y = (int32_t)(p / (uint32_t)pv.w);
+  camera_ray(pv, x, y + 1, 0.5f, 0.5f, 0.5f, 0.5f, uo, ud);
+  camera_ray(pv, x + 1, y, 0.5f, 0.5f, 0.5f, 0.5f, ro, rd);
 }
 
 // One path vertex: NEE (pixel_generate_arealight_sample_cpu,
@@ -1053,10 +1055,21 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
              ((f2.x * C.x + f2.y * C.y) + f2.z * C.z) + f2.w};
     }
     const f3 e1 = sub(B, A), e2 = sub(C, A);
-    const float4 ud4 = P.rdiff[st + slot];
+    // the first vertex's differentials are the camera's (not stored); later
+    // ones come from the previous vertex
+    const bool first = dim == kDimVertex0;
+    f3 duo, dud, dro, drd;
+    if (first) {
+      primary_differentials(P.pass, P.views, slot, duo, dud, dro, drd);
+    } else {
+      duo = xyz(P.rdiff[slot]);
+      dud = xyz(P.rdiff[st + slot]);
+      dro = xyz(P.rdiff[2 * st + slot]);
+      drd = xyz(P.rdiff[3 * st + slot]);
+    }
     float ub, ug, rb, rg;
-    plane_bary(A, e1, e2, xyz(P.rdiff[slot]), xyz(ud4), ub, ug);
-    plane_bary(A, e1, e2, xyz(P.rdiff[2 * st + slot]), xyz(P.rdiff[3 * st + slot]), rb, rg);
+    plane_bary(A, e1, e2, duo, dud, ub, ug);
+    plane_bary(A, e1, e2, dro, drd, rb, rg);
     const float u0 = (1.0f - ub) - ug, r0 = (1.0f - rb) - rg;
     if (m.texture >= 0) {
       const float4 tab = T.tab, tcm = T.tcm;
@@ -1073,10 +1086,11 @@ __device__ __forceinline__ void eval_vertex(const ShadeParams &P, uint32_t n, in
     ve.textured = m.texture >= 0;
     const f3 uo = add(add(scl(A, u0), scl(B, ub)), scl(C, ug));
     const f3 ro = add(add(scl(A, r0), scl(B, rb)), scl(C, rg));
-    const f3 ud = reflect_dir(xyz(ud4), N0);
+    const f3 ud = reflect_dir(dud, N0);
     P.rdiff[slot] = make_float4(uo.x, uo.y, uo.z, 0.0f);
     P.rdiff[st + slot] = make_float4(ud.x, ud.y, ud.z, 0.0f);
     P.rdiff[2 * st + slot] = make_float4(ro.x, ro.y, ro.z, 0.0f);
+    if (first) P.rdiff[3 * st + slot] = make_float4(drd.x, drd.y, drd.z, 0.0f);  // kept from here on
   }
   ve.org_dir = org_dir;
   ve.P = Pp;
@@ -1175,7 +1189,8 @@ __device__ __forceinline__ void shade_vertices(
     bool emit_next = false, emit_shadow = false;
     float4 o_next, d_next, tp_next, s_o, s_d, s_c;
     if (i < count) {
-      const float4 ro = ray_o[i], rd = ray_d[i], tpv = tp_in[i], h = hits[i];
+      const float4 ro = ray_o[i], rd = ray_d[i], h = hits[i];
+      const float4 tpv = b == 0 ? make_float4(1.f, 1.f, 1.f, 0.f) : tp_in[i];  // a new path's throughput
       const uint32_t n = fbits(ro.w), slot = fbits(rd.w), id = fbits(h.w);
       const f3 TP = xyz(tpv);
       const f3 org_dir = normalize(xyz(rd));
@@ -1498,7 +1513,8 @@ __global__ __launch_bounds__(256) void k_shade_adjoint(int32_t b, const uint32_t
     float *tex_a[4] = {nullptr, nullptr, nullptr, nullptr};
     float tex_w[4] = {0.f, 0.f, 0.f, 0.f}, tex_g[3] = {0.f, 0.f, 0.f};
     if (i < count) {
-      const float4 ro = ray_o[i], rd = ray_d[i], tpv = tp_in[i];
+      const float4 ro = ray_o[i], rd = ray_d[i];
+      const float4 tpv = b == 0 ? make_float4(1.f, 1.f, 1.f, 0.f) : tp_in[i];  // a new path's throughput
       const uint32_t n = fbits(ro.w), slot = fbits(rd.w);
       // the hit the forward pass recorded for this path's vertex b (the
       // replay's rays are the forward pass's, bit for bit)
@@ -2486,6 +2502,8 @@ ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
   P.rdiff_stride = (uint32_t)ctx->slot_cap;
   P.ks = ctx->d_ks.as<float4>();
   P.lsum = ctx->d_lsum.as<float>();
+  P.pass = nullptr;  // set per pass
+  P.views = ctx->d_views.as<DevView>();
   return P;
 }
 
@@ -2529,7 +2547,8 @@ std::vector<uint8_t> launch_key(gi_ctx *ctx, const PassPlan &plan, int32_t depth
                         (const void *)P.H.chunks, (const void *)P.sky.rgb, (const void *)P.sky.ycdf,
                         (const void *)P.sky.ypdf, (const void *)P.sky.xcdf, (const void *)P.sky.xpdf,
                         (const void *)P.is.inst, (const void *)P.is.vbase, (const void *)P.is.vtab,
-                        (const void *)P.tex, (const void *)P.rdiff, (const void *)P.ks, (const void *)P.lsum})
+                        (const void *)P.tex, (const void *)P.rdiff, (const void *)P.ks, (const void *)P.lsum,
+                        (const void *)P.views})
     key_put(k, q);
   key_put(k, ctx->opts.flags & GI_FLAG_PHONG);  // picks the shading kernel
   key_put(k, P.nlights);
@@ -2579,12 +2598,14 @@ int enqueue_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_c
   for (size_t pi = 0; pi < npass; ++pi) {
     const PassDesc &pd = plan.passes[pi];
     const PassDesc *dp = dpass + pi;
+    ShadeParams SPp = SP;
+    SPp.pass = dp;
     uint32_t *cnt = ctx->d_counters.as<uint32_t>() + pi * ncnt;
     hipLaunchKernelGGL(k_pass_setup, dim3(1), dim3(256), 0, ctx->stream, dp, cnt, ncnt);
     const int rg = (int)std::min<uint64_t>((pd.total_slots + 255) / 256, (uint64_t)persist);
     hipLaunchKernelGGL(k_raygen, dim3(std::max(1, rg)), dim3(256), 0, ctx->stream, dp, dviews, SP.H,
-                       ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->q_tp[0].as<float4>(),
-                       ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, depth, SP.rdiff, SP.rdiff_stride);
+                       ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->path_l.as<float4>(),
+                       (float4 *)nullptr, 0u, depth);
     uint32_t host_count = pd.total_slots;
     for (int32_t b = 0; b < depth; ++b) {
       const int cur = b & 1, nxt = cur ^ 1;
@@ -2604,7 +2625,7 @@ int enqueue_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, bool sync_c
       const bool phong = (ctx->opts.flags & GI_FLAG_PHONG) != 0;
       hipLaunchKernelGGL(phong ? (SP.rdiff ? k_shade_phong_tex : k_shade_phong) : (SP.rdiff ? k_shade_tex : k_shade),
                          dim3(sg), dim3(sblock), 0, ctx->stream, b,
-                         cnt + b, SP, ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(),
+                         cnt + b, SPp, ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(),
                          ctx->q_tp[cur].as<float4>(), ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(),
                          ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(), ctx->q_tp[nxt].as<float4>(),
                          cnt + b + 1, ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>(),
@@ -2757,6 +2778,8 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
     const PassDesc &pd = plan.passes[pi];
     const PassDesc *dp = dpass + pi;
     A.pass = dp;
+    ShadeParams SPp = SP;
+    SPp.pass = dp;
     const int rg = (int)std::min<uint64_t>((pd.total_slots + 255) / 256, (uint64_t)persist);
     const int sg = (int)std::max<uint64_t>(1, std::min<uint64_t>((pd.total_slots + sblock - 1) / sblock,
                                                                  (uint64_t)persist));
@@ -2764,9 +2787,8 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
       uint32_t *cnt = ctx->d_counters.as<uint32_t>() + (2 * pi + phase) * ncnt;
       hipLaunchKernelGGL(k_pass_setup, dim3(1), dim3(256), 0, ctx->stream, dp, cnt, ncnt);
       hipLaunchKernelGGL(k_raygen, dim3(std::max(1, rg)), dim3(256), 0, ctx->stream, dp, dviews, SP.H,
-                         ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->q_tp[0].as<float4>(),
-                         ctx->path_l.as<float4>(), phase == 0 ? rec : (float4 *)nullptr, stride, depth,
-                         SP.rdiff, SP.rdiff_stride);
+                         ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(), ctx->path_l.as<float4>(),
+                         phase == 0 ? rec : (float4 *)nullptr, stride, depth);
       for (int32_t b = 0; b < depth; ++b) {
         const int cur = b & 1, nxt = cur ^ 1;
         if (phase == 0) {
@@ -2775,7 +2797,7 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
                               ctx->q_d[cur].as<float4>(), ctx->q_hit.as<float4>());
           if (rc) return rc;
           hipLaunchKernelGGL(SP.rdiff ? k_shade_tex : k_shade, dim3(sg), dim3(sblock), 0, ctx->stream, b,
-                             cnt + b, SP,
+                             cnt + b, SPp,
                              ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
                              ctx->q_hit.as<float4>(), ctx->path_l.as<float4>(), ctx->q_o[nxt].as<float4>(),
                              ctx->q_d[nxt].as<float4>(), ctx->q_tp[nxt].as<float4>(), cnt + b + 1,
@@ -2788,7 +2810,7 @@ int run_adjoint_passes(gi_ctx *ctx, const PassPlan &plan, int32_t depth, AdjPara
           if (rc) return rc;
         } else {
           hipLaunchKernelGGL(SP.rdiff ? k_shade_adjoint<true> : k_shade_adjoint<false>, dim3(sg), dim3(sblock),
-                             lds_adj, ctx->stream, b, cnt + b, SP, A,
+                             lds_adj, ctx->stream, b, cnt + b, SPp, A,
                              ctx->q_o[cur].as<float4>(), ctx->q_d[cur].as<float4>(), ctx->q_tp[cur].as<float4>(),
                              ctx->q_o[nxt].as<float4>(), ctx->q_d[nxt].as<float4>(),
                              ctx->q_tp[nxt].as<float4>(), cnt + b + 1);
@@ -3683,8 +3705,7 @@ int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint3
   HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>()};
   hipLaunchKernelGGL(k_raygen, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_cam_pass.as<PassDesc>(),
                      ctx->d_cam_view.as<DevView>(), H, ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(),
-                     ctx->q_tp[0].as<float4>(), ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, 1,
-                     (float4 *)nullptr, 0u);
+                     ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, 1);
   GI_HIP(ctx, hipGetLastError());
   std::vector<float4> ro(npix), rd(npix);
   GI_HIP(ctx, hipMemcpyAsync(ro.data(), ctx->q_o[0].p, npix * 16, hipMemcpyDeviceToHost, ctx->stream));
