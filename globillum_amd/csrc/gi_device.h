@@ -175,6 +175,23 @@ __device__ __forceinline__ float halton(uint32_t n, int dim, HaltonDev H) {
   return f >= 1.0f ? 0x1.fffffep-1f : f;
 }
 
+// The chunk tables of any kDimsPerVertex consecutive dimensions of a render
+// of depth <= 32 fit this many entries (the largest such window, dimensions
+// 221..227 with P = p from 1399 to 1439, holds 9959); the host checks it.
+constexpr uint32_t kHaltonLdsEntries = 10240;
+// Copies the chunk tables of dimensions [d0, d0 + nd) into lds and returns
+// the sampler that reads them there: the same values, with LDS reads in
+// place of the scattered 16-bit global loads.  Called by every thread of
+// the block, before any of them samples.
+__device__ __forceinline__ HaltonDev halton_in_lds(HaltonDev H, int d0, int nd, uint16_t *lds) {
+  const HaltonDim first = H.dims[d0], last = H.dims[d0 + nd - 1];
+  const uint32_t off0 = first.chunk_off;
+  const uint32_t n = min(last.chunk_off + last.P - off0, kHaltonLdsEntries);
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = H.chunks[off0 + i];
+  __syncthreads();
+  return HaltonDev{H.dims, lds - off0};
+}
+
 // ---------------------------------------------------------- frames -------
 // Orthonormal basis around unit n: Duff et al., JCGT 6(1) 2017; T and B
 // normalised afterwards.

@@ -103,6 +103,23 @@ def test_every_dimension_needs_at_most_seven_chunks():
         assert P ** 7 >= 2 ** 32, (dim, p, P)
 
 
+def test_sampler_tables_fit_the_lds_copy():
+    # ray generation copies the chunk tables of dimensions 0..3 into LDS, the
+    # untextured shading those of its vertex's 7 dimensions: 10240 entries
+    # hold every such window of a depth-32 render (the largest, dimensions
+    # 221..227, has 9959)
+    lib = _lib.load()
+    sizes = []
+    for dim in range(4 + 7 * 32):
+        p = lib.gi_halton_prime(dim)
+        P = p
+        while P * p <= 1024:
+            P *= p
+        sizes.append(P)
+    windows = [sum(sizes[0:4])] + [sum(sizes[d:d + 7]) for d in range(4, 4 + 7 * 32, 7)]
+    assert max(windows) == 9959 and max(windows) <= 10240
+
+
 def test_fixed_chunk_count_keeps_the_rational(oracle):
     # the kernels split every index n < 2^32 into the same kmax chunks of its
     # dimension (the least K with P^K >= 2^32), zero chunks above the leading
