@@ -172,11 +172,14 @@ __device__ __forceinline__ void block_append2(bool p0, uint32_t *c0, bool p1, ui
 
 // -------------------------------------------------------------- traversal --
 // Ordered 4-wide BVH traversal, one ray per lane, persistent.  The queue is
-// cut into chunks of C consecutive rays (C = the refill threshold, a tuning
-// parameter: 8..64); wave w of W owns chunks w,
-// w + W, w + 2W, ... (no atomics).  A wave keeps its lanes busy: whenever
-// at least C lanes have finished their rays, they take the wave's next
-// whole chunks, one ray each.  Interior nodes are processed in an inner
+// cut into spans of S consecutive rays; wave w of W owns spans w, w + W,
+// w + 2W, ... (no atomics) and takes their rays in order.  A wave keeps its
+// lanes busy: whenever at least C of them have finished their rays (C, the
+// refill threshold, is a tuning parameter: 8..64), the idle lanes take the
+// wave's next rays, one each.  S is the largest power of two from C to 256
+// that still gives every wave 16 spans: successive refills of a wave take
+// neighbouring rays, whose traversals are more alike than those of chunks
+// W spans apart, and the last spans still balance the waves' work.  Interior nodes are processed in an inner
 // loop in which a lane that reaches a leaf parks it and goes on with its
 // stack (speculative traversal, Aila & Laine 2009), until every lane with
 // interior work left holds a parked leaf; then the parked leaves are
@@ -316,7 +319,7 @@ __device__ __forceinline__ bool special_ref(uint32_t ref) {
 // kCount: the lane also counts its rays, interior-node visits, triangle
 // tests and instance entries, and the wave adds them to stats[0..3] at the
 // end (traversal statistics; a separate instantiation).
-// cs: log2 of the chunk size (the refill threshold, 8..64 rays).
+// cs: log2 of the refill threshold (8..64 idle lanes).
 // kLdsScene: the block's LDS holds a copy of the f32 wide nodes (8 float4 each) at
 // lnodes and of the triangles at ltris; they are read from there.
 template <bool kAny, bool kInst, int kLdsDepth, bool kCount, bool kLdsScene, class Start, class Finish>
@@ -329,9 +332,19 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
   const uint32_t lane = __lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  const uint32_t chunk_n = 1u << cs;  // rays per chunk = idle lanes a refill needs
-  const uint32_t nchunks = (count + chunk_n - 1) >> cs;
-  uint32_t chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // wave-uniform: next chunk
+  const uint32_t chunk_n = 1u << cs;  // idle lanes a refill needs
+  // span length 2^ss (wave-uniform): 16 spans per wave at least, 256 rays at most
+  uint32_t ss = cs;
+  while (ss < 8u && (((uint64_t)waves * 16u) << (ss + 1)) <= count) ++ss;
+  // the start and the end of span k (both count past the queue's end)
+  const auto span_range = [&](uint32_t k, uint32_t &b, uint32_t &e) {
+    const uint64_t b64 = (uint64_t)k << ss;
+    b = b64 >= count ? count : (uint32_t)b64;
+    e = min(b + (1u << ss), count);
+  };
+  uint32_t span = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // wave-uniform: current span
+  uint32_t cur, end;  // the wave's next ray, the end of its current span
+  span_range(span, cur, end);
   uint32_t ray = kNoRay, ref = kDoneRef, parked = kDoneRef;
   f3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f}, idir{0.f, 0.f, 0.f}, oid{0.f, 0.f, 0.f};
   uint32_t neg = 0;  // dir_signs of the ray
@@ -339,16 +352,19 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
   uint32_t bid = GI_INVALID_REF;
   uint32_t vbase = 0;  // virtual id base of the current space (0: world)
   for (;;) {
-    // ---- refill: the idle lanes, in lane order, take whole chunks
+    // ---- refill: the idle lanes, in lane order, take the wave's next rays:
+    // the rest of its current span, then the start of its next one
     const uint64_t idle = __ballot(ray == kNoRay);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
-    if (n_idle >= chunk_n && chunk < nchunks) {
+    if (n_idle >= chunk_n && cur < count) {
+      const uint32_t rem = end - cur;  // >= 1
+      uint32_t nb, ne;
+      span_range(span + waves, nb, ne);
+      const uint32_t n_take = min(n_idle, rem + (ne - nb));
       if (ray == kNoRay) {
         const uint32_t r = (uint32_t)__popcll(idle & below);
-        const uint32_t c = chunk + (r >> cs) * waves;
-        const uint32_t i = (c << cs) + (r & (chunk_n - 1));
-        if (r < (n_idle & ~(chunk_n - 1)) && c < nchunks && i < count) {
-          ray = i;
+        if (r < n_take) {
+          ray = r < rem ? cur + r : nb + (r - rem);
           if constexpr (kCount) ++c_rays;
           start(ray, o, d, tmax);
           idir = f3{rcp_clamped(d.x), rcp_clamped(d.y), rcp_clamped(d.z)};
@@ -362,10 +378,20 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
           st.sp = 0;
         }
       }
-      chunk += (n_idle >> cs) * waves;
+      if (n_take < rem) {
+        cur += n_take;
+      } else {
+        span += waves;
+        cur = nb + (n_take - rem);
+        end = ne;
+        if (cur >= end) {
+          span += waves;
+          span_range(span, cur, end);
+        }
+      }
     }
     if (__ballot(ray != kNoRay) == 0) {
-      if (chunk >= nchunks) break;
+      if (cur >= count) break;
       continue;
     }
     if (ray == kNoRay) continue;
