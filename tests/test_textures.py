@@ -237,3 +237,50 @@ def test_texel_gradients_fp32_vs_fp64_and_white_texture(oracle):
     gp = oracle.adjoint(oracle.OracleScene(plain.tris, plain.mats, plain.lights), plain.views, plain.opts, dL,
                         threads=4)[0]
     assert np.allclose(gt, gp, rtol=1e-12, atol=0)
+
+
+def _rn32(fr):
+    """The float32 nearest to the rational fr (ties to even)."""
+    from fractions import Fraction
+    c = np.float32(float(fr))
+    best = None
+    for cand in (np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))):
+        d = abs(Fraction(float(cand)) - fr)
+        if best is None or d < best[0] or (d == best[0] and int(np.float32(cand).view(np.uint32)) & 1 == 0):
+            best = (d, np.float32(cand))
+    return best[1]
+
+
+def test_fma_corrected_division_by_255_is_the_ieee_quotient():
+    # the kernels take a texel channel c / 255 as q0 = c * RN(1/255) and
+    # fma(fma(-255, q0, c), RN(1/255), q0); with each fma rounded once
+    # (exact rationals here) that equals the IEEE quotient for every byte
+    from fractions import Fraction
+    r = np.float32(1) / np.float32(255)
+    for c in range(256):
+        x = np.float32(c)
+        q0 = np.float32(x * r)
+        e = _rn32(Fraction(-255) * Fraction(float(q0)) + Fraction(float(x)))
+        assert Fraction(float(e)) == Fraction(-255) * Fraction(float(q0)) + Fraction(float(x))  # exact residual
+        q = _rn32(Fraction(float(e)) * Fraction(float(r)) + Fraction(float(q0)))
+        assert q == x / np.float32(255), c
+
+
+def test_texel_index_wrap_by_selects():
+    # the lookup's indices lie in [-1, n + 1] (x = s n with s in [0, 1), and
+    # one neighbour either side); three selects give the modulo wrap there
+    def wrap(i, n):
+        i = i + n if i < 0 else i
+        i = i - n if i >= n else i
+        return i - n if i >= n else i
+    for n in range(1, 70):
+        for i in range(-1, n + 2):
+            assert wrap(i, n) == ((i % n) + n) % n == i % n
+    # x = s * W rounds to at most W for the largest s below 1, and 1 - t to
+    # at most 1 for t in [0, 1)
+    s_max = np.nextafter(np.float32(1), np.float32(0))
+    for w in list(range(1, 5000)) + [8191, 8192, 16383, 16384, 1 << 20]:
+        x = np.float32(s_max * np.float32(w))
+        assert 0 <= int(x) <= w
+    t = np.random.default_rng(3).random(10000).astype(np.float32)
+    assert np.all((np.float32(1) - t) <= 1) and np.all((np.float32(1) - t) > 0)
