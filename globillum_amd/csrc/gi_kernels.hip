@@ -336,7 +336,7 @@ __device__ __forceinline__ void trace_persistent(uint32_t count, uint32_t cs, co
   // span length 2^ss (wave-uniform): 16 spans per wave at least, 256 rays at most
   uint32_t ss = cs;
   while (ss < 8u && (((uint64_t)waves * 16u) << (ss + 1)) <= count) ++ss;
-  // the start and the end of span k (both count past the queue's end)
+  // the start and the end of span k, both clamped to count
   const auto span_range = [&](uint32_t k, uint32_t &b, uint32_t &e) {
     const uint64_t b64 = (uint64_t)k << ss;
     b = b64 >= count ? count : (uint32_t)b64;
