@@ -122,9 +122,13 @@ __device__ __forceinline__ float cdf_sample(const float *__restrict__ cdf, const
 }
 
 // ---------------------------------------------------------------- Halton --
+// chunks holds the chunk tables from entry chunk_base of the pool on (0: the
+// whole pool in global memory; an LDS copy of some dimensions' tables starts
+// at the first one's offset)
 struct HaltonDev {
   const HaltonDim *dims;
   const uint16_t *chunks;
+  uint32_t chunk_base;
 };
 // Scrambled radical inverse of n in the prime base p of dimension dim: the
 // digits d_k (K of them) map to perm[d_k] and the infinitely many leading
@@ -145,7 +149,7 @@ struct HaltonDev {
 constexpr int kHaltonMaxChunks = 7;  // kmax <= 7: every dimension has P >= 37
 __device__ __forceinline__ float halton(uint32_t n, int dim, HaltonDev H) {
   const HaltonDim hd = H.dims[dim];
-  const uint16_t *chunk = H.chunks + hd.chunk_off;
+  const uint16_t *chunk = H.chunks + (hd.chunk_off - H.chunk_base);
   const uint32_t mlo = (uint32_t)hd.magic, mhi = (uint32_t)(hd.magic >> 32);
   uint32_t c[kHaltonMaxChunks];
   uint32_t m = n;
@@ -182,14 +186,16 @@ constexpr uint32_t kHaltonLdsEntries = 10240;
 // Copies the chunk tables of dimensions [d0, d0 + nd) into lds and returns
 // the sampler that reads them there: the same values, with LDS reads in
 // place of the scattered 16-bit global loads.  Called by every thread of
-// the block, before any of them samples.
+// the block, before any of them samples.  (The copy is addressed from its
+// own start, chunk_base = the first table's offset: a pointer moved below
+// the array would leave it, and in a flat address the borrow is lost.)
 __device__ __forceinline__ HaltonDev halton_in_lds(HaltonDev H, int d0, int nd, uint16_t *lds) {
   const HaltonDim first = H.dims[d0], last = H.dims[d0 + nd - 1];
-  const uint32_t off0 = first.chunk_off;
-  const uint32_t n = min(last.chunk_off + last.P - off0, kHaltonLdsEntries);
+  const uint32_t off0 = first.chunk_off - H.chunk_base;
+  const uint32_t n = min(last.chunk_off + last.P - first.chunk_off, kHaltonLdsEntries);
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = H.chunks[off0 + i];
   __syncthreads();
-  return HaltonDev{H.dims, lds - off0};
+  return HaltonDev{H.dims, lds, first.chunk_off};
 }
 
 // ---------------------------------------------------------- frames -------

@@ -2530,7 +2530,7 @@ ShadeParams shade_params(gi_ctx *ctx, int32_t depth) {
   P.lights = ctx->d_lights.as<DevLight>();
   P.nlights = (int32_t)ctx->lights.size();
   P.power_total = ctx->power_total;
-  P.H = HaltonDev{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>()};
+  P.H = HaltonDev{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>(), 0u};
   P.off_b = ctx->opts.offset_bounce > 0.f ? ctx->opts.offset_bounce : 1e-4f * ctx->diag;
   P.off_s = ctx->opts.offset_shadow > 0.f ? ctx->opts.offset_shadow : 1e-4f * ctx->diag;
   for (int k = 0; k < 3; ++k) P.bg[k] = ctx->opts.background[k];
@@ -3746,7 +3746,7 @@ int gi_generate_camera_rays(gi_ctx *ctx, const gi_view *v, int32_t sample, uint3
   GI_HIP(ctx, copy_sync(ctx, ctx->d_cam_view.p, &dv, sizeof(dv), hipMemcpyHostToDevice));
   rc = ensure_workspace(ctx, npix);
   if (rc) return rc;
-  HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>()};
+  HaltonDev H{ctx->d_hdims.as<HaltonDim>(), ctx->d_hchunks.as<uint16_t>(), 0u};
   hipLaunchKernelGGL(k_raygen, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_cam_pass.as<PassDesc>(),
                      ctx->d_cam_view.as<DevView>(), H, ctx->q_o[0].as<float4>(), ctx->q_d[0].as<float4>(),
                      ctx->path_l.as<float4>(), (float4 *)nullptr, 0u, 1);

@@ -70,7 +70,7 @@ int main(int argc, char **argv) {
       !ok(hipMemcpy(d_dims, dims.data(), dims.size() * sizeof(gi::HaltonDim), hipMemcpyHostToDevice), "copy") ||
       !ok(hipMemcpy(d_chunks, chunks.data(), chunks.size() * sizeof(uint16_t), hipMemcpyHostToDevice), "copy"))
     return 2;
-  const gi::dev::HaltonDev H{static_cast<const gi::HaltonDim *>(d_dims), static_cast<const uint16_t *>(d_chunks)};
+  const gi::dev::HaltonDev H{static_cast<const gi::HaltonDim *>(d_dims), static_cast<const uint16_t *>(d_chunks), 0u};
   if (count > 0)
     hipLaunchKernelGGL(k_halton_eval, dim3((count + 255) / 256), dim3(256), 0, 0,
                        static_cast<const uint32_t *>(d_pairs), count, H, static_cast<float *>(d_out));

@@ -42,6 +42,23 @@ def test_c1_full_config_bit_exact(oracle, gpu_ctx_factory):
     assert c["samples"] == 256 * 256 * 16
 
 
+def test_depth_32_bit_exact(oracle, gpu_ctx_factory):
+    # the deepest render samples dimensions up to 227 (primes up to 1439);
+    # each shading launch copies its vertex's seven chunk tables into LDS,
+    # the last window (dimensions 221..227) holding 9959 entries.  No
+    # Russian roulette, so some paths of the open box reach the last vertices
+    sc = scenes.cornell_boxes(w=48, h=48, spp=16, depth=32)
+    ref, st = _oracle(oracle, sc)
+    _, st24 = _oracle(oracle, sc, opts=scenes.make_opts(24))
+    assert st["rays_closest"] > st24["rays_closest"]  # paths went past vertex 24
+    for flags in (0, FLAG_BATCHED):
+        with gpu_ctx_factory(sc) as ctx:
+            img = ctx.render(flags=flags).image(0).reshape(-1, 3)
+            c = ctx.counters()
+        _same(img, ref)
+        assert c["rays_closest"] == st["rays_closest"] and c["rays_shadow"] == st["rays_shadow"]
+
+
 @pytest.mark.parametrize("flags", [FLAG_BATCHED, FLAG_BATCHED | FLAG_GRAPH])
 def test_plain_batched_graph_identical(flags, gpu_ctx_factory):
     sc = scenes.cornell_boxes(w=128, h=96, spp=8)

@@ -117,7 +117,7 @@ def test_sampler_tables_fit_the_lds_copy():
             P *= p
         sizes.append(P)
     windows = [sum(sizes[0:4])] + [sum(sizes[d:d + 7]) for d in range(4, 4 + 7 * 32, 7)]
-    assert max(windows) == 9959 and max(windows) <= 10240
+    assert windows[0] == 2721 and max(windows) == 9959 and max(windows) <= 10240
 
 
 def test_fixed_chunk_count_keeps_the_rational(oracle):
